@@ -1,0 +1,211 @@
+! Golden-vector driver for the reference CPU path.
+!
+! This program is test infrastructure.  It is compiled together with the
+! reference's own CPU modules (src/half_{kinds,constants,math,types,basis,fft,
+! xc,potential,paw,dense_solver,forces,kpoints,energy,chgcar,potcar}.F90, read
+! in place from the reference tree by oracle/Makefile) and writes plain
+! "key value..." lines that tests/golden/make_golden.py turns into fixtures.
+!
+! Usage:
+!   half_ref_driver solve CHGCAR POTCAR ENCUT lda|pbe NBANDS KX KY KZ VEFF_OUT
+!   half_ref_driver kat
+!
+! "solve" mirrors the CPU context of src/half_library.F90:51-110 (create) and
+! :329-388 (solve_kpoint) plus the Harris-energy assembly of
+! app/half_cli.F90:896-920 and the non-local force of src/half_forces.F90:9-73.
+! "kat" evaluates the reference routines on the known-answer inputs of
+! app/half_energy_check.F90:19-72.
+program half_ref_driver
+  use half_kinds, only: dp, i32, i64
+  use half_types, only: crystal_t, charge_grid_t, plane_wave_basis_t, potcar_t
+  use half_chgcar, only: read_chgcar
+  use half_potcar, only: read_potcar, validate_potcar_structure
+  use half_basis, only: build_plane_wave_basis
+  use half_potential, only: build_veff_lda, build_veff_pbe
+  use half_paw, only: paw_species_t, build_paw_operators
+  use half_dense_solver, only: assemble_dense_gamma, solve_dense_gamma
+  use half_forces, only: add_nonlocal_paw_forces
+  use half_energy, only: compute_occupations, ewald_energy, ewald_forces
+  implicit none
+  character(len=1024) :: mode
+  call get_command_argument(1, mode)
+  select case (trim(mode))
+  case ('solve')
+    call run_solve()
+  case ('kat')
+    call run_kat()
+  case default
+    write(0,'(A)') 'usage: half_ref_driver solve|kat ...'
+    stop 2
+  end select
+contains
+  subroutine run_solve()
+    character(len=1024) :: chg_path, pot_path, arg, veff_path
+    type(crystal_t) :: crystal
+    type(charge_grid_t) :: charge
+    type(potcar_t), allocatable :: potcars(:)
+    type(plane_wave_basis_t) :: basis
+    type(paw_species_t), allocatable :: paw(:)
+    real(dp), allocatable :: veff(:), values(:), occ(:,:), eig2(:,:), charges(:), forces(:,:)
+    complex(dp), allocatable :: h(:,:), s(:,:), vectors(:,:), psi(:,:), hpsi(:,:), spsi(:,:)
+    real(dp) :: encut, kpt(3), eh, exc, exv, smin, smax, mu, band, entropy, ewald, local_g0, atomic, nelect
+    real(dp) :: weights(1)
+    integer :: nbands, i, j, it, ion, iat, unit, ib, ig
+    integer(i64) :: n
+    logical :: pbe
+
+    call get_command_argument(2, chg_path); call get_command_argument(3, pot_path)
+    call get_command_argument(4, arg); read(arg, *) encut
+    call get_command_argument(5, arg); pbe = trim(arg) == 'pbe'
+    call get_command_argument(6, arg); read(arg, *) nbands
+    do i = 1, 3
+      call get_command_argument(6 + i, arg); read(arg, *) kpt(i)
+    end do
+    call get_command_argument(10, veff_path)
+
+    call read_chgcar(trim(chg_path), crystal, charge)
+    call read_potcar(trim(pot_path), potcars)
+    call validate_potcar_structure(potcars, crystal)
+    if (pbe) then
+      call build_veff_pbe(charge, potcars, crystal, veff, eh, exc, exv)
+    else
+      call build_veff_lda(charge, potcars, crystal, veff, eh, exc, exv)
+    end if
+    write(*,'(A,I0)') 'ngrid ', size(veff)
+    write(*,'(A,3ES25.17)') 'energy_terms ', eh, exc, exv
+    write(*,'(A,ES25.17)') 'volume ', crystal%volume
+    write(*,'(A,ES25.17)') 'nelect_grid ', charge%electron_count()
+    open(newunit=unit, file=trim(veff_path), access='stream', form='unformatted', status='replace')
+    write(unit) veff
+    close(unit)
+
+    call build_plane_wave_basis(crystal, charge%shape, encut, kpt, basis)
+    n = basis%npw
+    write(*,'(A,I0)') 'npw ', n
+    call build_paw_operators(potcars, crystal, basis, paw)
+    call assemble_dense_gamma(veff, basis, paw, h, s)
+    do j = 1, int(n), 41
+      do i = 1, int(n), 37
+        write(*,'(A,2I7,4ES25.17)') 'hs ', i, j, real(h(i,j)), aimag(h(i,j)), real(s(i,j)), aimag(s(i,j))
+      end do
+    end do
+    do i = 1, int(n)
+      write(*,'(A,I7,2ES25.17)') 'hdiag ', i, real(h(i,i)), real(s(i,i))
+    end do
+    ! deterministic state block and its H/S images (half_apply_hs on the CPU
+    ! backend is matmul(h, psi), src/half_library.F90:322-326)
+    allocate(psi(n, 3))
+    do ib = 1, 3
+      do ig = 1, int(n)
+        psi(ig, ib) = cmplx(cos(0.37_dp*ig + ib), sin(0.11_dp*ig*ib), dp)
+      end do
+    end do
+    hpsi = matmul(h, psi); spsi = matmul(s, psi)
+    do ib = 1, 3
+      do ig = 1, int(n)
+        write(*,'(A,2I7,4ES25.17)') 'apply ', ig, ib, real(hpsi(ig,ib)), aimag(hpsi(ig,ib)), &
+          real(spsi(ig,ib)), aimag(spsi(ig,ib))
+      end do
+    end do
+    deallocate(h, s)
+
+    call solve_dense_gamma(veff, basis, paw, values, smin, smax, vectors)
+    write(*,'(A,2ES25.17)') 'overlap ', smin, smax
+    do i = 1, nbands
+      write(*,'(A,I5,ES25.17)') 'eig ', i, values(i)
+    end do
+
+    ! Harris energy with zero-temperature occupations of the lowest NBANDS
+    ! (app/half_cli.F90:896-920; use_uspp is off for these inputs)
+    nelect = 0
+    do it = 1, size(potcars)
+      nelect = nelect + potcars(it)%zval*real(crystal%counts(it), dp)
+    end do
+    weights = 1.0_dp
+    allocate(eig2(1, nbands)); eig2(1,:) = values(:nbands)
+    call compute_occupations(eig2, weights, nelect, 0.0_dp, occ, mu, band, entropy)
+    allocate(charges(crystal%nions)); ion = 0
+    do it = 1, size(potcars)
+      do iat = 1, crystal%counts(it); ion = ion + 1; charges(ion) = potcars(it)%zval; end do
+    end do
+    call ewald_energy(crystal, charges, ewald)
+    atomic = 0; local_g0 = 0
+    do it = 1, size(potcars)
+      atomic = atomic + potcars(it)%eatom*real(crystal%counts(it), dp)
+      local_g0 = local_g0 + potcars(it)%psp_local(1)*real(crystal%counts(it), dp)
+    end do
+    local_g0 = local_g0*nelect/crystal%volume
+    write(*,'(A,ES25.17)') 'nelect ', nelect
+    write(*,'(A,ES25.17)') 'band_energy ', band
+    write(*,'(A,ES25.17)') 'fermi ', mu
+    write(*,'(A,ES25.17)') 'ewald ', ewald
+    write(*,'(A,ES25.17)') 'local_g0 ', local_g0
+    write(*,'(A,ES25.17)') 'atomic_reference ', atomic
+    write(*,'(A,ES25.17)') 'harris_energy ', band - eh - exv + exc + ewald + local_g0 + atomic
+    do i = 1, nbands
+      write(*,'(A,I5,ES25.17)') 'occ ', i, occ(1, i)
+    end do
+
+    ! non-local projector force for the occupied bands (k weight 1)
+    allocate(forces(crystal%nions, 3)); forces = 0
+    call add_nonlocal_paw_forces(basis, paw, values(:nbands), occ(1,:), 1.0_dp, vectors(:, :nbands), forces)
+    do i = 1, crystal%nions
+      write(*,'(A,I5,3ES25.17)') 'force_nl ', i, forces(i, :)
+    end do
+  end subroutine run_solve
+
+  subroutine run_kat()
+    ! Known-answer inputs of app/half_energy_check.F90:19-72.
+    type(crystal_t) :: crystal
+    type(plane_wave_basis_t) :: basis
+    type(paw_species_t), allocatable :: paw(:)
+    real(dp) :: eig(2,3), weight(2), mu, band, entropy, e1, e2, analytic(2,3), phase
+    real(dp) :: nlforce(1,3), eval(1), focc(1), r0(3)
+    complex(dp) :: waves(3,1), projector0(2,3)
+    real(dp), allocatable :: occ(:,:)
+    integer :: ig, i
+
+    eig = 0.0_dp; eig(:,1:2) = reshape([0.0_dp,1.0_dp,2.0_dp,3.0_dp],[2,2]); weight = [0.5_dp,0.5_dp]
+    call compute_occupations(eig(:,1:2), weight, 2.0_dp, 0.0_dp, occ, mu, band, entropy)
+    write(*,'(A,4ES25.17,3ES25.17)') 'occ_zero ', occ, mu, band, entropy
+    eig = reshape([-1.0_dp,-0.8_dp,0.5_dp,0.7_dp,2.0_dp,2.2_dp],[2,3]); weight = [0.25_dp,0.75_dp]
+    call compute_occupations(eig, weight, 2.0_dp, 0.1_dp, occ, mu, band, entropy, -1)
+    write(*,'(A,6ES25.17,3ES25.17)') 'occ_fermi ', occ, mu, band, entropy
+    call compute_occupations(eig, weight, 2.0_dp, 0.1_dp, occ, mu, band, entropy, 0)
+    write(*,'(A,6ES25.17,3ES25.17)') 'occ_gauss ', occ, mu, band, entropy
+
+    crystal%nions = 2; crystal%ntypes = 1; crystal%lattice = 0.0_dp
+    crystal%lattice(1,1) = 4.0_dp; crystal%lattice(2,2) = 4.0_dp; crystal%lattice(3,3) = 4.0_dp
+    allocate(crystal%counts(1), crystal%species(1), crystal%positions(2,3))
+    crystal%counts = 2; crystal%species = 'X'
+    crystal%positions(1,:) = [0.07_dp,0.11_dp,0.19_dp]
+    crystal%positions(2,:) = [0.31_dp,0.46_dp,0.73_dp]
+    call crystal%update_geometry()
+    call ewald_energy(crystal, [1.0_dp,1.0_dp], e1, eta=0.35_dp, tolerance=1e-12_dp)
+    call ewald_energy(crystal, [1.0_dp,1.0_dp], e2, eta=0.80_dp, tolerance=1e-12_dp)
+    write(*,'(A,2ES25.17)') 'ewald_kat ', e1, e2
+    call ewald_forces(crystal, [1.0_dp,1.0_dp], analytic, eta=0.35_dp, tolerance=1e-12_dp)
+    do i = 1, 2
+      write(*,'(A,I3,3ES25.17)') 'ewald_force_kat ', i, analytic(i,:)
+    end do
+
+    basis%npw = 3; allocate(basis%qvectors(3,3))
+    basis%qvectors = reshape([0.2_dp,0.7_dp,-0.4_dp, -0.3_dp,0.5_dp,0.8_dp, 0.6_dp,-0.2_dp,0.9_dp],[3,3])
+    allocate(paw(1)); paw(1)%natoms = 1; paw(1)%nlm = 2
+    allocate(paw(1)%projectors(1,2,3), paw(1)%dij(2,2), paw(1)%qij(2,2), paw(1)%dij_atom(1,2,2))
+    projector0 = reshape([cmplx(0.4_dp,0.1_dp,dp), cmplx(-0.2_dp,0.3_dp,dp), &
+      cmplx(0.1_dp,-0.2_dp,dp), cmplx(0.5_dp,0.4_dp,dp), cmplx(-0.3_dp,0.2_dp,dp), &
+      cmplx(0.2_dp,0.6_dp,dp)], [2,3])
+    waves(:,1) = [cmplx(0.3_dp,-0.1_dp,dp), cmplx(-0.2_dp,0.4_dp,dp), cmplx(0.5_dp,0.2_dp,dp)]
+    paw(1)%dij_atom(1,:,:) = reshape([1.2_dp,0.15_dp,0.15_dp,0.8_dp],[2,2])
+    paw(1)%dij = paw(1)%dij_atom(1,:,:); paw(1)%qij = reshape([0.1_dp,0.02_dp,0.02_dp,0.07_dp],[2,2])
+    eval = [0.37_dp]; focc = [1.6_dp]; r0 = [0.13_dp,-0.21_dp,0.08_dp]
+    do ig = 1, 3
+      phase = dot_product(basis%qvectors(ig,:), r0)
+      paw(1)%projectors(1,:,ig) = projector0(:,ig)*cmplx(cos(phase), -sin(phase), dp)
+    end do
+    nlforce = 0.0_dp
+    call add_nonlocal_paw_forces(basis, paw, eval, focc, 0.4_dp, waves, nlforce)
+    write(*,'(A,3ES25.17)') 'nonlocal_force_kat ', nlforce(1,:)
+  end subroutine run_kat
+end program half_ref_driver

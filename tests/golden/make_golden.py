@@ -1,0 +1,134 @@
+"""Write the golden fixtures of this directory from the reference CPU path.
+
+Runs oracle/_ref/half_ref_driver (the reference's own Fortran CPU modules,
+compiled by oracle/Makefile against MKL) on the seeded synthetic inputs and
+stores its outputs as JSON.  Run from the repository root after
+`make -C oracle`:
+
+    python tests/golden/make_golden.py [case ...]
+
+Cases: si2_pbe, si2_lda, si2_k (PBE at k = (0.1, 0.2, 0.3)), hfo2_pbe, kat.
+The si2 input files are committed next to this script; the hfo2 inputs are
+regenerated by the seeded generator and their sha256 is recorded so a test can
+refuse to compare against goldens made from different inputs.
+"""
+import json
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "deepaw-half_amd"))
+from half_amd import synth  # noqa: E402
+
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "half_ref_driver")
+INPUTS = os.path.join(HERE, "inputs")
+
+CASES = {
+    "si2_pbe": ("si2", "pbe", (0.0, 0.0, 0.0), 16),
+    "si2_lda": ("si2", "lda", (0.0, 0.0, 0.0), 16),
+    "si2_k": ("si2", "pbe", (0.1, 0.2, 0.3), 16),
+    "hfo2_pbe": ("hfo2", "pbe", (0.0, 0.0, 0.0), 64),
+}
+
+_FLOAT = re.compile(r"^([-+]?\d+\.\d+)([-+]\d{3})$")
+
+
+def _num(tok):
+    # Fortran ES format drops the exponent letter for 3-digit exponents
+    m = _FLOAT.match(tok)
+    if m:
+        tok = m.group(1) + "E" + m.group(2)
+    return float(tok)
+
+
+def parse(text):
+    out = {}
+    for line in text.splitlines():
+        parts = line.split()
+        if not parts:
+            continue
+        out.setdefault(parts[0], []).append([_num(t) for t in parts[1:]])
+    return out
+
+
+def run_case(name):
+    wl, xc, k, nb = CASES[name]
+    if wl == "si2":
+        chg = os.path.join(INPUTS, "si2.CHGCAR")
+        pot = os.path.join(INPUTS, "si2.POTCAR")
+        if not os.path.exists(chg):
+            os.makedirs(INPUTS, exist_ok=True)
+            chg, pot, _ = synth.make_inputs("si2", INPUTS)
+    else:
+        chg, pot, _ = synth.make_inputs(wl, os.path.join(ROOT, "_work", "inputs"))
+    w = synth.workload(wl)
+    veff_path = os.path.join(HERE, f"{name}.veff.bin.tmp")
+    cmd = [DRIVER, "solve", chg, pot, repr(w.encut), xc, str(nb), *map(repr, k), veff_path]
+    text = subprocess.run(cmd, check=True, capture_output=True, text=True).stdout
+    d = parse(text)
+    veff = np.fromfile(veff_path, dtype="<f8")
+    os.remove(veff_path)
+    g = {
+        "workload": wl, "xc": xc, "kpoint": list(k), "nbands": nb, "encut": w.encut,
+        "chgcar_sha256": synth.sha256(chg), "potcar_sha256": synth.sha256(pot),
+        "ngrid": int(d["ngrid"][0][0]), "npw": int(d["npw"][0][0]),
+        "e_hartree": d["energy_terms"][0][0], "e_xc": d["energy_terms"][0][1],
+        "e_xc_potential": d["energy_terms"][0][2], "volume": d["volume"][0][0],
+        "overlap_min": d["overlap"][0][0], "overlap_max": d["overlap"][0][1],
+        "eigenvalues": [r[1] for r in d["eig"]],
+        "occupations": [r[1] for r in d["occ"]],
+        "band_energy": d["band_energy"][0][0], "fermi": d["fermi"][0][0],
+        "ewald": d["ewald"][0][0], "local_g0": d["local_g0"][0][0],
+        "atomic_reference": d["atomic_reference"][0][0],
+        "harris_energy": d["harris_energy"][0][0],
+        "force_nl": [r[1:] for r in d["force_nl"]],
+        # (i, j) one-based, Re H, Im H, Re S, Im S
+        "hs_samples": d["hs"],
+        "h_diag": [r[1] for r in d["hdiag"]],
+        "s_diag": [r[2] for r in d["hdiag"]],
+        "veff_sum": float(veff.sum()), "veff_sumsq": float((veff * veff).sum()),
+        "veff_min": float(veff.min()), "veff_max": float(veff.max()),
+    }
+    if wl == "si2":
+        # full C-order Veff grid and the three-column apply image
+        np.save(os.path.join(HERE, f"{name}.veff.npy"), veff)
+        ap = np.array(d["apply"])
+        n = g["npw"]
+        hpsi = (ap[:, 2] + 1j * ap[:, 3]).reshape(3, n).T
+        spsi = (ap[:, 4] + 1j * ap[:, 5]).reshape(3, n).T
+        np.save(os.path.join(HERE, f"{name}.apply.npy"), np.stack([hpsi, spsi]))
+    else:
+        idx = np.arange(0, veff.size, max(1, veff.size // 2000))
+        g["veff_sample_index"] = idx.tolist()
+        g["veff_sample"] = veff[idx].tolist()
+    with open(os.path.join(HERE, f"{name}.json"), "w") as fh:
+        json.dump(g, fh, indent=1)
+    print(name, "npw", g["npw"], "eig0", g["eigenvalues"][0])
+
+
+def run_kat():
+    text = subprocess.run([DRIVER, "kat"], check=True, capture_output=True, text=True).stdout
+    d = parse(text)
+    g = {
+        "occ_zero": d["occ_zero"][0], "occ_fermi": d["occ_fermi"][0],
+        "occ_gauss": d["occ_gauss"][0], "ewald_kat": d["ewald_kat"][0],
+        "ewald_force_kat": [r[1:] for r in d["ewald_force_kat"]],
+        "nonlocal_force_kat": d["nonlocal_force_kat"][0],
+    }
+    with open(os.path.join(HERE, "kat.json"), "w") as fh:
+        json.dump(g, fh, indent=1)
+    print("kat", g["nonlocal_force_kat"])
+
+
+if __name__ == "__main__":
+    todo = sys.argv[1:] or list(CASES) + ["kat"]
+    for c in todo:
+        if c == "kat":
+            run_kat()
+        else:
+            run_case(c)
