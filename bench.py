@@ -1,0 +1,186 @@
+"""H.psi band-application throughput and all-band solve time on one MI355X.
+
+Workload (default): the monoclinic HfO2-like 12-atom cell (4 Hf + 8 O),
+Gamma, 3407 plane waves, 48^3 box, 136 projectors, PBE, 64 bands; seeded
+synthetic density and PAW datasets (no real inputs exist offline).
+
+One step = one application of H and S (FP64, complex128) to a resident
+64-band block: scatter, backward FFT, Veff multiply, forward FFT, gather +
+kinetic, projector contraction B^H psi on the matrix cores, D/Q mixing,
+expansion B(Dc), B(Qc).  value = band applications per second over the
+timed steps (inputs resident in HBM, no host traffic in the timed region).
+
+Also reported: the FP32 operator's rate, the all-band solve times of the
+plain FP64 and the mixed form to <= 1e-8 eV of the reference eigenvalues,
+the dominant kernel's roofline, and the oracle's CPU rate on a bounded
+sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "deepaw-half_amd"))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+MFMA_FP64_TFLOPS = 78.6        # dense FP64 matrix
+MFMA_FP32_TFLOPS = 157.3
+
+
+def algorithmic_bytes(nr, n, fp64=True):
+    """Per band application: each FFT pass pair reads and writes the box
+    (2 transforms x (read + write)), Veff read once, and the plane-wave side
+    psi, H psi, S psi, kinetic and the int32 index map."""
+    sc, sr = (16, 8) if fp64 else (8, 4)
+    return 4 * nr * sc + nr * sr + n * (3 * sc + sr + 4)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="hfo2")
+    ap.add_argument("--bands", type=int, default=0)
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the k-point and band sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import half_amd as H
+    from half_amd import synth
+
+    w = synth.workload(args.workload)
+    nb = args.bands or w.nbands
+    tmp = tempfile.mkdtemp(prefix="halfbench")
+    chg, pot, _ = synth.make_inputs(args.workload, tmp)
+    dev_name, cus = H.device_info()
+    print(f"bench: {dev_name} ({cus} CUs), workload {args.workload}", file=sys.stderr, flush=True)
+    ctx = H.Context(chg, pot, w.encut, w.xc, H.BACKEND_CUDA, H.SOLVER_EVD)
+    k = tuple(w.kpoints[0])
+    info = ctx.system_info()
+    nr = int(np.prod(info["grid"]))
+    dev = torch.device("cuda")
+    stream = torch.cuda.current_stream()
+    rng = np.random.default_rng(11)
+
+    rates = {}
+    per_step = {}
+    for prec, dt in ((64, torch.complex128), (32, torch.complex64)):
+        op = ctx.operator(k, prec)
+        n = op.npw
+        x = torch.from_numpy(rng.standard_normal((nb, n)) + 1j * rng.standard_normal((nb, n))).to(dev, dt)
+        hx, sx = torch.empty_like(x), torch.empty_like(x)
+
+        def step():
+            op.apply_device(nb, x.data_ptr(), hx.data_ptr(), sx.data_ptr(), ld=n, stream=stream.cuda_stream)
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(args.steps):
+            step()
+        t1.record(stream)
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / args.steps
+        per_step[prec] = ms
+        rates[prec] = nb / (ms * 1e-3)
+        print(f"bench: fp{prec} H.psi {ms:.3f} ms/step, {rates[prec]:.1f} band-applications/s",
+              file=sys.stderr, flush=True)
+        op.close()
+        del x, hx, sx
+    n = ctx.basis_size(k)
+    nproj = ctx.operator(k, 64).nproj
+
+    solve = {}
+    if not args.no_solve:
+        g = None
+        gpath = os.path.join(ROOT, "tests", "golden", f"{w.extra.get('inputs', w.name)}_pbe.json")
+        if os.path.exists(gpath):
+            with open(gpath) as fh:
+                g = json.load(fh)
+        for mixed in (False, True):
+            t = time.perf_counter()
+            wv, _, st = ctx.solve_acc(nb, k, mixed=mixed, tol=1e-5, maxiter=400, extra=8)
+            el = time.perf_counter() - t
+            key = "mixed" if mixed else "fp64"
+            solve[key] = {"seconds": el, "iterations": st["iterations"], "fp64_iterations": st["fp64_iterations"],
+                          "max_residual_eV": st["max_residual_eV"], "band_applications": st["band_applications"]}
+            if g is not None and len(g["eigenvalues"]) >= nb:
+                solve[key]["max_abs_err_vs_reference_eV"] = float(np.abs(wv - np.array(g["eigenvalues"][:nb])).max())
+            print(f"bench: {key} all-band solve {el:.3f} s {solve[key]}", file=sys.stderr, flush=True)
+        if "fp64" in solve and "mixed" in solve:
+            solve["speedup_mixed_vs_fp64"] = solve["fp64"]["seconds"] / solve["mixed"]["seconds"]
+
+    # roofline of one band application (local stage bytes + projector flops)
+    b_alg = algorithmic_bytes(nr, n, True)
+    f_proj = 24.0 * n * nproj
+    t_band = per_step[64] * 1e-3 / nb
+    achieved_gbs = b_alg / t_band / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "whole H.psi band application (FP64)",
+                "combined_frac": (b_alg / (HBM_PEAK_GBS * 1e9) + f_proj / (MFMA_FP64_TFLOPS * 1e12)) / t_band}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(chg, pot, w, k, nb)
+
+    out = {
+        "metric": "H.psi band-applications/s + all-band solve time at 1e-8 eV, mixed vs FP64, 1 GPU",
+        "value": rates[64], "unit": "band-applications/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": per_step[64], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic (seeded Gaussian density, synthetic PAW datasets)",
+        "config": {"workload": f"{args.workload}: {sum(w.counts)} atoms, {n} plane waves, box {info['grid']}, "
+                               f"{nproj} projectors, {nb} bands, Gamma, {w.xc.upper()}",
+                   "npw": n, "box": list(info["grid"]), "nproj": nproj, "bands": nb},
+        "fp32_value": rates[32], "fp32_ms_per_step": per_step[32],
+        "all_band_solve": solve,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline(chg, pot, w, k, nb):
+    """The oracle's matrix-free H.psi (numpy, FP64) on the host cores for a
+    bounded sample: repeated 8-band applications until ~10 s have elapsed."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import half_oracle as O
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:  # pragma: no cover
+        threadpool_limits = None
+    ctx = O.Context(chg, pot, w.encut, w.xc)
+    basis, paw = ctx.prepare(k)
+    rng = np.random.default_rng(5)
+    psi = rng.standard_normal((basis.kinetic.size, 8)) + 1j * rng.standard_normal((basis.kinetic.size, 8))
+    lim = threadpool_limits(1) if threadpool_limits else None
+    t0 = time.perf_counter()
+    done = 0
+    while time.perf_counter() - t0 < 10.0:
+        O.apply_hs(ctx.veff, basis, paw, psi)
+        done += 8
+    el = time.perf_counter() - t0
+    if lim is not None:
+        lim.unregister() if hasattr(lim, "unregister") else None
+    return {"value": done / el, "unit": "band-applications/s", "cores": 1, "kind": "port",
+            "sample": f"{done} band applications of the oracle's numpy H.psi (FP64) in {el:.1f} s, "
+                      f"same workload, 8-band blocks, BLAS limited to 1 thread"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,710 @@
+// Matrix-free PAW operator H.psi / S.psi and dense H/S assembly for gfx950.
+//
+//   H psi = T psi + F[Veff F^-1[psi]]/N_r + B D B^H psi
+//   S psi = psi + B Q B^H psi
+// (reference: src/half_cuda_assembly.cuf:468-519 for the operator,
+//  :327-400 for dense assembly, :128-170 for the projectors)
+//
+// Local stage: plane waves are scattered into a batch of FFT boxes, the
+// backward transform runs axis 2 then axis 1, then one fused axis-0 pass does
+// backward transform * Veff/N_r * forward transform, then axes 1 and 2
+// forward, and the gather adds T psi.  Boxes are processed box_batch bands at
+// a time to bound the workspace.
+//
+// Projector stage: c = B^H psi, (dc, qc) = (D c, Q c) per atom block,
+// H psi += B dc, S psi += B qc.  The two contractions run on the matrix cores
+// (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32) with complex products built
+// from four real MFMAs; c = B^H psi is split over the plane-wave dimension and
+// reduced in a fixed order.
+#include <algorithm>
+#include <cmath>
+
+#include "cplx.hpp"
+#include "paw.hpp"
+
+namespace half {
+
+namespace {
+
+constexpr int kThreads = 256;
+inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename R>
+struct Acc;
+template <>
+struct Acc<double> {
+  using type = v4d;
+  __device__ static inline v4d mfma(double a, double b, v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Acc<float> {
+  using type = v4f;
+  __device__ static inline v4f mfma(float a, float b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// Complex 16x16 tile accumulate: (cr, ci) += A(16x4) * B(4x16), A and B
+// given as this lane's real/imaginary parts.
+template <typename R>
+__device__ inline void cmfma(R ar, R ai, R br, R bi, typename Acc<R>::type &cr, typename Acc<R>::type &ci) {
+  cr = Acc<R>::mfma(ar, br, cr);
+  cr = Acc<R>::mfma(-ai, bi, cr);
+  ci = Acc<R>::mfma(ar, bi, ci);
+  ci = Acc<R>::mfma(ai, br, ci);
+}
+
+// ---------------------------------------------------------------------------
+// projector generation (reference: src/half_cuda_assembly.cuf:84-170,
+// src/half_paw.F90:52-66)
+// ---------------------------------------------------------------------------
+
+__device__ inline double assoc_legendre_d(int l, int m, double x) {
+  double pmm = 1;
+  if (m > 0) {
+    double fact = 1;
+    const double s = sqrt(fmax(0.0, (1 - x) * (1 + x)));
+    for (int i = 1; i <= m; ++i) {
+      pmm = -pmm * fact * s;
+      fact += 2;
+    }
+  }
+  if (l == m) return pmm;
+  double pmmp1 = x * (2 * m + 1) * pmm;
+  if (l == m + 1) return pmmp1;
+  double pll = 0;
+  for (int ll = m + 2; ll <= l; ++ll) {
+    pll = ((2 * ll - 1) * x * pmmp1 - (ll + m - 1) * pmm) / (ll - m);
+    pmm = pmmp1;
+    pmmp1 = pll;
+  }
+  return pmmp1;
+}
+
+__device__ inline double factorial_d(int n) {
+  double v = 1;
+  for (int i = 2; i <= n; ++i) v *= i;
+  return v;
+}
+
+__device__ inline double real_ylm_d(int l, int m, double theta, double phi) {
+  const int am = abs(m);
+  const double x = cos(theta);
+  const double p = assoc_legendre_d(l, am, x);
+  const double norm = sqrt((2 * l + 1) / (4 * kPi) * factorial_d(l - am) / factorial_d(l + am));
+  const double sgn = (m & 1) ? -1.0 : 1.0;
+  if (m > 0) return sqrt(2.0) * sgn * norm * p * cos(am * phi);
+  if (m < 0) return sqrt(2.0) * sgn * norm * p * sin(am * phi);
+  return norm * p;
+}
+
+__device__ inline double spline_eval_p(const double *y, const double *m2, int n, double xmax, double x) {
+  const double h = xmax / n;
+  int k = static_cast<int>(floor(x / h)) + 1;
+  k = max(1, min(n - 1, k));
+  const double t = (x - (k - 1) * h) / h;
+  const double a = 1 - t;
+  return a * y[k - 1] + t * y[k] + h * h * ((a * a * a - a) * m2[k - 1] + (t * t * t - t) * m2[k]) / 6;
+}
+
+struct ProjArgs {
+  const double *q;  // npw x 3
+  const double *pos;
+  const int *p_atom, *p_l, *p_m, *p_table;
+  const double *p_gmax, *tables, *tables_m2;
+  int n, np, ntab;
+  int64_t ldb;
+  double pref;
+};
+
+// grid: x over plane waves, y over chunks of 16 projectors
+template <typename R>
+__global__ void projector_kernel(cplx_t<R> *B, double2 *B64, ProjArgs a) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.n) return;
+  const double qx = a.q[3 * g], qy = a.q[3 * g + 1], qz = a.q[3 * g + 2];
+  const double qn = sqrt(qx * qx + qy * qy + qz * qz);
+  double theta = 0, phi = 0;
+  if (qn > 1e-10) {
+    theta = acos(fmax(-1.0, fmin(1.0, qz / qn)));
+    phi = atan2(qy, qx);
+  }
+  const int p0 = blockIdx.y * 16;
+  const int p1 = min(a.np, p0 + 16);
+  int last_atom = -1;
+  double c = 1, s = 0;
+  for (int p = p0; p < p1; ++p) {
+    const int at = a.p_atom[p];
+    if (at != last_atom) {
+      const double ph = qx * a.pos[3 * at] + qy * a.pos[3 * at + 1] + qz * a.pos[3 * at + 2];
+      sincos(ph, &s, &c);
+      last_atom = at;
+    }
+    const int l = a.p_l[p], m = a.p_m[p];
+    const double gmax = a.p_gmax[p];
+    const double *tab = a.tables + int64_t(a.p_table[p]) * a.ntab;
+    const double *m2 = a.tables_m2 + int64_t(a.p_table[p]) * a.ntab;
+    double rval = spline_eval_p(tab, m2, a.ntab, gmax, qn);
+    if (qn > 0.99 * gmax) rval = 0;
+    const double v = a.pref * rval * real_ylm_d(l, m, theta, phi);
+    // (-i)^l * v * (cos - i sin)
+    double re = v * c, im = -v * s;
+    for (int k = 0; k < (l & 3); ++k) {
+      const double t = re;
+      re = im;
+      im = -t;
+    }
+    const int64_t idx = g + int64_t(p) * a.ldb;
+    B[idx] = mk<cplx_t<R>>(static_cast<R>(re), static_cast<R>(im));
+    if (B64) B64[idx] = make_double2(re, im);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// local stage
+// ---------------------------------------------------------------------------
+
+template <typename V>
+__global__ void zero_kernel(V *x, int64_t n) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < n) x[i] = mk<V>(0, 0);
+}
+
+template <typename V>
+__global__ void scatter_kernel(const V *psi, int64_t ldpsi, V *box, int64_t nr, const int32_t *idx, int n,
+                               int nb) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(n) * nb) return;
+  const int b = static_cast<int>(t / n);
+  const int g = static_cast<int>(t - int64_t(b) * n);
+  box[b * nr + idx[g]] = psi[g + b * ldpsi];
+}
+
+template <typename V, typename R>
+__global__ void gather_kernel(const V *psi, int64_t ldpsi, const V *box, int64_t nr, const int32_t *idx,
+                              const R *kin, V *hpsi, V *spsi, int64_t ldout, int n, int nb) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(n) * nb) return;
+  const int b = static_cast<int>(t / n);
+  const int g = static_cast<int>(t - int64_t(b) * n);
+  const V p = psi[g + b * ldpsi];
+  const V v = box[b * nr + idx[g]];
+  hpsi[g + b * ldout] = mk<V>(v.x + kin[g] * p.x, v.y + kin[g] * p.y);
+  spsi[g + b * ldout] = p;
+}
+
+// ---------------------------------------------------------------------------
+// projector contractions on the matrix cores
+// ---------------------------------------------------------------------------
+
+// partial[chunk][b][p] = sum_{G in chunk} conj(B[G,p]) psi[G,b]
+// One workgroup = 4 waves = one 16x16 (p, b) tile and one G chunk; the four
+// waves interleave over the chunk's 4-row k steps and reduce in LDS.
+template <typename R>
+__global__ __launch_bounds__(256) void coeff_partial_kernel(const cplx_t<R> *B, int64_t ldb,
+                                                            const cplx_t<R> *psi, int64_t ldpsi,
+                                                            cplx_t<R> *partial, int n, int np, int nb,
+                                                            int chunk, int tiles_p) {
+  using A = typename Acc<R>::type;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int tile = blockIdx.x;
+  const int tp = tile % tiles_p, tb = tile / tiles_p;
+  const int p0 = tp * 16, b0 = tb * 16;
+  const int g_begin = blockIdx.y * chunk;
+  const int g_end = min(n, g_begin + chunk);
+  const int row = lane & 15, kq = lane >> 4;
+  const int p = p0 + row, b = b0 + row;
+  const bool pv = p < np, bv = b < nb;
+  A cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+  for (int g4 = g_begin + 4 * wave; g4 < g_end; g4 += 16) {
+    const int g = g4 + kq;
+    R ar = 0, ai = 0, br = 0, bi = 0;
+    if (g < g_end) {
+      if (pv) {
+        const cplx_t<R> v = B[g + int64_t(p) * ldb];
+        ar = v.x;
+        ai = -v.y;  // conj
+      }
+      if (bv) {
+        const cplx_t<R> v = psi[g + int64_t(b) * ldpsi];
+        br = v.x;
+        bi = v.y;
+      }
+    }
+    cmfma<R>(ar, ai, br, bi, cr, ci);
+  }
+  __shared__ R red[4][2][4][64];
+  for (int i = 0; i < 4; ++i) {
+    red[wave][0][i][lane] = cr[i];
+    red[wave][1][i][lane] = ci[i];
+  }
+  __syncthreads();
+  if (wave == 0) {
+    for (int i = 0; i < 4; ++i) {
+      R sr = red[0][0][i][lane] + red[1][0][i][lane] + red[2][0][i][lane] + red[3][0][i][lane];
+      R si = red[0][1][i][lane] + red[1][1][i][lane] + red[2][1][i][lane] + red[3][1][i][lane];
+      // D layout: row = 4*(lane/16) + i (projector), column = lane%16 (band)
+      const int pr = p0 + 4 * (lane >> 4) + i;
+      const int bc = b0 + (lane & 15);
+      if (pr < np && bc < nb)
+        partial[(int64_t(blockIdx.y) * nb + bc) * np + pr] = mk<cplx_t<R>>(sr, si);
+    }
+  }
+}
+
+template <typename V>
+__global__ void reduce_chunks_kernel(const V *partial, V *coeff, int64_t count, int nchunks) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= count) return;
+  V acc = partial[i];
+  for (int c = 1; c < nchunks; ++c) acc = cadd(acc, partial[c * count + i]);
+  coeff[i] = acc;
+}
+
+// dc = D_atom c, qc = Q_atom c (reference: src/half_cuda_assembly.cuf:220-237)
+template <typename V, typename R>
+__global__ void mix_kernel(const V *coeff, V *dc, V *qc, const int *p_atom, const int *p_lm,
+                           const int *atom_first, const int *atom_nlm, const int *atom_boff, const R *dblk,
+                           const R *qblk, int np, int nb) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(np) * nb) return;
+  const int b = static_cast<int>(t / np);
+  const int p = static_cast<int>(t - int64_t(b) * np);
+  const int at = p_atom[p], a = p_lm[p];
+  const int first = atom_first[at], nlm = atom_nlm[at];
+  const R *drow = dblk + atom_boff[at] + a * nlm;
+  const R *qrow = qblk + atom_boff[at] + a * nlm;
+  const V *c = coeff + int64_t(b) * np + first;
+  V zd = mk<V>(0, 0), zq = mk<V>(0, 0);
+  for (int k = 0; k < nlm; ++k) {
+    const V v = c[k];
+    zd = mk<V>(zd.x + drow[k] * v.x, zd.y + drow[k] * v.y);
+    zq = mk<V>(zq.x + qrow[k] * v.x, zq.y + qrow[k] * v.y);
+  }
+  dc[t] = zd;
+  qc[t] = zq;
+}
+
+// hpsi[G,b] += sum_p B[G,p] dc[p,b];  spsi[G,b] += sum_p B[G,p] qc[p,b]
+// One wave = one 16x16 (G, b) tile for both outputs.
+template <typename R>
+__global__ __launch_bounds__(256) void expand_kernel(const cplx_t<R> *B, int64_t ldb, const cplx_t<R> *dc,
+                                                     const cplx_t<R> *qc, cplx_t<R> *hpsi, cplx_t<R> *spsi,
+                                                     int64_t ldout, int n, int np, int nb, int tiles_g) {
+  using A = typename Acc<R>::type;
+  const int lane = threadIdx.x & 63;
+  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int tiles_b = (nb + 15) / 16;
+  if (tile >= tiles_g * tiles_b) return;
+  const int tg = tile % tiles_g, tb = tile / tiles_g;
+  const int g0 = tg * 16, b0 = tb * 16;
+  const int row = lane & 15, kq = lane >> 4;
+  const int g = g0 + row, b = b0 + row;
+  const bool gv = g < n, bv = b < nb;
+  A hr = {0, 0, 0, 0}, hi = {0, 0, 0, 0}, sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
+  for (int p4 = 0; p4 < np; p4 += 4) {
+    const int p = p4 + kq;
+    R ar = 0, ai = 0, dr = 0, di = 0, qr = 0, qi = 0;
+    if (p < np) {
+      if (gv) {
+        const cplx_t<R> v = B[g + int64_t(p) * ldb];
+        ar = v.x;
+        ai = v.y;
+      }
+      if (bv) {
+        const cplx_t<R> d = dc[p + int64_t(b) * np];
+        const cplx_t<R> q = qc[p + int64_t(b) * np];
+        dr = d.x;
+        di = d.y;
+        qr = q.x;
+        qi = q.y;
+      }
+    }
+    cmfma<R>(ar, ai, dr, di, hr, hi);
+    cmfma<R>(ar, ai, qr, qi, sr, si);
+  }
+  const int bc = b0 + (lane & 15);
+  if (bc >= nb) return;
+  for (int i = 0; i < 4; ++i) {
+    const int gr = g0 + 4 * (lane >> 4) + i;
+    if (gr >= n) continue;
+    const int64_t o = gr + int64_t(bc) * ldout;
+    cplx_t<R> h = hpsi[o], s = spsi[o];
+    hpsi[o] = mk<cplx_t<R>>(h.x + hr[i], h.y + hi[i]);
+    spsi[o] = mk<cplx_t<R>>(s.x + sr[i], s.y + si[i]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// dense assembly (reference: src/half_dense_solver.F90:53-88,
+// src/half_cuda_assembly.cuf:35-57, 239-274)
+// ---------------------------------------------------------------------------
+
+// E[G,p] = sum_q B[G,q] D(q,p), F[G,p] = sum_q B[G,q] Q(q,p) within an atom
+__global__ void row_mix_kernel(const double2 *B, int64_t ldb, double2 *E, double2 *F, const int *p_atom,
+                               const int *p_lm, const int *atom_first, const int *atom_nlm,
+                               const int *atom_boff, const double *dblk, const double *qblk, int n, int np) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(n) * np) return;
+  const int p = static_cast<int>(t / n);
+  const int g = static_cast<int>(t - int64_t(p) * n);
+  const int at = p_atom[p], b = p_lm[p];
+  const int first = atom_first[at], nlm = atom_nlm[at];
+  const double *dcol = dblk + atom_boff[at];
+  const double *qcol = qblk + atom_boff[at];
+  double2 e = make_double2(0, 0), f = make_double2(0, 0);
+  for (int a = 0; a < nlm; ++a) {
+    const double2 v = B[g + int64_t(first + a) * ldb];
+    const double d = dcol[a * nlm + b], q = qcol[a * nlm + b];
+    e.x += v.x * d;
+    e.y += v.y * d;
+    f.x += v.x * q;
+    f.y += v.y * q;
+  }
+  E[g + int64_t(p) * n] = e;
+  F[g + int64_t(p) * n] = f;
+}
+
+// H[i,j] = Vg((Gi-Gj) mod n) + delta_ij T_i + sum_p E[i,p] conj(B[j,p]);
+// S[i,j] = delta_ij + sum_p F[i,p] conj(B[j,p]).  One wave per 16x16 tile.
+__global__ __launch_bounds__(256) void dense_kernel(const double2 *E, const double2 *F, const double2 *B,
+                                                    int64_t ldb, const double2 *vg, const int32_t *gvec,
+                                                    const double *kin, double2 *H, double2 *S, int64_t ld,
+                                                    int n, int np, int tiles, int n1, int n2, int n3) {
+  const int lane = threadIdx.x & 63;
+  const int64_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= int64_t(tiles) * tiles) return;
+  const int ti = static_cast<int>(tile % tiles), tj = static_cast<int>(tile / tiles);
+  const int i0 = ti * 16, j0 = tj * 16;
+  const int row = lane & 15, kq = lane >> 4;
+  const int i = i0 + row, j = j0 + row;
+  const bool iv = i < n, jv = j < n;
+  v4d hr = {0, 0, 0, 0}, hi = {0, 0, 0, 0}, sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
+  for (int p4 = 0; p4 < np; p4 += 4) {
+    const int p = p4 + kq;
+    double er = 0, ei = 0, fr = 0, fi = 0, br = 0, bi = 0;
+    if (p < np) {
+      if (iv) {
+        const double2 e = E[i + int64_t(p) * n], f = F[i + int64_t(p) * n];
+        er = e.x;
+        ei = e.y;
+        fr = f.x;
+        fi = f.y;
+      }
+      if (jv) {
+        const double2 v = B[j + int64_t(p) * ldb];
+        br = v.x;
+        bi = -v.y;  // conj
+      }
+    }
+    cmfma<double>(er, ei, br, bi, hr, hi);
+    cmfma<double>(fr, fi, br, bi, sr, si);
+  }
+  const int jc = j0 + (lane & 15);
+  if (jc >= n) return;
+  for (int k = 0; k < 4; ++k) {
+    const int ir = i0 + 4 * (lane >> 4) + k;
+    if (ir >= n) continue;
+    int a = (gvec[3 * ir] - gvec[3 * jc]) % n1;
+    int b = (gvec[3 * ir + 1] - gvec[3 * jc + 1]) % n2;
+    int c = (gvec[3 * ir + 2] - gvec[3 * jc + 2]) % n3;
+    a += a < 0 ? n1 : 0;
+    b += b < 0 ? n2 : 0;
+    c += c < 0 ? n3 : 0;
+    const double2 v = vg[(int64_t(a) * n2 + b) * n3 + c];
+    double2 h = make_double2(v.x + hr[k], v.y + hi[k]);
+    double2 s = make_double2(sr[k], si[k]);
+    if (ir == jc) {
+      h.x += kin[ir];
+      s.x += 1.0;
+    }
+    H[ir + int64_t(jc) * ld] = h;
+    S[ir + int64_t(jc) * ld] = s;
+  }
+}
+
+// X <- (X + X^H)/2 for H and S (reference: src/half_cuda_assembly.cuf:260-274)
+__global__ void hermitize_kernel(double2 *H, double2 *S, int64_t ld, int n) {
+  const int i = blockIdx.x * 16 + threadIdx.x;
+  const int j = blockIdx.y * 16 + threadIdx.y;
+  if (i >= n || j >= n || i > j) return;
+  const int64_t ij = i + int64_t(j) * ld, ji = j + int64_t(i) * ld;
+  double2 a = H[ij], b = H[ji];
+  double2 zh = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+  H[ij] = zh;
+  H[ji] = make_double2(zh.x, -zh.y);
+  a = S[ij];
+  b = S[ji];
+  double2 zs = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+  S[ij] = zs;
+  S[ji] = make_double2(zs.x, -zs.y);
+}
+
+template <typename VI, typename VO>
+__global__ void convert_kernel(const VI *in, int64_t ldin, VO *out, int64_t ldout, int n, int nb) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(n) * nb) return;
+  const int b = static_cast<int>(t / n);
+  const int g = static_cast<int>(t - int64_t(b) * n);
+  const VI v = in[g + b * ldin];
+  VO o;
+  o.x = v.x;
+  o.y = v.y;
+  out[g + b * ldout] = o;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// System
+// ---------------------------------------------------------------------------
+
+int System::max_nlm() const {
+  int m = 0;
+  for (int v : atom_nlm) m = std::max(m, v);
+  return m;
+}
+
+void System::build(hipStream_t s) {
+  stream = s;
+  validate_datasets(sets, crystal);
+  fft.init(charge.shape, s);
+  // projector metadata, atom-major global index
+  proj_atom.clear();
+  proj_lm.clear();
+  proj_l.clear();
+  proj_m.clear();
+  proj_table.clear();
+  proj_gmax.clear();
+  atom_first.clear();
+  atom_nlm.clear();
+  atom_species.clear();
+  atom_boff.clear();
+  tables.clear();
+  tables_m2.clear();
+  positions_cart.clear();
+  dblk.clear();
+  qblk.clear();
+  int ion = 0, table0 = 0;
+  for (size_t it = 0; it < sets.size(); ++it) {
+    const Dataset &d = sets[it];
+    for (int ic = 0; ic < d.channels(); ++ic) {
+      std::vector<double> tab(d.pspnl.begin() + ic * 100, d.pspnl.begin() + (ic + 1) * 100);
+      auto m2 = spline_second(tab, d.pspnl_gmax);
+      tables.insert(tables.end(), tab.begin(), tab.end());
+      tables_m2.insert(tables_m2.end(), m2.begin(), m2.end());
+    }
+    std::vector<double> dl, ql;
+    lm_expand(d, dl, ql);
+    const int nlm = d.nlm();
+    for (int a = 0; a < crystal.counts[it]; ++a, ++ion) {
+      atom_first.push_back(static_cast<int>(proj_atom.size()));
+      atom_nlm.push_back(nlm);
+      atom_species.push_back(static_cast<int>(it));
+      atom_boff.push_back(static_cast<int>(dblk.size()));
+      dblk.insert(dblk.end(), dl.begin(), dl.end());
+      qblk.insert(qblk.end(), ql.begin(), ql.end());
+      const double *f = &crystal.positions[3 * ion];
+      for (int j = 0; j < 3; ++j)
+        positions_cart.push_back(f[0] * crystal.lattice[0][j] + f[1] * crystal.lattice[1][j] +
+                                 f[2] * crystal.lattice[2][j]);
+      int ilm = 0;
+      for (int ic = 0; ic < d.channels(); ++ic) {
+        const int l = d.lps[ic];
+        for (int m = -l; m <= l; ++m, ++ilm) {
+          proj_atom.push_back(ion);
+          proj_lm.push_back(ilm);
+          proj_l.push_back(l);
+          proj_m.push_back(m);
+          proj_table.push_back(table0 + ic);
+          proj_gmax.push_back(d.pspnl_gmax);
+        }
+      }
+    }
+    table0 += d.channels();
+  }
+  natoms = ion;
+  nproj = static_cast<int>(proj_atom.size());
+  d_proj_atom.upload(proj_atom, s);
+  d_proj_lm.upload(proj_lm, s);
+  d_proj_l.upload(proj_l, s);
+  d_proj_m.upload(proj_m, s);
+  d_proj_table.upload(proj_table, s);
+  d_proj_gmax.upload(proj_gmax, s);
+  d_tables.upload(tables, s);
+  d_tables_m2.upload(tables_m2, s);
+  d_positions.upload(positions_cart, s);
+  d_atom_first.upload(atom_first, s);
+  d_atom_nlm.upload(atom_nlm, s);
+  d_atom_boff.upload(atom_boff, s);
+  d_dblk.upload(dblk, s);
+  d_qblk.upload(qblk, s);
+  std::vector<float> d32(dblk.begin(), dblk.end()), q32(qblk.begin(), qblk.end());
+  d_dblk32.upload(d32, s);
+  d_qblk32.upload(q32, s);
+  build_veff_device(*this, s);
+  ready = true;
+}
+
+// ---------------------------------------------------------------------------
+// KOperator
+// ---------------------------------------------------------------------------
+
+template <typename R>
+void KOperator<R>::build(const System &s, const double kpoint[3], hipStream_t st) {
+  sys = &s;
+  basis = build_basis(s.crystal, s.charge.shape, s.encut, kpoint);
+  n = static_cast<int>(basis.npw());
+  np = s.nproj;
+  ldb = n;
+  fft_index.upload(basis.fft_index, st);
+  std::vector<R> kin(basis.kinetic.begin(), basis.kinetic.end());
+  kinetic.upload(kin, st);
+  qvec.upload(basis.q, st);
+  B.resize(size_t(ldb) * std::max(np, 1));
+  if (np > 0 && n > 0) {
+    ProjArgs a{qvec.get(), s.d_positions.get(), s.d_proj_atom.get(), s.d_proj_l.get(), s.d_proj_m.get(),
+               s.d_proj_table.get(), s.d_proj_gmax.get(), s.d_tables.get(), s.d_tables_m2.get(), n, np,
+               100, ldb, 1.0 / std::sqrt(s.crystal.volume)};
+    dim3 grid((n + 255) / 256, (np + 15) / 16);
+    hipLaunchKernelGGL((projector_kernel<R>), grid, dim3(256), 0, st, B.get(), static_cast<double2 *>(nullptr), a);
+    HALF_HIP(hipGetLastError());
+  }
+}
+
+template <typename R>
+void KOperator<R>::apply_local(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
+                               int64_t ldout, hipStream_t st) {
+  using V = cplx_t<R>;
+  const int64_t nr = sys->fft.nr;
+  const int batch = std::max(1, std::min(box_batch, nb));
+  boxes.ensure(size_t(nr) * batch);
+  const R *veff = nullptr;
+  if constexpr (std::is_same<R, double>::value)
+    veff = sys->veff.get();
+  else
+    veff = sys->veff32.get();
+  for (int b0 = 0; b0 < nb; b0 += batch) {
+    const int cnt = std::min(batch, nb - b0);
+    hipLaunchKernelGGL((zero_kernel<V>), dim3(blocks_for(nr * cnt)), dim3(kThreads), 0, st, boxes.get(),
+                       nr * cnt);
+    hipLaunchKernelGGL((scatter_kernel<V>), dim3(blocks_for(int64_t(n) * cnt)), dim3(kThreads), 0, st,
+                       psi + b0 * ldpsi, ldpsi, boxes.get(), nr, fft_index.get(), n, cnt);
+    HALF_HIP(hipGetLastError());
+    fft_axis<R>(sys->fft, boxes.get(), cnt, 2, +1, st);
+    fft_axis<R>(sys->fft, boxes.get(), cnt, 1, +1, st);
+    fft_axis0_veff<R>(sys->fft, boxes.get(), cnt, veff, R(1.0 / double(nr)), st);
+    fft_axis<R>(sys->fft, boxes.get(), cnt, 1, -1, st);
+    fft_axis<R>(sys->fft, boxes.get(), cnt, 2, -1, st);
+    hipLaunchKernelGGL((gather_kernel<V, R>), dim3(blocks_for(int64_t(n) * cnt)), dim3(kThreads), 0, st,
+                       psi + b0 * ldpsi, ldpsi, boxes.get(), nr, fft_index.get(), kinetic.get(),
+                       hpsi + b0 * ldout, spsi + b0 * ldout, ldout, n, cnt);
+    HALF_HIP(hipGetLastError());
+  }
+}
+
+template <typename R>
+void KOperator<R>::project(int nb, const cplx_t<R> *psi, int64_t ldpsi, hipStream_t st) {
+  using V = cplx_t<R>;
+  const int tiles_p = (np + 15) / 16, tiles_b = (nb + 15) / 16;
+  const int tiles = tiles_p * tiles_b;
+  // choose the G chunk so that the launch has a few thousand workgroups
+  int nchunks = std::max(1, std::min((n + 63) / 64, 2048 / std::max(tiles, 1)));
+  int chunk = (n + nchunks - 1) / nchunks;
+  chunk = (chunk + 15) / 16 * 16;
+  nchunks = (n + chunk - 1) / chunk;
+  const int64_t count = int64_t(np) * nb;
+  coeff.ensure(count);
+  partial.ensure(count * nchunks);
+  hipLaunchKernelGGL((coeff_partial_kernel<R>), dim3(tiles, nchunks), dim3(256), 0, st, B.get(), ldb, psi,
+                     ldpsi, partial.get(), n, np, nb, chunk, tiles_p);
+  HALF_HIP(hipGetLastError());
+  hipLaunchKernelGGL((reduce_chunks_kernel<V>), dim3(blocks_for(count)), dim3(kThreads), 0, st, partial.get(),
+                     coeff.get(), count, nchunks);
+  HALF_HIP(hipGetLastError());
+}
+
+template <typename R>
+void KOperator<R>::apply_projectors(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi,
+                                    cplx_t<R> *spsi, int64_t ldout, hipStream_t st) {
+  using V = cplx_t<R>;
+  if (np == 0) return;
+  project(nb, psi, ldpsi, st);
+  const int64_t count = int64_t(np) * nb;
+  dcoeff.ensure(count);
+  qcoeff.ensure(count);
+  const R *db, *qb;
+  if constexpr (std::is_same<R, double>::value) {
+    db = sys->d_dblk.get();
+    qb = sys->d_qblk.get();
+  } else {
+    db = sys->d_dblk32.get();
+    qb = sys->d_qblk32.get();
+  }
+  hipLaunchKernelGGL((mix_kernel<V, R>), dim3(blocks_for(count)), dim3(kThreads), 0, st, coeff.get(),
+                     dcoeff.get(), qcoeff.get(), sys->d_proj_atom.get(), sys->d_proj_lm.get(),
+                     sys->d_atom_first.get(), sys->d_atom_nlm.get(), sys->d_atom_boff.get(), db, qb, np, nb);
+  HALF_HIP(hipGetLastError());
+  const int tiles_g = (n + 15) / 16, tiles_b = (nb + 15) / 16;
+  const int tiles = tiles_g * tiles_b;
+  hipLaunchKernelGGL((expand_kernel<R>), dim3((tiles + 3) / 4), dim3(256), 0, st, B.get(), ldb, dcoeff.get(),
+                     qcoeff.get(), hpsi, spsi, ldout, n, np, nb, tiles_g);
+  HALF_HIP(hipGetLastError());
+}
+
+template <typename R>
+void KOperator<R>::apply(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
+                         int64_t ldout, hipStream_t st) {
+  if (nb <= 0) return;
+  apply_local(nb, psi, ldpsi, hpsi, spsi, ldout, st);
+  apply_projectors(nb, psi, ldpsi, hpsi, spsi, ldout, st);
+}
+
+template struct KOperator<double>;
+template struct KOperator<float>;
+
+void assemble_dense(const KOperator<double> &op, double2 *h, double2 *s, int64_t ld, hipStream_t st) {
+  const System &sys = *op.sys;
+  const int n = op.n, np = op.np;
+  DeviceArray<double2> E(size_t(n) * std::max(np, 1)), F(size_t(n) * std::max(np, 1));
+  DeviceArray<int32_t> gvec;
+  DeviceArray<double> kin;
+  gvec.upload(op.basis.gvec, st);
+  kin.upload(op.basis.kinetic, st);
+  if (np > 0) {
+    hipLaunchKernelGGL(row_mix_kernel, dim3(blocks_for(int64_t(n) * np)), dim3(kThreads), 0, st, op.B.get(),
+                       op.ldb, E.get(), F.get(), sys.d_proj_atom.get(), sys.d_proj_lm.get(),
+                       sys.d_atom_first.get(), sys.d_atom_nlm.get(), sys.d_atom_boff.get(), sys.d_dblk.get(),
+                       sys.d_qblk.get(), n, np);
+    HALF_HIP(hipGetLastError());
+  }
+  const int tiles = (n + 15) / 16;
+  const int64_t ntiles = int64_t(tiles) * tiles;
+  hipLaunchKernelGGL(dense_kernel, dim3(static_cast<unsigned>((ntiles + 3) / 4)), dim3(256), 0, st, E.get(),
+                     F.get(), op.B.get(), op.ldb, sys.veff_g.get(), gvec.get(), kin.get(), h, s, ld, n, np, tiles,
+                     sys.charge.shape[0], sys.charge.shape[1], sys.charge.shape[2]);
+  HALF_HIP(hipGetLastError());
+  hipLaunchKernelGGL(hermitize_kernel, dim3(tiles, tiles), dim3(16, 16), 0, st, h, s, ld, n);
+  HALF_HIP(hipGetLastError());
+  HALF_HIP(hipStreamSynchronize(st));
+}
+
+void convert_states(const double2 *in, int64_t ldin, float2 *out, int64_t ldout, int n, int nb, hipStream_t s) {
+  hipLaunchKernelGGL((convert_kernel<double2, float2>), dim3(blocks_for(int64_t(n) * nb)), dim3(kThreads), 0, s,
+                     in, ldin, out, ldout, n, nb);
+  HALF_HIP(hipGetLastError());
+}
+
+void convert_states(const float2 *in, int64_t ldin, double2 *out, int64_t ldout, int n, int nb, hipStream_t s) {
+  hipLaunchKernelGGL((convert_kernel<float2, double2>), dim3(blocks_for(int64_t(n) * nb)), dim3(kThreads), 0, s,
+                     in, ldin, out, ldout, n, nb);
+  HALF_HIP(hipGetLastError());
+}
+
+}  // namespace half

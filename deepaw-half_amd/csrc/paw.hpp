@@ -1,0 +1,98 @@
+// Device-resident PAW system and per-k-point operator.
+//
+// System  : built once per context.  Veff on the C-order box (FP64 master
+//           plus an FP32 copy), Veff(G)/N_r for dense assembly, the FFT plan,
+//           radial projector tables with their spline second derivatives,
+//           Cartesian atom positions and the lm-expanded D/Q blocks.
+// KOperator<R>: built once per (context, k point, precision).  Basis index
+//           map and kinetic energies, the projector matrix
+//           B (npw x nproj, column-major, column = atom-major (atom, lm)),
+//           per-atom D and per-species Q blocks, FFT workspace.
+#pragma once
+
+#include <memory>
+
+#include "common.hpp"
+#include "fft.hpp"
+
+namespace half {
+
+struct System {
+  Crystal crystal;
+  ChargeGrid charge;
+  std::vector<Dataset> sets;
+  bool use_pbe = true;
+  double encut = 0;
+  hipStream_t stream = nullptr;
+
+  FftPlan fft;
+  DeviceArray<double> veff;    // C-order, eV
+  DeviceArray<float> veff32;
+  DeviceArray<double2> veff_g;  // forward FFT(veff) / N_r
+  double e_hartree = 0, e_xc = 0, e_xc_potential = 0;
+  bool ready = false;
+
+  // projector description (global projector index p = atom-major (atom, lm))
+  int nproj = 0;
+  int natoms = 0;
+  std::vector<int> proj_atom, proj_lm, proj_l, proj_m, proj_table;  // per projector
+  std::vector<double> proj_gmax;
+  std::vector<int> atom_first, atom_nlm, atom_species;              // per atom
+  std::vector<double> tables, tables_m2;                           // 100 points per channel
+  std::vector<double> positions_cart;                              // natoms x 3
+  std::vector<double> dblk, qblk;   // per atom nlm x nlm row-major (D and Q)
+  std::vector<int> atom_boff;       // offset of an atom's block in dblk/qblk
+  DeviceArray<int> d_proj_atom, d_proj_lm, d_proj_l, d_proj_m, d_proj_table;
+  DeviceArray<double> d_proj_gmax, d_tables, d_tables_m2, d_positions;
+  DeviceArray<int> d_atom_first, d_atom_nlm, d_atom_boff;
+  DeviceArray<double> d_dblk, d_qblk;
+  DeviceArray<float> d_dblk32, d_qblk32;
+
+  void build(hipStream_t s);  // Veff + tables; reads crystal/charge/sets
+  int max_nlm() const;
+};
+
+template <typename R>
+struct KOperator {
+  const System *sys = nullptr;
+  Basis basis;
+  int n = 0;       // plane waves
+  int np = 0;      // projectors
+  int64_t ldb = 0;  // leading dimension of B
+  DeviceArray<int32_t> fft_index;
+  DeviceArray<R> kinetic;
+  DeviceArray<double> qvec;  // npw x 3 (for the force adjoint)
+  DeviceArray<cplx_t<R>> B;
+  // workspace
+  int box_batch = 16;
+  DeviceArray<cplx_t<R>> boxes;
+  DeviceArray<cplx_t<R>> coeff, dcoeff, qcoeff, partial;
+
+  void build(const System &s, const double kpoint[3], hipStream_t st);
+  // H psi and S psi for nb states (column-major, leading dimensions ld*).
+  void apply(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
+             int64_t ldout, hipStream_t st);
+  // Stages, exposed for the local-only / projector-only checks and timing.
+  void apply_local(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
+                   int64_t ldout, hipStream_t st);
+  void apply_projectors(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi,
+                        cplx_t<R> *spsi, int64_t ldout, hipStream_t st);
+  // c = B^H psi  (np x nb, leading dimension np) into `coeff`
+  void project(int nb, const cplx_t<R> *psi, int64_t ldpsi, hipStream_t st);
+};
+
+// Dense H/S assembly on the device (FP64), column-major n x n with
+// leading dimension ld, hermitised (reference: src/half_dense_solver.F90:53-88).
+void assemble_dense(const KOperator<double> &op, double2 *h, double2 *s, int64_t ld,
+                    hipStream_t st);
+
+// Device Veff construction (reference: src/half_potential.F90:175-238).
+void build_veff_device(System &sys, hipStream_t s);
+
+// Conversions between precisions of state blocks.
+void convert_states(const double2 *in, int64_t ldin, float2 *out, int64_t ldout, int n, int nb,
+                    hipStream_t s);
+void convert_states(const float2 *in, int64_t ldin, double2 *out, int64_t ldout, int n, int nb,
+                    hipStream_t s);
+
+}  // namespace half

@@ -1,0 +1,423 @@
+// Dense and all-band eigensolvers on the device (see solver.hpp).
+//
+// The all-band minimiser keeps Z = [X | P], HZ = [HX | HP], SZ = [SX | SP]
+// contiguous so the 2m x 2m Rayleigh-Ritz projection needs no block copies;
+// Gram matrices and rotations are rocBLAS ZGEMMs, the m x m metric and the
+// 2m x 2m generalized problem are rocSOLVER zheevd / zhegvd, all in FP64.
+// Handles and workspaces live for the whole solve (the reference creates
+// cuBLAS/cuSOLVER handles per call and reallocates per iteration).  The one
+// host round trip per iteration is the m residual norms used for the
+// convergence test and the seed selection.
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "cplx.hpp"
+#include "solver.hpp"
+
+namespace half {
+
+namespace {
+
+constexpr int kThreads = 256;
+inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
+
+#define HALF_BLAS(call)                                                                         \
+  do {                                                                                          \
+    rocblas_status st_ = (call);                                                                \
+    if (st_ != rocblas_status_success)                                                          \
+      throw Error(kInternal, std::string("rocBLAS/rocSOLVER status ") + std::to_string(int(st_)) + \
+                                 " in " #call);                                                 \
+  } while (0)
+
+inline rocblas_double_complex *rz(double2 *p) { return reinterpret_cast<rocblas_double_complex *>(p); }
+inline const rocblas_double_complex *rz(const double2 *p) {
+  return reinterpret_cast<const rocblas_double_complex *>(p);
+}
+
+// C = op(A) * op(B) (+ beta C), all column-major FP64 complex
+void zgemm(Blas &b, rocblas_operation ta, rocblas_operation tb, int m, int n, int k, double2 alpha,
+           const double2 *A, int64_t lda, const double2 *B, int64_t ldb, double2 beta, double2 *C, int64_t ldc) {
+  rocblas_double_complex al{alpha.x, alpha.y}, be{beta.x, beta.y};
+  HALF_BLAS(rocblas_zgemm(b.h, ta, tb, m, n, k, &al, rz(A), static_cast<rocblas_int>(lda), rz(B),
+                          static_cast<rocblas_int>(ldb), &be, rz(C), static_cast<rocblas_int>(ldc)));
+}
+
+__global__ void init_states_kernel(double2 *x, int64_t ld, const int *indices, int n, int m) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(n) * m) return;
+  const int b = static_cast<int>(t / n);
+  const int g = static_cast<int>(t - int64_t(b) * n);
+  x[g + b * ld] = make_double2(g == indices[b] ? 1.0 : 0.0, 0.0);
+}
+
+// R = HX - SX diag(w) (reference: src/half_cuda_iterative_solver.cuf:29-40)
+__global__ void residual_kernel(const double2 *hx, const double2 *sx, const double *w, double2 *r, int n, int m,
+                                int64_t ld) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(n) * m) return;
+  const int b = static_cast<int>(t / n);
+  const int g = static_cast<int>(t - int64_t(b) * n);
+  const int64_t o = g + b * ld;
+  r[o] = make_double2(hx[o].x - w[b] * sx[o].x, hx[o].y - w[b] * sx[o].y);
+}
+
+// squared column norms, fixed-order per column
+__global__ void column_norm2_kernel(const double2 *r, int64_t ld, int n, double *out) {
+  __shared__ double red[kThreads];
+  const double2 *c = r + blockIdx.x * ld;
+  double acc = 0;
+  for (int g = threadIdx.x; g < n; g += blockDim.x) acc += c[g].x * c[g].x + c[g].y * c[g].y;
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// P = -R/(|T - eps|+1), or a unit seed vector for converged bands
+// (reference: src/half_cuda_iterative_solver.cuf:42-64)
+__global__ void precondition_kernel(const double2 *r, const double *kin, const double *w, const int *seeds,
+                                    double2 *p, int n, int m, int64_t ld) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(n) * m) return;
+  const int b = static_cast<int>(t / n);
+  const int g = static_cast<int>(t - int64_t(b) * n);
+  const int64_t o = g + b * ld;
+  if (seeds[b] >= 0) {
+    p[o] = make_double2(g == seeds[b] ? 1.0 : 0.0, 0.0);
+  } else {
+    const double d = fabs(kin[g] - w[b]) + 1.0;
+    p[o] = make_double2(-r[o].x / d, -r[o].y / d);
+  }
+}
+
+// U <- U diag(max(mu, floor))^-1/2 (reference: :66-77)
+__global__ void scale_columns_kernel(double2 *u, const double *mu, int m, double floor_value) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(m) * m) return;
+  const int col = static_cast<int>(t / m);
+  const double f = 1.0 / sqrt(fmax(mu[col], floor_value));
+  u[t] = make_double2(u[t].x * f, u[t].y * f);
+}
+
+void check_info(const DeviceArray<rocblas_int> &info, hipStream_t s, const char *what) {
+  rocblas_int h = 0;
+  HALF_HIP(hipMemcpyAsync(&h, info.get(), sizeof(h), hipMemcpyDeviceToHost, s));
+  HALF_HIP(hipStreamSynchronize(s));
+  if (h != 0) throw Error(kInternal, std::string(what) + " failed (info " + std::to_string(h) + ")");
+}
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+void Blas::init(hipStream_t s) {
+  stream = s;
+  if (!h) HALF_BLAS(rocblas_create_handle(&h));
+  HALF_BLAS(rocblas_set_stream(h, s));
+}
+
+Blas::~Blas() {
+  if (h) rocblas_destroy_handle(h);
+}
+
+std::vector<int> kinetic_order(const std::vector<double> &kin) {
+  const int n = static_cast<int>(kin.size());
+  std::vector<int> idx(n + 1);
+  for (int i = 1; i <= n; ++i) idx[i] = i;
+  auto K = [&](int i) { return kin[idx[i] - 1]; };
+  for (int i = 2; i <= n; ++i) {
+    int child = i;
+    while (child > 1) {
+      const int parent = child / 2;
+      if (K(parent) >= K(child)) break;
+      std::swap(idx[parent], idx[child]);
+      child = parent;
+    }
+  }
+  for (int last = n; last >= 2; --last) {
+    std::swap(idx[1], idx[last]);
+    int parent = 1;
+    while (true) {
+      int child = 2 * parent;
+      if (child > last - 1) break;
+      if (child + 1 <= last - 1 && K(child + 1) > K(child)) child = child + 1;
+      if (K(parent) >= K(child)) break;
+      std::swap(idx[parent], idx[child]);
+      parent = child;
+    }
+  }
+  std::vector<int> out(n);
+  for (int i = 0; i < n; ++i) out[i] = idx[i + 1] - 1;
+  return out;
+}
+
+void solve_dense(Blas &blas, const KOperator<double> &op, int solver, int nbands, bool want_vectors,
+                 DeviceArray<double2> &vectors, SolveResult &res) {
+  const int n = op.n;
+  hipStream_t s = blas.stream;
+  if (nbands < 1 || nbands > n) throw Error(kInvalidArgument, "invalid requested band count");
+  const double t0 = now();
+  DeviceArray<double2> H(size_t(n) * n), S(size_t(n) * n);
+  assemble_dense(op, H.get(), S.get(), n, s);
+  const double t1 = now();
+  DeviceArray<double> W(n), E(n);
+  DeviceArray<rocblas_int> info(1);
+  res.overlap_min = res.overlap_max = -1.0;
+  if (solver != kSolverEvx) {
+    // overlap spectrum (reference: src/half_cuda_solver.cuf:145-150)
+    DeviceArray<double2> S2(size_t(n) * n);
+    HALF_HIP(hipMemcpyAsync(S2.get(), S.get(), sizeof(double2) * size_t(n) * n, hipMemcpyDeviceToDevice, s));
+    HALF_BLAS(rocsolver_zheevd(blas.h, rocblas_evect_none, rocblas_fill_upper, n, rz(S2.get()), n, W.get(),
+                               E.get(), info.get()));
+    check_info(info, s, "overlap eigensolve");
+    double lo = 0, hi = 0;
+    HALF_HIP(hipMemcpyAsync(&lo, W.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+    HALF_HIP(hipMemcpyAsync(&hi, W.get() + (n - 1), sizeof(double), hipMemcpyDeviceToHost, s));
+    HALF_HIP(hipStreamSynchronize(s));
+    res.overlap_min = lo;
+    res.overlap_max = hi;
+  }
+  const rocblas_evect ev = want_vectors ? rocblas_evect_original : rocblas_evect_none;
+  if (solver == kSolverEvx) {
+    // lowest nbands only (reference: src/half_cuda_solver.cuf:119-143)
+    rocblas_int nev = 0;
+    DeviceArray<rocblas_int> dnev(1);
+    DeviceArray<double2> Z(want_vectors ? size_t(n) * nbands : 1);
+    HALF_BLAS(rocsolver_zhegvdx(blas.h, rocblas_eform_ax, ev, rocblas_erange_index, rocblas_fill_upper, n,
+                                rz(H.get()), n, rz(S.get()), n, 0.0, 0.0, 1, nbands, dnev.get(), W.get(),
+                                rz(Z.get()), n, info.get()));
+    check_info(info, s, "index-range generalized eigensolve");
+    HALF_HIP(hipMemcpyAsync(&nev, dnev.get(), sizeof(nev), hipMemcpyDeviceToHost, s));
+    HALF_HIP(hipStreamSynchronize(s));
+    if (nev != nbands) throw Error(kInternal, "index-range eigensolve returned a different band count");
+    if (want_vectors) vectors = std::move(Z);
+  } else if (solver == kSolverEvj) {
+    double resid = 0;
+    rocblas_int sweeps = 0;
+    DeviceArray<double> dres(1);
+    DeviceArray<rocblas_int> dsw(1);
+    HALF_BLAS(rocsolver_zhegvj(blas.h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, n,
+                               rz(H.get()), n, rz(S.get()), n, 1e-12, dres.get(), 100, dsw.get(), W.get(),
+                               info.get()));
+    check_info(info, s, "Jacobi generalized eigensolve");
+    (void)resid;
+    (void)sweeps;
+  } else {
+    HALF_BLAS(rocsolver_zhegvd(blas.h, rocblas_eform_ax, ev, rocblas_fill_upper, n, rz(H.get()), n, rz(S.get()), n,
+                               W.get(), E.get(), info.get()));
+    check_info(info, s, "generalized eigensolve");
+  }
+  res.eigenvalues.resize(nbands);
+  HALF_HIP(hipMemcpyAsync(res.eigenvalues.data(), W.get(), sizeof(double) * nbands, hipMemcpyDeviceToHost, s));
+  if (want_vectors && solver != kSolverEvx) {
+    vectors.resize(size_t(n) * nbands);
+    HALF_HIP(hipMemcpyAsync(vectors.get(), H.get(), sizeof(double2) * size_t(n) * nbands,
+                            hipMemcpyDeviceToDevice, s));
+  }
+  HALF_HIP(hipStreamSynchronize(s));
+  res.seconds_operator = t1 - t0;
+  res.seconds_solver = now() - t1;
+}
+
+namespace {
+
+struct AccWork {
+  int n, m, m2;
+  DeviceArray<double2> Z, HZ, SZ, R, tmp, cross, metric, hsub, ssub;
+  DeviceArray<float2> p32, hp32, sp32;
+  DeviceArray<double> w, mu, E, res2, kin;
+  DeviceArray<int> seeds;
+  DeviceArray<rocblas_int> info;
+};
+
+}  // namespace
+
+void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int nbands, const AccControls &ctl,
+               bool want_vectors, DeviceArray<double2> &vectors, SolveResult &res) {
+  hipStream_t s = blas.stream;
+  const int n = op64.n;
+  const int mreq = nbands;
+  const int m = nbands + std::max(0, ctl.extra);
+  if (mreq < 1 || mreq > n) throw Error(kInvalidArgument, "invalid requested band count");
+  if (2 * m > n) throw Error(kInvalidArgument, "ACC solver requires NBANDS <= NPL/2");
+  if (!(ctl.tol > 0) || ctl.maxiter < 1 || ctl.block < 1) throw Error(kInvalidArgument, "invalid ACC solver controls");
+  if (ctl.mixed && !op32) throw Error(kInternal, "mixed ACC needs an FP32 operator");
+  op64.box_batch = ctl.block;
+  if (op32) op32->box_batch = ctl.block;
+  const double t0 = now();
+  const int m2 = 2 * m;
+  const int64_t ld = n;
+  AccWork wk;
+  wk.Z.resize(size_t(n) * m2);
+  wk.HZ.resize(size_t(n) * m2);
+  wk.SZ.resize(size_t(n) * m2);
+  wk.R.resize(size_t(n) * m);
+  wk.tmp.resize(size_t(n) * m);
+  wk.cross.resize(size_t(m) * m);
+  wk.metric.resize(size_t(m) * m);
+  wk.hsub.resize(size_t(m2) * m2);
+  wk.ssub.resize(size_t(m2) * m2);
+  wk.w.resize(m2);
+  wk.mu.resize(m);
+  wk.E.resize(m2);
+  wk.res2.resize(m);
+  wk.seeds.resize(m);
+  wk.info.resize(1);
+  wk.kin.upload(op64.basis.kinetic, s);
+  if (ctl.mixed) {
+    wk.p32.resize(size_t(n) * m);
+    wk.hp32.resize(size_t(n) * m);
+    wk.sp32.resize(size_t(n) * m);
+  }
+  double2 *X = wk.Z.get(), *P = wk.Z.get() + size_t(n) * m;
+  double2 *HX = wk.HZ.get(), *HP = wk.HZ.get() + size_t(n) * m;
+  double2 *SX = wk.SZ.get(), *SP = wk.SZ.get() + size_t(n) * m;
+  const double2 one = make_double2(1, 0), zero = make_double2(0, 0), mone = make_double2(-1, 0);
+  bool fp64_phase = !ctl.mixed;
+  res.applies = 0;
+
+  auto apply = [&](double2 *in, double2 *hout, double2 *sout) {
+    if (fp64_phase) {
+      op64.apply(m, in, ld, hout, sout, ld, s);
+    } else {
+      convert_states(in, ld, wk.p32.get(), ld, n, m, s);
+      op32->apply(m, wk.p32.get(), ld, wk.hp32.get(), wk.sp32.get(), ld, s);
+      convert_states(wk.hp32.get(), ld, hout, ld, n, m, s);
+      convert_states(wk.sp32.get(), ld, sout, ld, n, m, s);
+    }
+    res.applies += m;
+  };
+  auto copy = [&](double2 *dst, const double2 *src) {
+    HALF_HIP(hipMemcpyAsync(dst, src, sizeof(double2) * size_t(n) * m, hipMemcpyDeviceToDevice, s));
+  };
+  // generalized k x k problem in (hsub, ssub), vectors in hsub, values in w
+  auto small_generalized = [&](int k) {
+    HALF_BLAS(rocsolver_zhegvd(blas.h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, k,
+                               rz(wk.hsub.get()), k, rz(wk.ssub.get()), k, wk.w.get(), wk.E.get(), wk.info.get()));
+    check_info(wk.info, s, "ACC small generalized eigensolve");
+  };
+  // rotate the first k columns of (Z, HZ, SZ) by hsub(:, :m) into X, HX, SX
+  auto ritz_rotate = [&](int k) {
+    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, k, one, wk.Z.get(), ld, wk.hsub.get(), k,
+          zero, wk.tmp.get(), ld);
+    copy(X, wk.tmp.get());
+    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, k, one, wk.HZ.get(), ld, wk.hsub.get(), k,
+          zero, wk.tmp.get(), ld);
+    copy(HX, wk.tmp.get());
+    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, k, one, wk.SZ.get(), ld, wk.hsub.get(), k,
+          zero, wk.tmp.get(), ld);
+    copy(SX, wk.tmp.get());
+  };
+  auto rayleigh_ritz_x = [&]() {
+    zgemm(blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, m, n, one, X, ld, HX, ld, zero,
+          wk.hsub.get(), m);
+    zgemm(blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, m, n, one, X, ld, SX, ld, zero,
+          wk.ssub.get(), m);
+    small_generalized(m);
+    ritz_rotate(m);
+  };
+
+  // initial subspace: unit vectors at the m lowest kinetic energies
+  {
+    auto order = kinetic_order(op64.basis.kinetic);
+    std::vector<int> first(order.begin(), order.begin() + m);
+    DeviceArray<int> d_idx;
+    d_idx.upload(first, s);
+    hipLaunchKernelGGL(init_states_kernel, dim3(blocks_for(int64_t(n) * m)), dim3(kThreads), 0, s, X, ld,
+                       d_idx.get(), n, m);
+    HALF_HIP(hipGetLastError());
+    apply(X, HX, SX);
+    rayleigh_ritz_x();
+    HALF_HIP(hipStreamSynchronize(s));
+  }
+  std::vector<double> r2(m);
+  std::vector<int> seeds(m);
+  double max_res = HUGE_VAL;
+  int iter = 0;
+  bool converged = false;
+  int fp64_iters = 0;
+  for (iter = 1; iter <= ctl.maxiter; ++iter) {
+    hipLaunchKernelGGL(residual_kernel, dim3(blocks_for(int64_t(n) * m)), dim3(kThreads), 0, s, HX, SX,
+                       wk.w.get(), wk.R.get(), n, m, ld);
+    hipLaunchKernelGGL(column_norm2_kernel, dim3(m), dim3(kThreads), 0, s, wk.R.get(), ld, n, wk.res2.get());
+    HALF_HIP(hipGetLastError());
+    HALF_HIP(hipMemcpyAsync(r2.data(), wk.res2.get(), sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    HALF_HIP(hipStreamSynchronize(s));
+    max_res = 0;
+    for (int j = 0; j < m; ++j) {
+      const double r = std::sqrt(std::max(0.0, r2[j]));
+      if (j < mreq) max_res = std::max(max_res, r);
+      seeds[j] = r < std::max(ctl.tol * 1e-2, 1e-12) ? (m + j) % n : -1;
+    }
+    if (fp64_phase) ++fp64_iters;
+    if (!fp64_phase && max_res <= std::max(ctl.switch_tol, ctl.tol)) {
+      // FP64 correction sweeps: exact HX, SX for the current subspace
+      fp64_phase = true;
+      apply(X, HX, SX);
+      rayleigh_ritz_x();
+      --iter;
+      continue;
+    }
+    if (fp64_phase && max_res <= ctl.tol) {
+      converged = true;
+      break;
+    }
+    wk.seeds.upload(seeds, s);
+    hipLaunchKernelGGL(precondition_kernel, dim3(blocks_for(int64_t(n) * m)), dim3(kThreads), 0, s, wk.R.get(),
+                       wk.kin.get(), wk.w.get(), wk.seeds.get(), P, n, m, ld);
+    HALF_HIP(hipGetLastError());
+    apply(P, HP, SP);
+    // S-orthogonalise the correction block against X
+    zgemm(blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, m, n, one, X, ld, SP, ld, zero,
+          wk.cross.get(), m);
+    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, m, mone, X, ld, wk.cross.get(), m, one, P, ld);
+    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, m, mone, HX, ld, wk.cross.get(), m, one, HP,
+          ld);
+    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, m, mone, SX, ld, wk.cross.get(), m, one, SP,
+          ld);
+    // normalise it in the S metric
+    zgemm(blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, m, n, one, P, ld, SP, ld, zero,
+          wk.metric.get(), m);
+    HALF_BLAS(rocsolver_zheevd(blas.h, rocblas_evect_original, rocblas_fill_upper, m, rz(wk.metric.get()), m,
+                               wk.mu.get(), wk.E.get(), wk.info.get()));
+    check_info(wk.info, s, "ACC correction metric eigensolve");
+    hipLaunchKernelGGL(scale_columns_kernel, dim3(blocks_for(int64_t(m) * m)), dim3(kThreads), 0, s,
+                       wk.metric.get(), wk.mu.get(), m, 1e-14);
+    HALF_HIP(hipGetLastError());
+    for (double2 *blk : {P, HP, SP}) {
+      zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, m, one, blk, ld, wk.metric.get(), m, zero,
+            wk.tmp.get(), ld);
+      copy(blk, wk.tmp.get());
+    }
+    // Rayleigh-Ritz in span[X P]
+    zgemm(blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m2, m2, n, one, wk.Z.get(), ld,
+          wk.HZ.get(), ld, zero, wk.hsub.get(), m2);
+    zgemm(blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m2, m2, n, one, wk.Z.get(), ld,
+          wk.SZ.get(), ld, zero, wk.ssub.get(), m2);
+    small_generalized(m2);
+    ritz_rotate(m2);
+  }
+  res.iterations = std::min(iter, ctl.maxiter);
+  res.fp64_iterations = fp64_iters;
+  res.residual = max_res;
+  res.eigenvalues.resize(mreq);
+  HALF_HIP(hipMemcpyAsync(res.eigenvalues.data(), wk.w.get(), sizeof(double) * mreq, hipMemcpyDeviceToHost, s));
+  if (want_vectors) {
+    vectors.resize(size_t(n) * mreq);
+    HALF_HIP(hipMemcpyAsync(vectors.get(), X, sizeof(double2) * size_t(n) * mreq, hipMemcpyDeviceToDevice, s));
+  }
+  HALF_HIP(hipStreamSynchronize(s));
+  res.overlap_min = res.overlap_max = -1.0;
+  res.seconds_solver = now() - t0;
+  (void)converged;
+}
+
+}  // namespace half

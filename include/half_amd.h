@@ -1,0 +1,92 @@
+/* Device-level extension of the HALF C ABI for the MI355X backend.
+ *
+ * The reference ABI (half.h) exchanges host buffers only.  These entry
+ * points expose the device-resident operator and solver controls that the
+ * reference keeps internal, for hosts that already hold states in HBM and for
+ * measurement.  All pointers named d_* are device pointers on the current
+ * HIP device; complex data are interleaved (re, im) pairs of the operator's
+ * precision (complex128 for precision 64, complex64 for precision 32);
+ * matrices are column-major.  Every function returns a half_status code and
+ * fills the caller-owned error buffer like the reference ABI.
+ */
+#ifndef DEEPAW_HALF_AMD_H
+#define DEEPAW_HALF_AMD_H
+
+#include <stdint.h>
+
+#include "half.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int64_t half_amd_operator;
+
+/* Matrix-free operator at one k point, built on the device: basis index
+ * map, kinetic energies, projector matrix B (npw x nproj) and workspace.
+ * Counterpart of prepare_hs_operator_cuda (reference:
+ * src/half_cuda_assembly.cuf:402-466).  precision is 64 or 32. */
+int half_amd_operator_create(half_handle handle, const double kpoint[3], int precision,
+                             half_amd_operator *op, int64_t *npw, int32_t *nproj, char *error,
+                             int error_capacity);
+/* H.psi and S.psi for nstates device columns (reference:
+ * src/half_cuda_assembly.cuf:468-519).  stage: 0 full operator, 1 local
+ * part only (T + Veff), 2 projector part only (B D B^H, B Q B^H added to
+ * the outputs).  stream: a hipStream_t, or null for the context stream.
+ * box_batch: bands per FFT batch (0 keeps the current value). */
+int half_amd_operator_apply(half_amd_operator op, int stage, int nstates, const void *d_psi,
+                            int64_t ldpsi, void *d_hpsi, void *d_spsi, int64_t ldout, int box_batch,
+                            void *stream, char *error, int error_capacity);
+int half_amd_operator_destroy(half_amd_operator op, char *error, int error_capacity);
+
+/* Solver statistics of the last solve. */
+typedef struct half_amd_solve_stats {
+  int32_t iterations;
+  int32_t fp64_iterations;
+  double max_residual_eV;
+  double seconds;
+  int64_t band_applications;
+} half_amd_solve_stats;
+
+/* All-band solve with explicit controls (the reference ABI fixes tol 1e-4 eV,
+ * 40 iterations, 16-band blocks: src/half_cuda_iterative_solver.cuf:113-119).
+ * mixed != 0 runs FP32 H.psi iterations until max residual <= switch_tol,
+ * then FP64 H.psi sweeps to tol; Rayleigh-Ritz, Cholesky-type metric
+ * normalisation and the subspace eigensolves are FP64 throughout.  extra
+ * buffer bands are carried internally and not reported.  eigenvectors
+ * (host, ld >= npw) may be null. */
+int half_amd_solve_acc(half_handle handle, const double kpoint[3], int nbands, int mixed, double tol,
+                       double switch_tol, int maxiter, int block, int extra, double *eigenvalues_eV,
+                       HALF_COMPLEX *eigenvectors, int64_t ld, half_amd_solve_stats *stats,
+                       char *error, int error_capacity);
+
+/* Select the solver used by half_solve_kpoint for this context (1..4). */
+int half_amd_set_solver(half_handle handle, int solver, char *error, int error_capacity);
+
+/* Harris energy terms of the context's potential (reference:
+ * src/half_cuda_potential.cuf:338-348): e_H, e_xc, e_xc,pot in eV. */
+int half_amd_energy_terms(half_handle handle, double *e_hartree, double *e_xc,
+                          double *e_xc_potential, char *error, int error_capacity);
+/* Veff on the C-order box (n1 slowest), eV; capacity in doubles. */
+int half_amd_get_veff(half_handle handle, int64_t capacity, double *veff, char *error,
+                      int error_capacity);
+
+/* Hellmann-Feynman non-local projector force (reference:
+ * src/half_forces.F90:9-73) through the adjoint projector contraction on
+ * the device: F_I = -2 w sum_n f_n Re[(D_I - eps_n Q)c_n . conj(dc_n)],
+ * c = B^H psi, dc_alpha = B^H (i q_alpha psi).  vectors: host, npw x nbands,
+ * leading dimension ld; forces: natoms x 3 (atom-major, species order),
+ * accumulated into (not overwritten). */
+int half_amd_nonlocal_forces(half_handle handle, const double kpoint[3], int nbands,
+                             const double *eigenvalues_eV, const double *occupations,
+                             double kweight, const HALF_COMPLEX *vectors, int64_t ld,
+                             double *forces, char *error, int error_capacity);
+
+/* Name and compute-unit count of the current HIP device. */
+int half_amd_device_info(char *name, int name_capacity, int32_t *compute_units, char *error,
+                         int error_capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,115 @@
+"""Drop-in boundary checks that need no GPU: the library loads, exports every
+symbol the public headers declare, and honours the reference ABI's argument
+validation, status codes and error-buffer contract (reference:
+tests/test_c_api.c, src/half_c_api.F90:213-288, 627-664)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+from conftest import ROOT
+
+import half_amd as H
+
+SI2 = (os.path.join(ROOT, "tests/golden/inputs/si2.CHGCAR"), os.path.join(ROOT, "tests/golden/inputs/si2.POTCAR"))
+
+
+def declared_symbols():
+    names = set()
+    for hdr in ("half.h", "half_amd.h"):
+        text = open(os.path.join(ROOT, "include", hdr)).read()
+        names |= set(re.findall(r"^\s*(?:int|const char \*)\s*(half_\w+)\s*\(", text, re.M))
+    return names
+
+
+def test_headers_and_binding_agree():
+    assert declared_symbols() == set(H.ABI_SYMBOLS) | set(H.EXT_SYMBOLS)
+    assert len(H.ABI_SYMBOLS) == 18
+
+
+def test_library_exports_every_declared_symbol():
+    lib = H.load()
+    out = subprocess.run(["nm", "-D", "--defined-only", H.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (half_\w+)", out))
+    missing = declared_symbols() - exported
+    assert not missing, missing
+    for name in declared_symbols():
+        assert getattr(lib, name)
+
+
+def test_c_boundary_program(tmp_path):
+    exe = tmp_path / "abi_check"
+    subprocess.run(["gcc", "-std=c11", "-O1", "-I", os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "abi_check.c"),
+                    "-L", os.path.dirname(H.LIB_PATH), "-lhalf", "-Wl,-rpath," + os.path.dirname(H.LIB_PATH),
+                    "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+    assert "ok" in r.stdout
+
+
+def _create(chg, pot, backend=H.BACKEND_AUTO, solver=H.SOLVER_EVD, xc=H.XC_PBE, uspp=0, encut=400.0, cap=256):
+    lib = H.load()
+    h = C.c_int64(7)
+    e = C.create_string_buffer(cap)
+    st = lib.half_create_from_files_backend(chg.encode(), pot.encode(), encut, xc, uspp, backend, solver,
+                                            C.byref(h), e, cap)
+    return st, h.value, e.value.decode()
+
+
+def test_create_argument_validation():
+    assert _create("", "")[0] == H.INVALID_ARGUMENT
+    assert _create(*SI2, backend=99)[0] == H.INVALID_ARGUMENT
+    assert _create(*SI2, solver=0)[0] == H.INVALID_ARGUMENT
+    assert _create(*SI2, solver=5)[0] == H.INVALID_ARGUMENT
+    assert _create(*SI2, xc=3)[0] == H.INVALID_ARGUMENT
+    st, h, msg = _create(*SI2, encut=-1.0)
+    assert st == H.INVALID_ARGUMENT and h == 0 and "ENCUT" in msg
+    st, h, msg = _create(*SI2, encut=float("nan"))
+    assert st == H.INVALID_ARGUMENT
+    st, h, msg = _create("/nonexistent", SI2[1])
+    assert st == H.IO and "charge" in msg
+    st, h, msg = _create(SI2[0], "/nonexistent")
+    assert st == H.IO and "potential" in msg
+    # this library carries no CPU numerics and no QDEP
+    assert _create(*SI2, backend=H.BACKEND_CPU)[0] == H.UNAVAILABLE
+    assert _create(*SI2, uspp=1)[0] == H.UNAVAILABLE
+
+
+def test_escn_entry_points():
+    lib = H.load()
+    e = C.create_string_buffer(128)
+    h = C.c_int64()
+    assert lib.half_escn_health(b"http://127.0.0.1:9", 1, e, 128) == H.UNAVAILABLE
+    assert lib.half_create_from_escn(b"", b"", None, 400.0, 2, 0, 0, 1, 0, 1, C.byref(h), e, 128) == H.INVALID_ARGUMENT
+    g = H.RequestGeometry()
+    assert lib.half_create_from_escn(b"http://x", SI2[1].encode(), C.byref(g), 400.0, 2, 0, 0, 1, 0, 1, C.byref(h), e,
+                                     128) == H.UNAVAILABLE
+
+
+def test_invalid_handles_everywhere():
+    lib = H.load()
+    e = C.create_string_buffer(64)
+    k = (C.c_double * 3)(0, 0, 0)
+    n = C.c_int64()
+    for bad in (0, -1, 12345, 32, 33):
+        assert lib.half_destroy(bad, e, 64) == H.INVALID_HANDLE
+        assert lib.half_get_basis_size(bad, k, C.byref(n), e, 64) == H.INVALID_HANDLE
+        assert e.value == b"invalid HALF context handle"
+    # capacity 0 leaves the buffer untouched
+    buf = C.create_string_buffer(b"keep", 16)
+    assert lib.half_destroy(5, buf, 0) == H.INVALID_HANDLE
+    assert buf.value == b"keep"
+
+
+def test_no_device_is_reported_not_aborted():
+    """Without a HIP device a valid create returns UNAVAILABLE (CPU-only
+    containers); with a device it succeeds."""
+    import torch
+    st, h, msg = _create(*SI2, solver=H.SOLVER_ACC)
+    if torch.cuda.device_count() == 0:
+        assert st == H.UNAVAILABLE and h == 0 and "device" in msg
+    else:
+        assert st == H.SUCCESS and h > 0
+        e = C.create_string_buffer(64)
+        assert H.load().half_destroy(h, e, 64) == H.SUCCESS
