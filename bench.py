@@ -55,9 +55,8 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
     import half_amd as H
-    from half_amd import synth
+    from half_amd import hip, synth
 
     w = synth.workload(args.workload)
     nb = args.bands or w.nbands
@@ -69,37 +68,43 @@ def main():
     k = tuple(w.kpoints[0])
     info = ctx.system_info()
     nr = int(np.prod(info["grid"]))
-    dev = torch.device("cuda")
-    stream = torch.cuda.current_stream()
+    stream = hip.Stream()
     rng = np.random.default_rng(11)
 
     rates = {}
     per_step = {}
-    for prec, dt in ((64, torch.complex128), (32, torch.complex64)):
+    kernels = {}
+    for prec, dt in ((64, np.complex128), (32, np.complex64)):
         op = ctx.operator(k, prec)
         n = op.npw
-        x = torch.from_numpy(rng.standard_normal((nb, n)) + 1j * rng.standard_normal((nb, n))).to(dev, dt)
-        hx, sx = torch.empty_like(x), torch.empty_like(x)
+        x = hip.Buffer.from_numpy((rng.standard_normal((nb, n)) + 1j * rng.standard_normal((nb, n))).astype(dt))
+        hx, sx = hip.Buffer(x.shape, dt), hip.Buffer(x.shape, dt)
 
         def step():
-            op.apply_device(nb, x.data_ptr(), hx.data_ptr(), sx.data_ptr(), ld=n, stream=stream.cuda_stream)
+            op.apply_device(nb, x.ptr, hx.ptr, sx.ptr, ld=n, stream=stream.ptr)
 
         for _ in range(args.warmup):
             step()
-        torch.cuda.synchronize()
-        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        stream.synchronize()
+        t0, t1 = hip.Event(), hip.Event()
         t0.record(stream)
         for _ in range(args.steps):
             step()
         t1.record(stream)
-        torch.cuda.synchronize()
-        ms = t0.elapsed_time(t1) / args.steps
+        ms = t0.elapsed_ms(t1) / args.steps
         per_step[prec] = ms
         rates[prec] = nb / (ms * 1e-3)
         print(f"bench: fp{prec} H.psi {ms:.3f} ms/step, {rates[prec]:.1f} band-applications/s",
               file=sys.stderr, flush=True)
+        # same steps again with the launch-boundary timeline on: per-kernel durations
+        H.trace(True)
+        for _ in range(args.steps):
+            step()
+        stream.synchronize()
+        kernels[prec] = H.trace_read()
+        H.trace(False)
         op.close()
-        del x, hx, sx
+        x.free(), hx.free(), sx.free()
     n = ctx.basis_size(k)
     nproj = ctx.operator(k, 64).nproj
 
@@ -123,15 +128,12 @@ def main():
         if "fp64" in solve and "mixed" in solve:
             solve["speedup_mixed_vs_fp64"] = solve["fp64"]["seconds"] / solve["mixed"]["seconds"]
 
-    # roofline of one band application (local stage bytes + projector flops)
+    roofline = kernel_roofline(kernels[64], nr, n, nproj, nb, args.steps, fp64=True)
+    roofline32 = kernel_roofline(kernels[32], nr, n, nproj, nb, args.steps, fp64=False)
     b_alg = algorithmic_bytes(nr, n, True)
     f_proj = 24.0 * n * nproj
     t_band = per_step[64] * 1e-3 / nb
-    achieved_gbs = b_alg / t_band / 1e9
-    roofline = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "whole H.psi band application (FP64)",
-                "combined_frac": (b_alg / (HBM_PEAK_GBS * 1e9) + f_proj / (MFMA_FP64_TFLOPS * 1e12)) / t_band}
+    roofline["operator_frac"] = (b_alg / (HBM_PEAK_GBS * 1e9) + f_proj / (MFMA_FP64_TFLOPS * 1e12)) / t_band
 
     cpu = None
     if not args.no_cpu:
@@ -149,9 +151,44 @@ def main():
         "fp32_value": rates[32], "fp32_ms_per_step": per_step[32],
         "all_band_solve": solve,
         "roofline": roofline,
+        "roofline_fp32": roofline32,
+        "kernels_ms_per_step": {str(p): {k: v[0] / args.steps for k, v in kernels[p].items()} for p in kernels},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def kernel_roofline(kern, nr, n, nproj, nb, steps, fp64=True, box_batch=16):
+    """Roofline of the kernel with the largest share of the traced time:
+    algorithmic bytes (or flops) of all its launches over its summed launch
+    durations."""
+    sc, sr = (16, 8) if fp64 else (8, 4)
+    nbatch_full, rem = divmod(nb, box_batch)
+    boxes = nb  # band boxes processed per step, summed over batches
+    per_step_bytes = {
+        "zero_boxes": boxes * nr * sc,
+        "scatter": boxes * n * (2 * sc + 4),
+        "fft_axis2_bwd": 2 * boxes * nr * sc, "fft_axis1_bwd": 2 * boxes * nr * sc,
+        "fft_axis0_veff": 2 * boxes * nr * sc + boxes * nr * sr,
+        "fft_axis1_fwd": 2 * boxes * nr * sc, "fft_axis2_fwd": 2 * boxes * nr * sc,
+        "gather_kinetic": boxes * n * (4 * sc + sr + 4),
+        "proj_coeff_reduce": None, "proj_mix": None,
+    }
+    per_step_flops = {"proj_coeff_mfma": 8.0 * n * nproj * nb, "proj_expand_mfma": 16.0 * n * nproj * nb}
+    name = max(kern, key=lambda k: kern[k][0])
+    total_ms, launches = kern[name]
+    t = total_ms * 1e-3
+    if name in per_step_flops:
+        peak = MFMA_FP64_TFLOPS if fp64 else MFMA_FP32_TFLOPS
+        ach = per_step_flops[name] * steps / t / 1e12
+        return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                "traffic": None, "kernel": name, "avg_launch_ms": total_ms / launches}
+    b = per_step_bytes.get(name)
+    ach = (b * steps / t / 1e9) if b else None
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (ach / HBM_PEAK_GBS) if ach else None, "traffic": None, "kernel": name,
+            "avg_launch_ms": total_ms / launches,
+            "bytes_per_launch": (b / max(1, launches // steps)) if b else None}
 
 
 def cpu_baseline(chg, pot, w, k, nb):

@@ -40,8 +40,10 @@ struct Context {
   bool use_uspp = false;
   hipStream_t stream = nullptr;
   Blas blas;
-  std::map<KeyK, std::unique_ptr<KOperator<double>>> ops64;
-  std::map<KeyK, std::unique_ptr<KOperator<float>>> ops32;
+  // operators cached per k point; device-level handles share ownership, so
+  // evicting a cache entry never frees an operator a host still holds
+  std::map<KeyK, std::shared_ptr<KOperator<double>>> ops64;
+  std::map<KeyK, std::shared_ptr<KOperator<float>>> ops32;
   // request geometry (reference: src/half_library.F90:31-35)
   bool has_request = false;
   int32_t req_nions = 0, req_ntypes = 0, req_grid[3] = {0, 0, 0};
@@ -50,32 +52,28 @@ struct Context {
   std::vector<double> req_positions;
 
   ~Context() {
+    // Drain the stream, release device objects, destroy the rocBLAS handle
+    // while its stream still exists, and only then the stream.
+    if (stream) (void)hipStreamSynchronize(stream);
     ops64.clear();
     ops32.clear();
+    blas.release();
     if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
   }
-  KOperator<double> &op64(const double k[3]) {
+  template <typename R>
+  std::shared_ptr<KOperator<R>> cached(std::map<KeyK, std::shared_ptr<KOperator<R>>> &cache, const double k[3]) {
     KeyK key{{k[0], k[1], k[2]}};
-    auto it = ops64.find(key);
-    if (it != ops64.end()) return *it->second;
-    if (ops64.size() >= 8) ops64.clear();
-    auto op = std::make_unique<KOperator<double>>();
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    if (cache.size() >= 8) cache.clear();
+    auto op = std::make_shared<KOperator<R>>();
     op->build(sys, k, stream);
-    auto &ref = *op;
-    ops64[key] = std::move(op);
-    return ref;
+    cache[key] = op;
+    return op;
   }
-  KOperator<float> &op32(const double k[3]) {
-    KeyK key{{k[0], k[1], k[2]}};
-    auto it = ops32.find(key);
-    if (it != ops32.end()) return *it->second;
-    if (ops32.size() >= 8) ops32.clear();
-    auto op = std::make_unique<KOperator<float>>();
-    op->build(sys, k, stream);
-    auto &ref = *op;
-    ops32[key] = std::move(op);
-    return ref;
-  }
+  KOperator<double> &op64(const double k[3]) { return *cached(ops64, k); }
+  KOperator<float> &op32(const double k[3]) { return *cached(ops32, k); }
 };
 
 Context *g_contexts[kMaxContexts + 1] = {};
@@ -84,8 +82,8 @@ int64_t g_generation[kMaxContexts + 1] = {};
 struct OperatorEntry {
   int64_t owner;
   int precision;
-  KOperator<double> *op64;
-  KOperator<float> *op32;
+  std::shared_ptr<KOperator<double>> op64;
+  std::shared_ptr<KOperator<float>> op32;
   Context *ctx;
 };
 std::unordered_map<int64_t, OperatorEntry> g_ops;
@@ -541,11 +539,11 @@ int half_amd_operator_create(half_handle handle, const double kpoint[3], int pre
     int64_t n;
     int32_t np;
     if (precision == 64) {
-      e.op64 = &c.op64(kpoint);
+      e.op64 = c.cached(c.ops64, kpoint);
       n = e.op64->n;
       np = e.op64->np;
     } else {
-      e.op32 = &c.op32(kpoint);
+      e.op32 = c.cached(c.ops32, kpoint);
       n = e.op32->n;
       np = e.op32->np;
     }
@@ -582,9 +580,9 @@ int half_amd_operator_apply(half_amd_operator op, int stage, int nstates, const 
         k->apply_projectors(nstates, psi, ldpsi, hp, sp, ldout, s);
     };
     if (e.precision == 64)
-      run(e.op64, static_cast<const double2 *>(d_psi), static_cast<double2 *>(d_hpsi), static_cast<double2 *>(d_spsi));
+      run(e.op64.get(), static_cast<const double2 *>(d_psi), static_cast<double2 *>(d_hpsi), static_cast<double2 *>(d_spsi));
     else
-      run(e.op32, static_cast<const float2 *>(d_psi), static_cast<float2 *>(d_hpsi), static_cast<float2 *>(d_spsi));
+      run(e.op32.get(), static_cast<const float2 *>(d_psi), static_cast<float2 *>(d_hpsi), static_cast<float2 *>(d_spsi));
     return int(kSuccess);
   });
 }

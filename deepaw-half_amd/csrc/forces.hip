@@ -74,7 +74,7 @@ __global__ __launch_bounds__(64) void adjoint_partial_kernel(const double2 *B, i
   const int bc = b0 + (lane & 15);
   if (bc >= nb) return;
   for (int i = 0; i < 4; ++i) {
-    const int pr = p0 + 4 * (lane >> 4) + i;
+    const int pr = p0 + 4 * i + (lane >> 4);  // FP64 MFMA rows interleave across lane groups
     if (pr >= np) continue;
     for (int c = 0; c < 4; ++c)
       partial[((int64_t(blockIdx.y) * 4 + c) * nb + bc) * np + pr] = make_double2(cr[c][i], ci[c][i]);

@@ -1,0 +1,313 @@
+// Pruned local stage of H.psi: T psi + F[Veff F^-1[psi]] / N_r.
+//
+// Only the plane-wave sphere is populated in G space, so most 1D transforms
+// of a full 3D FFT act on zeros.  With the C-order box (i1 slowest, i3
+// contiguous) and the basis enumerated n1, n2, n3 ascending (so the plane
+// waves of one (i1, i2) line are contiguous), the transform is done in four
+// kernels that touch only what is needed:
+//
+//   z_bwd   active (i1,i2) lines: plane waves scattered into LDS straight
+//           from psi (no box clearing), backward transform along i3
+//   y_bwd   active i1 planes: backward transform along i2, loading only the
+//           active lines of the plane (the rest are zeros, never stored)
+//   x_veff  every (i2,i3) column: load the active planes, backward transform
+//           along i1, multiply by Veff/N_r, forward transform, store the
+//           active planes only
+//   y_fwd   active planes: forward transform along i2, store active lines
+//   z_fwd   active lines: forward transform along i3, gather the plane-wave
+//           coefficients and add T psi (S psi = psi is written here too)
+//
+// Box entries outside the active lines/planes are never read or written.
+// Numerically this is the same circular convolution as the reference's
+// scatter / inverse FFT / multiply / forward FFT / gather sequence
+// (reference: src/half_cuda_assembly.cuf:180-218, 484-498).
+#include <algorithm>
+
+#include "local.hpp"
+#include "stockham.hpp"
+#include "trace.hpp"
+
+namespace half {
+
+namespace {
+
+struct Tables {
+  int n1, n2, n3;
+  long long nr;
+  int nlines;
+  const int *line_i12, *line_start, *line_count, *line_of;
+  int nplanes;
+  const int *plane_i1, *plane_on;
+  const int *pw_i3;
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void z_bwd_kernel(const cplx_t<R> *psi, long long ldpsi, cplx_t<R> *box,
+                                                    Tables t, FftArgs a) {
+  using V = cplx_t<R>;
+  extern __shared__ double2 smem_raw[];
+  V *buf0 = reinterpret_cast<V *>(smem_raw);
+  V *buf1 = buf0 + a.M * a.n;
+  V *tws = buf1 + a.M * a.n;
+  load_twiddles(tws, a);
+  const long long L0 = static_cast<long long>(blockIdx.x) * a.M;
+  const int MN = a.M * a.n;
+  for (int e = threadIdx.x; e < MN; e += blockDim.x) buf0[e] = mk<V>(R(0), R(0));
+  __syncthreads();
+  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
+    const int l = e / a.n, j = e - l * a.n;
+    const long long L = L0 + l;
+    if (L >= a.lines_total) continue;
+    const int band = static_cast<int>(L / t.nlines);
+    const int line = static_cast<int>(L - static_cast<long long>(band) * t.nlines);
+    if (j < t.line_count[line]) {
+      const int g = t.line_start[line] + j;
+      buf0[l * a.n + t.pw_i3[g]] = psi[g + band * ldpsi];
+    }
+  }
+  __syncthreads();
+  V *res = run_stages(buf0, buf1, tws, a, +1);
+  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
+    const int l = e / a.n, p = e - l * a.n;
+    const long long L = L0 + l;
+    if (L >= a.lines_total) continue;
+    const int band = static_cast<int>(L / t.nlines);
+    const int line = static_cast<int>(L - static_cast<long long>(band) * t.nlines);
+    box[band * t.nr + static_cast<long long>(t.line_i12[line]) * t.n3 + p] = res[e];
+  }
+}
+
+// Along i2 on the active planes; BWD loads active lines only and stores all,
+// the forward pass loads all and stores active lines only.
+template <typename R, bool BWD>
+__global__ __launch_bounds__(256) void y_kernel(cplx_t<R> *box, Tables t, FftArgs a) {
+  using V = cplx_t<R>;
+  extern __shared__ double2 smem_raw[];
+  V *buf0 = reinterpret_cast<V *>(smem_raw);
+  V *buf1 = buf0 + a.M * a.n;
+  V *tws = buf1 + a.M * a.n;
+  load_twiddles(tws, a);
+  const long long L0 = static_cast<long long>(blockIdx.x) * a.M;
+  const int MN = a.M * a.n;
+  const long long per_band = static_cast<long long>(t.nplanes) * t.n3;
+  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
+    const int p = e / a.M, l = e - p * a.M;
+    const long long L = L0 + l;
+    V v = mk<V>(R(0), R(0));
+    if (L < a.lines_total) {
+      const int band = static_cast<int>(L / per_band);
+      const long long rem = L - band * per_band;
+      const int ip = static_cast<int>(rem / t.n3), i3 = static_cast<int>(rem - static_cast<long long>(ip) * t.n3);
+      const int i1 = t.plane_i1[ip];
+      if (!BWD || t.line_of[i1 * t.n2 + p] >= 0)
+        v = box[band * t.nr + (static_cast<long long>(i1) * t.n2 + p) * t.n3 + i3];
+    }
+    buf0[l * a.n + p] = v;
+  }
+  __syncthreads();
+  V *res = run_stages(buf0, buf1, tws, a, BWD ? +1 : -1);
+  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
+    const int p = e / a.M, l = e - p * a.M;
+    const long long L = L0 + l;
+    if (L >= a.lines_total) continue;
+    const int band = static_cast<int>(L / per_band);
+    const long long rem = L - band * per_band;
+    const int ip = static_cast<int>(rem / t.n3), i3 = static_cast<int>(rem - static_cast<long long>(ip) * t.n3);
+    const int i1 = t.plane_i1[ip];
+    if (BWD || t.line_of[i1 * t.n2 + p] >= 0)
+      box[band * t.nr + (static_cast<long long>(i1) * t.n2 + p) * t.n3 + i3] = res[l * a.n + p];
+  }
+}
+
+// Along i1 for every (i2, i3): backward, * Veff/N_r, forward.
+template <typename R>
+__global__ __launch_bounds__(256) void x_veff_kernel(cplx_t<R> *box, const R *veff, R scale, Tables t, FftArgs a) {
+  using V = cplx_t<R>;
+  extern __shared__ double2 smem_raw[];
+  V *buf0 = reinterpret_cast<V *>(smem_raw);
+  V *buf1 = buf0 + a.M * a.n;
+  V *tws = buf1 + a.M * a.n;
+  load_twiddles(tws, a);
+  const long long L0 = static_cast<long long>(blockIdx.x) * a.M;
+  const int MN = a.M * a.n;
+  const long long C = a.C;  // n2 * n3
+  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
+    const int p = e / a.M, l = e - p * a.M;
+    const long long L = L0 + l;
+    V v = mk<V>(R(0), R(0));
+    if (L < a.lines_total && t.plane_on[p]) {
+      const long long band = L / C, c = L - band * C;
+      v = box[band * t.nr + c + p * C];
+    }
+    buf0[l * a.n + p] = v;
+  }
+  __syncthreads();
+  V *res = run_stages(buf0, buf1, tws, a, +1);
+  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
+    const int p = e / a.M, l = e - p * a.M;
+    const long long L = L0 + l;
+    if (L < a.lines_total) {
+      const long long c = L % C;
+      res[l * a.n + p] = cscale(res[l * a.n + p], veff[c + p * C] * scale);
+    }
+  }
+  __syncthreads();
+  res = run_stages(res, res == buf0 ? buf1 : buf0, tws, a, -1);
+  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
+    const int p = e / a.M, l = e - p * a.M;
+    const long long L = L0 + l;
+    if (L < a.lines_total && t.plane_on[p]) {
+      const long long band = L / C, c = L - band * C;
+      box[band * t.nr + c + p * C] = res[l * a.n + p];
+    }
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void z_fwd_kernel(const cplx_t<R> *psi, long long ldpsi, const cplx_t<R> *box,
+                                                    const R *kin, cplx_t<R> *hpsi, cplx_t<R> *spsi,
+                                                    long long ldout, Tables t, FftArgs a) {
+  using V = cplx_t<R>;
+  extern __shared__ double2 smem_raw[];
+  V *buf0 = reinterpret_cast<V *>(smem_raw);
+  V *buf1 = buf0 + a.M * a.n;
+  V *tws = buf1 + a.M * a.n;
+  load_twiddles(tws, a);
+  const long long L0 = static_cast<long long>(blockIdx.x) * a.M;
+  const int MN = a.M * a.n;
+  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
+    const int l = e / a.n, p = e - l * a.n;
+    const long long L = L0 + l;
+    V v = mk<V>(R(0), R(0));
+    if (L < a.lines_total) {
+      const int band = static_cast<int>(L / t.nlines);
+      const int line = static_cast<int>(L - static_cast<long long>(band) * t.nlines);
+      v = box[band * t.nr + static_cast<long long>(t.line_i12[line]) * t.n3 + p];
+    }
+    buf0[e] = v;
+  }
+  __syncthreads();
+  V *res = run_stages(buf0, buf1, tws, a, -1);
+  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
+    const int l = e / a.n, j = e - l * a.n;
+    const long long L = L0 + l;
+    if (L >= a.lines_total) continue;
+    const int band = static_cast<int>(L / t.nlines);
+    const int line = static_cast<int>(L - static_cast<long long>(band) * t.nlines);
+    if (j < t.line_count[line]) {
+      const int g = t.line_start[line] + j;
+      const V p = psi[g + band * ldpsi];
+      const V v = res[l * a.n + t.pw_i3[g]];
+      hpsi[g + band * ldout] = mk<V>(v.x + kin[g] * p.x, v.y + kin[g] * p.y);
+      spsi[g + band * ldout] = p;
+    }
+  }
+}
+
+template <typename R>
+FftArgs axis_args(const FftPlan &plan, int axis, long long lines, int C) {
+  const FftAxis &ax = plan.axis[axis];
+  FftArgs a{};
+  a.n = ax.n;
+  a.nstages = ax.nstages;
+  for (int i = 0; i < 8; ++i) a.radix[i] = ax.radix[i];
+  a.tw = ax.twiddle.get();
+  a.lines_total = lines;
+  a.C = C;
+  const int budget = std::is_same<R, double>::value ? 1536 : 3072;
+  int M = 1;
+  while (M * 2 * ax.n <= budget && M < 64) M *= 2;
+  a.M = M;
+  return a;
+}
+
+template <typename R>
+size_t lds_bytes(const FftArgs &a) {
+  return (2 * size_t(a.M) * a.n + a.n) * sizeof(cplx_t<R>);
+}
+
+template <typename R>
+void launch(const char *name, const void *fn, const FftArgs &a, hipStream_t s, void **args) {
+  trace_mark(name, s);
+  const long long blocks = (a.lines_total + a.M - 1) / a.M;
+  HALF_HIP(hipLaunchKernel(fn, dim3(static_cast<unsigned>(blocks)), dim3(256), args, lds_bytes<R>(a), s));
+}
+
+}  // namespace
+
+void LocalMap::build(const Basis &b, hipStream_t s) {
+  n1 = b.shape[0];
+  n2 = b.shape[1];
+  n3 = b.shape[2];
+  const int n = static_cast<int>(b.npw());
+  std::vector<int> i12(n), i3(n), line_of_h(size_t(n1) * n2, -1), plane_on_h(n1, 0);
+  std::vector<int> starts, counts, lines;
+  for (int g = 0; g < n; ++g) {
+    i12[g] = b.fft_index[g] / n3;
+    i3[g] = b.fft_index[g] % n3;
+    if (g == 0 || i12[g] != i12[g - 1]) {
+      if (line_of_h[i12[g]] >= 0)
+        throw Error(kInternal, "plane waves of one FFT line are not contiguous in the basis");
+      line_of_h[i12[g]] = static_cast<int>(lines.size());
+      lines.push_back(i12[g]);
+      starts.push_back(g);
+      counts.push_back(0);
+    }
+    ++counts.back();
+    plane_on_h[i12[g] / n2] = 1;
+  }
+  std::vector<int> planes;
+  for (int i = 0; i < n1; ++i)
+    if (plane_on_h[i]) planes.push_back(i);
+  nlines = static_cast<int>(lines.size());
+  nplanes = static_cast<int>(planes.size());
+  line_i12.upload(lines, s);
+  line_start.upload(starts, s);
+  line_count.upload(counts, s);
+  line_of.upload(line_of_h, s);
+  plane_i1.upload(planes, s);
+  plane_on.upload(plane_on_h, s);
+  pw_i3.upload(i3, s);
+  HALF_HIP(hipStreamSynchronize(s));
+}
+
+template <typename R>
+void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const R *kin, int nb, const cplx_t<R> *psi,
+                 int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi, int64_t ldout, cplx_t<R> *boxes, hipStream_t s) {
+  if (nb <= 0 || m.nlines == 0) return;
+  Tables t{m.n1, m.n2, m.n3, static_cast<long long>(m.n1) * m.n2 * m.n3, m.nlines, m.line_i12.get(),
+           m.line_start.get(), m.line_count.get(), m.line_of.get(), m.nplanes, m.plane_i1.get(), m.plane_on.get(),
+           m.pw_i3.get()};
+  long long lp = ldpsi, lo = ldout;
+  FftArgs az = axis_args<R>(plan, 2, static_cast<long long>(nb) * m.nlines, 1);
+  FftArgs ay = axis_args<R>(plan, 1, static_cast<long long>(nb) * m.nplanes * m.n3, m.n3);
+  FftArgs ax = axis_args<R>(plan, 0, static_cast<long long>(nb) * m.n2 * m.n3, m.n2 * m.n3);
+  R scale = static_cast<R>(1.0 / double(t.nr));
+  {
+    void *args[] = {&psi, &lp, &boxes, &t, &az};
+    launch<R>("local_z_bwd", reinterpret_cast<const void *>(&z_bwd_kernel<R>), az, s, args);
+  }
+  {
+    void *args[] = {&boxes, &t, &ay};
+    launch<R>("local_y_bwd", reinterpret_cast<const void *>(&y_kernel<R, true>), ay, s, args);
+  }
+  {
+    void *args[] = {&boxes, &veff, &scale, &t, &ax};
+    launch<R>("local_x_veff", reinterpret_cast<const void *>(&x_veff_kernel<R>), ax, s, args);
+  }
+  {
+    void *args[] = {&boxes, &t, &ay};
+    launch<R>("local_y_fwd", reinterpret_cast<const void *>(&y_kernel<R, false>), ay, s, args);
+  }
+  {
+    void *args[] = {&psi, &lp, &boxes, &kin, &hpsi, &spsi, &lo, &t, &az};
+    launch<R>("local_z_fwd", reinterpret_cast<const void *>(&z_fwd_kernel<R>), az, s, args);
+  }
+}
+
+template void local_apply<double>(const FftPlan &, const LocalMap &, const double *, const double *, int,
+                                  const double2 *, int64_t, double2 *, double2 *, int64_t, double2 *, hipStream_t);
+template void local_apply<float>(const FftPlan &, const LocalMap &, const float *, const float *, int, const float2 *,
+                                 int64_t, float2 *, float2 *, int64_t, float2 *, hipStream_t);
+
+}  // namespace half

@@ -21,6 +21,7 @@
 
 #include "cplx.hpp"
 #include "paw.hpp"
+#include "trace.hpp"
 
 namespace half {
 
@@ -48,6 +49,15 @@ struct Acc<float> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
 };
+
+// Row of the 16x16 result held in accumulator element r of this lane.  The
+// lane's column is lane % 16 for both instructions; rows differ: the FP32
+// instruction keeps four consecutive rows per lane group, the FP64 one
+// interleaves them (row = 4 r + lane / 16).
+template <typename R>
+__device__ inline int mfma_row(int lane, int r) {
+  return std::is_same<R, double>::value ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r;
+}
 
 // Complex 16x16 tile accumulate: (cr, ci) += A(16x4) * B(4x16), A and B
 // given as this lane's real/imaginary parts.
@@ -249,8 +259,7 @@ __global__ __launch_bounds__(256) void coeff_partial_kernel(const cplx_t<R> *B, 
     for (int i = 0; i < 4; ++i) {
       R sr = red[0][0][i][lane] + red[1][0][i][lane] + red[2][0][i][lane] + red[3][0][i][lane];
       R si = red[0][1][i][lane] + red[1][1][i][lane] + red[2][1][i][lane] + red[3][1][i][lane];
-      // D layout: row = 4*(lane/16) + i (projector), column = lane%16 (band)
-      const int pr = p0 + 4 * (lane >> 4) + i;
+      const int pr = p0 + mfma_row<R>(lane, i);  // projector row, band column lane % 16
       const int bc = b0 + (lane & 15);
       if (pr < np && bc < nb)
         partial[(int64_t(blockIdx.y) * nb + bc) * np + pr] = mk<cplx_t<R>>(sr, si);
@@ -332,7 +341,7 @@ __global__ __launch_bounds__(256) void expand_kernel(const cplx_t<R> *B, int64_t
   const int bc = b0 + (lane & 15);
   if (bc >= nb) return;
   for (int i = 0; i < 4; ++i) {
-    const int gr = g0 + 4 * (lane >> 4) + i;
+    const int gr = g0 + mfma_row<R>(lane, i);
     if (gr >= n) continue;
     const int64_t o = gr + int64_t(bc) * ldout;
     cplx_t<R> h = hpsi[o], s = spsi[o];
@@ -409,7 +418,7 @@ __global__ __launch_bounds__(256) void dense_kernel(const double2 *E, const doub
   const int jc = j0 + (lane & 15);
   if (jc >= n) return;
   for (int k = 0; k < 4; ++k) {
-    const int ir = i0 + 4 * (lane >> 4) + k;
+    const int ir = i0 + mfma_row<double>(lane, k);
     if (ir >= n) continue;
     int a = (gvec[3 * ir] - gvec[3 * jc]) % n1;
     int b = (gvec[3 * ir + 1] - gvec[3 * jc + 1]) % n2;
@@ -564,6 +573,7 @@ void KOperator<R>::build(const System &s, const double kpoint[3], hipStream_t st
   np = s.nproj;
   ldb = n;
   fft_index.upload(basis.fft_index, st);
+  lmap.build(basis, st);
   std::vector<R> kin(basis.kinetic.begin(), basis.kinetic.end());
   kinetic.upload(kin, st);
   qvec.upload(basis.q, st);
@@ -576,12 +586,13 @@ void KOperator<R>::build(const System &s, const double kpoint[3], hipStream_t st
     hipLaunchKernelGGL((projector_kernel<R>), grid, dim3(256), 0, st, B.get(), static_cast<double2 *>(nullptr), a);
     HALF_HIP(hipGetLastError());
   }
+  // the uploads above read host vectors local to this call
+  HALF_HIP(hipStreamSynchronize(st));
 }
 
 template <typename R>
 void KOperator<R>::apply_local(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
                                int64_t ldout, hipStream_t st) {
-  using V = cplx_t<R>;
   const int64_t nr = sys->fft.nr;
   const int batch = std::max(1, std::min(box_batch, nb));
   boxes.ensure(size_t(nr) * batch);
@@ -592,20 +603,8 @@ void KOperator<R>::apply_local(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx
     veff = sys->veff32.get();
   for (int b0 = 0; b0 < nb; b0 += batch) {
     const int cnt = std::min(batch, nb - b0);
-    hipLaunchKernelGGL((zero_kernel<V>), dim3(blocks_for(nr * cnt)), dim3(kThreads), 0, st, boxes.get(),
-                       nr * cnt);
-    hipLaunchKernelGGL((scatter_kernel<V>), dim3(blocks_for(int64_t(n) * cnt)), dim3(kThreads), 0, st,
-                       psi + b0 * ldpsi, ldpsi, boxes.get(), nr, fft_index.get(), n, cnt);
-    HALF_HIP(hipGetLastError());
-    fft_axis<R>(sys->fft, boxes.get(), cnt, 2, +1, st);
-    fft_axis<R>(sys->fft, boxes.get(), cnt, 1, +1, st);
-    fft_axis0_veff<R>(sys->fft, boxes.get(), cnt, veff, R(1.0 / double(nr)), st);
-    fft_axis<R>(sys->fft, boxes.get(), cnt, 1, -1, st);
-    fft_axis<R>(sys->fft, boxes.get(), cnt, 2, -1, st);
-    hipLaunchKernelGGL((gather_kernel<V, R>), dim3(blocks_for(int64_t(n) * cnt)), dim3(kThreads), 0, st,
-                       psi + b0 * ldpsi, ldpsi, boxes.get(), nr, fft_index.get(), kinetic.get(),
-                       hpsi + b0 * ldout, spsi + b0 * ldout, ldout, n, cnt);
-    HALF_HIP(hipGetLastError());
+    local_apply<R>(sys->fft, lmap, veff, kinetic.get(), cnt, psi + b0 * ldpsi, ldpsi, hpsi + b0 * ldout,
+                   spsi + b0 * ldout, ldout, boxes.get(), st);
   }
 }
 
@@ -622,9 +621,11 @@ void KOperator<R>::project(int nb, const cplx_t<R> *psi, int64_t ldpsi, hipStrea
   const int64_t count = int64_t(np) * nb;
   coeff.ensure(count);
   partial.ensure(count * nchunks);
+  trace_mark("proj_coeff_mfma", st);
   hipLaunchKernelGGL((coeff_partial_kernel<R>), dim3(tiles, nchunks), dim3(256), 0, st, B.get(), ldb, psi,
                      ldpsi, partial.get(), n, np, nb, chunk, tiles_p);
   HALF_HIP(hipGetLastError());
+  trace_mark("proj_coeff_reduce", st);
   hipLaunchKernelGGL((reduce_chunks_kernel<V>), dim3(blocks_for(count)), dim3(kThreads), 0, st, partial.get(),
                      coeff.get(), count, nchunks);
   HALF_HIP(hipGetLastError());
@@ -647,12 +648,14 @@ void KOperator<R>::apply_projectors(int nb, const cplx_t<R> *psi, int64_t ldpsi,
     db = sys->d_dblk32.get();
     qb = sys->d_qblk32.get();
   }
+  trace_mark("proj_mix", st);
   hipLaunchKernelGGL((mix_kernel<V, R>), dim3(blocks_for(count)), dim3(kThreads), 0, st, coeff.get(),
                      dcoeff.get(), qcoeff.get(), sys->d_proj_atom.get(), sys->d_proj_lm.get(),
                      sys->d_atom_first.get(), sys->d_atom_nlm.get(), sys->d_atom_boff.get(), db, qb, np, nb);
   HALF_HIP(hipGetLastError());
   const int tiles_g = (n + 15) / 16, tiles_b = (nb + 15) / 16;
   const int tiles = tiles_g * tiles_b;
+  trace_mark("proj_expand_mfma", st);
   hipLaunchKernelGGL((expand_kernel<R>), dim3((tiles + 3) / 4), dim3(256), 0, st, B.get(), ldb, dcoeff.get(),
                      qcoeff.get(), hpsi, spsi, ldout, n, np, nb, tiles_g);
   HALF_HIP(hipGetLastError());
@@ -664,6 +667,7 @@ void KOperator<R>::apply(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> 
   if (nb <= 0) return;
   apply_local(nb, psi, ldpsi, hpsi, spsi, ldout, st);
   apply_projectors(nb, psi, ldpsi, hpsi, spsi, ldout, st);
+  trace_close(st);
 }
 
 template struct KOperator<double>;

@@ -14,6 +14,7 @@
 
 #include "common.hpp"
 #include "fft.hpp"
+#include "local.hpp"
 
 namespace half {
 
@@ -63,6 +64,7 @@ struct KOperator {
   DeviceArray<R> kinetic;
   DeviceArray<double> qvec;  // npw x 3 (for the force adjoint)
   DeviceArray<cplx_t<R>> B;
+  LocalMap lmap;  // plane-wave sphere on the box (pruned transforms)
   // workspace
   int box_batch = 16;
   DeviceArray<cplx_t<R>> boxes;

@@ -124,9 +124,12 @@ void Blas::init(hipStream_t s) {
   HALF_BLAS(rocblas_set_stream(h, s));
 }
 
-Blas::~Blas() {
+void Blas::release() {
   if (h) rocblas_destroy_handle(h);
+  h = nullptr;
 }
+
+Blas::~Blas() { release(); }
 
 std::vector<int> kinetic_order(const std::vector<double> &kin) {
   const int n = static_cast<int>(kin.size());
@@ -190,7 +193,9 @@ void solve_dense(Blas &blas, const KOperator<double> &op, int solver, int nbands
     // lowest nbands only (reference: src/half_cuda_solver.cuf:119-143)
     rocblas_int nev = 0;
     DeviceArray<rocblas_int> dnev(1);
-    DeviceArray<double2> Z(want_vectors ? size_t(n) * nbands : 1);
+    // Z must hold n columns: rocSOLVER may use all of them as workspace,
+    // whatever the index range asks for
+    DeviceArray<double2> Z(size_t(n) * n);
     HALF_BLAS(rocsolver_zhegvdx(blas.h, rocblas_eform_ax, ev, rocblas_erange_index, rocblas_fill_upper, n,
                                 rz(H.get()), n, rz(S.get()), n, 0.0, 0.0, 1, nbands, dnev.get(), W.get(),
                                 rz(Z.get()), n, info.get()));
@@ -198,7 +203,12 @@ void solve_dense(Blas &blas, const KOperator<double> &op, int solver, int nbands
     HALF_HIP(hipMemcpyAsync(&nev, dnev.get(), sizeof(nev), hipMemcpyDeviceToHost, s));
     HALF_HIP(hipStreamSynchronize(s));
     if (nev != nbands) throw Error(kInternal, "index-range eigensolve returned a different band count");
-    if (want_vectors) vectors = std::move(Z);
+    if (want_vectors) {
+      vectors.resize(size_t(n) * nbands);
+      HALF_HIP(hipMemcpyAsync(vectors.get(), Z.get(), sizeof(double2) * size_t(n) * nbands, hipMemcpyDeviceToDevice,
+                              s));
+      HALF_HIP(hipStreamSynchronize(s));
+    }
   } else if (solver == kSolverEvj) {
     double resid = 0;
     rocblas_int sweeps = 0;

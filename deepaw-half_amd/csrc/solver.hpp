@@ -15,6 +15,7 @@ struct Blas {
   rocblas_handle h = nullptr;
   hipStream_t stream = nullptr;
   void init(hipStream_t s);
+  void release();
   ~Blas();
 };
 

@@ -1,0 +1,162 @@
+// Device core of the LDS-staged mixed-radix Stockham FFT: radix butterflies
+// and the stage loop over M lines held in LDS (used by fft.hip for whole
+// boxes and by local.hip for the pruned plane-wave transforms).
+#pragma once
+
+#include "cplx.hpp"
+#include "fft.hpp"
+
+namespace half {
+
+struct FftArgs {
+  int n;
+  int nstages;
+  int radix[8];
+  const double2 *tw;
+  long long lines_total;  // number of 1D lines in the batch
+  int C;                  // element stride of the transform axis
+  int M;                  // lines per workgroup
+};
+
+template <typename V>
+__device__ inline V twiddle(const V *tws, int q, int sign) {
+  V w = tws[q];
+  if (sign > 0) w.y = -w.y;
+  return w;
+}
+
+// In-register DFT of length R with exponent sign `sign`.
+template <typename V>
+__device__ inline void butterfly(V *v, int R, int sign, const V *tws, int n) {
+  using S = typename scalar_of<V>::type;
+  const S sg = static_cast<S>(sign);
+  switch (R) {
+    case 2: {
+      V a = v[0], b = v[1];
+      v[0] = cadd(a, b);
+      v[1] = csub(a, b);
+      break;
+    }
+    case 3: {
+      const S c = S(-0.5), s = sg * S(0.86602540378443864676372317075294);
+      V a1 = cadd(v[1], v[2]);
+      V t = cadd(v[0], cscale(a1, c));
+      V u = cmul_i(csub(v[1], v[2]), s);
+      v[0] = cadd(v[0], a1);
+      v[1] = cadd(t, u);
+      v[2] = csub(t, u);
+      break;
+    }
+    case 4: {
+      V t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+      V t2 = cadd(v[1], v[3]), t3 = cmul_i(csub(v[1], v[3]), sg);
+      v[0] = cadd(t0, t2);
+      v[1] = cadd(t1, t3);
+      v[2] = csub(t0, t2);
+      v[3] = csub(t1, t3);
+      break;
+    }
+    case 5: {
+      const S c1 = S(0.30901699437494742410229341718282);
+      const S c2 = S(-0.80901699437494742410229341718282);
+      const S s1 = sg * S(0.95105651629515357211643933337938);
+      const S s2 = sg * S(0.58778525229247312916870595463907);
+      V a1 = cadd(v[1], v[4]), a2 = cadd(v[2], v[3]);
+      V b1 = csub(v[1], v[4]), b2 = csub(v[2], v[3]);
+      V r1 = cadd(v[0], cadd(cscale(a1, c1), cscale(a2, c2)));
+      V r2 = cadd(v[0], cadd(cscale(a1, c2), cscale(a2, c1)));
+      V i1 = cmul_i(cadd(cscale(b1, s1), cscale(b2, s2)), S(1));
+      V i2 = cmul_i(csub(cscale(b1, s2), cscale(b2, s1)), S(1));
+      v[0] = cadd(v[0], cadd(a1, a2));
+      v[1] = cadd(r1, i1);
+      v[4] = csub(r1, i1);
+      v[2] = cadd(r2, i2);
+      v[3] = csub(r2, i2);
+      break;
+    }
+    case 8: {
+      const S h = S(0.70710678118654752440084436210485);
+      // radix-2 split into two radix-4 halves
+      V e0 = cadd(v[0], v[4]), e1 = csub(v[0], v[4]);
+      V e2 = cadd(v[2], v[6]), e3 = cmul_i(csub(v[2], v[6]), sg);
+      V E0 = cadd(e0, e2), E2 = csub(e0, e2), E1 = cadd(e1, e3), E3 = csub(e1, e3);
+      V o0 = cadd(v[1], v[5]), o1 = csub(v[1], v[5]);
+      V o2 = cadd(v[3], v[7]), o3 = cmul_i(csub(v[3], v[7]), sg);
+      V O0 = cadd(o0, o2), O2 = csub(o0, o2), O1 = cadd(o1, o3), O3 = csub(o1, o3);
+      // twiddles w8^s = exp(sign i pi s / 4)
+      V w1 = mk<V>(h, sg * h), w3 = mk<V>(-h, sg * h);
+      O1 = cmul(O1, w1);
+      O2 = cmul_i(O2, sg);
+      O3 = cmul(O3, w3);
+      v[0] = cadd(E0, O0);
+      v[4] = csub(E0, O0);
+      v[1] = cadd(E1, O1);
+      v[5] = csub(E1, O1);
+      v[2] = cadd(E2, O2);
+      v[6] = csub(E2, O2);
+      v[3] = cadd(E3, O3);
+      v[7] = csub(E3, O3);
+      break;
+    }
+    default: {  // generic small radix (7): direct DFT from the twiddle table
+      V y[8];
+      const int step = n / R;
+      for (int s = 0; s < R; ++s) {
+        V acc = v[0];
+        for (int r = 1; r < R; ++r) acc = cadd(acc, cmul(v[r], twiddle(tws, (r * s % R) * step, sign)));
+        y[s] = acc;
+      }
+      for (int s = 0; s < R; ++s) v[s] = y[s];
+    }
+  }
+}
+
+// Stockham stages on M lines held in LDS; returns the buffer with the result.
+template <typename V>
+__device__ V *run_stages(V *in, V *out, const V *tws, const FftArgs &a, int sign) {
+  const int n = a.n;
+  int Ns = 1;
+  for (int s = 0; s < a.nstages; ++s) {
+    const int R = a.radix[s];
+    const int nr = n / R;
+    const int tasks = a.M * nr;
+    const int tstep = n / (Ns * R);
+    for (int t = threadIdx.x; t < tasks; t += blockDim.x) {
+      const int l = t / nr;
+      const int j = t - l * nr;
+      const int k = j % Ns;
+      V v[8];
+      const V *src = in + l * n + j;
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        if (r < R) v[r] = src[r * nr];
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < 8; ++r)
+          if (r < R) v[r] = cmul(v[r], twiddle(tws, k * r * tstep, sign));
+      }
+      butterfly(v, R, sign, tws, n);
+      V *dst = out + l * n + (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        if (r < R) dst[r * Ns] = v[r];
+    }
+    __syncthreads();
+    V *tmp = in;
+    in = out;
+    out = tmp;
+    Ns *= R;
+  }
+  return in;
+}
+
+
+// Twiddle table of one axis into LDS (converted to the working precision).
+template <typename V>
+__device__ inline void load_twiddles(V *tws, const FftArgs &a) {
+  using S = typename scalar_of<V>::type;
+  for (int i = threadIdx.x; i < a.n; i += blockDim.x)
+    tws[i] = mk<V>(static_cast<S>(a.tw[i].x), static_cast<S>(a.tw[i].y));
+}
+
+}  // namespace half

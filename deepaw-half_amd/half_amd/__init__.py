@@ -38,6 +38,7 @@ EXT_SYMBOLS = (
     "half_amd_operator_create", "half_amd_operator_apply", "half_amd_operator_destroy",
     "half_amd_solve_acc", "half_amd_set_solver", "half_amd_energy_terms",
     "half_amd_get_veff", "half_amd_nonlocal_forces", "half_amd_device_info",
+    "half_amd_trace", "half_amd_trace_read",
 )
 
 
@@ -104,6 +105,8 @@ def load(path: str | None = None):
         "half_amd_get_veff": [i64, i64, pd, cp, i],
         "half_amd_nonlocal_forces": [i64, pd, i, pd, pd, d, vp, i64, pd, cp, i],
         "half_amd_device_info": [cp, i, pi32, cp, i],
+        "half_amd_trace": [i, cp, i],
+        "half_amd_trace_read": [i, cp, pd, pi64, pi32, cp, i],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -137,6 +140,31 @@ def device_info():
     e = _err()
     _check(lib.half_amd_device_info(name, 256, C.byref(cu), e, 512), e)
     return name.value.decode(), cu.value
+
+
+def trace(enable: bool):
+    """Start (clearing earlier marks) or stop the launch-boundary timeline."""
+    lib = load()
+    e = _err()
+    _check(lib.half_amd_trace(int(bool(enable)), e, 512), e)
+
+
+def trace_read():
+    """{kernel name: (total ms, launches)} of the recorded timeline."""
+    lib = load()
+    cap = 64
+    names = C.create_string_buffer(64 * cap)
+    ms = np.zeros(cap)
+    cnt = np.zeros(cap, np.int64)
+    nk = C.c_int32()
+    e = _err()
+    _check(lib.half_amd_trace_read(cap, names, _pd(ms), cnt.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(nk),
+                                   e, 512), e)
+    out = {}
+    for i in range(min(nk.value, cap)):
+        nm = names.raw[64 * i:64 * (i + 1)].split(b"\0", 1)[0].decode()
+        out[nm] = (float(ms[i]), int(cnt[i]))
+    return out
 
 
 class Context:

@@ -82,6 +82,15 @@ int half_amd_nonlocal_forces(half_handle handle, const double kpoint[3], int nba
                              double kweight, const HALF_COMPLEX *vectors, int64_t ld,
                              double *forces, char *error, int error_capacity);
 
+/* Launch-boundary timeline for measurement.  half_amd_trace(1) starts
+ * recording an event before every operator/FFT kernel launch (clearing
+ * earlier marks), half_amd_trace(0) stops.  half_amd_trace_read returns, per
+ * kernel name (64-byte slots), the summed duration between a launch and the
+ * next mark and the launch count; nkinds receives the number of names. */
+int half_amd_trace(int enable, char *error, int error_capacity);
+int half_amd_trace_read(int max_kernels, char *names, double *total_ms, int64_t *launches,
+                        int32_t *nkinds, char *error, int error_capacity);
+
 /* Name and compute-unit count of the current HIP device. */
 int half_amd_device_info(char *name, int name_capacity, int32_t *compute_units, char *error,
                          int error_capacity);

@@ -46,3 +46,14 @@ def psi_block(n, nb=3):
     ig = np.arange(1, n + 1)[:, None]
     ib = np.arange(1, nb + 1)[None, :]
     return np.cos(0.37 * ig + ib) + 1j * np.sin(0.11 * ig * ib)
+
+
+def pytest_runtest_logstart(nodeid, location):
+    print(f"\n[start] {nodeid}", flush=True)
+
+
+def pytest_runtest_logreport(report):
+    # failure details right away, so they survive a run that ends abruptly
+    if report.failed:
+        text = str(report.longrepr)
+        print(f"[failed] {report.nodeid} ({report.when}):\n" + text[-1500:], flush=True)

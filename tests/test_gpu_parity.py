@@ -118,25 +118,30 @@ def test_forces_adjoint(si2):
 
 
 def test_fp32_operator_close_to_fp64(si2):
+    from half_amd import hip
     name, g, ctx = si2
-    import torch
     n = g["npw"]
     psi = psi_block(n, 8)
-    op64, op32 = ctx.operator(g["kpoint"], 64), ctx.operator(g["kpoint"], 32)
-    dev = torch.device("cuda")
     res = {}
-    for op, dt in ((op64, torch.complex128), (op32, torch.complex64)):
-        x = torch.from_numpy(np.ascontiguousarray(psi.T)).to(dev, dt)  # column-major (n, nb)
-        hp, sp = torch.zeros_like(x), torch.zeros_like(x)
-        op.apply_device(8, x.data_ptr(), hp.data_ptr(), sp.data_ptr(), stream=torch.cuda.current_stream().cuda_stream)
-        torch.cuda.synchronize()
-        res[op.precision] = (hp.cpu().numpy().T.astype(np.complex128), sp.cpu().numpy().T.astype(np.complex128))
+    stream = hip.Stream()
+    for prec, dt in ((64, np.complex128), (32, np.complex64)):
+        op = ctx.operator(g["kpoint"], prec)
+        x = hip.Buffer.from_numpy(np.ascontiguousarray(psi.T).astype(dt))  # column-major (n, nb)
+        hp, sp = hip.Buffer(x.shape, dt), hip.Buffer(x.shape, dt)
+        op.apply_device(8, x.ptr, hp.ptr, sp.ptr, stream=stream.ptr)
+        stream.synchronize()
+        res[prec] = (hp.numpy().T.astype(np.complex128), sp.numpy().T.astype(np.complex128))
+        # the stages compose: local then projector part equals the full operator
+        hl, sl = hip.Buffer(x.shape, dt), hip.Buffer(x.shape, dt)
+        op.apply_device(8, x.ptr, hl.ptr, sl.ptr, stage=1, stream=stream.ptr)
+        op.apply_device(8, x.ptr, hl.ptr, sl.ptr, stage=2, stream=stream.ptr)
+        stream.synchronize()
+        assert np.abs(hl.numpy() - hp.numpy()).max() <= 1e-12 * np.abs(hp.numpy()).max()
+        op.close()
     h64, s64 = ctx.apply_hs(psi, g["kpoint"])
     assert np.abs(res[64][0] - h64).max() / np.abs(h64).max() < 1e-14
     assert np.abs(res[32][0] - h64).max() / np.abs(h64).max() < 2e-6
     assert np.abs(res[32][1] - s64).max() / np.abs(s64).max() < 2e-6
-    op64.close()
-    op32.close()
 
 
 def test_mapped_solve_permutation(si2):
