@@ -7,7 +7,8 @@ stores its outputs as JSON.  Run from the repository root after
 
     python tests/golden/make_golden.py [case ...]
 
-Cases: si2_pbe, si2_lda, si2_k (PBE at k = (0.1, 0.2, 0.3)), hfo2_pbe, kat.
+Cases: si2_pbe, si2_lda, si2_k (PBE at k = (0.1, 0.2, 0.3)), hfo2_pbe, si8k_k
+(8-atom cell at k = (1/6, -1/3, 1/2) of the 6x6x6 mesh), kat.
 The si2 input files are committed next to this script; the hfo2 inputs are
 regenerated by the seeded generator and their sha256 is recorded so a test can
 refuse to compare against goldens made from different inputs.
@@ -33,6 +34,7 @@ CASES = {
     "si2_lda": ("si2", "lda", (0.0, 0.0, 0.0), 16),
     "si2_k": ("si2", "pbe", (0.1, 0.2, 0.3), 16),
     "hfo2_pbe": ("hfo2", "pbe", (0.0, 0.0, 0.0), 64),
+    "si8k_k": ("si8k", "pbe", (1.0 / 6.0, -1.0 / 3.0, 0.5), 32),
 }
 
 _FLOAT = re.compile(r"^([-+]?\d+\.\d+)([-+]\d{3})$")
