@@ -128,8 +128,10 @@ def main():
         if "fp64" in solve and "mixed" in solve:
             solve["speedup_mixed_vs_fp64"] = solve["fp64"]["seconds"] / solve["mixed"]["seconds"]
 
-    roofline = kernel_roofline(kernels[64], nr, n, nproj, nb, args.steps, fp64=True)
-    roofline32 = kernel_roofline(kernels[32], nr, n, nproj, nb, args.steps, fp64=False)
+    gv, _ = ctx.basis(k)
+    nlines, nplanes = sphere_counts(gv, info["grid"])
+    roofline = kernel_roofline(kernels[64], nr, info["grid"], n, nproj, nb, args.steps, nlines, nplanes, True)
+    roofline32 = kernel_roofline(kernels[32], nr, info["grid"], n, nproj, nb, args.steps, nlines, nplanes, False)
     b_alg = algorithmic_bytes(nr, n, True)
     f_proj = 24.0 * n * nproj
     t_band = per_step[64] * 1e-3 / nb
@@ -147,7 +149,8 @@ def main():
         "data": "synthetic (seeded Gaussian density, synthetic PAW datasets)",
         "config": {"workload": f"{args.workload}: {sum(w.counts)} atoms, {n} plane waves, box {info['grid']}, "
                                f"{nproj} projectors, {nb} bands, Gamma, {w.xc.upper()}",
-                   "npw": n, "box": list(info["grid"]), "nproj": nproj, "bands": nb},
+                   "npw": n, "box": list(info["grid"]), "nproj": nproj, "bands": nb,
+                   "active_lines": nlines, "active_planes": nplanes},
         "fp32_value": rates[32], "fp32_ms_per_step": per_step[32],
         "all_band_solve": solve,
         "roofline": roofline,
@@ -158,37 +161,51 @@ def main():
     print(json.dumps(out))
 
 
-def kernel_roofline(kern, nr, n, nproj, nb, steps, fp64=True, box_batch=16):
-    """Roofline of the kernel with the largest share of the traced time:
-    algorithmic bytes (or flops) of all its launches over its summed launch
-    durations."""
-    sc, sr = (16, 8) if fp64 else (8, 4)
-    nbatch_full, rem = divmod(nb, box_batch)
-    boxes = nb  # band boxes processed per step, summed over batches
-    per_step_bytes = {
-        "zero_boxes": boxes * nr * sc,
-        "scatter": boxes * n * (2 * sc + 4),
-        "fft_axis2_bwd": 2 * boxes * nr * sc, "fft_axis1_bwd": 2 * boxes * nr * sc,
-        "fft_axis0_veff": 2 * boxes * nr * sc + boxes * nr * sr,
-        "fft_axis1_fwd": 2 * boxes * nr * sc, "fft_axis2_fwd": 2 * boxes * nr * sc,
-        "gather_kinetic": boxes * n * (4 * sc + sr + 4),
-        "proj_coeff_reduce": None, "proj_mix": None,
+def sphere_counts(gvec, grid):
+    """Active (i1, i2) lines and i1 planes of the plane-wave sphere."""
+    import numpy as np
+    i1 = np.mod(gvec[:, 0], grid[0])
+    i2 = np.mod(gvec[:, 1], grid[1])
+    return len(set(zip(i1.tolist(), i2.tolist()))), len(set(i1.tolist()))
+
+
+def kernel_bytes(name, nr, grid, n, nlines, nplanes, nb, sc, sr):
+    """Bytes a kernel must move per step (nb bands) in the pruned scheme:
+    each active line/plane read and written once, Veff read once per band,
+    plane-wave vectors read/written once."""
+    n1, n2, n3 = grid
+    line = nlines * n3 * sc
+    plane = nplanes * n2 * n3 * sc
+    table = {
+        "local_z_bwd": n * (sc + 8) + line,
+        "local_y_bwd": line + plane,
+        "local_x_veff": 2 * plane + nr * sr,
+        "local_y_fwd": plane + line,
+        "local_z_fwd": line + n * (3 * sc + sr + 8),
     }
+    return None if name not in table else nb * table[name]
+
+
+def kernel_roofline(kern, nr, grid, n, nproj, nb, steps, nlines, nplanes, fp64=True):
+    """Roofline of the kernel with the largest share of the traced time:
+    its algorithmic bytes (or flops) over its summed launch durations."""
+    sc, sr = (16, 8) if fp64 else (8, 4)
     per_step_flops = {"proj_coeff_mfma": 8.0 * n * nproj * nb, "proj_expand_mfma": 16.0 * n * nproj * nb}
     name = max(kern, key=lambda k: kern[k][0])
     total_ms, launches = kern[name]
     t = total_ms * 1e-3
+    out = {"kernel": name, "avg_launch_ms": total_ms / launches, "launches_per_step": launches / steps}
     if name in per_step_flops:
         peak = MFMA_FP64_TFLOPS if fp64 else MFMA_FP32_TFLOPS
         ach = per_step_flops[name] * steps / t / 1e12
-        return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
-                "traffic": None, "kernel": name, "avg_launch_ms": total_ms / launches}
-    b = per_step_bytes.get(name)
-    ach = (b * steps / t / 1e9) if b else None
-    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": (ach / HBM_PEAK_GBS) if ach else None, "traffic": None, "kernel": name,
-            "avg_launch_ms": total_ms / launches,
-            "bytes_per_launch": (b / max(1, launches // steps)) if b else None}
+        out.update(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak, traffic=None)
+        return out
+    b = kernel_bytes(name, nr, grid, n, nlines, nplanes, nb, sc, sr)
+    ach = b * steps / t / 1e9 if b else None
+    out.update(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
+               frac=(ach / HBM_PEAK_GBS) if ach else None, traffic=None,
+               bytes_per_launch=(b * steps / launches) if b else None)
+    return out
 
 
 def cpu_baseline(chg, pot, w, k, nb):

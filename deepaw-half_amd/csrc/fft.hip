@@ -16,8 +16,8 @@ __global__ __launch_bounds__(256) void fft_lines_kernel(cplx_t<R> *data, FftArgs
   using V = cplx_t<R>;
   extern __shared__ double2 smem_raw[];
   V *buf0 = reinterpret_cast<V *>(smem_raw);
-  V *buf1 = buf0 + a.M * a.n;
-  V *tws = buf1 + a.M * a.n;
+  V *buf1 = buf0 + a.M * a.ld;
+  V *tws = buf1 + a.M * a.ld;
   for (int i = threadIdx.x; i < a.n; i += blockDim.x)
     tws[i] = mk<V>(static_cast<R>(a.tw[i].x), static_cast<R>(a.tw[i].y));
   const long long L0 = static_cast<long long>(blockIdx.x) * a.M;
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) void fft_lines_kernel(cplx_t<R> *data, FftArgs
       const long long c = L - blk * a.C;
       v = data[blk * nC + c + static_cast<long long>(p) * a.C];
     }
-    buf0[l * a.n + p] = v;
+    buf0[l * a.ld + p] = v;
   }
   __syncthreads();
   V *res = run_stages(buf0, buf1, tws, a, VEFF ? +1 : sign);
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void fft_lines_kernel(cplx_t<R> *data, FftArgs
       const long long L = L0 + l;
       if (L < a.lines_total) {
         const long long c = L % a.C;
-        res[e] = cscale(res[e], veff[c + static_cast<long long>(p) * a.C] * scale);
+        res[l * a.ld + p] = cscale(res[l * a.ld + p], veff[c + static_cast<long long>(p) * a.C] * scale);
       }
     }
     __syncthreads();
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void fft_lines_kernel(cplx_t<R> *data, FftArgs
     if (L < a.lines_total) {
       const long long blk = L / a.C;
       const long long c = L - blk * a.C;
-      data[blk * nC + c + static_cast<long long>(p) * a.C] = res[l * a.n + p];
+      data[blk * nC + c + static_cast<long long>(p) * a.C] = res[l * a.ld + p];
     }
   }
 }
@@ -102,7 +102,8 @@ FftArgs make_args(const FftPlan &plan, int axis, int64_t nbatch, int &M, size_t 
   M = 1;
   while (M * 2 * ax.n <= budget && M < 64) M *= 2;
   a.M = M;
-  lds = (2 * size_t(M) * ax.n + ax.n) * sizeof(cplx_t<R>);
+  a.ld = ax.n + 1;
+  lds = (2 * size_t(M) * a.ld + ax.n) * sizeof(cplx_t<R>);
   return a;
 }
 

@@ -31,6 +31,9 @@ namespace half {
 
 namespace {
 
+int g_tile_points64 = 768;   // tunable: FFT points per workgroup tile, FP64
+int g_tile_points32 = 1536;  // FP32
+
 struct Tables {
   int n1, n2, n3;
   long long nr;
@@ -41,81 +44,97 @@ struct Tables {
   const int *pw_i3;
 };
 
+// Elements per thread of a tile (M * n <= tile budget <= 256 * KMAX).
+template <typename R>
+struct Tile {
+  static constexpr int KMAX = std::is_same<R, double>::value ? 8 : 16;
+};
+
 template <typename R>
 __global__ __launch_bounds__(256) void z_bwd_kernel(const cplx_t<R> *psi, long long ldpsi, cplx_t<R> *box,
                                                     Tables t, FftArgs a) {
   using V = cplx_t<R>;
+  constexpr int K = Tile<R>::KMAX;
   extern __shared__ double2 smem_raw[];
   V *buf0 = reinterpret_cast<V *>(smem_raw);
-  V *buf1 = buf0 + a.M * a.n;
-  V *tws = buf1 + a.M * a.n;
+  V *buf1 = buf0 + a.M * a.ld;
+  V *tws = buf1 + a.M * a.ld;
   load_twiddles(tws, a);
-  const long long L0 = static_cast<long long>(blockIdx.x) * a.M;
+  const int L0 = blockIdx.x * a.M;
   const int MN = a.M * a.n;
-  for (int e = threadIdx.x; e < MN; e += blockDim.x) buf0[e] = mk<V>(R(0), R(0));
+  for (int e = threadIdx.x; e < a.M * a.ld; e += blockDim.x) buf0[e] = mk<V>(R(0), R(0));
   __syncthreads();
-  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
-    const int l = e / a.n, j = e - l * a.n;
-    const long long L = L0 + l;
-    if (L >= a.lines_total) continue;
-    const int band = static_cast<int>(L / t.nlines);
-    const int line = static_cast<int>(L - static_cast<long long>(band) * t.nlines);
-    if (j < t.line_count[line]) {
-      const int g = t.line_start[line] + j;
-      buf0[l * a.n + t.pw_i3[g]] = psi[g + band * ldpsi];
+  // plane waves of each line, placed at their i3 (the rest of the line is 0)
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int e = threadIdx.x + k * 256;
+    if (e < MN) {
+      const int l = e / a.n, j = e - l * a.n;
+      const int L = L0 + l;
+      if (L < a.lines_total) {
+        const int band = L / t.nlines, line = L - band * t.nlines;
+        if (j < t.line_count[line]) {
+          const int g = t.line_start[line] + j;
+          buf0[l * a.ld + t.pw_i3[g]] = psi[g + band * ldpsi];
+        }
+      }
     }
   }
   __syncthreads();
   V *res = run_stages(buf0, buf1, tws, a, +1);
-  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int e = threadIdx.x + k * 256;
+    if (e >= MN) break;
     const int l = e / a.n, p = e - l * a.n;
-    const long long L = L0 + l;
-    if (L >= a.lines_total) continue;
-    const int band = static_cast<int>(L / t.nlines);
-    const int line = static_cast<int>(L - static_cast<long long>(band) * t.nlines);
-    box[band * t.nr + static_cast<long long>(t.line_i12[line]) * t.n3 + p] = res[e];
+    const int L = L0 + l;
+    if (L < a.lines_total) {
+      const int band = L / t.nlines, line = L - band * t.nlines;
+      box[band * t.nr + static_cast<long long>(t.line_i12[line]) * t.n3 + p] = res[l * a.ld + p];
+    }
   }
 }
 
 // Along i2 on the active planes; BWD loads active lines only and stores all,
-// the forward pass loads all and stores active lines only.
+// the forward pass loads all and stores active lines only.  Each thread keeps
+// one line (M divides the 256 threads) and walks its points.
 template <typename R, bool BWD>
 __global__ __launch_bounds__(256) void y_kernel(cplx_t<R> *box, Tables t, FftArgs a) {
   using V = cplx_t<R>;
+  constexpr int K = Tile<R>::KMAX;
   extern __shared__ double2 smem_raw[];
   V *buf0 = reinterpret_cast<V *>(smem_raw);
-  V *buf1 = buf0 + a.M * a.n;
-  V *tws = buf1 + a.M * a.n;
+  V *buf1 = buf0 + a.M * a.ld;
+  V *tws = buf1 + a.M * a.ld;
   load_twiddles(tws, a);
-  const long long L0 = static_cast<long long>(blockIdx.x) * a.M;
-  const int MN = a.M * a.n;
-  const long long per_band = static_cast<long long>(t.nplanes) * t.n3;
-  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
-    const int p = e / a.M, l = e - p * a.M;
-    const long long L = L0 + l;
-    V v = mk<V>(R(0), R(0));
-    if (L < a.lines_total) {
-      const int band = static_cast<int>(L / per_band);
-      const long long rem = L - band * per_band;
-      const int ip = static_cast<int>(rem / t.n3), i3 = static_cast<int>(rem - static_cast<long long>(ip) * t.n3);
-      const int i1 = t.plane_i1[ip];
-      if (!BWD || t.line_of[i1 * t.n2 + p] >= 0)
-        v = box[band * t.nr + (static_cast<long long>(i1) * t.n2 + p) * t.n3 + i3];
-    }
-    buf0[l * a.n + p] = v;
+  const int l = threadIdx.x % a.M, p0 = threadIdx.x / a.M, pstep = blockDim.x / a.M;
+  const int L = blockIdx.x * a.M + l;
+  const bool live = L < a.lines_total;
+  const int per_band = t.nplanes * t.n3;
+  const int band = live ? L / per_band : 0;
+  const int rem = L - band * per_band;
+  const int ip = rem / t.n3, i3 = rem - ip * t.n3;
+  const int i1 = live ? t.plane_i1[ip] : 0;
+  const long long base = band * t.nr + static_cast<long long>(i1) * t.n2 * t.n3 + i3;
+  const int *lof = t.line_of + i1 * t.n2;
+  V v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int p = p0 + k * pstep;
+    v[k] = mk<V>(R(0), R(0));
+    if (live && p < a.n && (!BWD || lof[p] >= 0)) v[k] = box[base + static_cast<long long>(p) * t.n3];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int p = p0 + k * pstep;
+    if (p < a.n) buf0[l * a.ld + p] = v[k];
   }
   __syncthreads();
   V *res = run_stages(buf0, buf1, tws, a, BWD ? +1 : -1);
-  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
-    const int p = e / a.M, l = e - p * a.M;
-    const long long L = L0 + l;
-    if (L >= a.lines_total) continue;
-    const int band = static_cast<int>(L / per_band);
-    const long long rem = L - band * per_band;
-    const int ip = static_cast<int>(rem / t.n3), i3 = static_cast<int>(rem - static_cast<long long>(ip) * t.n3);
-    const int i1 = t.plane_i1[ip];
-    if (BWD || t.line_of[i1 * t.n2 + p] >= 0)
-      box[band * t.nr + (static_cast<long long>(i1) * t.n2 + p) * t.n3 + i3] = res[l * a.n + p];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int p = p0 + k * pstep;
+    if (live && p < a.n && (BWD || lof[p] >= 0)) box[base + static_cast<long long>(p) * t.n3] = res[l * a.ld + p];
   }
 }
 
@@ -123,43 +142,49 @@ __global__ __launch_bounds__(256) void y_kernel(cplx_t<R> *box, Tables t, FftArg
 template <typename R>
 __global__ __launch_bounds__(256) void x_veff_kernel(cplx_t<R> *box, const R *veff, R scale, Tables t, FftArgs a) {
   using V = cplx_t<R>;
+  constexpr int K = Tile<R>::KMAX;
   extern __shared__ double2 smem_raw[];
   V *buf0 = reinterpret_cast<V *>(smem_raw);
-  V *buf1 = buf0 + a.M * a.n;
-  V *tws = buf1 + a.M * a.n;
+  V *buf1 = buf0 + a.M * a.ld;
+  V *tws = buf1 + a.M * a.ld;
   load_twiddles(tws, a);
-  const long long L0 = static_cast<long long>(blockIdx.x) * a.M;
-  const int MN = a.M * a.n;
-  const long long C = a.C;  // n2 * n3
-  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
-    const int p = e / a.M, l = e - p * a.M;
-    const long long L = L0 + l;
-    V v = mk<V>(R(0), R(0));
-    if (L < a.lines_total && t.plane_on[p]) {
-      const long long band = L / C, c = L - band * C;
-      v = box[band * t.nr + c + p * C];
+  const int l = threadIdx.x % a.M, p0 = threadIdx.x / a.M, pstep = blockDim.x / a.M;
+  const int L = blockIdx.x * a.M + l;
+  const bool live = L < a.lines_total;
+  const int C = a.C;  // n2 * n3
+  const int band = live ? L / C : 0;
+  const int c = L - band * C;
+  const long long base = band * t.nr + c;
+  V v[K];
+  R w[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int p = p0 + k * pstep;
+    v[k] = mk<V>(R(0), R(0));
+    w[k] = R(0);
+    if (live && p < a.n) {
+      if (t.plane_on[p]) v[k] = box[base + static_cast<long long>(p) * C];
+      w[k] = veff[c + static_cast<long long>(p) * C] * scale;
     }
-    buf0[l * a.n + p] = v;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int p = p0 + k * pstep;
+    if (p < a.n) buf0[l * a.ld + p] = v[k];
   }
   __syncthreads();
   V *res = run_stages(buf0, buf1, tws, a, +1);
-  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
-    const int p = e / a.M, l = e - p * a.M;
-    const long long L = L0 + l;
-    if (L < a.lines_total) {
-      const long long c = L % C;
-      res[l * a.n + p] = cscale(res[l * a.n + p], veff[c + p * C] * scale);
-    }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int p = p0 + k * pstep;
+    if (p < a.n) res[l * a.ld + p] = cscale(res[l * a.ld + p], w[k]);
   }
   __syncthreads();
   res = run_stages(res, res == buf0 ? buf1 : buf0, tws, a, -1);
-  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
-    const int p = e / a.M, l = e - p * a.M;
-    const long long L = L0 + l;
-    if (L < a.lines_total && t.plane_on[p]) {
-      const long long band = L / C, c = L - band * C;
-      box[band * t.nr + c + p * C] = res[l * a.n + p];
-    }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int p = p0 + k * pstep;
+    if (live && p < a.n && t.plane_on[p]) box[base + static_cast<long long>(p) * C] = res[l * a.ld + p];
   }
 }
 
@@ -168,37 +193,51 @@ __global__ __launch_bounds__(256) void z_fwd_kernel(const cplx_t<R> *psi, long l
                                                     const R *kin, cplx_t<R> *hpsi, cplx_t<R> *spsi,
                                                     long long ldout, Tables t, FftArgs a) {
   using V = cplx_t<R>;
+  constexpr int K = Tile<R>::KMAX;
   extern __shared__ double2 smem_raw[];
   V *buf0 = reinterpret_cast<V *>(smem_raw);
-  V *buf1 = buf0 + a.M * a.n;
-  V *tws = buf1 + a.M * a.n;
+  V *buf1 = buf0 + a.M * a.ld;
+  V *tws = buf1 + a.M * a.ld;
   load_twiddles(tws, a);
-  const long long L0 = static_cast<long long>(blockIdx.x) * a.M;
+  const int L0 = blockIdx.x * a.M;
   const int MN = a.M * a.n;
-  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
-    const int l = e / a.n, p = e - l * a.n;
-    const long long L = L0 + l;
-    V v = mk<V>(R(0), R(0));
-    if (L < a.lines_total) {
-      const int band = static_cast<int>(L / t.nlines);
-      const int line = static_cast<int>(L - static_cast<long long>(band) * t.nlines);
-      v = box[band * t.nr + static_cast<long long>(t.line_i12[line]) * t.n3 + p];
+  V v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int e = threadIdx.x + k * 256;
+    v[k] = mk<V>(R(0), R(0));
+    if (e < MN) {
+      const int l = e / a.n, p = e - l * a.n;
+      const int L = L0 + l;
+      if (L < a.lines_total) {
+        const int band = L / t.nlines, line = L - band * t.nlines;
+        v[k] = box[band * t.nr + static_cast<long long>(t.line_i12[line]) * t.n3 + p];
+      }
     }
-    buf0[e] = v;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int e = threadIdx.x + k * 256;
+    if (e < MN) {
+      const int l = e / a.n, p = e - l * a.n;
+      buf0[l * a.ld + p] = v[k];
+    }
   }
   __syncthreads();
   V *res = run_stages(buf0, buf1, tws, a, -1);
-  for (int e = threadIdx.x; e < MN; e += blockDim.x) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int e = threadIdx.x + k * 256;
+    if (e >= MN) break;
     const int l = e / a.n, j = e - l * a.n;
-    const long long L = L0 + l;
+    const int L = L0 + l;
     if (L >= a.lines_total) continue;
-    const int band = static_cast<int>(L / t.nlines);
-    const int line = static_cast<int>(L - static_cast<long long>(band) * t.nlines);
+    const int band = L / t.nlines, line = L - band * t.nlines;
     if (j < t.line_count[line]) {
       const int g = t.line_start[line] + j;
       const V p = psi[g + band * ldpsi];
-      const V v = res[l * a.n + t.pw_i3[g]];
-      hpsi[g + band * ldout] = mk<V>(v.x + kin[g] * p.x, v.y + kin[g] * p.y);
+      const V r = res[l * a.ld + t.pw_i3[g]];
+      hpsi[g + band * ldout] = mk<V>(r.x + kin[g] * p.x, r.y + kin[g] * p.y);
       spsi[g + band * ldout] = p;
     }
   }
@@ -214,16 +253,20 @@ FftArgs axis_args(const FftPlan &plan, int axis, long long lines, int C) {
   a.tw = ax.twiddle.get();
   a.lines_total = lines;
   a.C = C;
-  const int budget = std::is_same<R, double>::value ? 1536 : 3072;
+  // lines per workgroup: the largest power of two with M * n <= budget
+  // (points per tile; at most 256 threads x KMAX elements)
+  const int budget = std::min(256 * Tile<R>::KMAX, std::is_same<R, double>::value ? g_tile_points64 : g_tile_points32);
   int M = 1;
-  while (M * 2 * ax.n <= budget && M < 64) M *= 2;
+  while (2 * M * ax.n <= budget && M < 64) M *= 2;
+  if (M * ax.n > 256 * Tile<R>::KMAX) throw Error(kUnavailable, "FFT line too long for the local-stage tile");
   a.M = M;
+  a.ld = ax.n + 1;
   return a;
 }
 
 template <typename R>
 size_t lds_bytes(const FftArgs &a) {
-  return (2 * size_t(a.M) * a.n + a.n) * sizeof(cplx_t<R>);
+  return (2 * size_t(a.M) * a.ld + a.n) * sizeof(cplx_t<R>);
 }
 
 template <typename R>
@@ -275,6 +318,9 @@ template <typename R>
 void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const R *kin, int nb, const cplx_t<R> *psi,
                  int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi, int64_t ldout, cplx_t<R> *boxes, hipStream_t s) {
   if (nb <= 0 || m.nlines == 0) return;
+  if (static_cast<long long>(nb) * (static_cast<long long>(m.n1) * m.n2 + static_cast<long long>(m.n2) * m.n3 +
+                                   static_cast<long long>(m.n1) * m.n3) >= (1ll << 31))
+    throw Error(kInvalidArgument, "band batch too large for the local stage");
   Tables t{m.n1, m.n2, m.n3, static_cast<long long>(m.n1) * m.n2 * m.n3, m.nlines, m.line_i12.get(),
            m.line_start.get(), m.line_count.get(), m.line_of.get(), m.nplanes, m.plane_i1.get(), m.plane_on.get(),
            m.pw_i3.get()};
@@ -303,6 +349,11 @@ void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const R 
     void *args[] = {&psi, &lp, &boxes, &kin, &hpsi, &spsi, &lo, &t, &az};
     launch<R>("local_z_fwd", reinterpret_cast<const void *>(&z_fwd_kernel<R>), az, s, args);
   }
+}
+
+void set_local_tile(int points64, int points32) {
+  if (points64 > 0) g_tile_points64 = points64;
+  if (points32 > 0) g_tile_points32 = points32;
 }
 
 template void local_apply<double>(const FftPlan &, const LocalMap &, const double *, const double *, int,

@@ -20,4 +20,8 @@ template <typename R>
 void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const R *kin, int nb, const cplx_t<R> *psi,
                  int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi, int64_t ldout, cplx_t<R> *boxes, hipStream_t s);
 
+// Tile size (FFT points per workgroup) of the FP64 and FP32 local-stage
+// kernels; values <= 0 keep the current setting.
+void set_local_tile(int points64, int points32);
+
 }  // namespace half

@@ -594,7 +594,10 @@ template <typename R>
 void KOperator<R>::apply_local(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
                                int64_t ldout, hipStream_t st) {
   const int64_t nr = sys->fft.nr;
-  const int batch = std::max(1, std::min(box_batch, nb));
+  // bands per launch: all of them unless the boxes would exceed 2 GiB
+  int batch = box_batch > 0 ? box_batch
+                            : static_cast<int>(std::max<int64_t>(1, (int64_t(2) << 30) / (nr * int64_t(sizeof(cplx_t<R>)))));
+  batch = std::max(1, std::min(batch, nb));
   boxes.ensure(size_t(nr) * batch);
   const R *veff = nullptr;
   if constexpr (std::is_same<R, double>::value)
@@ -613,8 +616,9 @@ void KOperator<R>::project(int nb, const cplx_t<R> *psi, int64_t ldpsi, hipStrea
   using V = cplx_t<R>;
   const int tiles_p = (np + 15) / 16, tiles_b = (nb + 15) / 16;
   const int tiles = tiles_p * tiles_b;
-  // choose the G chunk so that the launch has a few thousand workgroups
-  int nchunks = std::max(1, std::min((n + 63) / 64, 2048 / std::max(tiles, 1)));
+  // split G so the launch has about a thousand workgroups, at most 16 chunks
+  // (the fixed-order reduction reads every chunk's partial tile)
+  int nchunks = std::max(1, std::min({(n + 63) / 64, 1024 / std::max(tiles, 1), 16}));
   int chunk = (n + nchunks - 1) / nchunks;
   chunk = (chunk + 15) / 16 * 16;
   nchunks = (n + chunk - 1) / chunk;

@@ -66,7 +66,7 @@ struct KOperator {
   DeviceArray<cplx_t<R>> B;
   LocalMap lmap;  // plane-wave sphere on the box (pruned transforms)
   // workspace
-  int box_batch = 16;
+  int box_batch = 0;  // bands per FFT launch; 0 = all bands within a 2 GiB workspace
   DeviceArray<cplx_t<R>> boxes;
   DeviceArray<cplx_t<R>> coeff, dcoeff, qcoeff, partial;
 

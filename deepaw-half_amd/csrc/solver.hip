@@ -258,7 +258,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   const int m = nbands + std::max(0, ctl.extra);
   if (mreq < 1 || mreq > n) throw Error(kInvalidArgument, "invalid requested band count");
   if (2 * m > n) throw Error(kInvalidArgument, "ACC solver requires NBANDS <= NPL/2");
-  if (!(ctl.tol > 0) || ctl.maxiter < 1 || ctl.block < 1) throw Error(kInvalidArgument, "invalid ACC solver controls");
+  if (!(ctl.tol > 0) || ctl.maxiter < 1 || ctl.block < 0) throw Error(kInvalidArgument, "invalid ACC solver controls");
   if (ctl.mixed && !op32) throw Error(kInternal, "mixed ACC needs an FP32 operator");
   op64.box_batch = ctl.block;
   if (op32) op32->box_batch = ctl.block;

@@ -16,6 +16,7 @@ struct FftArgs {
   long long lines_total;  // number of 1D lines in the batch
   int C;                  // element stride of the transform axis
   int M;                  // lines per workgroup
+  int ld;                 // LDS line stride (n + 1: odd, so strided LDS accesses spread over banks)
 };
 
 template <typename V>
@@ -98,16 +99,22 @@ __device__ inline void butterfly(V *v, int R, int sign, const V *tws, int n) {
       v[7] = csub(E3, O3);
       break;
     }
-    default: {  // generic small radix (7): direct DFT from the twiddle table
-      V y[8];
-      const int step = n / R;
-      for (int s = 0; s < R; ++s) {
+    case 7: {  // direct DFT with twiddles from the table (all indices compile-time)
+      V y[7];
+      const int step = n / 7;
+#pragma unroll
+      for (int q = 0; q < 7; ++q) {
         V acc = v[0];
-        for (int r = 1; r < R; ++r) acc = cadd(acc, cmul(v[r], twiddle(tws, (r * s % R) * step, sign)));
-        y[s] = acc;
+#pragma unroll
+        for (int r = 1; r < 7; ++r) acc = cadd(acc, cmul(v[r], twiddle(tws, ((r * q) % 7) * step, sign)));
+        y[q] = acc;
       }
-      for (int s = 0; s < R; ++s) v[s] = y[s];
+#pragma unroll
+      for (int q = 0; q < 7; ++q) v[q] = y[q];
+      break;
     }
+    default:
+      break;  // the plan only produces radices 8, 7, 5, 4, 3, 2
   }
 }
 
@@ -126,7 +133,7 @@ __device__ V *run_stages(V *in, V *out, const V *tws, const FftArgs &a, int sign
       const int j = t - l * nr;
       const int k = j % Ns;
       V v[8];
-      const V *src = in + l * n + j;
+      const V *src = in + l * a.ld + j;
 #pragma unroll
       for (int r = 0; r < 8; ++r)
         if (r < R) v[r] = src[r * nr];
@@ -136,7 +143,7 @@ __device__ V *run_stages(V *in, V *out, const V *tws, const FftArgs &a, int sign
           if (r < R) v[r] = cmul(v[r], twiddle(tws, k * r * tstep, sign));
       }
       butterfly(v, R, sign, tws, n);
-      V *dst = out + l * n + (j / Ns) * Ns * R + k;
+      V *dst = out + l * a.ld + (j / Ns) * Ns * R + k;
 #pragma unroll
       for (int r = 0; r < 8; ++r)
         if (r < R) dst[r * Ns] = v[r];

@@ -293,7 +293,7 @@ class Context:
         return w, v[:, :npw].T.copy(), smin.value, smax.value
 
     def solve_acc(self, nbands, k=(0, 0, 0), mixed=False, tol=1e-4, switch_tol=0.0, maxiter=40,
-                  block=16, extra=0, vectors=False):
+                  block=0, extra=0, vectors=False):
         n = self.basis_size(k)
         w = np.zeros(nbands)
         v = np.zeros((nbands, n), np.complex128) if vectors else None

@@ -50,6 +50,7 @@ typedef struct half_amd_solve_stats {
 
 /* All-band solve with explicit controls (the reference ABI fixes tol 1e-4 eV,
  * 40 iterations, 16-band blocks: src/half_cuda_iterative_solver.cuf:113-119).
+ * block: bands per FFT launch, 0 = all bands within a 2 GiB workspace.
  * mixed != 0 runs FP32 H.psi iterations until max residual <= switch_tol,
  * then FP64 H.psi sweeps to tol; Rayleigh-Ritz, Cholesky-type metric
  * normalisation and the subspace eigensolves are FP64 throughout.  extra
