@@ -120,8 +120,14 @@ def main():
             wv, _, st = ctx.solve_acc(nb, k, mixed=mixed, tol=1e-5, maxiter=400, extra=8)
             el = time.perf_counter() - t
             key = "mixed" if mixed else "fp64"
+            # the same solve again on the launch-boundary timeline: where the time goes
+            H.trace(True)
+            ctx.solve_acc(nb, k, mixed=mixed, tol=1e-5, maxiter=400, extra=8)
+            phases = H.trace_read()
+            H.trace(False)
             solve[key] = {"seconds": el, "iterations": st["iterations"], "fp64_iterations": st["fp64_iterations"],
-                          "max_residual_eV": st["max_residual_eV"], "band_applications": st["band_applications"]}
+                          "max_residual_eV": st["max_residual_eV"], "band_applications": st["band_applications"],
+                          "phase_ms": {p: round(v[0], 3) for p, v in sorted(phases.items(), key=lambda x: -x[1][0])}}
             if g is not None and len(g["eigenvalues"]) >= nb:
                 solve[key]["max_abs_err_vs_reference_eV"] = float(np.abs(wv - np.array(g["eigenvalues"][:nb])).max())
             print(f"bench: {key} all-band solve {el:.3f} s {solve[key]}", file=sys.stderr, flush=True)

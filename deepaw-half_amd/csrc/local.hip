@@ -31,8 +31,15 @@ namespace half {
 
 namespace {
 
-int g_tile_points64 = 768;   // tunable: FFT points per workgroup tile, FP64
-int g_tile_points32 = 1536;  // FP32
+// Tunables (FFT points per workgroup tile per kernel class and precision,
+// and bands sharing one Veff tile in the x pass); set through
+// half_amd_set_tuning, chosen by the autotuner in half_amd/autotune.py.
+struct Tuning {
+  // defaults from the autotuner on the 48^3 HfO2-like box, 64 bands
+  int tile[2][3] = {{1024, 1536, 768}, {1536, 1536, 768}};  // [fp64|fp32][x|y|z]
+  int xgroup[2] = {4, 8};
+};
+Tuning g_tune;
 
 struct Tables {
   int n1, n2, n3;
@@ -138,34 +145,40 @@ __global__ __launch_bounds__(256) void y_kernel(cplx_t<R> *box, Tables t, FftArg
   }
 }
 
-// Along i1 for every (i2, i3): backward, * Veff/N_r, forward.
+// Along i1 for every (i2, i3): backward, * Veff/N_r, forward.  A workgroup
+// takes Mc consecutive (i2,i3) columns for G bands (M = G * Mc lines); the
+// Veff tile of the Mc columns is read once into LDS and shared by the bands.
 template <typename R>
-__global__ __launch_bounds__(256) void x_veff_kernel(cplx_t<R> *box, const R *veff, R scale, Tables t, FftArgs a) {
+__global__ __launch_bounds__(256) void x_veff_kernel(cplx_t<R> *box, const R *veff, R scale, Tables t, FftArgs a,
+                                                     int G, int nb) {
   using V = cplx_t<R>;
   constexpr int K = Tile<R>::KMAX;
   extern __shared__ double2 smem_raw[];
   V *buf0 = reinterpret_cast<V *>(smem_raw);
   V *buf1 = buf0 + a.M * a.ld;
   V *tws = buf1 + a.M * a.ld;
+  R *wbuf = reinterpret_cast<R *>(tws + a.n);
   load_twiddles(tws, a);
-  const int l = threadIdx.x % a.M, p0 = threadIdx.x / a.M, pstep = blockDim.x / a.M;
-  const int L = blockIdx.x * a.M + l;
-  const bool live = L < a.lines_total;
   const int C = a.C;  // n2 * n3
-  const int band = live ? L / C : 0;
-  const int c = L - band * C;
-  const long long base = band * t.nr + c;
+  const int Mc = a.M / G;
+  const int ncb = (C + Mc - 1) / Mc;
+  const int cblock = blockIdx.x % ncb, bgroup = blockIdx.x / ncb;
+  for (int e = threadIdx.x; e < Mc * a.n; e += blockDim.x) {
+    const int p = e / Mc, ci = e - p * Mc;
+    const int c = cblock * Mc + ci;
+    wbuf[ci * a.n + p] = c < C ? veff[c + static_cast<long long>(p) * C] * scale : R(0);
+  }
+  const int l = threadIdx.x % a.M, p0 = threadIdx.x / a.M, pstep = blockDim.x / a.M;
+  const int gi = l / Mc, ci = l - gi * Mc;
+  const int c = cblock * Mc + ci, band = bgroup * G + gi;
+  const bool live = c < C && band < nb;
+  const long long base = static_cast<long long>(live ? band : 0) * t.nr + (live ? c : 0);
   V v[K];
-  R w[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int p = p0 + k * pstep;
     v[k] = mk<V>(R(0), R(0));
-    w[k] = R(0);
-    if (live && p < a.n) {
-      if (t.plane_on[p]) v[k] = box[base + static_cast<long long>(p) * C];
-      w[k] = veff[c + static_cast<long long>(p) * C] * scale;
-    }
+    if (live && p < a.n && t.plane_on[p]) v[k] = box[base + static_cast<long long>(p) * C];
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -177,7 +190,7 @@ __global__ __launch_bounds__(256) void x_veff_kernel(cplx_t<R> *box, const R *ve
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int p = p0 + k * pstep;
-    if (p < a.n) res[l * a.ld + p] = cscale(res[l * a.ld + p], w[k]);
+    if (p < a.n) res[l * a.ld + p] = cscale(res[l * a.ld + p], wbuf[ci * a.n + p]);
   }
   __syncthreads();
   res = run_stages(res, res == buf0 ? buf1 : buf0, tws, a, -1);
@@ -244,7 +257,7 @@ __global__ __launch_bounds__(256) void z_fwd_kernel(const cplx_t<R> *psi, long l
 }
 
 template <typename R>
-FftArgs axis_args(const FftPlan &plan, int axis, long long lines, int C) {
+FftArgs axis_args(const FftPlan &plan, int axis, long long lines, int C, int group = 1) {
   const FftAxis &ax = plan.axis[axis];
   FftArgs a{};
   a.n = ax.n;
@@ -255,9 +268,11 @@ FftArgs axis_args(const FftPlan &plan, int axis, long long lines, int C) {
   a.C = C;
   // lines per workgroup: the largest power of two with M * n <= budget
   // (points per tile; at most 256 threads x KMAX elements)
-  const int budget = std::min(256 * Tile<R>::KMAX, std::is_same<R, double>::value ? g_tile_points64 : g_tile_points32);
+  const int prec = std::is_same<R, double>::value ? 0 : 1;
+  const int budget = std::min(256 * Tile<R>::KMAX, g_tune.tile[prec][axis]);
   int M = 1;
   while (2 * M * ax.n <= budget && M < 64) M *= 2;
+  M = std::max(M, group);  // at least one column per band of the group
   if (M * ax.n > 256 * Tile<R>::KMAX) throw Error(kUnavailable, "FFT line too long for the local-stage tile");
   a.M = M;
   a.ld = ax.n + 1;
@@ -270,10 +285,11 @@ size_t lds_bytes(const FftArgs &a) {
 }
 
 template <typename R>
-void launch(const char *name, const void *fn, const FftArgs &a, hipStream_t s, void **args) {
+void launch(const char *name, const void *fn, const FftArgs &a, hipStream_t s, void **args, long long blocks = -1,
+            size_t extra_lds = 0) {
   trace_mark(name, s);
-  const long long blocks = (a.lines_total + a.M - 1) / a.M;
-  HALF_HIP(hipLaunchKernel(fn, dim3(static_cast<unsigned>(blocks)), dim3(256), args, lds_bytes<R>(a), s));
+  if (blocks < 0) blocks = (a.lines_total + a.M - 1) / a.M;
+  HALF_HIP(hipLaunchKernel(fn, dim3(static_cast<unsigned>(blocks)), dim3(256), args, lds_bytes<R>(a) + extra_lds, s));
 }
 
 }  // namespace
@@ -327,7 +343,13 @@ void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const R 
   long long lp = ldpsi, lo = ldout;
   FftArgs az = axis_args<R>(plan, 2, static_cast<long long>(nb) * m.nlines, 1);
   FftArgs ay = axis_args<R>(plan, 1, static_cast<long long>(nb) * m.nplanes * m.n3, m.n3);
-  FftArgs ax = axis_args<R>(plan, 0, static_cast<long long>(nb) * m.n2 * m.n3, m.n2 * m.n3);
+  const int prec = std::is_same<R, double>::value ? 0 : 1;
+  int G = std::max(1, std::min(g_tune.xgroup[prec], nb));
+  while (G > 1 && (G & (G - 1))) --G;  // power of two, divides M
+  FftArgs ax = axis_args<R>(plan, 0, static_cast<long long>(nb) * m.n2 * m.n3, m.n2 * m.n3, G);
+  if (ax.M % G) G = 1;
+  const int Mc = ax.M / G;
+  const long long xblocks = static_cast<long long>((m.n2 * m.n3 + Mc - 1) / Mc) * ((nb + G - 1) / G);
   R scale = static_cast<R>(1.0 / double(t.nr));
   {
     void *args[] = {&psi, &lp, &boxes, &t, &az};
@@ -338,8 +360,10 @@ void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const R 
     launch<R>("local_y_bwd", reinterpret_cast<const void *>(&y_kernel<R, true>), ay, s, args);
   }
   {
-    void *args[] = {&boxes, &veff, &scale, &t, &ax};
-    launch<R>("local_x_veff", reinterpret_cast<const void *>(&x_veff_kernel<R>), ax, s, args);
+    int nbi = nb;
+    void *args[] = {&boxes, &veff, &scale, &t, &ax, &G, &nbi};
+    launch<R>("local_x_veff", reinterpret_cast<const void *>(&x_veff_kernel<R>), ax, s, args, xblocks,
+              size_t(Mc) * ax.n * sizeof(R));
   }
   {
     void *args[] = {&boxes, &t, &ay};
@@ -351,9 +375,40 @@ void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const R 
   }
 }
 
-void set_local_tile(int points64, int points32) {
-  if (points64 > 0) g_tile_points64 = points64;
-  if (points32 > 0) g_tile_points32 = points32;
+bool set_local_tuning(const std::string &key, int value) {
+  static const char *axes[3] = {"x", "y", "z"};
+  for (int p = 0; p < 2; ++p) {
+    const std::string pre = std::string("local.") + (p == 0 ? "fp64" : "fp32");
+    for (int a = 0; a < 3; ++a)
+      if (key == pre + ".tile." + axes[a]) {
+        if (value < 16 || value > 256 * 16) return false;
+        g_tune.tile[p][a] = value;
+        return true;
+      }
+    if (key == pre + ".xgroup") {
+      if (value < 1 || value > 64) return false;
+      g_tune.xgroup[p] = value;
+      return true;
+    }
+  }
+  return false;
+}
+
+bool get_local_tuning(const std::string &key, int &value) {
+  static const char *axes[3] = {"x", "y", "z"};
+  for (int p = 0; p < 2; ++p) {
+    const std::string pre = std::string("local.") + (p == 0 ? "fp64" : "fp32");
+    for (int a = 0; a < 3; ++a)
+      if (key == pre + ".tile." + axes[a]) {
+        value = g_tune.tile[p][a];
+        return true;
+      }
+    if (key == pre + ".xgroup") {
+      value = g_tune.xgroup[p];
+      return true;
+    }
+  }
+  return false;
 }
 
 template void local_apply<double>(const FftPlan &, const LocalMap &, const double *, const double *, int,

@@ -1,6 +1,8 @@
 // Pruned local stage of the PAW operator (see local.hip).
 #pragma once
 
+#include <string>
+
 #include "fft.hpp"
 
 namespace half {
@@ -20,8 +22,10 @@ template <typename R>
 void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const R *kin, int nb, const cplx_t<R> *psi,
                  int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi, int64_t ldout, cplx_t<R> *boxes, hipStream_t s);
 
-// Tile size (FFT points per workgroup) of the FP64 and FP32 local-stage
-// kernels; values <= 0 keep the current setting.
-void set_local_tile(int points64, int points32);
+// Tunables of the local stage, keys "local.{fp64|fp32}.tile.{x|y|z}" (FFT
+// points per workgroup tile) and "local.{fp64|fp32}.xgroup" (bands sharing a
+// Veff tile).  Return false for an unknown key or an out-of-range value.
+bool set_local_tuning(const std::string &key, int value);
+bool get_local_tuning(const std::string &key, int &value);
 
 }  // namespace half

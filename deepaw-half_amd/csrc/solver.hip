@@ -16,6 +16,7 @@
 
 #include "cplx.hpp"
 #include "solver.hpp"
+#include "trace.hpp"
 
 namespace half {
 
@@ -355,10 +356,12 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   bool converged = false;
   int fp64_iters = 0;
   for (iter = 1; iter <= ctl.maxiter; ++iter) {
+    trace_mark("acc_residual", s);
     hipLaunchKernelGGL(residual_kernel, dim3(blocks_for(int64_t(n) * m)), dim3(kThreads), 0, s, HX, SX,
                        wk.w.get(), wk.R.get(), n, m, ld);
     hipLaunchKernelGGL(column_norm2_kernel, dim3(m), dim3(kThreads), 0, s, wk.R.get(), ld, n, wk.res2.get());
     HALF_HIP(hipGetLastError());
+    trace_mark("acc_norms_to_host", s);
     HALF_HIP(hipMemcpyAsync(r2.data(), wk.res2.get(), sizeof(double) * m, hipMemcpyDeviceToHost, s));
     HALF_HIP(hipStreamSynchronize(s));
     max_res = 0;
@@ -380,12 +383,14 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
       converged = true;
       break;
     }
+    trace_mark("acc_precondition", s);
     wk.seeds.upload(seeds, s);
     hipLaunchKernelGGL(precondition_kernel, dim3(blocks_for(int64_t(n) * m)), dim3(kThreads), 0, s, wk.R.get(),
                        wk.kin.get(), wk.w.get(), wk.seeds.get(), P, n, m, ld);
     HALF_HIP(hipGetLastError());
     apply(P, HP, SP);
     // S-orthogonalise the correction block against X
+    trace_mark("acc_orthogonalise_gemm", s);
     zgemm(blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, m, n, one, X, ld, SP, ld, zero,
           wk.cross.get(), m);
     zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, m, mone, X, ld, wk.cross.get(), m, one, P, ld);
@@ -396,24 +401,30 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     // normalise it in the S metric
     zgemm(blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, m, n, one, P, ld, SP, ld, zero,
           wk.metric.get(), m);
+    trace_mark("acc_metric_zheevd", s);
     HALF_BLAS(rocsolver_zheevd(blas.h, rocblas_evect_original, rocblas_fill_upper, m, rz(wk.metric.get()), m,
                                wk.mu.get(), wk.E.get(), wk.info.get()));
     check_info(wk.info, s, "ACC correction metric eigensolve");
     hipLaunchKernelGGL(scale_columns_kernel, dim3(blocks_for(int64_t(m) * m)), dim3(kThreads), 0, s,
                        wk.metric.get(), wk.mu.get(), m, 1e-14);
     HALF_HIP(hipGetLastError());
+    trace_mark("acc_normalise_gemm", s);
     for (double2 *blk : {P, HP, SP}) {
       zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, m, one, blk, ld, wk.metric.get(), m, zero,
             wk.tmp.get(), ld);
       copy(blk, wk.tmp.get());
     }
     // Rayleigh-Ritz in span[X P]
+    trace_mark("acc_projection_gemm", s);
     zgemm(blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m2, m2, n, one, wk.Z.get(), ld,
           wk.HZ.get(), ld, zero, wk.hsub.get(), m2);
     zgemm(blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m2, m2, n, one, wk.Z.get(), ld,
           wk.SZ.get(), ld, zero, wk.ssub.get(), m2);
+    trace_mark("acc_subspace_zhegvd", s);
     small_generalized(m2);
+    trace_mark("acc_rotate_gemm", s);
     ritz_rotate(m2);
+    trace_close(s);
   }
   res.iterations = std::min(iter, ctl.maxiter);
   res.fp64_iterations = fp64_iters;

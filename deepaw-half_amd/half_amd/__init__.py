@@ -38,7 +38,7 @@ EXT_SYMBOLS = (
     "half_amd_operator_create", "half_amd_operator_apply", "half_amd_operator_destroy",
     "half_amd_solve_acc", "half_amd_set_solver", "half_amd_energy_terms",
     "half_amd_get_veff", "half_amd_nonlocal_forces", "half_amd_device_info",
-    "half_amd_trace", "half_amd_trace_read",
+    "half_amd_trace", "half_amd_trace_read", "half_amd_set_tuning", "half_amd_get_tuning",
 )
 
 
@@ -107,6 +107,8 @@ def load(path: str | None = None):
         "half_amd_device_info": [cp, i, pi32, cp, i],
         "half_amd_trace": [i, cp, i],
         "half_amd_trace_read": [i, cp, pd, pi64, pi32, cp, i],
+        "half_amd_set_tuning": [cp, i, cp, i],
+        "half_amd_get_tuning": [cp, C.POINTER(i), cp, i],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -140,6 +142,20 @@ def device_info():
     e = _err()
     _check(lib.half_amd_device_info(name, 256, C.byref(cu), e, 512), e)
     return name.value.decode(), cu.value
+
+
+def set_tuning(key: str, value: int):
+    lib = load()
+    e = _err()
+    _check(lib.half_amd_set_tuning(key.encode(), int(value), e, 512), e)
+
+
+def get_tuning(key: str) -> int:
+    lib = load()
+    v = C.c_int()
+    e = _err()
+    _check(lib.half_amd_get_tuning(key.encode(), C.byref(v), e, 512), e)
+    return v.value
 
 
 def trace(enable: bool):

@@ -1,0 +1,119 @@
+"""Autotuner for the local-stage tile shapes of the H.psi operator.
+
+For one workload it times the FP64 and FP32 operators on a resident band
+block while sweeping, per precision:
+  local.<p>.tile.{x,y,z}   FFT points per workgroup tile of each pass
+  local.<p>.xgroup         bands sharing one Veff tile in the x pass
+Each candidate is applied to all three passes at once and every pass keeps the
+value that minimised its own traced time (the passes are independent
+kernels); xgroup is swept afterwards with the chosen tiles.
+
+    python -m half_amd.autotune --workload hfo2 --bands 64 --out tuning.json
+
+The result maps each key to its best value and records all timings; the
+library's defaults are set from it (csrc/local.hip, struct Tuning).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+from . import BACKEND_CUDA, SOLVER_EVD, Context, get_tuning, hip, set_tuning, synth, trace, trace_read
+
+PASSES = {"x": "local_x_veff", "y": ("local_y_bwd", "local_y_fwd"), "z": ("local_z_bwd", "local_z_fwd")}
+
+
+def _pass_ms(kern, axis, steps):
+    names = PASSES[axis]
+    names = (names,) if isinstance(names, str) else names
+    return sum(kern[n][0] for n in names if n in kern) / steps
+
+
+def tune(workload="hfo2", bands=0, steps=10, verbose=True):
+    w = synth.workload(workload)
+    nb = bands or w.nbands
+    tmp = tempfile.mkdtemp(prefix="halftune")
+    chg, pot, _ = synth.make_inputs(workload, tmp)
+    ctx = Context(chg, pot, w.encut, w.xc, BACKEND_CUDA, SOLVER_EVD)
+    k = tuple(w.kpoints[0])
+    stream = hip.Stream()
+    rng = np.random.default_rng(3)
+    result = {"workload": workload, "bands": nb, "best": {}, "timings": {}}
+    for prec, dt, cands in ((64, np.complex128, (256, 384, 512, 768, 1024, 1536)),
+                            (32, np.complex64, (512, 768, 1024, 1536, 2048, 3072))):
+        p = f"fp{prec}"
+        op = ctx.operator(k, prec)
+        n = op.npw
+        x = hip.Buffer.from_numpy((rng.standard_normal((nb, n)) + 1j * rng.standard_normal((nb, n))).astype(dt))
+        hx, sx = hip.Buffer(x.shape, dt), hip.Buffer(x.shape, dt)
+
+        def run():
+            for _ in range(2):
+                op.apply_device(nb, x.ptr, hx.ptr, sx.ptr, stream=stream.ptr)
+            stream.synchronize()
+            trace(True)
+            for _ in range(steps):
+                op.apply_device(nb, x.ptr, hx.ptr, sx.ptr, stream=stream.ptr)
+            stream.synchronize()
+            kern = trace_read()
+            trace(False)
+            return kern
+
+        best = {a: (None, float("inf")) for a in "xyz"}
+        for c in cands:
+            for a in "xyz":
+                set_tuning(f"local.{p}.tile.{a}", c)
+            try:
+                kern = run()
+            except Exception as exc:  # a tile too large for this box is refused, not run
+                result["timings"][f"{p}.tile={c}"] = str(exc)
+                continue
+            for a in "xyz":
+                ms = _pass_ms(kern, a, steps)
+                result["timings"][f"{p}.tile.{a}={c}"] = ms
+                if ms < best[a][1]:
+                    best[a] = (c, ms)
+        for a in "xyz":
+            set_tuning(f"local.{p}.tile.{a}", best[a][0])
+            result["best"][f"local.{p}.tile.{a}"] = best[a][0]
+        gbest = (1, float("inf"))
+        for g in (1, 2, 4, 8):
+            set_tuning(f"local.{p}.xgroup", g)
+            kern = run()
+            ms = _pass_ms(kern, "x", steps)
+            result["timings"][f"{p}.xgroup={g}"] = ms
+            if ms < gbest[1]:
+                gbest = (g, ms)
+        set_tuning(f"local.{p}.xgroup", gbest[0])
+        result["best"][f"local.{p}.xgroup"] = gbest[0]
+        kern = run()
+        result["timings"][f"{p}.tuned_step_ms"] = sum(v[0] for v in kern.values()) / steps
+        if verbose:
+            print(f"autotune {p}: {result['best']} step {result['timings'][f'{p}.tuned_step_ms']:.4f} ms",
+                  file=sys.stderr, flush=True)
+        op.close()
+        x.free(), hx.free(), sx.free()
+    ctx.close()
+    return result
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="hfo2")
+    ap.add_argument("--bands", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--out", default="")
+    args = ap.parse_args()
+    res = tune(args.workload, args.bands, args.steps)
+    text = json.dumps(res, indent=1)
+    if args.out:
+        with open(args.out, "w") as fh:
+            fh.write(text)
+    print(text)
+
+
+if __name__ == "__main__":
+    main()

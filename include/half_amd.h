@@ -92,6 +92,12 @@ int half_amd_trace(int enable, char *error, int error_capacity);
 int half_amd_trace_read(int max_kernels, char *names, double *total_ms, int64_t *launches,
                         int32_t *nkinds, char *error, int error_capacity);
 
+/* Kernel tunables, e.g. "local.fp64.tile.x" (FFT points per workgroup tile
+ * of the x/y/z local-stage passes) and "local.fp64.xgroup" (bands sharing a
+ * Veff tile); "fp32" keys for the FP32 operator.  Process-wide. */
+int half_amd_set_tuning(const char *key, int value, char *error, int error_capacity);
+int half_amd_get_tuning(const char *key, int *value, char *error, int error_capacity);
+
 /* Name and compute-unit count of the current HIP device. */
 int half_amd_device_info(char *name, int name_capacity, int32_t *compute_units, char *error,
                          int error_capacity);
