@@ -631,6 +631,8 @@ int half_amd_solve_acc(half_handle handle, const double kpoint[3], int nbands, i
       stats->max_residual_eV = res.residual;
       stats->seconds = res.seconds_solver;
       stats->band_applications = res.applies;
+      stats->metric_fallbacks = res.metric_fallbacks;
+      stats->reserved = 0;
     }
     return int(kSuccess);
   });

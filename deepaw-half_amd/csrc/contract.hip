@@ -1,0 +1,355 @@
+// Dense complex contractions on the gfx950 matrix cores.
+//
+// Two shapes occur on the operator and solver paths:
+//   gram       C = A^H B    with A, B tall (n rows >> columns): the projector
+//              coefficients B^H psi and the subspace Grams Z^H H Z, Z^H S Z,
+//              X^H S P, P^H S P;
+//   gemm_tall  C = A W      with A tall and W small: the projector expansion
+//              B (D c), B (Q c).
+// Both run 256-thread workgroups of 2 x 2 waves.  Each wave owns WT x WT
+// 16x16 complex tiles (16 WT square), so a workgroup covers a 32 WT square
+// of the output.  Operand slabs of 16 inner-index rows are read from HBM
+// with each thread loading a short contiguous run of one column, stored to
+// LDS split into real and imaginary planes, and consumed by
+// v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32 (four real products per
+// complex tile product).  The next slab is loaded into registers while the
+// current one is multiplied.  gram splits the rows into chunks whose partial
+// tiles are summed in a fixed order by a second kernel.
+#include <algorithm>
+
+#include "contract.hpp"
+#include "cplx.hpp"
+#include "mfma.hpp"
+
+namespace half {
+
+namespace {
+
+constexpr int KS = 16;  // inner-index rows per LDS slab
+constexpr int PAD = 4;  // LDS row padding (elements)
+
+template <typename R>
+struct GramArgs {
+  const cplx_t<R> *A;
+  int64_t lda;
+  const cplx_t<R> *B[2];
+  int64_t ldb[2];
+  cplx_t<R> *out[2];
+  int64_t ldo[2];
+  int64_t chunk_stride;  // distance between the partial tiles of consecutive chunks
+  int n, ka, kb, chunk, tiles_a;
+};
+
+template <typename R, int WT, int NB>
+__global__ __launch_bounds__(256) void gram_kernel(GramArgs<R> g) {
+  using V = cplx_t<R>;
+  using A4 = typename Acc<R>::type;
+  constexpr int TW = 32 * WT;        // workgroup tile width
+  constexpr int LW = TW + PAD;       // LDS row length
+  constexpr int L = TW * KS / 256;   // elements per thread and operand
+  constexpr int TPC = KS / L;        // threads per column
+  __shared__ R sa[2][KS][LW];
+  __shared__ R sb[NB][2][KS][LW];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int row = lane & 15, kq = lane >> 4;
+  const int wa = (wave & 1) * 16 * WT, wb = (wave >> 1) * 16 * WT;
+  const int a0 = (blockIdx.x % g.tiles_a) * TW, b0 = (blockIdx.x / g.tiles_a) * TW;
+  const int r_begin = blockIdx.y * g.chunk;
+  const int r_end = min(g.n, r_begin + g.chunk);
+  // loader: column lc of the tile, rows lr .. lr + L - 1 of the slab
+  const int lc = tid / TPC, lr = (tid % TPC) * L;
+  const bool ca = a0 + lc < g.ka, cb = b0 + lc < g.kb;
+  const V *pa = g.A + int64_t(min(a0 + lc, g.ka - 1)) * g.lda;
+  const V *pb[NB];
+  for (int j = 0; j < NB; ++j) pb[j] = g.B[j] + int64_t(min(b0 + lc, g.kb - 1)) * g.ldb[j];
+  V ra[L], rb[NB][L];
+  auto load = [&](int r0) {
+    for (int i = 0; i < L; ++i) {
+      const int r = r0 + lr + i;
+      const bool rv = r < r_end;
+      ra[i] = (rv && ca) ? pa[r] : mk<V>(0, 0);
+      for (int j = 0; j < NB; ++j) rb[j][i] = (rv && cb) ? pb[j][r] : mk<V>(0, 0);
+    }
+  };
+  A4 acc[NB][WT][WT][2];
+  for (int j = 0; j < NB; ++j)
+    for (int t = 0; t < WT; ++t)
+      for (int u = 0; u < WT; ++u) acc[j][t][u][0] = acc[j][t][u][1] = A4{0, 0, 0, 0};
+  if (r_begin < r_end) load(r_begin);
+  for (int r0 = r_begin; r0 < r_end; r0 += KS) {
+    __syncthreads();
+    for (int i = 0; i < L; ++i) {
+      sa[0][lr + i][lc] = ra[i].x;
+      sa[1][lr + i][lc] = -ra[i].y;  // conj(A)
+      for (int j = 0; j < NB; ++j) {
+        sb[j][0][lr + i][lc] = rb[j][i].x;
+        sb[j][1][lr + i][lc] = rb[j][i].y;
+      }
+    }
+    __syncthreads();
+    if (r0 + KS < r_end) load(r0 + KS);
+#pragma unroll
+    for (int k4 = 0; k4 < KS / 4; ++k4) {
+      const int kk = 4 * k4 + kq;
+      R ar[WT], ai[WT];
+      for (int t = 0; t < WT; ++t) {
+        ar[t] = sa[0][kk][wa + 16 * t + row];
+        ai[t] = sa[1][kk][wa + 16 * t + row];
+      }
+      for (int j = 0; j < NB; ++j)
+        for (int u = 0; u < WT; ++u) {
+          const R br = sb[j][0][kk][wb + 16 * u + row], bi = sb[j][1][kk][wb + 16 * u + row];
+          for (int t = 0; t < WT; ++t) cmfma<R>(ar[t], ai[t], br, bi, acc[j][t][u][0], acc[j][t][u][1]);
+        }
+    }
+  }
+  const int64_t off = int64_t(blockIdx.y) * g.chunk_stride;
+  for (int j = 0; j < NB; ++j)
+    for (int t = 0; t < WT; ++t)
+      for (int u = 0; u < WT; ++u) {
+        const int cb2 = b0 + wb + 16 * u + row;
+        if (cb2 >= g.kb) continue;
+        for (int i = 0; i < 4; ++i) {
+          const int ra2 = a0 + wa + 16 * t + mfma_row<R>(lane, i);
+          if (ra2 < g.ka) g.out[j][off + ra2 + int64_t(cb2) * g.ldo[j]] = mk<V>(acc[j][t][u][0][i], acc[j][t][u][1][i]);
+        }
+      }
+}
+
+// C[p + b ldc] = sum over chunks c of partial[c stride + p + b ka], in chunk order
+template <typename V>
+__global__ void gram_reduce_kernel(const V *partial, int64_t stride, int nchunks, int ka, int kb, V *C,
+                                   int64_t ldc) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(ka) * kb) return;
+  V acc = partial[t];
+  for (int c = 1; c < nchunks; ++c) acc = cadd(acc, partial[c * stride + t]);
+  const int b = static_cast<int>(t / ka), p = static_cast<int>(t - int64_t(b) * ka);
+  C[p + int64_t(b) * ldc] = acc;
+}
+
+template <typename R>
+struct TallArgs {
+  const cplx_t<R> *A;
+  int64_t lda;
+  const cplx_t<R> *W[2];
+  int64_t ldw[2];
+  cplx_t<R> *C[2];
+  int64_t ldc[2];
+  int n, m, k;
+  R alpha, beta;
+};
+
+template <typename R, int WT, int NB>
+__global__ __launch_bounds__(256) void tall_kernel(TallArgs<R> g) {
+  using V = cplx_t<R>;
+  using A4 = typename Acc<R>::type;
+  constexpr int TW = 32 * WT;
+  constexpr int LW = TW + PAD;
+  constexpr int L = TW * KS / 256;
+  constexpr int TPA = TW / L;  // A loader: threads per inner-index column
+  constexpr int TPW = KS / L;  // W loader: threads per output column
+  __shared__ R sa[2][KS][LW];       // [inner][row]
+  __shared__ R sw[NB][2][KS][LW];   // [inner][column]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int row = lane & 15, kq = lane >> 4;
+  const int wr = (wave & 1) * 16 * WT, wc = (wave >> 1) * 16 * WT;
+  const int g0 = blockIdx.x * TW, c0 = blockIdx.y * TW;
+  const int akc = tid / TPA, ar0 = (tid % TPA) * L;  // A: inner column akc, rows ar0..
+  const int wcol = tid / TPW, wk0 = (tid % TPW) * L;  // W: column wcol, inner rows wk0..
+  const bool wcv = c0 + wcol < g.m;
+  V ra[L], rw[NB][L];
+  auto load = [&](int k0) {
+    const int kc = k0 + akc;
+    const bool kv = kc < g.k;
+    const V *pa = g.A + int64_t(min(kc, g.k - 1)) * g.lda;
+    for (int i = 0; i < L; ++i) {
+      const int r = g0 + ar0 + i;
+      ra[i] = (kv && r < g.n) ? pa[r] : mk<V>(0, 0);
+    }
+    for (int j = 0; j < NB; ++j) {
+      const V *pw = g.W[j] + int64_t(min(c0 + wcol, g.m - 1)) * g.ldw[j];
+      for (int i = 0; i < L; ++i) {
+        const int kr = k0 + wk0 + i;
+        rw[j][i] = (wcv && kr < g.k) ? pw[kr] : mk<V>(0, 0);
+      }
+    }
+  };
+  A4 acc[NB][WT][WT][2];
+  for (int j = 0; j < NB; ++j)
+    for (int t = 0; t < WT; ++t)
+      for (int u = 0; u < WT; ++u) acc[j][t][u][0] = acc[j][t][u][1] = A4{0, 0, 0, 0};
+  load(0);
+  for (int k0 = 0; k0 < g.k; k0 += KS) {
+    __syncthreads();
+    for (int i = 0; i < L; ++i) {
+      sa[0][akc][ar0 + i] = ra[i].x;
+      sa[1][akc][ar0 + i] = ra[i].y;
+      for (int j = 0; j < NB; ++j) {
+        sw[j][0][wk0 + i][wcol] = rw[j][i].x;
+        sw[j][1][wk0 + i][wcol] = rw[j][i].y;
+      }
+    }
+    __syncthreads();
+    if (k0 + KS < g.k) load(k0 + KS);
+#pragma unroll
+    for (int k4 = 0; k4 < KS / 4; ++k4) {
+      const int kk = 4 * k4 + kq;
+      R xr[WT], xi[WT];
+      for (int t = 0; t < WT; ++t) {
+        xr[t] = sa[0][kk][wr + 16 * t + row];
+        xi[t] = sa[1][kk][wr + 16 * t + row];
+      }
+      for (int j = 0; j < NB; ++j)
+        for (int u = 0; u < WT; ++u) {
+          const R yr = sw[j][0][kk][wc + 16 * u + row], yi = sw[j][1][kk][wc + 16 * u + row];
+          for (int t = 0; t < WT; ++t) cmfma<R>(xr[t], xi[t], yr, yi, acc[j][t][u][0], acc[j][t][u][1]);
+        }
+    }
+  }
+  for (int j = 0; j < NB; ++j)
+    for (int u = 0; u < WT; ++u) {
+      const int col = c0 + wc + 16 * u + row;
+      if (col >= g.m) continue;
+      for (int t = 0; t < WT; ++t)
+        for (int i = 0; i < 4; ++i) {
+          const int r = g0 + wr + 16 * t + mfma_row<R>(lane, i);
+          if (r >= g.n) continue;
+          V *c = g.C[j] + r + int64_t(col) * g.ldc[j];
+          R vr = g.alpha * acc[j][t][u][0][i], vi = g.alpha * acc[j][t][u][1][i];
+          if (g.beta != R(0)) {
+            const V o = *c;
+            vr += g.beta * o.x;
+            vi += g.beta * o.y;
+          }
+          *c = mk<V>(vr, vi);
+        }
+    }
+}
+
+template <typename R, int WT, int NB>
+void launch_gram(dim3 grid, const GramArgs<R> &a, hipStream_t s) {
+  hipLaunchKernelGGL((gram_kernel<R, WT, NB>), grid, dim3(256), 0, s, a);
+}
+
+template <typename R, int WT, int NB>
+void launch_tall(dim3 grid, const TallArgs<R> &a, hipStream_t s) {
+  hipLaunchKernelGGL((tall_kernel<R, WT, NB>), grid, dim3(256), 0, s, a);
+}
+
+}  // namespace
+
+template <typename R>
+void gram(int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, const cplx_t<R> *B, int64_t ldb, cplx_t<R> *C,
+          int64_t ldc, const cplx_t<R> *B2, int64_t ldb2, cplx_t<R> *C2, int64_t ldc2,
+          DeviceArray<cplx_t<R>> &work, hipStream_t s) {
+  using V = cplx_t<R>;
+  if (ka <= 0 || kb <= 0) return;
+  const int nb = B2 ? 2 : 1;
+  if (n <= 0) {
+    HALF_HIP(hipMemset2DAsync(C, sizeof(V) * ldc, 0, sizeof(V) * ka, kb, s));
+    if (B2) HALF_HIP(hipMemset2DAsync(C2, sizeof(V) * ldc2, 0, sizeof(V) * ka, kb, s));
+    return;
+  }
+  if (lda < n || ldb < n || ldc < ka || (B2 && (ldb2 < n || ldc2 < ka)))
+    throw Error(kInvalidArgument, "gram: leading dimension too small");
+  const int wt = std::max(ka, kb) <= 32 ? 1 : 2;
+  const int tw = 32 * wt;
+  const int tiles_a = (ka + tw - 1) / tw, tiles = tiles_a * ((kb + tw - 1) / tw);
+  // about a thousand workgroups, each chunk at least 64 rows
+  int nchunks = std::max(1, std::min({1024 / tiles, (n + 63) / 64, 64}));
+  int chunk = (n + nchunks - 1) / nchunks;
+  chunk = (chunk + KS - 1) / KS * KS;
+  nchunks = (n + chunk - 1) / chunk;
+  GramArgs<R> a{};
+  a.A = A;
+  a.lda = lda;
+  a.B[0] = B;
+  a.ldb[0] = ldb;
+  a.B[1] = B2 ? B2 : B;
+  a.ldb[1] = B2 ? ldb2 : ldb;
+  a.n = n;
+  a.ka = ka;
+  a.kb = kb;
+  a.chunk = chunk;
+  a.tiles_a = tiles_a;
+  const int64_t tile_elems = int64_t(ka) * kb;
+  if (nchunks == 1) {
+    a.out[0] = C;
+    a.ldo[0] = ldc;
+    a.out[1] = C2;
+    a.ldo[1] = ldc2;
+    a.chunk_stride = 0;
+  } else {
+    work.ensure(size_t(tile_elems) * nchunks * nb);
+    a.out[0] = work.get();
+    a.out[1] = work.get() + tile_elems * nchunks;
+    a.ldo[0] = a.ldo[1] = ka;
+    a.chunk_stride = tile_elems;
+  }
+  const dim3 grid(tiles, nchunks);
+  if (wt == 1)
+    nb == 1 ? launch_gram<R, 1, 1>(grid, a, s) : launch_gram<R, 1, 2>(grid, a, s);
+  else
+    nb == 1 ? launch_gram<R, 2, 1>(grid, a, s) : launch_gram<R, 2, 2>(grid, a, s);
+  HALF_HIP(hipGetLastError());
+  if (nchunks > 1) {
+    const unsigned blocks = static_cast<unsigned>((tile_elems + 255) / 256);
+    hipLaunchKernelGGL((gram_reduce_kernel<V>), dim3(blocks), dim3(256), 0, s, a.out[0], tile_elems, nchunks, ka,
+                       kb, C, ldc);
+    if (B2)
+      hipLaunchKernelGGL((gram_reduce_kernel<V>), dim3(blocks), dim3(256), 0, s, a.out[1], tile_elems, nchunks,
+                         ka, kb, C2, ldc2);
+    HALF_HIP(hipGetLastError());
+  }
+}
+
+template <typename R>
+void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, const cplx_t<R> *W, int64_t ldw,
+               R beta, cplx_t<R> *C, int64_t ldc, const cplx_t<R> *W2, int64_t ldw2, cplx_t<R> *C2, int64_t ldc2,
+               hipStream_t s) {
+  if (n <= 0 || m <= 0) return;
+  if (k <= 0) throw Error(kInvalidArgument, "gemm_tall: empty inner dimension");
+  if (lda < n || ldw < k || ldc < n || (W2 && (ldw2 < k || ldc2 < n)))
+    throw Error(kInvalidArgument, "gemm_tall: leading dimension too small");
+  const int nb = W2 ? 2 : 1;
+  // 64-wide tiles when they still give a few hundred workgroups
+  const int64_t t64 = int64_t((n + 63) / 64) * ((m + 63) / 64);
+  const int wt = t64 >= 512 ? 2 : 1;
+  const int tw = 32 * wt;
+  TallArgs<R> a{};
+  a.A = A;
+  a.lda = lda;
+  a.W[0] = W;
+  a.ldw[0] = ldw;
+  a.C[0] = C;
+  a.ldc[0] = ldc;
+  a.W[1] = W2 ? W2 : W;
+  a.ldw[1] = W2 ? ldw2 : ldw;
+  a.C[1] = W2 ? C2 : C;
+  a.ldc[1] = W2 ? ldc2 : ldc;
+  a.n = n;
+  a.m = m;
+  a.k = k;
+  a.alpha = alpha;
+  a.beta = beta;
+  const dim3 grid((n + tw - 1) / tw, (m + tw - 1) / tw);
+  if (grid.y > 65535) throw Error(kInvalidArgument, "gemm_tall: too many columns");
+  if (wt == 1)
+    nb == 1 ? launch_tall<R, 1, 1>(grid, a, s) : launch_tall<R, 1, 2>(grid, a, s);
+  else
+    nb == 1 ? launch_tall<R, 2, 1>(grid, a, s) : launch_tall<R, 2, 2>(grid, a, s);
+  HALF_HIP(hipGetLastError());
+}
+
+#define HALF_CONTRACT(R)                                                                                     \
+  template void gram<R>(int, int, int, const cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *, \
+                        int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *, int64_t, DeviceArray<cplx_t<R>> &, \
+                        hipStream_t);                                                                        \
+  template void gemm_tall<R>(int, int, int, R, const cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, R,       \
+                             cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *, int64_t, hipStream_t);
+HALF_CONTRACT(double)
+HALF_CONTRACT(float)
+#undef HALF_CONTRACT
+
+}  // namespace half

@@ -1,0 +1,28 @@
+// Dense complex contractions on the matrix cores with LDS-staged tiles
+// (see contract.hip).  All blocks are column-major.
+#pragma once
+
+#include "common.hpp"
+#include "fft.hpp"
+
+namespace half {
+
+// C = A^H B summed over the n rows of A (n x ka) and B (n x kb); C is ka x kb
+// with leading dimension ldc.  When B2 is given, also C2 = A^H B2 (A is read
+// once for both).  The row range is split into chunks whose partial tiles are
+// added in a fixed order, so the result does not depend on scheduling.
+// `work` grows to hold the partial tiles.
+template <typename R>
+void gram(int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, const cplx_t<R> *B, int64_t ldb, cplx_t<R> *C,
+          int64_t ldc, const cplx_t<R> *B2, int64_t ldb2, cplx_t<R> *C2, int64_t ldc2,
+          DeviceArray<cplx_t<R>> &work, hipStream_t s);
+
+// C = beta C + alpha A W for A (n x k) and W (k x m); C is n x m.  When W2 is
+// given, also C2 = beta C2 + alpha A W2.  beta = 0 does not read C.  C must not
+// overlap A or W.
+template <typename R>
+void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, const cplx_t<R> *W, int64_t ldw,
+               R beta, cplx_t<R> *C, int64_t ldc, const cplx_t<R> *W2, int64_t ldw2, cplx_t<R> *C2, int64_t ldc2,
+               hipStream_t s);
+
+}  // namespace half

@@ -10,9 +10,14 @@
 //           from psi (no box clearing), backward transform along i3
 //   y_bwd   active i1 planes: backward transform along i2, loading only the
 //           active lines of the plane (the rest are zeros, never stored)
-//   x_veff  every (i2,i3) column: load the active planes, backward transform
-//           along i1, multiply by Veff/N_r, forward transform, store the
-//           active planes only
+//   x_conv  every (i2,i3) column: restricted to the active i1 planes the
+//           backward transform along i1, the multiply by Veff/N_r and the
+//           forward transform are one circular convolution,
+//             out[p'] = sum_p Vx[(p' - p) mod n1][c] in[p],
+//           with Vx[d][c] = sum_x Veff[x][c]/N_r e^{-2 pi i d x/n1} computed
+//           once per potential; each active plane is read and written once
+//           (x_veff, the transform-based form, serves boxes whose active
+//           plane count exceeds the register budget)
 //   y_fwd   active planes: forward transform along i2, store active lines
 //   z_fwd   active lines: forward transform along i3, gather the plane-wave
 //           coefficients and add T psi (S psi = psi is written here too)
@@ -38,6 +43,7 @@ struct Tuning {
   // defaults from the autotuner on the 48^3 HfO2-like box, 64 bands
   int tile[2][3] = {{1024, 1536, 768}, {1536, 1536, 768}};  // [fp64|fp32][x|y|z]
   int xgroup[2] = {4, 8};
+  int xconv[2] = {1, 1};  // x pass as a per-column convolution when it fits
 };
 Tuning g_tune;
 
@@ -49,6 +55,7 @@ struct Tables {
   int nplanes;
   const int *plane_i1, *plane_on;
   const int *pw_i3;
+  int plane_s0;  // first signed plane of the contiguous run (x_conv_kernel)
 };
 
 // Elements per thread of a tile (M * n <= tile budget <= 256 * KMAX).
@@ -201,6 +208,82 @@ __global__ __launch_bounds__(256) void x_veff_kernel(cplx_t<R> *box, const R *ve
   }
 }
 
+// x pass as a convolution over the active planes (see the header), for a
+// contiguous run of P planes s0 .. s0 + P - 1 (signed, modulo n1): with the
+// planes in that order, plane p' takes sum_p Vx[(p' - p) mod n1] in[p].  A
+// workgroup takes kXcCols consecutive (i2,i3) columns and kXcBands bands;
+// lanes 0-31 and 32-63 of a wave hold two bands of the same columns.  LDS
+// holds the 2P - 1 needed rows of Vx ordered by the difference p' - p + P - 1,
+// so every term of the sum is one LDS read at a fixed offset; each thread
+// keeps its column's P values in registers (PMAX >= P) and overwrites them.
+constexpr int kXcCols = 32;
+constexpr int kXcBands = 8;
+
+template <typename R, int PMAX>
+__global__ __launch_bounds__(256) void x_conv_kernel(cplx_t<R> *box, const cplx_t<R> *vx, Tables t, int nb) {
+  using V = cplx_t<R>;
+  extern __shared__ double2 smem_raw[];
+  V *vs = reinterpret_cast<V *>(smem_raw);  // [2P - 1][kXcCols]
+  const int C = t.n2 * t.n3;
+  const int P = t.nplanes;
+  const int c0 = blockIdx.x * kXcCols;
+  for (int e = threadIdx.x; e < (2 * P - 1) * kXcCols; e += blockDim.x) {
+    const int k = e / kXcCols, c = c0 + (e - k * kXcCols);
+    int d = (k - (P - 1)) % t.n1;
+    d += d < 0 ? t.n1 : 0;
+    vs[e] = c < C ? vx[static_cast<long long>(d) * C + c] : mk<V>(R(0), R(0));
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ci = lane & 31;
+  const int c = c0 + ci;
+  const int band = blockIdx.y * kXcBands + wave * 2 + (lane >> 5);
+  if (c >= C || band >= nb) return;  // no barrier follows
+  V *col = box + static_cast<long long>(band) * t.nr + c;
+  const int s0 = t.plane_s0 < 0 ? t.plane_s0 + t.n1 : t.plane_s0;
+  V in[PMAX];
+#pragma unroll
+  for (int q = 0; q < PMAX; ++q) {
+    int i1 = s0 + q;
+    i1 -= i1 >= t.n1 ? t.n1 : 0;
+    in[q] = q < P ? col[static_cast<long long>(i1) * C] : mk<V>(R(0), R(0));
+  }
+  for (int op = 0; op < P; ++op) {
+    const V *w = vs + (op + P - 1) * kXcCols + ci;  // w[-q kXcCols] = Vx[(op - q) mod n1]
+    R ar = 0, ai = 0;
+#pragma unroll
+    for (int q = 0; q < PMAX; ++q) {
+      if (q < P) {
+        const V v = w[-q * kXcCols];
+        ar += v.x * in[q].x - v.y * in[q].y;
+        ai += v.x * in[q].y + v.y * in[q].x;
+      }
+    }
+    int i1 = s0 + op;
+    i1 -= i1 >= t.n1 ? t.n1 : 0;
+    col[static_cast<long long>(i1) * C] = mk<V>(ar, ai);
+  }
+}
+
+// Vx[d][c] = scale sum_x veff[x][c] e^{-2 pi i d x / n1} for the C = n2 n3
+// columns of the C-order box, in FP64 and FP32
+__global__ void veff_x_dft_kernel(const double *veff, double2 *vx, float2 *vx32, int n1, int C, double scale) {
+  const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<long long>(n1) * C) return;
+  const int d = static_cast<int>(t / C);
+  const int c = static_cast<int>(t - static_cast<long long>(d) * C);
+  double sr = 0, si = 0;
+  for (int x = 0; x < n1; ++x) {
+    double sn, cs;
+    sincospi(-2.0 * ((d * x) % n1) / n1, &sn, &cs);
+    const double v = veff[static_cast<long long>(x) * C + c];
+    sr += v * cs;
+    si += v * sn;
+  }
+  vx[t] = make_double2(sr * scale, si * scale);
+  vx32[t] = make_float2(static_cast<float>(sr * scale), static_cast<float>(si * scale));
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void z_fwd_kernel(const cplx_t<R> *psi, long long ldpsi, const cplx_t<R> *box,
                                                     const R *kin, cplx_t<R> *hpsi, cplx_t<R> *spsi,
@@ -318,6 +401,14 @@ void LocalMap::build(const Basis &b, hipStream_t s) {
   std::vector<int> planes;
   for (int i = 0; i < n1; ++i)
     if (plane_on_h[i]) planes.push_back(i);
+  // signed plane indices in (-n1/2, n1/2]: one contiguous run for a sphere
+  std::vector<int> signed_planes;
+  for (int i : planes) signed_planes.push_back(i > n1 / 2 ? i - n1 : i);
+  std::sort(signed_planes.begin(), signed_planes.end());
+  planes_contiguous = !signed_planes.empty();
+  for (size_t q = 1; q < signed_planes.size(); ++q)
+    if (signed_planes[q] != signed_planes[0] + static_cast<int>(q)) planes_contiguous = false;
+  plane_s0 = signed_planes.empty() ? 0 : signed_planes[0];
   nlines = static_cast<int>(lines.size());
   nplanes = static_cast<int>(planes.size());
   line_i12.upload(lines, s);
@@ -330,16 +421,42 @@ void LocalMap::build(const Basis &b, hipStream_t s) {
   HALF_HIP(hipStreamSynchronize(s));
 }
 
+// register budget of x_conv_kernel per precision: the largest PMAX instantiated
 template <typename R>
-void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const R *kin, int nb, const cplx_t<R> *psi,
-                 int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi, int64_t ldout, cplx_t<R> *boxes, hipStream_t s) {
+const void *x_conv_for(int planes, int &pmax) {
+  if constexpr (std::is_same<R, double>::value) {
+    if (planes <= 16) return pmax = 16, reinterpret_cast<const void *>(&x_conv_kernel<R, 16>);
+    if (planes <= 24) return pmax = 24, reinterpret_cast<const void *>(&x_conv_kernel<R, 24>);
+    if (planes <= 32) return pmax = 32, reinterpret_cast<const void *>(&x_conv_kernel<R, 32>);
+  } else {
+    if (planes <= 16) return pmax = 16, reinterpret_cast<const void *>(&x_conv_kernel<R, 16>);
+    if (planes <= 32) return pmax = 32, reinterpret_cast<const void *>(&x_conv_kernel<R, 32>);
+    if (planes <= 48) return pmax = 48, reinterpret_cast<const void *>(&x_conv_kernel<R, 48>);
+    if (planes <= 64) return pmax = 64, reinterpret_cast<const void *>(&x_conv_kernel<R, 64>);
+  }
+  pmax = 0;
+  return nullptr;
+}
+
+void build_veff_x_dft(const double *veff, int n1, int n2, int n3, double2 *vx, float2 *vx32, hipStream_t s) {
+  const long long total = static_cast<long long>(n1) * n2 * n3;
+  const double scale = 1.0 / static_cast<double>(total);
+  hipLaunchKernelGGL(veff_x_dft_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s, veff, vx,
+                     vx32, n1, n2 * n3, scale);
+  HALF_HIP(hipGetLastError());
+}
+
+template <typename R>
+void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const cplx_t<R> *vx, const R *kin, int nb,
+                 const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi, int64_t ldout,
+                 cplx_t<R> *boxes, hipStream_t s) {
   if (nb <= 0 || m.nlines == 0) return;
   if (static_cast<long long>(nb) * (static_cast<long long>(m.n1) * m.n2 + static_cast<long long>(m.n2) * m.n3 +
                                    static_cast<long long>(m.n1) * m.n3) >= (1ll << 31))
     throw Error(kInvalidArgument, "band batch too large for the local stage");
   Tables t{m.n1, m.n2, m.n3, static_cast<long long>(m.n1) * m.n2 * m.n3, m.nlines, m.line_i12.get(),
            m.line_start.get(), m.line_count.get(), m.line_of.get(), m.nplanes, m.plane_i1.get(), m.plane_on.get(),
-           m.pw_i3.get()};
+           m.pw_i3.get(), m.plane_s0};
   long long lp = ldpsi, lo = ldout;
   FftArgs az = axis_args<R>(plan, 2, static_cast<long long>(nb) * m.nlines, 1);
   FftArgs ay = axis_args<R>(plan, 1, static_cast<long long>(nb) * m.nplanes * m.n3, m.n3);
@@ -359,7 +476,19 @@ void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const R 
     void *args[] = {&boxes, &t, &ay};
     launch<R>("local_y_bwd", reinterpret_cast<const void *>(&y_kernel<R, true>), ay, s, args);
   }
-  {
+  int pmax = 0;
+  const size_t xconv_lds = size_t(2 * m.nplanes - 1) * kXcCols * sizeof(cplx_t<R>);
+  const bool xconv_ok = vx && g_tune.xconv[prec] && m.planes_contiguous && m.nplanes <= m.n1 && xconv_lds <= 65536;
+  const void *xconv = xconv_ok ? x_conv_for<R>(m.nplanes, pmax) : nullptr;
+  if (xconv) {
+    int nbi = nb;
+    void *args[] = {&boxes, &vx, &t, &nbi};
+    const int C = m.n2 * m.n3;
+    const size_t lds = xconv_lds;
+    trace_mark("local_x_conv", s);
+    HALF_HIP(hipLaunchKernel(xconv, dim3((C + kXcCols - 1) / kXcCols, (nb + kXcBands - 1) / kXcBands), dim3(256),
+                             args, lds, s));
+  } else {
     int nbi = nb;
     void *args[] = {&boxes, &veff, &scale, &t, &ax, &G, &nbi};
     launch<R>("local_x_veff", reinterpret_cast<const void *>(&x_veff_kernel<R>), ax, s, args, xblocks,
@@ -390,6 +519,11 @@ bool set_local_tuning(const std::string &key, int value) {
       g_tune.xgroup[p] = value;
       return true;
     }
+    if (key == pre + ".xconv") {
+      if (value < 0 || value > 1) return false;
+      g_tune.xconv[p] = value;
+      return true;
+    }
   }
   return false;
 }
@@ -407,13 +541,18 @@ bool get_local_tuning(const std::string &key, int &value) {
       value = g_tune.xgroup[p];
       return true;
     }
+    if (key == pre + ".xconv") {
+      value = g_tune.xconv[p];
+      return true;
+    }
   }
   return false;
 }
 
-template void local_apply<double>(const FftPlan &, const LocalMap &, const double *, const double *, int,
-                                  const double2 *, int64_t, double2 *, double2 *, int64_t, double2 *, hipStream_t);
-template void local_apply<float>(const FftPlan &, const LocalMap &, const float *, const float *, int, const float2 *,
-                                 int64_t, float2 *, float2 *, int64_t, float2 *, hipStream_t);
+template void local_apply<double>(const FftPlan &, const LocalMap &, const double *, const double2 *,
+                                  const double *, int, const double2 *, int64_t, double2 *, double2 *, int64_t,
+                                  double2 *, hipStream_t);
+template void local_apply<float>(const FftPlan &, const LocalMap &, const float *, const float2 *, const float *, int,
+                                 const float2 *, int64_t, float2 *, float2 *, int64_t, float2 *, hipStream_t);
 
 }  // namespace half

@@ -12,14 +12,15 @@
 // a time to bound the workspace.
 //
 // Projector stage: c = B^H psi, (dc, qc) = (D c, Q c) per atom block,
-// H psi += B dc, S psi += B qc.  The two contractions run on the matrix cores
-// (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32) with complex products built
-// from four real MFMAs; c = B^H psi is split over the plane-wave dimension and
-// reduced in a fixed order.
+// H psi += B dc, S psi += B qc.  The two contractions are the LDS-tiled
+// matrix-core kernels of contract.hip (gram and gemm_tall); c = B^H psi is
+// split over the plane-wave dimension and reduced in a fixed order.
 #include <algorithm>
 #include <cmath>
 
+#include "contract.hpp"
 #include "cplx.hpp"
+#include "mfma.hpp"
 #include "paw.hpp"
 #include "trace.hpp"
 
@@ -29,45 +30,6 @@ namespace {
 
 constexpr int kThreads = 256;
 inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
-
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-template <typename R>
-struct Acc;
-template <>
-struct Acc<double> {
-  using type = v4d;
-  __device__ static inline v4d mfma(double a, double b, v4d c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-};
-template <>
-struct Acc<float> {
-  using type = v4f;
-  __device__ static inline v4f mfma(float a, float b, v4f c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-};
-
-// Row of the 16x16 result held in accumulator element r of this lane.  The
-// lane's column is lane % 16 for both instructions; rows differ: the FP32
-// instruction keeps four consecutive rows per lane group, the FP64 one
-// interleaves them (row = 4 r + lane / 16).
-template <typename R>
-__device__ inline int mfma_row(int lane, int r) {
-  return std::is_same<R, double>::value ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r;
-}
-
-// Complex 16x16 tile accumulate: (cr, ci) += A(16x4) * B(4x16), A and B
-// given as this lane's real/imaginary parts.
-template <typename R>
-__device__ inline void cmfma(R ar, R ai, R br, R bi, typename Acc<R>::type &cr, typename Acc<R>::type &ci) {
-  cr = Acc<R>::mfma(ar, br, cr);
-  cr = Acc<R>::mfma(-ai, bi, cr);
-  ci = Acc<R>::mfma(ar, bi, ci);
-  ci = Acc<R>::mfma(ai, br, ci);
-}
 
 // ---------------------------------------------------------------------------
 // projector generation (reference: src/half_cuda_assembly.cuf:84-170,
@@ -179,102 +141,9 @@ __global__ void projector_kernel(cplx_t<R> *B, double2 *B64, ProjArgs a) {
 // local stage
 // ---------------------------------------------------------------------------
 
-template <typename V>
-__global__ void zero_kernel(V *x, int64_t n) {
-  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (i < n) x[i] = mk<V>(0, 0);
-}
-
-template <typename V>
-__global__ void scatter_kernel(const V *psi, int64_t ldpsi, V *box, int64_t nr, const int32_t *idx, int n,
-                               int nb) {
-  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (t >= int64_t(n) * nb) return;
-  const int b = static_cast<int>(t / n);
-  const int g = static_cast<int>(t - int64_t(b) * n);
-  box[b * nr + idx[g]] = psi[g + b * ldpsi];
-}
-
-template <typename V, typename R>
-__global__ void gather_kernel(const V *psi, int64_t ldpsi, const V *box, int64_t nr, const int32_t *idx,
-                              const R *kin, V *hpsi, V *spsi, int64_t ldout, int n, int nb) {
-  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (t >= int64_t(n) * nb) return;
-  const int b = static_cast<int>(t / n);
-  const int g = static_cast<int>(t - int64_t(b) * n);
-  const V p = psi[g + b * ldpsi];
-  const V v = box[b * nr + idx[g]];
-  hpsi[g + b * ldout] = mk<V>(v.x + kin[g] * p.x, v.y + kin[g] * p.y);
-  spsi[g + b * ldout] = p;
-}
-
 // ---------------------------------------------------------------------------
-// projector contractions on the matrix cores
+// projector mixing
 // ---------------------------------------------------------------------------
-
-// partial[chunk][b][p] = sum_{G in chunk} conj(B[G,p]) psi[G,b]
-// One workgroup = 4 waves = one 16x16 (p, b) tile and one G chunk; the four
-// waves interleave over the chunk's 4-row k steps and reduce in LDS.
-template <typename R>
-__global__ __launch_bounds__(256) void coeff_partial_kernel(const cplx_t<R> *B, int64_t ldb,
-                                                            const cplx_t<R> *psi, int64_t ldpsi,
-                                                            cplx_t<R> *partial, int n, int np, int nb,
-                                                            int chunk, int tiles_p) {
-  using A = typename Acc<R>::type;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int tile = blockIdx.x;
-  const int tp = tile % tiles_p, tb = tile / tiles_p;
-  const int p0 = tp * 16, b0 = tb * 16;
-  const int g_begin = blockIdx.y * chunk;
-  const int g_end = min(n, g_begin + chunk);
-  const int row = lane & 15, kq = lane >> 4;
-  const int p = p0 + row, b = b0 + row;
-  const bool pv = p < np, bv = b < nb;
-  A cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-  for (int g4 = g_begin + 4 * wave; g4 < g_end; g4 += 16) {
-    const int g = g4 + kq;
-    R ar = 0, ai = 0, br = 0, bi = 0;
-    if (g < g_end) {
-      if (pv) {
-        const cplx_t<R> v = B[g + int64_t(p) * ldb];
-        ar = v.x;
-        ai = -v.y;  // conj
-      }
-      if (bv) {
-        const cplx_t<R> v = psi[g + int64_t(b) * ldpsi];
-        br = v.x;
-        bi = v.y;
-      }
-    }
-    cmfma<R>(ar, ai, br, bi, cr, ci);
-  }
-  __shared__ R red[4][2][4][64];
-  for (int i = 0; i < 4; ++i) {
-    red[wave][0][i][lane] = cr[i];
-    red[wave][1][i][lane] = ci[i];
-  }
-  __syncthreads();
-  if (wave == 0) {
-    for (int i = 0; i < 4; ++i) {
-      R sr = red[0][0][i][lane] + red[1][0][i][lane] + red[2][0][i][lane] + red[3][0][i][lane];
-      R si = red[0][1][i][lane] + red[1][1][i][lane] + red[2][1][i][lane] + red[3][1][i][lane];
-      const int pr = p0 + mfma_row<R>(lane, i);  // projector row, band column lane % 16
-      const int bc = b0 + (lane & 15);
-      if (pr < np && bc < nb)
-        partial[(int64_t(blockIdx.y) * nb + bc) * np + pr] = mk<cplx_t<R>>(sr, si);
-    }
-  }
-}
-
-template <typename V>
-__global__ void reduce_chunks_kernel(const V *partial, V *coeff, int64_t count, int nchunks) {
-  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (i >= count) return;
-  V acc = partial[i];
-  for (int c = 1; c < nchunks; ++c) acc = cadd(acc, partial[c * count + i]);
-  coeff[i] = acc;
-}
 
 // dc = D_atom c, qc = Q_atom c (reference: src/half_cuda_assembly.cuf:220-237)
 template <typename V, typename R>
@@ -298,56 +167,6 @@ __global__ void mix_kernel(const V *coeff, V *dc, V *qc, const int *p_atom, cons
   }
   dc[t] = zd;
   qc[t] = zq;
-}
-
-// hpsi[G,b] += sum_p B[G,p] dc[p,b];  spsi[G,b] += sum_p B[G,p] qc[p,b]
-// One wave = one 16x16 (G, b) tile for both outputs.
-template <typename R>
-__global__ __launch_bounds__(256) void expand_kernel(const cplx_t<R> *B, int64_t ldb, const cplx_t<R> *dc,
-                                                     const cplx_t<R> *qc, cplx_t<R> *hpsi, cplx_t<R> *spsi,
-                                                     int64_t ldout, int n, int np, int nb, int tiles_g) {
-  using A = typename Acc<R>::type;
-  const int lane = threadIdx.x & 63;
-  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int tiles_b = (nb + 15) / 16;
-  if (tile >= tiles_g * tiles_b) return;
-  const int tg = tile % tiles_g, tb = tile / tiles_g;
-  const int g0 = tg * 16, b0 = tb * 16;
-  const int row = lane & 15, kq = lane >> 4;
-  const int g = g0 + row, b = b0 + row;
-  const bool gv = g < n, bv = b < nb;
-  A hr = {0, 0, 0, 0}, hi = {0, 0, 0, 0}, sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
-  for (int p4 = 0; p4 < np; p4 += 4) {
-    const int p = p4 + kq;
-    R ar = 0, ai = 0, dr = 0, di = 0, qr = 0, qi = 0;
-    if (p < np) {
-      if (gv) {
-        const cplx_t<R> v = B[g + int64_t(p) * ldb];
-        ar = v.x;
-        ai = v.y;
-      }
-      if (bv) {
-        const cplx_t<R> d = dc[p + int64_t(b) * np];
-        const cplx_t<R> q = qc[p + int64_t(b) * np];
-        dr = d.x;
-        di = d.y;
-        qr = q.x;
-        qi = q.y;
-      }
-    }
-    cmfma<R>(ar, ai, dr, di, hr, hi);
-    cmfma<R>(ar, ai, qr, qi, sr, si);
-  }
-  const int bc = b0 + (lane & 15);
-  if (bc >= nb) return;
-  for (int i = 0; i < 4; ++i) {
-    const int gr = g0 + mfma_row<R>(lane, i);
-    if (gr >= n) continue;
-    const int64_t o = gr + int64_t(bc) * ldout;
-    cplx_t<R> h = hpsi[o], s = spsi[o];
-    hpsi[o] = mk<cplx_t<R>>(h.x + hr[i], h.y + hi[i]);
-    spsi[o] = mk<cplx_t<R>>(s.x + sr[i], s.y + si[i]);
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -558,6 +377,10 @@ void System::build(hipStream_t s) {
   d_dblk32.upload(d32, s);
   d_qblk32.upload(q32, s);
   build_veff_device(*this, s);
+  const int64_t nr = int64_t(fft.n[0]) * fft.n[1] * fft.n[2];
+  veff_x.resize(size_t(nr));
+  veff_x32.resize(size_t(nr));
+  build_veff_x_dft(veff.get(), fft.n[0], fft.n[1], fft.n[2], veff_x.get(), veff_x32.get(), s);
   ready = true;
 }
 
@@ -600,39 +423,26 @@ void KOperator<R>::apply_local(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx
   batch = std::max(1, std::min(batch, nb));
   boxes.ensure(size_t(nr) * batch);
   const R *veff = nullptr;
-  if constexpr (std::is_same<R, double>::value)
+  const cplx_t<R> *vx = nullptr;
+  if constexpr (std::is_same<R, double>::value) {
     veff = sys->veff.get();
-  else
+    vx = sys->veff_x.get();
+  } else {
     veff = sys->veff32.get();
+    vx = sys->veff_x32.get();
+  }
   for (int b0 = 0; b0 < nb; b0 += batch) {
     const int cnt = std::min(batch, nb - b0);
-    local_apply<R>(sys->fft, lmap, veff, kinetic.get(), cnt, psi + b0 * ldpsi, ldpsi, hpsi + b0 * ldout,
+    local_apply<R>(sys->fft, lmap, veff, vx, kinetic.get(), cnt, psi + b0 * ldpsi, ldpsi, hpsi + b0 * ldout,
                    spsi + b0 * ldout, ldout, boxes.get(), st);
   }
 }
 
 template <typename R>
 void KOperator<R>::project(int nb, const cplx_t<R> *psi, int64_t ldpsi, hipStream_t st) {
-  using V = cplx_t<R>;
-  const int tiles_p = (np + 15) / 16, tiles_b = (nb + 15) / 16;
-  const int tiles = tiles_p * tiles_b;
-  // split G so the launch has about a thousand workgroups, at most 16 chunks
-  // (the fixed-order reduction reads every chunk's partial tile)
-  int nchunks = std::max(1, std::min({(n + 63) / 64, 1024 / std::max(tiles, 1), 16}));
-  int chunk = (n + nchunks - 1) / nchunks;
-  chunk = (chunk + 15) / 16 * 16;
-  nchunks = (n + chunk - 1) / chunk;
-  const int64_t count = int64_t(np) * nb;
-  coeff.ensure(count);
-  partial.ensure(count * nchunks);
+  coeff.ensure(int64_t(np) * nb);
   trace_mark("proj_coeff_mfma", st);
-  hipLaunchKernelGGL((coeff_partial_kernel<R>), dim3(tiles, nchunks), dim3(256), 0, st, B.get(), ldb, psi,
-                     ldpsi, partial.get(), n, np, nb, chunk, tiles_p);
-  HALF_HIP(hipGetLastError());
-  trace_mark("proj_coeff_reduce", st);
-  hipLaunchKernelGGL((reduce_chunks_kernel<V>), dim3(blocks_for(count)), dim3(kThreads), 0, st, partial.get(),
-                     coeff.get(), count, nchunks);
-  HALF_HIP(hipGetLastError());
+  gram<R>(n, np, nb, B.get(), ldb, psi, ldpsi, coeff.get(), np, nullptr, 0, nullptr, 0, partial, st);
 }
 
 template <typename R>
@@ -657,12 +467,9 @@ void KOperator<R>::apply_projectors(int nb, const cplx_t<R> *psi, int64_t ldpsi,
                      dcoeff.get(), qcoeff.get(), sys->d_proj_atom.get(), sys->d_proj_lm.get(),
                      sys->d_atom_first.get(), sys->d_atom_nlm.get(), sys->d_atom_boff.get(), db, qb, np, nb);
   HALF_HIP(hipGetLastError());
-  const int tiles_g = (n + 15) / 16, tiles_b = (nb + 15) / 16;
-  const int tiles = tiles_g * tiles_b;
   trace_mark("proj_expand_mfma", st);
-  hipLaunchKernelGGL((expand_kernel<R>), dim3((tiles + 3) / 4), dim3(256), 0, st, B.get(), ldb, dcoeff.get(),
-                     qcoeff.get(), hpsi, spsi, ldout, n, np, nb, tiles_g);
-  HALF_HIP(hipGetLastError());
+  gemm_tall<R>(n, nb, np, R(1), B.get(), ldb, dcoeff.get(), np, R(1), hpsi, ldout, qcoeff.get(), np, spsi, ldout,
+               st);
 }
 
 template <typename R>

@@ -27,6 +27,7 @@ struct SolveResult {
   int fp64_iterations = 0;  // iterations done with the FP64 operator
   double seconds_operator = 0, seconds_solver = 0;
   int64_t applies = 0;      // band applications performed
+  int metric_fallbacks = 0; // iterations normalised by the metric's eigen-decomposition
 };
 
 struct AccControls {

@@ -46,6 +46,10 @@ typedef struct half_amd_solve_stats {
   double max_residual_eV;
   double seconds;
   int64_t band_applications;
+  int32_t metric_fallbacks; /* iterations whose correction block was normalised
+                               by the metric's eigen-decomposition instead of
+                               its Cholesky factor */
+  int32_t reserved;
 } half_amd_solve_stats;
 
 /* All-band solve with explicit controls (the reference ABI fixes tol 1e-4 eV,

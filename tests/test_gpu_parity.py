@@ -181,6 +181,37 @@ def test_abi_errors_on_live_context(si2):
     assert abs(info["volume"] - g["volume"]) < 1e-12
 
 
+def test_local_x_pass_forms(si2):
+    """The x pass as a per-column convolution (default) and as transform /
+    multiply / transform agree, and both reproduce the reference's H.psi."""
+    from half_amd import hip
+    name, g, ctx = si2
+    n = g["npw"]
+    psi = psi_block(n, 3)
+    ref = np.load(f"tests/golden/{name}.apply.npy")
+    stream = hip.Stream()
+    try:
+        for prec, dt in ((64, np.complex128), (32, np.complex64)):
+            op = ctx.operator(g["kpoint"], prec)
+            x = hip.Buffer.from_numpy(np.ascontiguousarray(psi.T).astype(dt))
+            out = {}
+            for form in (1, 0):
+                H.set_tuning(f"local.fp{prec}.xconv", form)
+                hp, sp = hip.Buffer(x.shape, dt), hip.Buffer(x.shape, dt)
+                op.apply_device(3, x.ptr, hp.ptr, sp.ptr, stream=stream.ptr)
+                stream.synchronize()
+                out[form] = hp.numpy().T.astype(np.complex128)
+            op.close()
+            scale = np.abs(ref[0]).max()
+            tol = 1e-13 if prec == 64 else 2e-6
+            assert np.abs(out[1] - out[0]).max() / scale < tol
+            for form in (1, 0):
+                assert np.abs(out[form] - ref[0]).max() / scale < (1e-11 if prec == 64 else 2e-6), (prec, form)
+    finally:
+        H.set_tuning("local.fp64.xconv", 1)
+        H.set_tuning("local.fp32.xconv", 1)
+
+
 @pytest.fixture(scope="module")
 def hfo2(tmp_path_factory):
     g, ctx = _ctx("hfo2_pbe", inputs_dir=str(tmp_path_factory.mktemp("hfo2")))
