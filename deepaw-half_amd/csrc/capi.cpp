@@ -50,6 +50,7 @@ struct Context {
   double req_lattice[9] = {};
   std::vector<int32_t> req_species;
   std::vector<double> req_positions;
+  std::vector<double> acc_history;  // residual history of the last all-band solve
 
   ~Context() {
     // Drain the stream, release device objects, destroy the rocBLAS handle
@@ -623,6 +624,7 @@ int half_amd_solve_acc(half_handle handle, const double kpoint[3], int nbands, i
     DeviceArray<double2> vec;
     SolveResult res;
     solve_acc(c.blas, op, op32, nbands, ctl, eigenvectors != nullptr, vec, res);
+    c.acc_history = res.history;
     std::copy(res.eigenvalues.begin(), res.eigenvalues.end(), eigenvalues);
     if (eigenvectors) copy_vectors_out(vec, static_cast<int>(n), nbands, eigenvectors, ld, c.stream);
     if (stats) {
@@ -636,6 +638,21 @@ int half_amd_solve_acc(half_handle handle, const double kpoint[3], int nbands, i
     }
     return int(kSuccess);
   });
+}
+
+int half_amd_solve_history(half_handle handle, int capacity, double *residuals, int32_t *count, char *error,
+                           int ecap) {
+  clear_error(error, ecap);
+  const int slot = context_slot(handle);
+  if (!slot) return bad_handle(error, ecap);
+  if (!count || (capacity > 0 && !residuals))
+    return export_error(kInvalidArgument, "output pointers are required", error, ecap);
+  const std::vector<double> &h = g_contexts[slot]->acc_history;
+  *count = static_cast<int32_t>(h.size());
+  if (capacity < static_cast<int64_t>(h.size()))
+    return export_error(kCapacity, "residual history capacity is too small", error, ecap);
+  std::copy(h.begin(), h.end(), residuals);
+  return int(kSuccess);
 }
 
 int half_amd_set_solver(half_handle handle, int solver, char *error, int capacity) {
@@ -657,6 +674,35 @@ int half_amd_energy_terms(half_handle handle, double *eh, double *exc, double *e
   *exc = s.e_xc;
   *exv = s.e_xc_potential;
   return int(kSuccess);
+}
+
+int half_amd_set_projector_positions(half_handle handle, int nions, const double *frac, char *error,
+                                     int capacity) {
+  clear_error(error, capacity);
+  const int slot = context_slot(handle);
+  if (!slot) return bad_handle(error, capacity);
+  return guarded(error, capacity, [&]() {
+    Context &c = *g_contexts[slot];
+    if (!frac) return export_error(kInvalidArgument, "positions pointer is required", error, capacity);
+    if (nions != c.sys.natoms) return export_error(kInvalidArgument, "ion count does not match the context", error,
+                                                   capacity);
+    std::vector<double> cart;
+    for (int a = 0; a < nions; ++a) {
+      const double *f = frac + 3 * a;
+      if (!std::isfinite(f[0]) || !std::isfinite(f[1]) || !std::isfinite(f[2]))
+        return export_error(kInvalidArgument, "invalid position values", error, capacity);
+      for (int j = 0; j < 3; ++j)
+        cart.push_back(f[0] * c.sys.crystal.lattice[0][j] + f[1] * c.sys.crystal.lattice[1][j] +
+                       f[2] * c.sys.crystal.lattice[2][j]);
+    }
+    HALF_HIP(hipStreamSynchronize(c.stream));
+    c.ops64.clear();
+    c.ops32.clear();
+    c.sys.positions_cart = cart;
+    c.sys.d_positions.upload(cart, c.stream);
+    HALF_HIP(hipStreamSynchronize(c.stream));
+    return int(kSuccess);
+  });
 }
 
 int half_amd_get_veff(half_handle handle, int64_t cap, double *veff, char *error, int capacity) {

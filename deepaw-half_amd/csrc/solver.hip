@@ -421,6 +421,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   std::vector<int> seeds(m);
   std::vector<double> pivots(m + 1);
   res.metric_fallbacks = 0;
+  res.history.clear();
   double max_res = HUGE_VAL;
   int iter = 0;
   bool converged = false;
@@ -440,6 +441,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
       if (j < mreq) max_res = std::max(max_res, r);
       seeds[j] = r < std::max(ctl.tol * 1e-2, 1e-12) ? (m + j) % n : -1;
     }
+    res.history.push_back(max_res);
     if (fp64_phase) ++fp64_iters;
     if (!fp64_phase && max_res <= std::max(ctl.switch_tol, ctl.tol)) {
       // FP64 correction sweeps: exact HX, SX for the current subspace

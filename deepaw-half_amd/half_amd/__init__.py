@@ -39,6 +39,7 @@ EXT_SYMBOLS = (
     "half_amd_solve_acc", "half_amd_set_solver", "half_amd_energy_terms",
     "half_amd_get_veff", "half_amd_nonlocal_forces", "half_amd_device_info",
     "half_amd_trace", "half_amd_trace_read", "half_amd_set_tuning", "half_amd_get_tuning",
+    "half_amd_set_projector_positions", "half_amd_solve_history",
 )
 
 
@@ -110,6 +111,8 @@ def load(path: str | None = None):
         "half_amd_trace_read": [i, cp, pd, pi64, pi32, cp, i],
         "half_amd_set_tuning": [cp, i, cp, i],
         "half_amd_get_tuning": [cp, C.POINTER(i), cp, i],
+        "half_amd_set_projector_positions": [i64, i, pd, cp, i],
+        "half_amd_solve_history": [i64, i, pd, pi32, cp, i],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -197,6 +200,7 @@ class Context:
             int(bool(use_uspp)), int(backend), int(solver), C.byref(h), e, 512)
         _check(st, e)
         self.handle = h.value
+        self.chgcar, self.potcar = chgcar, potcar
 
     def close(self):
         if self.handle:
@@ -333,6 +337,21 @@ class Context:
         _check(self.lib.half_amd_nonlocal_forces(self.handle, _kp(k), nb, _pd(eps), _pd(occ), float(kweight),
                                                  vec.ctypes.data, n, _pd(f), e, 512), e)
         return f
+
+    def solve_history(self):
+        """Max residual (eV) per iteration of the last solve_acc."""
+        cnt = C.c_int32()
+        e = _err()
+        self.lib.half_amd_solve_history(self.handle, 0, None, C.byref(cnt), e, 512)  # sizes only
+        out = np.zeros(max(cnt.value, 1))
+        _check(self.lib.half_amd_solve_history(self.handle, out.size, _pd(out), C.byref(cnt), e, 512), e)
+        return out[:cnt.value]
+
+    def set_projector_positions(self, positions_fractional):
+        """Frozen-potential displacement of the atoms seen by the projectors."""
+        pos = np.ascontiguousarray(positions_fractional, float)
+        e = _err()
+        _check(self.lib.half_amd_set_projector_positions(self.handle, pos.shape[0], _pd(pos.ravel()), e, 512), e)
 
     def operator(self, k=(0, 0, 0), precision=64):
         return Operator(self, k, precision)

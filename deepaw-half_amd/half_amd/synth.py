@@ -264,7 +264,37 @@ def species_tables(sp: Species, seed: int):
     return dion, qpaw, radial, loc, rho_atom
 
 
-def write_potcar(path: str, w: Workload) -> None:
+PAW_NMAX = 256      # radial grid points of the partial-wave sets
+PAW_RMAX = 1.5      # augmentation sphere radius (A)
+
+
+def partial_waves(sp: Species, seed: int):
+    """Radial grid and r-weighted partial waves of one dataset.
+
+    Logarithmic grid from 1e-3 A to 2 A.  Pseudo wave of a channel with
+    angular momentum l: r^(l+1) exp(-r^2 / (2 rc^2)); the all-electron wave
+    adds b r^(l+1) (1 - (r/PAW_RMAX)^2)^3 inside the augmentation sphere
+    (b uniform in [-0.5, 0.5], seeded), so their difference vanishes smoothly
+    at PAW_RMAX."""
+    rng = np.random.default_rng(seed + 7919)
+    r = 1e-3 * (2.0 / 1e-3) ** (np.arange(PAW_NMAX) / (PAW_NMAX - 1))
+    wps, wae = [], []
+    seen = {}
+    for l in sp.lmax_channels:
+        j = seen.get(l, 0)
+        seen[l] = j + 1
+        rc = sp.rc[j % len(sp.rc)]
+        ps = r ** (l + 1) * np.exp(-0.5 * r * r / (rc * rc))
+        bump = np.where(r < PAW_RMAX, r ** (l + 1) * (1.0 - (r / PAW_RMAX) ** 2) ** 3, 0.0)
+        wps.append(ps)
+        wae.append(ps + rng.uniform(-0.5, 0.5) * bump)
+    return r, wps, wae
+
+
+def write_potcar(path: str, w: Workload, with_partial_waves: bool = False) -> None:
+    """POTCAR-format datasets; with_partial_waves adds the radial grid and the
+    pseudo / all-electron partial waves the potential-dependent D correction
+    (QDEP) integrates, otherwise the PAW section carries an 8-point stub."""
     chunks = []
     for it, s in enumerate(w.species):
         sp = SPECIES[s]
@@ -307,22 +337,27 @@ def write_potcar(path: str, w: Workload) -> None:
                 c.append(_fmt_block(np.zeros(N_NONLOCAL)))
             ich += npro
         c.append(" PAW radial sets")
-        nmax = 8
-        c.append(f"  {nmax}  1.50000000")
+        if with_partial_waves:
+            rgrid, wps, wae = partial_waves(sp, w.seed * 10 + it)
+        else:
+            rgrid = np.linspace(0.01, 1.5, 8)
+            wps = wae = [np.zeros(8)] * nch
+        nmax = rgrid.size
+        c.append(f"  {nmax}  {PAW_RMAX:.8f}")
         c.append(" (5E20.12)")
         c.append(" augmentation charges (non sperical)")
         c.append(_fmt_block(qpaw.ravel()))
         c.append(" grid")
-        c.append(_fmt_block(np.linspace(0.01, 1.5, nmax)))
+        c.append(_fmt_block(rgrid))
         c.append(" aepotential")
         c.append(_fmt_block(np.zeros(nmax)))
         c.append(" core charge-density")
         c.append(_fmt_block(np.zeros(nmax)))
         for j in range(nch):
             c.append(" pseudo wavefunction")
-            c.append(_fmt_block(np.zeros(nmax)))
+            c.append(_fmt_block(wps[j]))
             c.append(" ae wavefunction")
-            c.append(_fmt_block(np.zeros(nmax)))
+            c.append(_fmt_block(wae[j]))
         c.append(" End of Dataset")
         chunks.append("\n".join(c))
     with open(path, "w") as fh:
@@ -337,19 +372,21 @@ def sha256(path: str) -> str:
     return h.hexdigest()
 
 
-def make_inputs(name: str, outdir: str):
-    """Write CHGCAR/POTCAR for a workload into outdir (cached by content)."""
+def make_inputs(name: str, outdir: str, partial_waves: bool = False):
+    """Write CHGCAR/POTCAR for a workload into outdir (cached by content).
+    partial_waves selects the datasets with PAW partial waves (file
+    <name>.paw.POTCAR), needed by the potential-dependent D correction."""
     w = workload(name)
     os.makedirs(outdir, exist_ok=True)
     tag = w.extra.get("inputs", w.name)
     chg = os.path.join(outdir, f"{tag}.CHGCAR")
-    pot = os.path.join(outdir, f"{tag}.POTCAR")
+    pot = os.path.join(outdir, f"{tag}.paw.POTCAR" if partial_waves else f"{tag}.POTCAR")
     if not os.path.exists(chg):
         tmp = chg + f".tmp{os.getpid()}"
         write_chgcar(tmp, w)
         os.replace(tmp, chg)
     if not os.path.exists(pot):
         tmp = pot + f".tmp{os.getpid()}"
-        write_potcar(tmp, w)
+        write_potcar(tmp, w, partial_waves)
         os.replace(tmp, pot)
     return chg, pot, w

@@ -65,6 +65,13 @@ int half_amd_solve_acc(half_handle handle, const double kpoint[3], int nbands, i
                        HALF_COMPLEX *eigenvectors, int64_t ld, half_amd_solve_stats *stats,
                        char *error, int error_capacity);
 
+/* Residual history of the last half_amd_solve_acc on this context: the
+ * maximum residual norm (eV) over the requested bands at each iteration.
+ * count receives the number of iterations; HALF_ERROR_CAPACITY if capacity
+ * is smaller. */
+int half_amd_solve_history(half_handle handle, int capacity, double *residuals_eV, int32_t *count,
+                           char *error, int error_capacity);
+
 /* Select the solver used by half_solve_kpoint for this context (1..4). */
 int half_amd_set_solver(half_handle handle, int solver, char *error, int error_capacity);
 
@@ -87,6 +94,18 @@ int half_amd_nonlocal_forces(half_handle handle, const double kpoint[3], int nba
                              double kweight, const HALF_COMPLEX *vectors, int64_t ld,
                              double *forces, char *error, int error_capacity);
 
+/* Frozen-potential displacement: move the atoms seen by the projectors
+ * (and the force contraction) to `positions_fractional` (nions x 3, species
+ * order, fractional) while Veff, the basis and the energy terms keep the
+ * geometry the context was created with.  Operators built afterwards use
+ * the new positions; operator handles already created keep theirs.  The
+ * band energy sum_n f_n eps_n then varies with the positions exactly as the
+ * non-local force of half_amd_nonlocal_forces predicts, which is what a
+ * finite-difference check of that force needs.  No counterpart in the
+ * reference ABI. */
+int half_amd_set_projector_positions(half_handle handle, int nions, const double *positions_fractional,
+                                     char *error, int error_capacity);
+
 /* Launch-boundary timeline for measurement.  half_amd_trace(1) starts
  * recording an event before every operator/FFT kernel launch (clearing
  * earlier marks), half_amd_trace(0) stops.  half_amd_trace_read returns, per
@@ -97,8 +116,10 @@ int half_amd_trace_read(int max_kernels, char *names, double *total_ms, int64_t 
                         int32_t *nkinds, char *error, int error_capacity);
 
 /* Kernel tunables, e.g. "local.fp64.tile.x" (FFT points per workgroup tile
- * of the x/y/z local-stage passes) and "local.fp64.xgroup" (bands sharing a
- * Veff tile); "fp32" keys for the FP32 operator.  Process-wide. */
+ * of the x/y/z local-stage passes), "local.fp64.xgroup" (bands sharing a
+ * Veff tile in the transform-based x pass) and "local.fp64.xconv" (1: x pass
+ * as a per-column convolution, 0: transform-based); "fp32" keys for the
+ * FP32 operator.  Process-wide. */
 int half_amd_set_tuning(const char *key, int value, char *error, int error_capacity);
 int half_amd_get_tuning(const char *key, int *value, char *error, int error_capacity);
 

@@ -199,11 +199,36 @@ def read_potcar(path):
             dion[o:o + k, o:o + k] = blk
             o += k
         i += 1
-        i += 1  # nmax, rmax line
+        t = lines[i].split()  # nmax, paw_rmax
+        nmax, paw_rmax = int(t[0]), float(t[1])
         i += 1  # format line
-        i = nonblank(i)
+        i = nonblank(i + 1)
         flat, i = floats(i + 1, nch * nch)
-        p.update(lps=lps, dion=dion, pspnl=np.array(tables).T, qpaw=flat.reshape(nch, nch))
+        p.update(lps=lps, dion=dion, pspnl=np.array(tables).T, qpaw=flat.reshape(nch, nch),
+                 paw_rmax=paw_rmax, rgrid=np.zeros(nmax), wae=np.zeros((nmax, nch)), wps=np.zeros((nmax, nch)))
+        # PAW radial selectors (src/half_potcar.F90:171-205): keep the grid and
+        # the partial waves, skip the rest; the partial waves end the section
+        while True:
+            i = nonblank(i)
+            key = lines[i].strip().lower()
+            if "occupancies in atom" in key or "uccopancies in atom" in key:
+                _, i = floats(i + 1, nch * nch)
+            elif key == "grid":
+                p["rgrid"], i = floats(i + 1, nmax)
+            elif key in ("aepotential",) or "core charge-density" in key or "local pseudopotential core" in key \
+                    or "pspotential valence only" in key or "kinetic energy-density" in key:
+                _, i = floats(i + 1, nmax)
+            elif "pseudo wavefunction" in key:
+                for ich in range(nch):
+                    i = nonblank(i)
+                    p["wps"][:, ich], i = floats(i + 1, nmax)
+                    i = nonblank(i)
+                    p["wae"][:, ich], i = floats(i + 1, nmax)
+                break
+            elif key.startswith("t"):
+                _, i = floats(i + 1, nch * nch)
+            else:
+                raise ValueError("unknown POTCAR PAW selector: " + lines[i])
         while i < n and "End of Dataset" not in lines[i]:
             i += 1
         out.append(p)
@@ -378,9 +403,241 @@ def build_projectors(potcars, crystal, basis):
             ilfac = (-1j) ** l
             beta[:, ilm, :] = (pref * ilfac * rval * ylm)[None, :] * ephase.T
         out.append({"natoms": nat, "nlm": nlm, "beta": beta, "dij": d, "qij": qq,
-                    "dij_atom": np.repeat(d[None], nat, 0), "lm": lm})
+                    "dij_atom": np.repeat(d[None], nat, 0), "ddij_atom": np.zeros((nat, nlm, nlm, 3)), "lm": lm})
         ion0 += nat
     return out
+
+
+# --------------------------------------------------------------------------
+# potential-dependent D correction, QDEP (src/half_uspp.F90:112-397)
+# --------------------------------------------------------------------------
+
+def radial_weights(r):
+    """src/half_uspp.F90:296-311: composite Simpson weights on a non-uniform
+    grid (pairs of intervals), with a three-point end correction for an
+    even point count."""
+    n = r.size
+    w = np.zeros(n)
+    last = n if n % 2 == 1 else n - 1
+    for i in range(0, last - 2, 2):
+        h0, h1 = r[i + 1] - r[i], r[i + 2] - r[i + 1]
+        sc = (h0 + h1) / 6.0
+        w[i] += sc * (2 - h1 / h0)
+        w[i + 1] += sc * (h0 + h1) ** 2 / (h0 * h1)
+        w[i + 2] += sc * (2 - h0 / h1)
+    if n % 2 == 0:
+        h0, h1 = r[n - 2] - r[n - 3], r[n - 1] - r[n - 2]
+        w[n - 1] += (2 * h1 * h1 + 3 * h0 * h1) / (6 * (h0 + h1))
+        w[n - 2] += (h1 * h1 + 3 * h0 * h1) / (6 * h0)
+        w[n - 3] -= h1 ** 3 / (6 * h0 * (h0 + h1))
+    return w
+
+
+def sph_bessel(l, x):
+    """src/half_uspp.F90:313-323: power series for |x| < l + 2, upward
+    recurrence from j0, j1 otherwise (scalar)."""
+    if abs(x) < l + 2:
+        den = 1.0
+        for i in range(1, l + 1):
+            den *= 2 * i + 1
+        x2 = x * x
+        term = x ** l / den
+        v = term
+        for k in range(31):
+            term = -term * x2 / (2.0 * (k + 1) * (2 * l + 2 * k + 3))
+            v += term
+            if abs(term) < 1e-16 * max(1.0, abs(v)):
+                break
+        return v
+    jm = math.sin(x) / x
+    if l == 0:
+        return jm
+    j = math.sin(x) / (x * x) - math.cos(x) / x
+    for nn in range(1, l):
+        jm, j = j, (2 * nn + 1) * j / x - jm
+    return j
+
+
+def bessel_root(l, which):
+    """src/half_uspp.F90:325-336: scan in steps of 0.05 from 1e-4 for the
+    which-th sign change, then bisect to 1e-13."""
+    x0 = 1e-4
+    f0 = sph_bessel(l, x0)
+    found = 0
+    while found < which:
+        x1 = x0 + 0.05
+        f1 = sph_bessel(l, x1)
+        if f0 * f1 < 0:
+            found += 1
+            if found == which:
+                break
+        x0, f0 = x1, f1
+    while x1 - x0 > 1e-13:
+        root = 0.5 * (x0 + x1)
+        if f0 * sph_bessel(l, root) <= 0:
+            x1 = root
+        else:
+            x0 = root
+            f0 = sph_bessel(l, x0)
+    return 0.5 * (x0 + x1)
+
+
+def gauss_legendre(n):
+    """src/half_uspp.F90:364-376 (Newton iteration on P_n)."""
+    x, w = np.zeros(n), np.zeros(n)
+    for i in range(1, (n + 1) // 2 + 1):
+        z = math.cos(math.pi * (i - 0.25) / (n + 0.5))
+        while True:
+            p1, p2 = 1.0, 0.0
+            for j in range(1, n + 1):
+                p3, p2 = p2, p1
+                p1 = ((2 * j - 1) * z * p2 - (j - 1) * p3) / j
+            pp = n * (z * p1 - p2) / (z * z - 1)
+            z1 = z
+            z = z1 - p1 / pp
+            if abs(z - z1) < 1e-15:
+                break
+        x[i - 1], x[n - i] = -z, z
+        w[i - 1] = w[n - i] = 2 / ((1 - z * z) * pp * pp)
+    return x, w
+
+
+def compensation_coefficients(l, rc, z1, z2):
+    """src/half_uspp.F90:338-345: g_l = c1 j_l(z1 r/rc) + c2 j_l(z2 r/rc)
+    with zero slope at rc and unit l-th moment (32-point Gauss-Legendre)."""
+    def djl(x):
+        return -sph_bessel(1, x) if l == 0 else sph_bessel(l - 1, x) - (l + 1) / x * sph_bessel(l, x)
+    a11, a12 = z1 / rc * djl(z1), z2 / rc * djl(z2)
+    xg, wg = gauss_legendre(32)
+    rg, wg = 0.5 * rc * (xg + 1), 0.5 * rc * wg
+    a21 = sum(wg[i] * sph_bessel(l, z1 * rg[i] / rc) * rg[i] ** (l + 2) for i in range(32))
+    a22 = sum(wg[i] * sph_bessel(l, z2 * rg[i] / rc) * rg[i] ** (l + 2) for i in range(32))
+    det = a11 * a22 - a12 * a21
+    return -a12 / det, a11 / det
+
+
+def _ylm_x(l, m, x, phi):
+    """Real Ylm of src/half_uspp.F90:378-385 with x = cos(theta)."""
+    am = abs(m)
+    p = assoc_legendre(l, am, x)
+    norm = math.sqrt((2 * l + 1) / (4 * PI) * _factorial(l - am) / _factorial(l + am))
+    sgn = -1.0 if am % 2 else 1.0
+    if m > 0:
+        return math.sqrt(2.0) * sgn * norm * p * np.cos(am * phi)
+    if m < 0:
+        return math.sqrt(2.0) * sgn * norm * p * np.sin(am * phi)
+    return norm * p
+
+
+def gaunt_numeric(l1, m1, l2, m2, l3, m3):
+    """src/half_uspp.F90:354-362: 12 x 24 Gauss-Legendre x uniform-phi
+    quadrature of a product of three real Ylm."""
+    xg, wg = gauss_legendre(12)
+    g = 0.0
+    for it in range(12):
+        for ip in range(24):
+            phi = 2 * PI * ip / 24
+            g += wg[it] * 2 * PI / 24 * float(_ylm_x(l1, m1, xg[it], phi) * _ylm_x(l2, m2, xg[it], phi)
+                                               * _ylm_x(l3, m3, xg[it], phi))
+    return g
+
+
+def _bspline_weights(t):
+    """Cubic B-spline weights and derivatives (src/half_uspp.F90:278-294)."""
+    w = np.stack([(1 - t) ** 3 / 6, (3 * t ** 3 - 6 * t * t + 4) / 6, (-3 * t ** 3 + 3 * t * t + 3 * t + 1) / 6,
+                  t ** 3 / 6])
+    dw = np.stack([-0.5 * (1 - t) ** 2, 1.5 * t * t - 2 * t, -1.5 * t * t + t + 0.5, 0.5 * t * t])
+    return w, dw
+
+
+def _spline_sample(coeff, grid, frac, invlat):
+    """Periodic cubic B-spline value and Cartesian gradient at fractional
+    points (src/half_uspp.F90:244-276), vectorised over points."""
+    shape = np.array(grid)
+    coord = frac * shape
+    ix = np.floor(coord).astype(np.int64)
+    t = coord - ix
+    w, dw = zip(*(_bspline_weights(t[:, d]) for d in range(3)))
+    value = np.zeros(frac.shape[0])
+    df = np.zeros((frac.shape[0], 3))
+    for a in range(4):
+        ia = np.mod(ix[:, 0] + a - 1, grid[0])
+        for b in range(4):
+            ib = np.mod(ix[:, 1] + b - 1, grid[1])
+            for c in range(4):
+                ic = np.mod(ix[:, 2] + c - 1, grid[2])
+                v = coeff[ia, ib, ic]
+                value += w[0][a] * w[1][b] * w[2][c] * v
+                df[:, 0] += dw[0][a] * w[1][b] * w[2][c] * v * grid[0]
+                df[:, 1] += w[0][a] * dw[1][b] * w[2][c] * v * grid[1]
+                df[:, 2] += w[0][a] * w[1][b] * dw[2][c] * v * grid[2]
+    return value, df @ invlat.T
+
+
+def build_uspp_dij(veff, grid, potcars, crystal, paw):
+    """src/half_uspp.F90:112-242 (build_uspp_dij_cpu): for every atom
+    D_ab = DION_ab + sum_k M_ab,k int r^2 g_l(r) [sum_Omega Y_k V(R + r Omega)] dr
+    and dD_ab/dR from the spline gradient, both symmetrised; V is the
+    periodic cubic B-spline of Veff (FFT prefilter).  Updates paw in place
+    ("dij_atom", "ddij_atom")."""
+    n1, n2, n3 = grid
+    b = [(2.0 + np.cos(2 * PI * np.arange(nn) / nn)) / 3.0 for nn in grid]
+    bsym = b[0][:, None, None] * b[1][None, :, None] * b[2][None, None, :]
+    coeff = np.fft.ifftn(np.fft.fftn(np.asarray(veff, float).reshape(grid)) / bsym).real
+    invlat = np.linalg.inv(crystal.lattice)
+    xg, wg = gauss_legendre(12)
+    ion0 = 0
+    for it, (pc, sp) in enumerate(zip(potcars, paw)):
+        lm = [(ich, l, m) for ich, l in enumerate(pc["lps"]) for m in range(-l, l + 1)]
+        nlm = len(lm)
+        lmax = 2 * max(pc["lps"])
+        kl = [(l, m) for l in range(lmax + 1) for m in range(-l, l + 1)]
+        r = pc["rgrid"]
+        rc = pc["paw_rmax"]
+        rw = radial_weights(r)
+        gshape = np.zeros((r.size, lmax + 1))
+        for l in range(lmax + 1):
+            z1, z2 = bessel_root(l, 1), bessel_root(l, 2)
+            c1, c2 = compensation_coefficients(l, rc, z1, z2)
+            for ir in range(r.size):
+                if r[ir] <= rc:
+                    gshape[ir, l] = c1 * sph_bessel(l, z1 * r[ir] / rc) + c2 * sph_bessel(l, z2 * r[ir] / rc)
+        mult = np.zeros((nlm, nlm, len(kl)))
+        for a, (cha, la, ma) in enumerate(lm):
+            for bb, (chb, lb, mb) in enumerate(lm):
+                for k, (l, m) in enumerate(kl):
+                    if abs(la - lb) <= l <= la + lb and (la + lb + l) % 2 == 0:
+                        mom = np.sum(rw * (pc["wae"][:, cha] * pc["wae"][:, chb] - pc["wps"][:, cha] * pc["wps"][:, chb])
+                                     * r ** l)
+                        mult[a, bb, k] = gaunt_numeric(la, ma, lb, mb, l, m) * mom
+        dirs, yw = [], []
+        for itt in range(12):
+            for m in range(24):
+                phi = 2 * PI * m / 24
+                st = math.sqrt(max(0.0, 1 - xg[itt] * xg[itt]))
+                dirs.append([st * math.cos(phi), st * math.sin(phi), xg[itt]])
+                yw.append([float(_ylm_x(l, mm, xg[itt], phi)) * wg[itt] * 2 * PI / 24 for l, mm in kl])
+        dirs, yw = np.array(dirs), np.array(yw)  # (nang, 3), (nang, naug)
+        lk = np.array([l for l, _ in kl])
+        radial = rw * r * r  # (nrad,)
+        for iat in range(sp["natoms"]):
+            pos = crystal.positions[ion0 + iat] @ crystal.lattice
+            pts = pos[None, None, :] + r[:, None, None] * dirs[None, :, :]  # (nrad, nang, 3)
+            frac = np.mod(pts.reshape(-1, 3) @ invlat, 1.0)
+            val, grad = _spline_sample(coeff, grid, frac, invlat)
+            val = val.reshape(r.size, -1)
+            grad = grad.reshape(r.size, -1, 3)
+            vlm = val @ yw  # (nrad, naug)
+            rv = np.einsum("r,rk,rk->k", radial, gshape[:, lk], vlm)
+            d = sp["dij"] + mult @ rv
+            sp["dij_atom"][iat] = 0.5 * (d + d.T)
+            for al in range(3):
+                glm = grad[:, :, al] @ yw
+                rg = np.einsum("r,rk,rk->k", radial, gshape[:, lk], glm)
+                dd = mult @ rg
+                sp["ddij_atom"][iat, :, :, al] = 0.5 * (dd + dd.T)
+        ion0 += sp["natoms"]
+    return paw
 
 
 # --------------------------------------------------------------------------
@@ -632,9 +889,10 @@ def solve_dense(h, s, nbands=None, vectors=True):
 # --------------------------------------------------------------------------
 
 def nonlocal_forces(basis_q, paw, eigenvalues, occupations, kweight, vectors, nions=None):
-    """src/half_forces.F90:9-73 (projector term; dD/dR is zero without QDEP):
-    F_I,alpha = -2 w sum_n f_n Re[(M c_n)^H dc_n,alpha], M = D_I - eps_n Q,
-    c_a = sum_G conj(beta_a) psi_G, dc_a,alpha = sum_G i q_alpha conj(beta_a) psi_G."""
+    """src/half_forces.F90:9-73:
+    F_I,alpha = -2 w sum_n f_n Re[(M c_n)^H dc_n,alpha] - w sum_n f_n Re[c_n^H dD_I/dR_alpha c_n],
+    M = D_I - eps_n Q, c_a = sum_G conj(beta_a) psi_G,
+    dc_a,alpha = sum_G i q_alpha conj(beta_a) psi_G (dD/dR is zero without QDEP)."""
     nions = nions or sum(sp["natoms"] for sp in paw)
     forces = np.zeros((nions, 3))
     ion0 = 0
@@ -649,6 +907,9 @@ def nonlocal_forces(basis_q, paw, eigenvalues, occupations, kweight, vectors, ni
                 for al in range(3):
                     dc = bc @ (1j * basis_q[:, al] * vectors[:, ib])
                     forces[ion0 + iat, al] += -2.0 * kweight * f * float(np.vdot(mc, dc).real)
+                    dd = sp.get("ddij_atom")
+                    if dd is not None:
+                        forces[ion0 + iat, al] += -kweight * f * float(np.vdot(c[:, ib], dd[iat, :, :, al] @ c[:, ib]).real)
         ion0 += sp["natoms"]
     return forces
 

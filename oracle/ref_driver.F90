@@ -7,8 +7,13 @@
 ! "key value..." lines that tests/golden/make_golden.py turns into fixtures.
 !
 ! Usage:
-!   half_ref_driver solve CHGCAR POTCAR ENCUT lda|pbe NBANDS KX KY KZ VEFF_OUT
+!   half_ref_driver solve CHGCAR POTCAR ENCUT lda|pbe NBANDS KX KY KZ VEFF_OUT [uspp]
 !   half_ref_driver kat
+!
+! With "uspp" the potential-dependent D correction of src/half_uspp.F90
+! (build_uspp_dij_cpu, QDEP) is applied after the projectors are built, as
+! src/half_library.F90:277 does for a context created with use_uspp, and the
+! per-atom D and dD/dR are printed ("dij", "ddij").
 !
 ! "solve" mirrors the CPU context of src/half_library.F90:51-110 (create) and
 ! :329-388 (solve_kpoint) plus the Harris-energy assembly of
@@ -25,6 +30,7 @@ program half_ref_driver
   use half_paw, only: paw_species_t, build_paw_operators
   use half_dense_solver, only: assemble_dense_gamma, solve_dense_gamma
   use half_forces, only: add_nonlocal_paw_forces
+  use half_uspp, only: build_uspp_dij_cpu
   use half_energy, only: compute_occupations, ewald_energy, ewald_forces
   implicit none
   character(len=1024) :: mode
@@ -50,9 +56,9 @@ contains
     complex(dp), allocatable :: h(:,:), s(:,:), vectors(:,:), psi(:,:), hpsi(:,:), spsi(:,:)
     real(dp) :: encut, kpt(3), eh, exc, exv, smin, smax, mu, band, entropy, ewald, local_g0, atomic, nelect
     real(dp) :: weights(1)
-    integer :: nbands, i, j, it, ion, iat, unit, ib, ig
+    integer :: nbands, i, j, it, ion, iat, unit, ib, ig, a, b, al
     integer(i64) :: n
-    logical :: pbe
+    logical :: pbe, uspp
 
     call get_command_argument(2, chg_path); call get_command_argument(3, pot_path)
     call get_command_argument(4, arg); read(arg, *) encut
@@ -62,6 +68,7 @@ contains
       call get_command_argument(6 + i, arg); read(arg, *) kpt(i)
     end do
     call get_command_argument(10, veff_path)
+    call get_command_argument(11, arg); uspp = trim(arg) == 'uspp'
 
     call read_chgcar(trim(chg_path), crystal, charge)
     call read_potcar(trim(pot_path), potcars)
@@ -83,6 +90,23 @@ contains
     n = basis%npw
     write(*,'(A,I0)') 'npw ', n
     call build_paw_operators(potcars, crystal, basis, paw)
+    if (uspp) then
+      call build_uspp_dij_cpu(veff, charge%shape, potcars, crystal, paw)
+      ion = 0
+      do it = 1, size(paw)
+        do iat = 1, paw(it)%natoms
+          ion = ion + 1
+          do b = 1, paw(it)%nlm
+            do a = 1, paw(it)%nlm
+              write(*,'(A,3I5,ES25.17)') 'dij ', ion, a, b, paw(it)%dij_atom(iat, a, b)
+              do al = 1, 3
+                write(*,'(A,4I5,ES25.17)') 'ddij ', ion, a, b, al, paw(it)%ddij_atom(iat, a, b, al)
+              end do
+            end do
+          end do
+        end do
+      end do
+    end if
     call assemble_dense_gamma(veff, basis, paw, h, s)
     do j = 1, int(n), 41
       do i = 1, int(n), 37

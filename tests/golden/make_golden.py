@@ -8,7 +8,9 @@ stores its outputs as JSON.  Run from the repository root after
     python tests/golden/make_golden.py [case ...]
 
 Cases: si2_pbe, si2_lda, si2_k (PBE at k = (0.1, 0.2, 0.3)), hfo2_pbe, si8k_k
-(8-atom cell at k = (1/6, -1/3, 1/2) of the 6x6x6 mesh), kat.
+(8-atom cell at k = (1/6, -1/3, 1/2) of the 6x6x6 mesh), kat, and the
+potential-dependent D correction (QDEP, src/half_uspp.F90) on datasets with
+PAW partial waves: si2_qdep (k = (0.1, 0.2, 0.3)) and hfo2_qdep (Gamma).
 The si2 input files are committed next to this script; the hfo2 inputs are
 regenerated by the seeded generator and their sha256 is recorded so a test can
 refuse to compare against goldens made from different inputs.
@@ -29,12 +31,14 @@ from half_amd import synth  # noqa: E402
 DRIVER = os.path.join(ROOT, "oracle", "_ref", "half_ref_driver")
 INPUTS = os.path.join(HERE, "inputs")
 
-CASES = {
-    "si2_pbe": ("si2", "pbe", (0.0, 0.0, 0.0), 16),
-    "si2_lda": ("si2", "lda", (0.0, 0.0, 0.0), 16),
-    "si2_k": ("si2", "pbe", (0.1, 0.2, 0.3), 16),
-    "hfo2_pbe": ("hfo2", "pbe", (0.0, 0.0, 0.0), 64),
-    "si8k_k": ("si8k", "pbe", (1.0 / 6.0, -1.0 / 3.0, 0.5), 32),
+CASES = {  # workload, xc, k point, bands, QDEP
+    "si2_pbe": ("si2", "pbe", (0.0, 0.0, 0.0), 16, False),
+    "si2_lda": ("si2", "lda", (0.0, 0.0, 0.0), 16, False),
+    "si2_k": ("si2", "pbe", (0.1, 0.2, 0.3), 16, False),
+    "hfo2_pbe": ("hfo2", "pbe", (0.0, 0.0, 0.0), 64, False),
+    "si8k_k": ("si8k", "pbe", (1.0 / 6.0, -1.0 / 3.0, 0.5), 32, False),
+    "si2_qdep": ("si2", "pbe", (0.1, 0.2, 0.3), 16, True),
+    "hfo2_qdep": ("hfo2", "pbe", (0.0, 0.0, 0.0), 64, True),
 }
 
 _FLOAT = re.compile(r"^([-+]?\d+\.\d+)([-+]\d{3})$")
@@ -59,24 +63,26 @@ def parse(text):
 
 
 def run_case(name):
-    wl, xc, k, nb = CASES[name]
+    wl, xc, k, nb, qdep = CASES[name]
     if wl == "si2":
         chg = os.path.join(INPUTS, "si2.CHGCAR")
-        pot = os.path.join(INPUTS, "si2.POTCAR")
-        if not os.path.exists(chg):
+        pot = os.path.join(INPUTS, "si2.paw.POTCAR" if qdep else "si2.POTCAR")
+        if not os.path.exists(chg) or not os.path.exists(pot):
             os.makedirs(INPUTS, exist_ok=True)
-            chg, pot, _ = synth.make_inputs("si2", INPUTS)
+            chg, pot, _ = synth.make_inputs("si2", INPUTS, partial_waves=qdep)
     else:
-        chg, pot, _ = synth.make_inputs(wl, os.path.join(ROOT, "_work", "inputs"))
+        chg, pot, _ = synth.make_inputs(wl, os.path.join(ROOT, "_work", "inputs"), partial_waves=qdep)
     w = synth.workload(wl)
     veff_path = os.path.join(HERE, f"{name}.veff.bin.tmp")
     cmd = [DRIVER, "solve", chg, pot, repr(w.encut), xc, str(nb), *map(repr, k), veff_path]
+    if qdep:
+        cmd.append("uspp")
     text = subprocess.run(cmd, check=True, capture_output=True, text=True).stdout
     d = parse(text)
     veff = np.fromfile(veff_path, dtype="<f8")
     os.remove(veff_path)
     g = {
-        "workload": wl, "xc": xc, "kpoint": list(k), "nbands": nb, "encut": w.encut,
+        "workload": wl, "xc": xc, "kpoint": list(k), "nbands": nb, "encut": w.encut, "qdep": qdep,
         "chgcar_sha256": synth.sha256(chg), "potcar_sha256": synth.sha256(pot),
         "ngrid": int(d["ngrid"][0][0]), "npw": int(d["npw"][0][0]),
         "e_hartree": d["energy_terms"][0][0], "e_xc": d["energy_terms"][0][1],
@@ -96,7 +102,11 @@ def run_case(name):
         "veff_sum": float(veff.sum()), "veff_sumsq": float((veff * veff).sum()),
         "veff_min": float(veff.min()), "veff_max": float(veff.max()),
     }
-    if wl == "si2":
+    if qdep:
+        # per-ion D (one-based ion, a, b) and dD/dR (ion, a, b, alpha)
+        g["dij"] = [[int(r[0]), int(r[1]), int(r[2]), r[3]] for r in d["dij"]]
+        g["ddij"] = [[int(r[0]), int(r[1]), int(r[2]), int(r[3]), r[4]] for r in d["ddij"]]
+    if wl == "si2" and not qdep:
         # full C-order Veff grid and the three-column apply image
         np.save(os.path.join(HERE, f"{name}.veff.npy"), veff)
         ap = np.array(d["apply"])

@@ -117,6 +117,45 @@ def test_forces_adjoint(si2):
     assert np.abs(f - np.array(g["force_nl"])).max() < 1e-8
 
 
+def _fd_forces(ctx, g, k, nb, pairs, step=1e-4):
+    """Central differences of the band energy sum_n f_n eps_n (dense FP64
+    solve) under frozen-potential projector displacements."""
+    crystal = O.read_chgcar(ctx.chgcar)[0]
+    base = crystal.positions.copy()
+    inv = np.linalg.inv(crystal.lattice)
+    occ = np.array(g["occupations"][:nb])
+    out = {}
+    try:
+        for atom, axis in pairs:
+            e = []
+            for sgn in (1, -1):
+                pos = base.copy()
+                pos[atom] += sgn * step * inv[axis]
+                ctx.set_projector_positions(pos)
+                w = ctx.solve(nb, k, vectors=False)[0]
+                e.append(float(np.dot(occ, w)))
+            out[(atom, axis)] = (e[0] - e[1]) / (2 * step)
+    finally:
+        ctx.set_projector_positions(base)
+    return out
+
+
+def test_forces_finite_difference(si2):
+    """Adjoint projector forces equal minus the finite-difference derivative
+    of the band energy under frozen-potential displacements (h = 1e-4 A)."""
+    name, g, ctx = si2
+    if name != "si2_k":
+        pytest.skip("one k point suffices; si2_k has a clean gap above the occupied bands")
+    nb = 8
+    w, v, _, _ = ctx.solve(nb, g["kpoint"], vectors=True)
+    occ = np.array(g["occupations"][:nb])
+    f = ctx.nonlocal_forces(w, occ, v, g["kpoint"])
+    pairs = [(a, x) for a in range(2) for x in range(3)]
+    fd = _fd_forces(ctx, g, g["kpoint"], nb, pairs)
+    for (a, x), d in fd.items():
+        assert abs(d + f[a, x]) < 1e-7, (a, x, d, f[a, x])
+
+
 def test_fp32_operator_close_to_fp64(si2):
     from half_amd import hip
     name, g, ctx = si2
@@ -229,6 +268,18 @@ def test_hfo2_veff_and_dense(hfo2):
     w, _, smin, smax = ctx.solve(g["nbands"], vectors=False)
     assert np.abs(w - g["eigenvalues"]).max() < 1e-9
     assert abs(smin - g["overlap_min"]) < 1e-12 and abs(smax - g["overlap_max"]) < 1e-12
+
+
+def test_hfo2_forces_finite_difference(hfo2):
+    g, ctx = hfo2
+    nb = 32
+    w, v, _, _ = ctx.solve(nb, vectors=True)
+    occ = np.array(g["occupations"][:nb])
+    f = ctx.nonlocal_forces(w, occ, v)
+    assert np.abs(f - np.array(g["force_nl"])).max() < 1e-8
+    fd = _fd_forces(ctx, g, (0.0, 0.0, 0.0), nb, [(0, 0), (5, 2)])
+    for (a, x), d in fd.items():
+        assert abs(d + f[a, x]) < 1e-6, (a, x, d, f[a, x])
 
 
 def test_hfo2_acc_fp64_and_mixed(hfo2):
