@@ -22,6 +22,7 @@
 #include "common.hpp"
 #include "forces.hpp"
 #include "paw.hpp"
+#include "qdep.hpp"
 #include "solver.hpp"
 
 namespace half {
@@ -168,9 +169,6 @@ int create(const char *charge, const char *potential, double encut, int xc, int 
                           error, capacity);
     if (!file_exists(charge)) return export_error(kIo, "charge input does not exist", error, capacity);
     if (!file_exists(potential)) return export_error(kIo, "potential input does not exist", error, capacity);
-    if (use_uspp)
-      return export_error(kUnavailable, "MIMIC_US QDEP (use_uspp_dij) is not available in this backend", error,
-                          capacity);
     auto ctx = std::make_unique<Context>();
     read_chgcar(charge, ctx->sys.crystal, ctx->sys.charge);
     ctx->sys.sets = read_potcar(potential);
@@ -182,6 +180,7 @@ int create(const char *charge, const char *potential, double encut, int xc, int 
     ctx->sys.use_pbe = xc == 2;
     ctx->solver = solver;
     ctx->use_uspp = use_uspp != 0;
+    ctx->sys.use_qdep = ctx->use_uspp;
     require_device();
     HALF_HIP(hipStreamCreate(&ctx->stream));
     ctx->blas.init(ctx->stream);
@@ -700,6 +699,7 @@ int half_amd_set_projector_positions(half_handle handle, int nions, const double
     c.ops32.clear();
     c.sys.positions_cart = cart;
     c.sys.d_positions.upload(cart, c.stream);
+    if (c.sys.use_qdep) build_qdep(c.sys, c.stream);  // D depends on the atom positions
     HALF_HIP(hipStreamSynchronize(c.stream));
     return int(kSuccess);
   });

@@ -75,6 +75,10 @@ struct Dataset {
   std::vector<int> lps;                   // l of each channel
   std::vector<double> dion, qpaw;         // channels x channels, row-major
   std::vector<double> pspnl;              // channels x 100, channel-major
+  // PAW radial sets: augmentation-sphere radius, radial grid and the
+  // r-weighted pseudo / all-electron partial waves (channel-major, nmax each)
+  double paw_rmax = 0;
+  std::vector<double> rgrid, wps, wae;
   int channels() const { return static_cast<int>(lps.size()); }
   int nlm() const {
     int n = 0;

@@ -4,6 +4,7 @@
 //   c_n      = B^H psi_n
 //   dc_n,a   = B^H (i q_a psi_n)                      (a = x, y, z)
 //   F_I,a    = -2 w sum_n f_n Re[ ((D_I - eps_n Q) c_n)^H dc_n,a ]
+// plus, with QDEP, -w sum_n f_n Re[c_n^H dD_I/dR_a c_n]
 // (reference: src/half_forces.F90:9-73).  One fused matrix-core pass reads
 // B and psi once and accumulates the four contractions c, dc_x, dc_y, dc_z
 // (eight real v_mfma_f64_16x16x4f64 accumulators per tile), split over the
@@ -90,9 +91,10 @@ __global__ void adjoint_reduce_kernel(const double2 *partial, double2 *out, int6
 }
 
 // One workgroup per atom; threads over bands; fixed-order LDS reduction.
+// With QDEP (ddblk non-null) each band adds -w f Re[c^H dD/dR_alpha c].
 __global__ void force_kernel(const double2 *c4, const double *eps, const double *occ, double kweight,
                              const int *atom_first, const int *atom_nlm, const int *atom_boff, const double *dblk,
-                             const double *qblk, int np, int nb, double *forces) {
+                             const double *qblk, const double *ddblk, int np, int nb, double *forces) {
   __shared__ double red[3][256];
   const int at = blockIdx.x;
   const int first = atom_first[at], nlm = atom_nlm[at];
@@ -117,6 +119,22 @@ __global__ void force_kernel(const double2 *c4, const double *eps, const double 
       }
     }
     for (int al = 0; al < 3; ++al) f[al] += -2.0 * kweight * occ[b] * s[al];
+    if (ddblk) {
+      const double *dd = ddblk + 3 * int64_t(atom_boff[at]);
+      for (int al = 0; al < 3; ++al) {
+        double t = 0;
+        for (int a = 0; a < nlm; ++a) {
+          double2 m = make_double2(0, 0);
+          for (int k = 0; k < nlm; ++k) {
+            const double v = dd[(al * nlm + a) * nlm + k];
+            m.x += v * c[k].x;
+            m.y += v * c[k].y;
+          }
+          t += c[a].x * m.x + c[a].y * m.y;  // Re(conj(c_a) (dD c)_a)
+        }
+        f[al] += -kweight * occ[b] * t;
+      }
+    }
   }
   for (int al = 0; al < 3; ++al) red[al][threadIdx.x] = f[al];
   __syncthreads();
@@ -158,7 +176,7 @@ void nonlocal_forces(const KOperator<double> &op, int nb, const double2 *d_psi, 
   d_f.upload(zero, st);
   hipLaunchKernelGGL(force_kernel, dim3(sys.natoms), dim3(256), 0, st, c4.get(), d_eps.get(), d_occ.get(), kweight,
                      sys.d_atom_first.get(), sys.d_atom_nlm.get(), sys.d_atom_boff.get(), sys.d_dblk.get(),
-                     sys.d_qblk.get(), np, nb, d_f.get());
+                     sys.d_qblk.get(), sys.use_qdep ? sys.d_ddblk.get() : nullptr, np, nb, d_f.get());
   HALF_HIP(hipGetLastError());
   std::vector<double> f(size_t(sys.natoms) * 3);
   HALF_HIP(hipMemcpyAsync(f.data(), d_f.get(), sizeof(double) * f.size(), hipMemcpyDeviceToHost, st));

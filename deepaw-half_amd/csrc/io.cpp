@@ -298,13 +298,50 @@ void read_dataset(Cursor &c, const std::string &header, Dataset &p) {
   }
   p.pspnl.clear();
   for (auto &t : tables) p.pspnl.insert(p.pspnl.end(), t.begin(), t.end());
-  // PAW radial sets: only the augmentation charges are used here
-  // (reference: src/half_potcar.F90:148-159)
-  c.reals(2, "POTCAR PAW grid header");
+  // PAW radial sets: augmentation charges, radial grid and partial waves;
+  // the other radial tables are skipped (reference: src/half_potcar.F90:148-205)
+  auto hdr = c.reals(2, "POTCAR PAW grid header");
+  const int nmax = static_cast<int>(hdr[0]);
+  if (nmax < 1 || nmax > 100000) throw Error(kIo, "invalid POTCAR PAW grid size");
+  p.paw_rmax = hdr[1];
   c.line();  // format line
   if (lower(c.next_nonblank()).find("augmentation charges") == std::string::npos)
     throw Error(kIo, "expected augmentation charges");
   p.qpaw = c.reals(size_t(n) * n, "POTCAR QPAW");
+  p.rgrid.assign(nmax, 0.0);
+  p.wps.assign(size_t(nmax) * n, 0.0);
+  p.wae.assign(size_t(nmax) * n, 0.0);
+  while (true) {
+    std::string key = lower(c.next_nonblank());
+    const size_t b = key.find_first_not_of(" \t");
+    key = b == std::string::npos ? std::string() : key.substr(b);
+    while (!key.empty() && (key.back() == ' ' || key.back() == '\r' || key.back() == '\t')) key.pop_back();
+    if (key.find("occupancies in atom") != std::string::npos || key.find("uccopancies in atom") != std::string::npos) {
+      c.reals(size_t(n) * n, "POTCAR QATO");
+    } else if (key == "grid") {
+      p.rgrid = c.reals(nmax, "POTCAR radial grid");
+    } else if (key == "aepotential" || key.find("core charge-density") != std::string::npos ||
+               key.find("local pseudopotential core") != std::string::npos ||
+               key.find("pspotential valence only") != std::string::npos ||
+               key.find("kinetic energy-density") != std::string::npos) {
+      c.reals(nmax, "POTCAR PAW radial table");
+    } else if (key.find("pseudo wavefunction") != std::string::npos) {
+      for (int ich = 0; ich < n; ++ich) {
+        if (ich > 0 && lower(c.next_nonblank()).find("pseudo wavefunction") == std::string::npos)
+          throw Error(kIo, "expected POTCAR pseudo wavefunction");
+        auto ps = c.reals(nmax, "POTCAR pseudo wavefunction");
+        c.next_nonblank();  // ae wavefunction header
+        auto ae = c.reals(nmax, "POTCAR AE wavefunction");
+        std::copy(ps.begin(), ps.end(), p.wps.begin() + size_t(ich) * nmax);
+        std::copy(ae.begin(), ae.end(), p.wae.begin() + size_t(ich) * nmax);
+      }
+      break;
+    } else if (!key.empty() && key[0] == 't') {
+      c.reals(size_t(n) * n, "POTCAR total charge");
+    } else {
+      throw Error(kIo, "unknown POTCAR PAW selector");
+    }
+  }
   while (!c.eof()) {
     const std::string &l = c.line();
     if (l.find("End of Dataset") != std::string::npos) break;

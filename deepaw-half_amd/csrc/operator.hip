@@ -22,6 +22,7 @@
 #include "cplx.hpp"
 #include "mfma.hpp"
 #include "paw.hpp"
+#include "qdep.hpp"
 #include "trace.hpp"
 
 namespace half {
@@ -377,6 +378,7 @@ void System::build(hipStream_t s) {
   d_dblk32.upload(d32, s);
   d_qblk32.upload(q32, s);
   build_veff_device(*this, s);
+  if (use_qdep) build_qdep(*this, s);
   const int64_t nr = int64_t(fft.n[0]) * fft.n[1] * fft.n[2];
   veff_x.resize(size_t(nr));
   veff_x32.resize(size_t(nr));

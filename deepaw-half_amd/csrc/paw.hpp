@@ -44,6 +44,9 @@ struct System {
   std::vector<double> tables, tables_m2;                           // 100 points per channel
   std::vector<double> positions_cart;                              // natoms x 3
   std::vector<double> dblk, qblk;   // per atom nlm x nlm row-major (D and Q)
+  bool use_qdep = false;            // D of each atom corrected for Veff (QDEP)
+  std::vector<double> ddblk;        // with QDEP: dD/dR per atom, [alpha][a][b] at 3 x block offset
+  DeviceArray<double> d_ddblk;
   std::vector<int> atom_boff;       // offset of an atom's block in dblk/qblk
   DeviceArray<int> d_proj_atom, d_proj_lm, d_proj_l, d_proj_m, d_proj_table;
   DeviceArray<double> d_proj_gmax, d_tables, d_tables_m2, d_positions;

@@ -291,3 +291,76 @@ def test_hfo2_acc_fp64_and_mixed(hfo2):
     w, _, st = ctx.solve_acc(64, mixed=True, tol=1e-6, maxiter=300, extra=8)
     print("[gpu] hfo2 mixed acc", st, np.abs(w - ref).max(), flush=True)
     assert np.abs(w - ref).max() < 1e-8
+
+
+# --------------------------------------------------------------------------
+# potential-dependent D correction (QDEP) on the device
+# --------------------------------------------------------------------------
+
+SI2_PAW = ("tests/golden/inputs/si2.CHGCAR", "tests/golden/inputs/si2.paw.POTCAR")
+
+
+@pytest.fixture(scope="module")
+def si2q():
+    g = golden("si2_qdep")
+    print("[gpu] create si2_qdep with QDEP", flush=True)
+    ctx = H.Context(*SI2_PAW, g["encut"], g["xc"], H.BACKEND_CUDA, H.SOLVER_EVD, use_uspp=True)
+    yield g, ctx
+    ctx.close()
+
+
+def test_qdep_hs_and_eigenvalues(si2q):
+    g, ctx = si2q
+    k = g["kpoint"]
+    h, s = ctx.assemble_hs(k)
+    hs = np.array(g["hs_samples"])
+    i, j = hs[:, 0].astype(int) - 1, hs[:, 1].astype(int) - 1
+    assert np.abs(h[i, j] - (hs[:, 2] + 1j * hs[:, 3])).max() < 1e-9
+    assert np.abs(h.diagonal().real - g["h_diag"]).max() < 1e-9
+    w, v, smin, smax = ctx.solve(g["nbands"], k, vectors=True)
+    assert np.abs(w - g["eigenvalues"]).max() < 1e-9
+    # the correction moves the spectrum: the datasets without partial waves differ
+    assert np.abs(w - golden("si2_k")["eigenvalues"]).max() > 1e-3
+    occ = np.array(g["occupations"])
+    f = ctx.nonlocal_forces(w, occ, v, k)
+    assert np.abs(f - np.array(g["force_nl"])).max() < 1e-8
+
+
+def test_qdep_acc_and_mixed(si2q):
+    g, ctx = si2q
+    ref = np.array(g["eigenvalues"])
+    w, _, st = ctx.solve_acc(g["nbands"], g["kpoint"], tol=1e-7, maxiter=200)
+    assert np.abs(w - ref).max() < 1e-9
+    w, _, st = ctx.solve_acc(g["nbands"], g["kpoint"], mixed=True, tol=1e-6, maxiter=200)
+    assert np.abs(w - ref).max() < 1e-8
+
+
+def test_qdep_forces_finite_difference(si2q):
+    """With QDEP the force carries -w f Re[c^H dD/dR c]; moving an atom
+    recomputes its D at the new position, so central differences of the band
+    energy check both terms."""
+    g, ctx = si2q
+    k = g["kpoint"]
+    nb = 8
+    w, v, _, _ = ctx.solve(nb, k, vectors=True)
+    occ = np.array(g["occupations"][:nb])
+    f = ctx.nonlocal_forces(w, occ, v, k)
+    fd = _fd_forces(ctx, g, k, nb, [(0, 0), (1, 1), (1, 2)])
+    for (a, x), d in fd.items():
+        assert abs(d + f[a, x]) < 1e-6, (a, x, d, f[a, x])
+
+
+def test_qdep_hfo2(tmp_path_factory):
+    g = golden("hfo2_qdep")
+    from half_amd import synth
+    chg, pot, _ = synth.make_inputs("hfo2", str(tmp_path_factory.mktemp("hfo2q")), partial_waves=True)
+    require_same_inputs(g, chg, pot)
+    print("[gpu] create hfo2_qdep with QDEP", flush=True)
+    ctx = H.Context(chg, pot, g["encut"], g["xc"], H.BACKEND_CUDA, H.SOLVER_EVD, use_uspp=True)
+    try:
+        w, v, _, _ = ctx.solve(g["nbands"], vectors=True)
+        assert np.abs(w - g["eigenvalues"]).max() < 1e-9
+        f = ctx.nonlocal_forces(w, np.array(g["occupations"]), v)
+        assert np.abs(f - np.array(g["force_nl"])).max() < 1e-8
+    finally:
+        ctx.close()
