@@ -48,6 +48,9 @@ def main():
     ap.add_argument("--bands", type=int, default=0)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (the byte-counter calibration "
+                         "for profiles) and print their rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -57,6 +60,20 @@ def main():
     import numpy as np
     import half_amd as H
     from half_amd import hip, synth
+
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        src, dst = hip.Buffer((nbytes,), np.uint8), hip.Buffer((nbytes,), np.uint8)
+        stream = hip.Stream()
+        hip.memset(src, 1)
+        t0, t1 = hip.Event(), hip.Event()
+        t0.record(stream)
+        for _ in range(5):
+            hip.copy_async(dst, src, nbytes, stream)
+        t1.record(stream)
+        ms = t0.elapsed_ms(t1) / 5
+        print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": ms, "GBps": 2 * nbytes / ms / 1e6}))
+        return
 
     w = synth.workload(args.workload)
     nb = args.bands or w.nbands
@@ -192,6 +209,8 @@ def kernel_bytes(name, nr, grid, n, nlines, nplanes, nb, sc, sr):
         "local_y_bwd": line + plane,
         "local_x_veff": 2 * plane + nr * sr,
         "local_x_conv": 2 * plane,
+        "local_plane_bwd": n * (sc + 8) + plane,
+        "local_plane_fwd": plane + n * (3 * sc + sr + 8),
         "local_y_fwd": plane + line,
         "local_z_fwd": line + n * (3 * sc + sr + 8),
     }

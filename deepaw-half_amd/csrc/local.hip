@@ -22,6 +22,13 @@
 //   z_fwd   active lines: forward transform along i3, gather the plane-wave
 //           coefficients and add T psi (S psi = psi is written here too)
 //
+// Where a whole n2 x n3 plane fits in LDS, z and y are fused per active
+// plane instead (plane_bwd: scatter, z transforms of the plane's active
+// lines, y transforms of all its columns; plane_fwd: the reverse, ending in
+// the gather), and only the active planes are stored, compactly in signed
+// order, so the local stage is three kernels and each active plane crosses
+// HBM four times per band application.
+//
 // Box entries outside the active lines/planes are never read or written.
 // Numerically this is the same circular convolution as the reference's
 // scatter / inverse FFT / multiply / forward FFT / gather sequence
@@ -44,6 +51,10 @@ struct Tuning {
   int tile[2][3] = {{1024, 1536, 768}, {1536, 1536, 768}};  // [fp64|fp32][x|y|z]
   int xgroup[2] = {4, 8};
   int xconv[2] = {1, 1};  // x pass as a per-column convolution when it fits
+  // lines per LDS batch of the plane-fused z/y passes; off by default: with
+  // one plane per workgroup its transform batches keep too few threads busy
+  // (measured slower than the separate passes on the 48^3 box)
+  int plane[2] = {0, 0};
 };
 Tuning g_tune;
 
@@ -56,6 +67,7 @@ struct Tables {
   const int *plane_i1, *plane_on;
   const int *pw_i3;
   int plane_s0;  // first signed plane of the contiguous run (x_conv_kernel)
+  int compact;   // planes stored compactly: plane q of a band at (band P + q) n2 n3
 };
 
 // Elements per thread of a tile (M * n <= tile budget <= 256 * KMAX).
@@ -208,6 +220,141 @@ __global__ __launch_bounds__(256) void x_veff_kernel(cplx_t<R> *box, const R *ve
   }
 }
 
+// Plane-fused z and y passes.  Workgroup = one (band, active plane q).
+// LDS: the plane pb[n2][n3 + 1], two transform batches of M lines of
+// stride ldw = max(n2, n3) + 1, and both axes' twiddles.
+struct PlaneArgs {
+  const int *line0, *nline;  // lines of plane q (signed plane order)
+  int P, M, ldp, ldw;
+};
+
+template <typename R>
+__device__ inline void plane_lds(cplx_t<R> *smem, const PlaneArgs &pa, int n2, int n3, cplx_t<R> *&pb,
+                                 cplx_t<R> *&w0, cplx_t<R> *&w1, cplx_t<R> *&tz, cplx_t<R> *&ty) {
+  pb = smem;
+  w0 = pb + n2 * pa.ldp;
+  w1 = w0 + pa.M * pa.ldw;
+  tz = w1 + pa.M * pa.ldw;
+  ty = tz + n3;
+}
+
+// backward: psi -> plane q of `planes` (transform along i3 of the plane's
+// active lines, then along i2 of every column)
+template <typename R>
+__global__ __launch_bounds__(256) void plane_bwd_kernel(const cplx_t<R> *psi, long long ldpsi, cplx_t<R> *planes,
+                                                        Tables t, PlaneArgs pa, FftArgs az, FftArgs ay) {
+  using V = cplx_t<R>;
+  extern __shared__ double2 smem_raw[];
+  V *pb, *w0, *w1, *tz, *ty;
+  plane_lds<R>(reinterpret_cast<V *>(smem_raw), pa, t.n2, t.n3, pb, w0, w1, tz, ty);
+  load_twiddles(tz, az);
+  load_twiddles(ty, ay);
+  const int q = blockIdx.x % pa.P, band = blockIdx.x / pa.P;
+  const int n2 = t.n2, n3 = t.n3, M = pa.M;
+  for (int e = threadIdx.x; e < n2 * pa.ldp; e += blockDim.x) pb[e] = mk<V>(R(0), R(0));
+  const int l0 = pa.line0[q], nl = pa.nline[q];
+  const V *pband = psi + band * ldpsi;
+  for (int c0 = 0; c0 < nl; c0 += M) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < M * pa.ldw; e += blockDim.x) w0[e] = mk<V>(R(0), R(0));
+    __syncthreads();
+    for (int e = threadIdx.x; e < M * n3; e += blockDim.x) {
+      const int li = e / n3, j = e - li * n3;
+      if (c0 + li < nl) {
+        const int line = l0 + c0 + li;
+        if (j < t.line_count[line]) {
+          const int g = t.line_start[line] + j;
+          w0[li * pa.ldw + t.pw_i3[g]] = pband[g];
+        }
+      }
+    }
+    __syncthreads();
+    const V *res = run_stages(w0, w1, tz, az, +1);
+    for (int e = threadIdx.x; e < M * n3; e += blockDim.x) {
+      const int li = e / n3, j = e - li * n3;
+      if (c0 + li < nl) pb[(t.line_i12[l0 + c0 + li] % n2) * pa.ldp + j] = res[li * pa.ldw + j];
+    }
+  }
+  for (int j0 = 0; j0 < n3; j0 += M) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < M * n2; e += blockDim.x) {
+      const int ci = e / n2, i2 = e - ci * n2;
+      w0[ci * pa.ldw + i2] = j0 + ci < n3 ? pb[i2 * pa.ldp + j0 + ci] : mk<V>(R(0), R(0));
+    }
+    __syncthreads();
+    const V *res = run_stages(w0, w1, ty, ay, +1);
+    for (int e = threadIdx.x; e < M * n2; e += blockDim.x) {
+      const int ci = e / n2, i2 = e - ci * n2;
+      if (j0 + ci < n3) pb[i2 * pa.ldp + j0 + ci] = res[ci * pa.ldw + i2];
+    }
+  }
+  __syncthreads();
+  V *out = planes + (static_cast<long long>(band) * pa.P + q) * n2 * n3;
+  for (int e = threadIdx.x; e < n2 * n3; e += blockDim.x) {
+    const int i2 = e / n3, i3 = e - i2 * n3;
+    out[e] = pb[i2 * pa.ldp + i3];
+  }
+}
+
+// forward: plane q -> hpsi = (transform) + T psi, spsi = psi for the plane's
+// plane waves (transform along i2 of every column, then along i3 of the
+// active lines, then gather)
+template <typename R>
+__global__ __launch_bounds__(256) void plane_fwd_kernel(const cplx_t<R> *psi, long long ldpsi, const cplx_t<R> *planes,
+                                                        const R *kin, cplx_t<R> *hpsi, cplx_t<R> *spsi,
+                                                        long long ldout, Tables t, PlaneArgs pa, FftArgs az,
+                                                        FftArgs ay) {
+  using V = cplx_t<R>;
+  extern __shared__ double2 smem_raw[];
+  V *pb, *w0, *w1, *tz, *ty;
+  plane_lds<R>(reinterpret_cast<V *>(smem_raw), pa, t.n2, t.n3, pb, w0, w1, tz, ty);
+  load_twiddles(tz, az);
+  load_twiddles(ty, ay);
+  const int q = blockIdx.x % pa.P, band = blockIdx.x / pa.P;
+  const int n2 = t.n2, n3 = t.n3, M = pa.M;
+  const V *in = planes + (static_cast<long long>(band) * pa.P + q) * n2 * n3;
+  for (int e = threadIdx.x; e < n2 * n3; e += blockDim.x) {
+    const int i2 = e / n3, i3 = e - i2 * n3;
+    pb[i2 * pa.ldp + i3] = in[e];
+  }
+  for (int j0 = 0; j0 < n3; j0 += M) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < M * n2; e += blockDim.x) {
+      const int ci = e / n2, i2 = e - ci * n2;
+      w0[ci * pa.ldw + i2] = j0 + ci < n3 ? pb[i2 * pa.ldp + j0 + ci] : mk<V>(R(0), R(0));
+    }
+    __syncthreads();
+    const V *res = run_stages(w0, w1, ty, ay, -1);
+    for (int e = threadIdx.x; e < M * n2; e += blockDim.x) {
+      const int ci = e / n2, i2 = e - ci * n2;
+      if (j0 + ci < n3) pb[i2 * pa.ldp + j0 + ci] = res[ci * pa.ldw + i2];
+    }
+  }
+  const int l0 = pa.line0[q], nl = pa.nline[q];
+  const V *pband = psi + band * ldpsi;
+  for (int c0 = 0; c0 < nl; c0 += M) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < M * n3; e += blockDim.x) {
+      const int li = e / n3, j = e - li * n3;
+      w0[li * pa.ldw + j] = c0 + li < nl ? pb[(t.line_i12[l0 + c0 + li] % n2) * pa.ldp + j] : mk<V>(R(0), R(0));
+    }
+    __syncthreads();
+    const V *res = run_stages(w0, w1, tz, az, -1);
+    for (int e = threadIdx.x; e < M * n3; e += blockDim.x) {
+      const int li = e / n3, j = e - li * n3;
+      if (c0 + li >= nl) continue;
+      const int line = l0 + c0 + li;
+      if (j < t.line_count[line]) {
+        const int g = t.line_start[line] + j;
+        const V p = pband[g];
+        const V r = res[li * pa.ldw + t.pw_i3[g]];
+        hpsi[g + band * ldout] = mk<V>(r.x + kin[g] * p.x, r.y + kin[g] * p.y);
+        spsi[g + band * ldout] = p;
+      }
+    }
+  }
+}
+
 // x pass as a convolution over the active planes (see the header), for a
 // contiguous run of P planes s0 .. s0 + P - 1 (signed, modulo n1): with the
 // planes in that order, plane p' takes sum_p Vx[(p' - p) mod n1] in[p].  A
@@ -239,13 +386,14 @@ __global__ __launch_bounds__(256) void x_conv_kernel(cplx_t<R> *box, const cplx_
   const int c = c0 + ci;
   const int band = blockIdx.y * kXcBands + wave * 2 + (lane >> 5);
   if (c >= C || band >= nb) return;  // no barrier follows
-  V *col = box + static_cast<long long>(band) * t.nr + c;
+  V *col = box + (t.compact ? static_cast<long long>(band) * P * C : static_cast<long long>(band) * t.nr) + c;
   const int s0 = t.plane_s0 < 0 ? t.plane_s0 + t.n1 : t.plane_s0;
   V in[PMAX];
 #pragma unroll
   for (int q = 0; q < PMAX; ++q) {
     int i1 = s0 + q;
     i1 -= i1 >= t.n1 ? t.n1 : 0;
+    if (t.compact) i1 = q;
     in[q] = q < P ? col[static_cast<long long>(i1) * C] : mk<V>(R(0), R(0));
   }
   for (int op = 0; op < P; ++op) {
@@ -261,6 +409,7 @@ __global__ __launch_bounds__(256) void x_conv_kernel(cplx_t<R> *box, const cplx_
     }
     int i1 = s0 + op;
     i1 -= i1 >= t.n1 ? t.n1 : 0;
+    if (t.compact) i1 = op;
     col[static_cast<long long>(i1) * C] = mk<V>(ar, ai);
   }
 }
@@ -401,14 +550,34 @@ void LocalMap::build(const Basis &b, hipStream_t s) {
   std::vector<int> planes;
   for (int i = 0; i < n1; ++i)
     if (plane_on_h[i]) planes.push_back(i);
-  // signed plane indices in (-n1/2, n1/2]: one contiguous run for a sphere
+  // signed plane indices in [-n1/2, (n1-1)/2], the basis's enumeration
+  // range: one contiguous run for a sphere
+  auto signed_of = [&](int i) { return i <= (n1 - 1) / 2 ? i : i - n1; };
   std::vector<int> signed_planes;
-  for (int i : planes) signed_planes.push_back(i > n1 / 2 ? i - n1 : i);
+  for (int i : planes) signed_planes.push_back(signed_of(i));
   std::sort(signed_planes.begin(), signed_planes.end());
   planes_contiguous = !signed_planes.empty();
   for (size_t q = 1; q < signed_planes.size(); ++q)
     if (signed_planes[q] != signed_planes[0] + static_cast<int>(q)) planes_contiguous = false;
   plane_s0 = signed_planes.empty() ? 0 : signed_planes[0];
+  // lines of each plane in signed plane order; the basis enumerates the
+  // signed i1 outermost, so each plane's lines form one run
+  std::vector<int> q_of(n1, -1);
+  for (size_t q = 0; q < signed_planes.size(); ++q) q_of[(signed_planes[q] % n1 + n1) % n1] = static_cast<int>(q);
+  std::vector<int> qline0(signed_planes.size(), -1), qnline(signed_planes.size(), 0);
+  lines_by_plane = true;
+  int prev_q = -1;
+  for (size_t l = 0; l < lines.size(); ++l) {
+    const int q = q_of[lines[l] / n2];
+    if (q != prev_q) {
+      if (q < prev_q || qline0[q] >= 0) lines_by_plane = false;
+      qline0[q] = static_cast<int>(l);
+      prev_q = q;
+    }
+    ++qnline[q];
+  }
+  qplane_line0.upload(qline0, s);
+  qplane_nline.upload(qnline, s);
   nlines = static_cast<int>(lines.size());
   nplanes = static_cast<int>(planes.size());
   line_i12.upload(lines, s);
@@ -450,17 +619,62 @@ template <typename R>
 void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const cplx_t<R> *vx, const R *kin, int nb,
                  const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi, int64_t ldout,
                  cplx_t<R> *boxes, hipStream_t s) {
+  using V = cplx_t<R>;
   if (nb <= 0 || m.nlines == 0) return;
   if (static_cast<long long>(nb) * (static_cast<long long>(m.n1) * m.n2 + static_cast<long long>(m.n2) * m.n3 +
                                    static_cast<long long>(m.n1) * m.n3) >= (1ll << 31))
     throw Error(kInvalidArgument, "band batch too large for the local stage");
   Tables t{m.n1, m.n2, m.n3, static_cast<long long>(m.n1) * m.n2 * m.n3, m.nlines, m.line_i12.get(),
            m.line_start.get(), m.line_count.get(), m.line_of.get(), m.nplanes, m.plane_i1.get(), m.plane_on.get(),
-           m.pw_i3.get(), m.plane_s0};
+           m.pw_i3.get(), m.plane_s0, 0};
   long long lp = ldpsi, lo = ldout;
+  const int prec = std::is_same<R, double>::value ? 0 : 1;
+  const int C = m.n2 * m.n3;
+
+  // x pass form: per-column convolution where the active planes are one run
+  // and fit the register budget, else transform / multiply / transform
+  int pmax = 0;
+  const size_t xconv_lds = size_t(2 * m.nplanes - 1) * kXcCols * sizeof(V);
+  const bool xconv_ok = vx && g_tune.xconv[prec] && m.planes_contiguous && m.nplanes <= m.n1 && xconv_lds <= 65536;
+  const void *xconv = xconv_ok ? x_conv_for<R>(m.nplanes, pmax) : nullptr;
+
+  // z and y fused per plane when a plane and two transform batches fit LDS
+  const int ldp = m.n3 + 1, ldw = std::max(m.n2, m.n3) + 1;
+  int Mp = g_tune.plane[prec];
+  auto plane_bytes = [&](int M) { return (size_t(m.n2) * ldp + 2 * size_t(M) * ldw + m.n2 + m.n3) * sizeof(V); };
+  while (Mp > 1 && plane_bytes(Mp) > 65536) Mp /= 2;
+  const bool fused = xconv && Mp >= 1 && m.lines_by_plane && plane_bytes(Mp) <= 65536;
+
+  if (fused) {
+    t.compact = 1;
+    FftArgs az = axis_args<R>(plan, 2, 0, 1), ay = axis_args<R>(plan, 1, 0, m.n3);
+    az.M = ay.M = Mp;
+    az.ld = ay.ld = ldw;
+    PlaneArgs pa{m.qplane_line0.get(), m.qplane_nline.get(), m.nplanes, Mp, ldp, ldw};
+    const dim3 grid(static_cast<unsigned>(int64_t(m.nplanes) * nb));
+    const size_t lds = plane_bytes(Mp);
+    {
+      void *args[] = {&psi, &lp, &boxes, &t, &pa, &az, &ay};
+      trace_mark("local_plane_bwd", s);
+      HALF_HIP(hipLaunchKernel(reinterpret_cast<const void *>(&plane_bwd_kernel<R>), grid, dim3(256), args, lds, s));
+    }
+    {
+      int nbi = nb;
+      void *args[] = {&boxes, &vx, &t, &nbi};
+      trace_mark("local_x_conv", s);
+      HALF_HIP(hipLaunchKernel(xconv, dim3((C + kXcCols - 1) / kXcCols, (nb + kXcBands - 1) / kXcBands), dim3(256),
+                               args, xconv_lds, s));
+    }
+    {
+      void *args[] = {&psi, &lp, &boxes, &kin, &hpsi, &spsi, &lo, &t, &pa, &az, &ay};
+      trace_mark("local_plane_fwd", s);
+      HALF_HIP(hipLaunchKernel(reinterpret_cast<const void *>(&plane_fwd_kernel<R>), grid, dim3(256), args, lds, s));
+    }
+    return;
+  }
+
   FftArgs az = axis_args<R>(plan, 2, static_cast<long long>(nb) * m.nlines, 1);
   FftArgs ay = axis_args<R>(plan, 1, static_cast<long long>(nb) * m.nplanes * m.n3, m.n3);
-  const int prec = std::is_same<R, double>::value ? 0 : 1;
   int G = std::max(1, std::min(g_tune.xgroup[prec], nb));
   while (G > 1 && (G & (G - 1))) --G;  // power of two, divides M
   FftArgs ax = axis_args<R>(plan, 0, static_cast<long long>(nb) * m.n2 * m.n3, m.n2 * m.n3, G);
@@ -476,18 +690,12 @@ void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const cp
     void *args[] = {&boxes, &t, &ay};
     launch<R>("local_y_bwd", reinterpret_cast<const void *>(&y_kernel<R, true>), ay, s, args);
   }
-  int pmax = 0;
-  const size_t xconv_lds = size_t(2 * m.nplanes - 1) * kXcCols * sizeof(cplx_t<R>);
-  const bool xconv_ok = vx && g_tune.xconv[prec] && m.planes_contiguous && m.nplanes <= m.n1 && xconv_lds <= 65536;
-  const void *xconv = xconv_ok ? x_conv_for<R>(m.nplanes, pmax) : nullptr;
   if (xconv) {
     int nbi = nb;
     void *args[] = {&boxes, &vx, &t, &nbi};
-    const int C = m.n2 * m.n3;
-    const size_t lds = xconv_lds;
     trace_mark("local_x_conv", s);
     HALF_HIP(hipLaunchKernel(xconv, dim3((C + kXcCols - 1) / kXcCols, (nb + kXcBands - 1) / kXcBands), dim3(256),
-                             args, lds, s));
+                             args, xconv_lds, s));
   } else {
     int nbi = nb;
     void *args[] = {&boxes, &veff, &scale, &t, &ax, &G, &nbi};
@@ -524,6 +732,11 @@ bool set_local_tuning(const std::string &key, int value) {
       g_tune.xconv[p] = value;
       return true;
     }
+    if (key == pre + ".plane") {
+      if (value < 0 || value > 64 || (value & (value - 1))) return false;
+      g_tune.plane[p] = value;
+      return true;
+    }
   }
   return false;
 }
@@ -543,6 +756,10 @@ bool get_local_tuning(const std::string &key, int &value) {
     }
     if (key == pre + ".xconv") {
       value = g_tune.xconv[p];
+      return true;
+    }
+    if (key == pre + ".plane") {
+      value = g_tune.plane[p];
       return true;
     }
   }

@@ -16,7 +16,11 @@ struct LocalMap {
   // nplanes - 1 (modulo n1) when planes_contiguous (true for a sphere)
   int plane_s0 = 0;
   bool planes_contiguous = false;
+  // lines of each active plane in signed order q (plane q holds lines
+  // qplane_line0[q] .. + qplane_nline[q] - 1); valid when lines_by_plane
+  bool lines_by_plane = false;
   DeviceArray<int> line_i12, line_start, line_count, line_of, plane_i1, plane_on, pw_i3;
+  DeviceArray<int> qplane_line0, qplane_nline;
   void build(const Basis &b, hipStream_t s);
 };
 
@@ -34,9 +38,12 @@ void build_veff_x_dft(const double *veff, int n1, int n2, int n3, double2 *vx, f
 
 // Tunables of the local stage, keys "local.{fp64|fp32}.tile.{x|y|z}" (FFT
 // points per workgroup tile), "local.{fp64|fp32}.xgroup" (bands sharing a
-// Veff tile in the transform-based x pass) and "local.{fp64|fp32}.xconv"
+// Veff tile in the transform-based x pass), "local.{fp64|fp32}.xconv"
 // (1: x pass as a per-column convolution where the active planes fit, 0:
-// transform-based).  Return false for an unknown key or an out-of-range value.
+// transform-based) and "local.{fp64|fp32}.plane" (0: separate z and y
+// passes over full boxes; M > 0: z and y fused per active plane with M
+// lines per LDS transform batch, planes stored compactly).  Return false for
+// an unknown key or an out-of-range value.
 bool set_local_tuning(const std::string &key, int value);
 bool get_local_tuning(const std::string &key, int &value);
 

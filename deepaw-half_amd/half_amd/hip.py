@@ -32,6 +32,7 @@ def lib():
         _hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
         _hip.hipEventDestroy.argtypes = [C.c_void_p]
         _hip.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
+        _hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
     return _hip
 
 
@@ -47,6 +48,17 @@ def device_count() -> int:
 
 def synchronize():
     _ok(lib().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+def memset(buf: "Buffer", value: int):
+    _ok(lib().hipMemset(buf.ptr, int(value), buf.nbytes), "hipMemset")
+
+
+def copy_async(dst: "Buffer", src: "Buffer", nbytes: int, stream: "Stream"):
+    """Device-to-device copy on a stream."""
+    if nbytes > min(dst.nbytes, src.nbytes):
+        raise ValueError("copy larger than the buffers")
+    _ok(lib().hipMemcpyAsync(dst.ptr, src.ptr, nbytes, D2D, stream.ptr), "hipMemcpyAsync")
 
 
 class Buffer:
