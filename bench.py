@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--bands", type=int, default=0)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--extra", type=int, default=8, help="buffer bands of the all-band solves")
+    ap.add_argument("--tol", type=float, default=1e-5, help="residual tolerance (eV) of the all-band solves")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the byte-counter calibration "
                          "for profiles) and print their rate")
@@ -126,6 +128,7 @@ def main():
     nproj = ctx.operator(k, 64).nproj
 
     solve = {}
+    evals = {}
     if not args.no_solve:
         g = None
         gpath = os.path.join(ROOT, "tests", "golden", f"{w.extra.get('inputs', w.name)}_pbe.json")
@@ -135,15 +138,15 @@ def main():
         # warm-up: first-call library initialisation (rocBLAS/rocSOLVER kernels,
         # operator caches) stays out of the timed solves
         for mixed in (False, True):
-            ctx.solve_acc(nb, k, mixed=mixed, tol=1e-5, maxiter=2, extra=8)
+            ctx.solve_acc(nb, k, mixed=mixed, tol=args.tol, maxiter=2, extra=args.extra)
         for mixed in (False, True):
             t = time.perf_counter()
-            wv, _, st = ctx.solve_acc(nb, k, mixed=mixed, tol=1e-5, maxiter=400, extra=8)
+            wv, _, st = ctx.solve_acc(nb, k, mixed=mixed, tol=args.tol, maxiter=400, extra=args.extra)
             el = time.perf_counter() - t
             key = "mixed" if mixed else "fp64"
             # the same solve again on the launch-boundary timeline: where the time goes
             H.trace(True)
-            ctx.solve_acc(nb, k, mixed=mixed, tol=1e-5, maxiter=400, extra=8)
+            ctx.solve_acc(nb, k, mixed=mixed, tol=args.tol, maxiter=400, extra=args.extra)
             phases = H.trace_read()
             H.trace(False)
             solve[key] = {"seconds": el, "iterations": st["iterations"], "fp64_iterations": st["fp64_iterations"],
@@ -152,13 +155,18 @@ def main():
             if g is not None and len(g["eigenvalues"]) >= nb:
                 solve[key]["max_abs_err_vs_reference_eV"] = float(np.abs(wv - np.array(g["eigenvalues"][:nb])).max())
             print(f"bench: {key} all-band solve {el:.3f} s {solve[key]}", file=sys.stderr, flush=True)
+            evals[key] = wv
         if "fp64" in solve and "mixed" in solve:
             solve["speedup_mixed_vs_fp64"] = solve["fp64"]["seconds"] / solve["mixed"]["seconds"]
+            solve["max_abs_diff_mixed_vs_fp64_eV"] = float(np.abs(evals["mixed"] - evals["fp64"]).max())
+        solve["controls"] = {"tol_eV": args.tol, "extra_bands": args.extra, "maxiter": 400}
 
     gv, _ = ctx.basis(k)
     nlines, nplanes = sphere_counts(gv, info["grid"])
-    roofline = kernel_roofline(kernels[64], nr, info["grid"], n, nproj, nb, args.steps, nlines, nplanes, True)
-    roofline32 = kernel_roofline(kernels[32], nr, info["grid"], n, nproj, nb, args.steps, nlines, nplanes, False)
+    roofline = kernel_roofline(kernels[64], nr, info["grid"], n, nproj, nb, args.steps, nlines, nplanes, True,
+                               args.workload)
+    roofline32 = kernel_roofline(kernels[32], nr, info["grid"], n, nproj, nb, args.steps, nlines, nplanes, False,
+                                 args.workload)
     b_alg = algorithmic_bytes(nr, n, True)
     f_proj = 24.0 * n * nproj
     t_band = per_step[64] * 1e-3 / nb
@@ -219,7 +227,21 @@ def kernel_bytes(name, nr, grid, n, nlines, nplanes, nb, sc, sr):
     return None if name not in table else nb * table[name] + once.get(name, 0)
 
 
-def kernel_roofline(kern, nr, grid, n, nproj, nb, steps, nlines, nplanes, fp64=True):
+def measured_traffic(workload, nb, fp64, name):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (profiles/hbm_traffic_<workload>.json: FETCH_SIZE / WRITE_SIZE scaled by
+    a copy-kernel calibration) when it covers this workload and band count."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    if t.get("bands") != nb:
+        return None
+    return t["bytes_per_launch"].get("fp64" if fp64 else "fp32", {}).get(name)
+
+
+def kernel_roofline(kern, nr, grid, n, nproj, nb, steps, nlines, nplanes, fp64=True, workload=None):
     """Roofline of the kernel with the largest share of the traced time:
     its algorithmic bytes (or flops) over its summed launch durations."""
     sc, sr = (16, 8) if fp64 else (8, 4)
@@ -236,7 +258,8 @@ def kernel_roofline(kern, nr, grid, n, nproj, nb, steps, nlines, nplanes, fp64=T
     b = kernel_bytes(name, nr, grid, n, nlines, nplanes, nb, sc, sr)
     ach = b * steps / t / 1e9 if b else None
     out.update(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-               frac=(ach / HBM_PEAK_GBS) if ach else None, traffic=None,
+               frac=(ach / HBM_PEAK_GBS) if ach else None,
+               traffic=measured_traffic(workload, nb, fp64, name) if workload else None,
                bytes_per_launch=(b * steps / launches) if b else None)
     return out
 

@@ -371,3 +371,37 @@ def test_qdep_hfo2(tmp_path_factory):
         assert np.abs(f - np.array(g["force_nl"])).max() < 1e-8
     finally:
         ctx.close()
+
+
+def test_si8k_kpoint(tmp_path_factory):
+    """8-atom cubic cell at k = (1/6, -1/3, 1/2) of the 6x6x6 mesh, 40^3 box
+    (radix-5 transforms), 5896 plane waves: Veff terms, H/S samples, dense
+    eigenvalues, FP64 and mixed all-band solves, forces."""
+    g = golden("si8k_k")
+    chg, pot = workload_files("si8k", str(tmp_path_factory.mktemp("si8k")))
+    require_same_inputs(g, chg, pot)
+    print("[gpu] create si8k_k", flush=True)
+    ctx = H.Context(chg, pot, g["encut"], g["xc"], H.BACKEND_CUDA, H.SOLVER_EVD)
+    try:
+        k = g["kpoint"]
+        eh, exc, exv = ctx.energy_terms()
+        assert abs(eh - g["e_hartree"]) < 1e-9 and abs(exc - g["e_xc"]) < 1e-9 and abs(exv - g["e_xc_potential"]) < 1e-9
+        assert ctx.basis_size(k) == g["npw"]
+        h, s = ctx.assemble_hs(k)
+        hs = np.array(g["hs_samples"])
+        i, j = hs[:, 0].astype(int) - 1, hs[:, 1].astype(int) - 1
+        assert np.abs(h[i, j] - (hs[:, 2] + 1j * hs[:, 3])).max() < 1e-9
+        del h, s
+        ref = np.array(g["eigenvalues"])
+        w, v, smin, smax = ctx.solve(g["nbands"], k, vectors=True)
+        assert np.abs(w - ref).max() < 1e-9
+        f = ctx.nonlocal_forces(w, np.array(g["occupations"]), v, k)
+        assert np.abs(f - np.array(g["force_nl"])).max() < 1e-8
+        w, _, st = ctx.solve_acc(g["nbands"], k, tol=1e-6, maxiter=300, extra=8)
+        print("[gpu] si8k fp64 acc", st, np.abs(w - ref).max(), flush=True)
+        assert np.abs(w - ref).max() < 1e-9
+        w, _, st = ctx.solve_acc(g["nbands"], k, mixed=True, tol=1e-6, maxiter=300, extra=8)
+        print("[gpu] si8k mixed acc", st, np.abs(w - ref).max(), flush=True)
+        assert np.abs(w - ref).max() < 1e-8
+    finally:
+        ctx.close()
