@@ -38,6 +38,7 @@ struct GramArgs {
   int64_t ldo[2];
   int64_t chunk_stride;  // distance between the partial tiles of consecutive chunks
   int n, ka, kb, chunk, tiles_a;
+  int upper;             // grid enumerates only tiles with ta <= tb (tiles_a per side)
 };
 
 template <typename R, int WT, int NB>
@@ -53,7 +54,17 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs<R> g) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int row = lane & 15, kq = lane >> 4;
   const int wa = (wave & 1) * 16 * WT, wb = (wave >> 1) * 16 * WT;
-  const int a0 = (blockIdx.x % g.tiles_a) * TW, b0 = (blockIdx.x / g.tiles_a) * TW;
+  int ta = blockIdx.x % g.tiles_a, tb = blockIdx.x / g.tiles_a;
+  if (g.upper) {  // linear index over the upper-triangular tile pairs, column by column
+    int t = blockIdx.x;
+    tb = 0;
+    while (t > tb) {
+      t -= tb + 1;
+      ++tb;
+    }
+    ta = t;
+  }
+  const int a0 = ta * TW, b0 = tb * TW;
   const int r_begin = blockIdx.y * g.chunk;
   const int r_end = min(g.n, r_begin + g.chunk);
   // loader: column lc of the tile, rows lr .. lr + L - 1 of the slab
@@ -126,6 +137,17 @@ __global__ void gram_reduce_kernel(const V *partial, int64_t stride, int nchunks
   for (int c = 1; c < nchunks; ++c) acc = cadd(acc, partial[c * stride + t]);
   const int b = static_cast<int>(t / ka), p = static_cast<int>(t - int64_t(b) * ka);
   C[p + int64_t(b) * ldc] = acc;
+}
+
+// C[j + i ldc] = conj(C[i + j ldc]) for i < j (lower triangle from the upper)
+template <typename V>
+__global__ void mirror_lower_kernel(V *C, int64_t ldc, int k) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(k) * k) return;
+  const int j = static_cast<int>(t / k), i = static_cast<int>(t - int64_t(j) * k);
+  if (i <= j) return;  // (i, j) strictly lower: row i > column j
+  const V u = C[j + int64_t(i) * ldc];
+  C[i + int64_t(j) * ldc] = mk<V>(u.x, -u.y);
 }
 
 template <typename R>
@@ -242,9 +264,10 @@ void launch_tall(dim3 grid, const TallArgs<R> &a, hipStream_t s) {
 template <typename R>
 void gram(int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, const cplx_t<R> *B, int64_t ldb, cplx_t<R> *C,
           int64_t ldc, const cplx_t<R> *B2, int64_t ldb2, cplx_t<R> *C2, int64_t ldc2,
-          DeviceArray<cplx_t<R>> &work, hipStream_t s) {
+          DeviceArray<cplx_t<R>> &work, hipStream_t s, bool hermitian) {
   using V = cplx_t<R>;
   if (ka <= 0 || kb <= 0) return;
+  if (hermitian && ka != kb) throw Error(kInvalidArgument, "gram: a Hermitian result must be square");
   const int nb = B2 ? 2 : 1;
   if (n <= 0) {
     HALF_HIP(hipMemset2DAsync(C, sizeof(V) * ldc, 0, sizeof(V) * ka, kb, s));
@@ -255,7 +278,8 @@ void gram(int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, const cplx_t<R
     throw Error(kInvalidArgument, "gram: leading dimension too small");
   const int wt = std::max(ka, kb) <= 32 ? 1 : 2;
   const int tw = 32 * wt;
-  const int tiles_a = (ka + tw - 1) / tw, tiles = tiles_a * ((kb + tw - 1) / tw);
+  const int tiles_a = (ka + tw - 1) / tw;
+  const int tiles = hermitian ? tiles_a * (tiles_a + 1) / 2 : tiles_a * ((kb + tw - 1) / tw);
   // about a thousand workgroups, each chunk at least 64 rows
   int nchunks = std::max(1, std::min({1024 / tiles, (n + 63) / 64, 64}));
   int chunk = (n + nchunks - 1) / nchunks;
@@ -273,6 +297,7 @@ void gram(int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, const cplx_t<R
   a.kb = kb;
   a.chunk = chunk;
   a.tiles_a = tiles_a;
+  a.upper = hermitian ? 1 : 0;
   const int64_t tile_elems = int64_t(ka) * kb;
   if (nchunks == 1) {
     a.out[0] = C;
@@ -300,6 +325,12 @@ void gram(int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, const cplx_t<R
     if (B2)
       hipLaunchKernelGGL((gram_reduce_kernel<V>), dim3(blocks), dim3(256), 0, s, a.out[1], tile_elems, nchunks,
                          ka, kb, C2, ldc2);
+    HALF_HIP(hipGetLastError());
+  }
+  if (hermitian) {
+    const unsigned blocks = static_cast<unsigned>((int64_t(ka) * ka + 255) / 256);
+    hipLaunchKernelGGL((mirror_lower_kernel<V>), dim3(blocks), dim3(256), 0, s, C, ldc, ka);
+    if (B2) hipLaunchKernelGGL((mirror_lower_kernel<V>), dim3(blocks), dim3(256), 0, s, C2, ldc2, ka);
     HALF_HIP(hipGetLastError());
   }
 }
@@ -345,7 +376,7 @@ void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, co
 #define HALF_CONTRACT(R)                                                                                     \
   template void gram<R>(int, int, int, const cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *, \
                         int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *, int64_t, DeviceArray<cplx_t<R>> &, \
-                        hipStream_t);                                                                        \
+                        hipStream_t, bool);                                                                  \
   template void gemm_tall<R>(int, int, int, R, const cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, R,       \
                              cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *, int64_t, hipStream_t);
 HALF_CONTRACT(double)

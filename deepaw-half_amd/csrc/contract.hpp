@@ -12,10 +12,13 @@ namespace half {
 // once for both).  The row range is split into chunks whose partial tiles are
 // added in a fixed order, so the result does not depend on scheduling.
 // `work` grows to hold the partial tiles.
+// hermitian (ka == kb): the results are known to be Hermitian (A^H H A,
+// A^H S A); only the tiles on and above the diagonal are contracted and the
+// strictly lower triangle is set to the conjugate of the upper one.
 template <typename R>
 void gram(int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, const cplx_t<R> *B, int64_t ldb, cplx_t<R> *C,
           int64_t ldc, const cplx_t<R> *B2, int64_t ldb2, cplx_t<R> *C2, int64_t ldc2,
-          DeviceArray<cplx_t<R>> &work, hipStream_t s);
+          DeviceArray<cplx_t<R>> &work, hipStream_t s, bool hermitian = false);
 
 // C = beta C + alpha A W for A (n x k) and W (k x m); C is n x m.  When W2 is
 // given, also C2 = beta C2 + alpha A W2.  beta = 0 does not read C.  C must not

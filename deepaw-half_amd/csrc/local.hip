@@ -597,6 +597,7 @@ const void *x_conv_for(int planes, int &pmax) {
     if (planes <= 16) return pmax = 16, reinterpret_cast<const void *>(&x_conv_kernel<R, 16>);
     if (planes <= 24) return pmax = 24, reinterpret_cast<const void *>(&x_conv_kernel<R, 24>);
     if (planes <= 32) return pmax = 32, reinterpret_cast<const void *>(&x_conv_kernel<R, 32>);
+    if (planes <= 48) return pmax = 48, reinterpret_cast<const void *>(&x_conv_kernel<R, 48>);
   } else {
     if (planes <= 16) return pmax = 16, reinterpret_cast<const void *>(&x_conv_kernel<R, 16>);
     if (planes <= 32) return pmax = 32, reinterpret_cast<const void *>(&x_conv_kernel<R, 32>);

@@ -384,7 +384,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   };
   // (hsub, ssub) = (A^H HA, A^H SA) for k columns of Z-shaped blocks
   auto gram_hs = [&](int k, const double2 *A, const double2 *HA, const double2 *SA) {
-    gram<double>(n, k, k, A, ld, HA, ld, wk.hsub.get(), k, SA, ld, wk.ssub.get(), k, wk.gwork, s);
+    gram<double>(n, k, k, A, ld, HA, ld, wk.hsub.get(), k, SA, ld, wk.ssub.get(), k, wk.gwork, s, true);
   };
   // rotate the first k columns of (Z, HZ, SZ) by hsub(:, :m) into X, HX, SX
   auto ritz_rotate = [&](int k) {
@@ -476,7 +476,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     // pairs.  It is used when the unit-diagonal factor's squared pivots all
     // exceed 1e-10 (no column within 1e-5 of the span of the others);
     // otherwise the reference's eigen-decomposition form runs.
-    gram<double>(n, m, m, P, ld, SP, ld, wk.metric.get(), m, nullptr, 0, nullptr, 0, wk.gwork, s);
+    gram<double>(n, m, m, P, ld, SP, ld, wk.metric.get(), m, nullptr, 0, nullptr, 0, wk.gwork, s, true);
     trace_mark("acc_metric_cholesky", s);
     hipLaunchKernelGGL(unit_diagonal_kernel, dim3(blocks_for(m)), dim3(kThreads), 0, s, wk.metric.get(),
                        wk.dscale.get(), m);
