@@ -217,8 +217,6 @@ def kernel_bytes(name, nr, grid, n, nlines, nplanes, nb, sc, sr):
         "local_y_bwd": line + plane,
         "local_x_veff": 2 * plane + nr * sr,
         "local_x_conv": 2 * plane,
-        "local_plane_bwd": n * (sc + 8) + plane,
-        "local_plane_fwd": plane + n * (3 * sc + sr + 8),
         "local_y_fwd": plane + line,
         "local_z_fwd": line + n * (3 * sc + sr + 8),
     }

@@ -21,6 +21,7 @@
 #include "../../include/half_amd.h"
 #include "common.hpp"
 #include "forces.hpp"
+#include "kmesh.hpp"
 #include "paw.hpp"
 #include "qdep.hpp"
 #include "solver.hpp"
@@ -626,6 +627,57 @@ int half_amd_solve_acc(half_handle handle, const double kpoint[3], int nbands, i
     c.acc_history = res.history;
     std::copy(res.eigenvalues.begin(), res.eigenvalues.end(), eigenvalues);
     if (eigenvectors) copy_vectors_out(vec, static_cast<int>(n), nbands, eigenvectors, ld, c.stream);
+    if (stats) {
+      stats->iterations = res.iterations;
+      stats->fp64_iterations = res.fp64_iterations;
+      stats->max_residual_eV = res.residual;
+      stats->seconds = res.seconds_solver;
+      stats->band_applications = res.applies;
+      stats->metric_fallbacks = res.metric_fallbacks;
+      stats->reserved = 0;
+    }
+    return int(kSuccess);
+  });
+}
+
+int half_amd_solve_kmesh(half_handle handle, int nk, const double *kpoints, int nbands, int mixed, double tol,
+                         double switch_tol, int maxiter, int extra, int graph, double *eigenvalues,
+                         half_amd_solve_stats *stats, char *error, int capacity) {
+  clear_error(error, capacity);
+  const int slot = context_slot(handle);
+  if (!slot) return bad_handle(error, capacity);
+  if (nbands < 1 || !eigenvalues || !kpoints || nk < 1 || nk > 65535)
+    return export_error(kInvalidArgument, "k points (1..65535), band count and outputs are required", error,
+                        capacity);
+  return guarded(error, capacity, [&]() {
+    Context &c = *g_contexts[slot];
+    std::vector<std::array<double, 3>> ks(nk);
+    for (int j = 0; j < nk; ++j) {
+      ks[j] = {kpoints[3 * j], kpoints[3 * j + 1], kpoints[3 * j + 2]};
+      if (2 * int64_t(nbands + std::max(extra, 0)) > make_basis(c, ks[j].data()).npw())
+        return export_error(kInvalidArgument, "ACC solver requires NBANDS <= NPL/2 at every k point", error,
+                            capacity);
+    }
+    AccControls ctl;
+    ctl.tol = tol;
+    ctl.maxiter = maxiter;
+    ctl.mixed = mixed != 0;
+    if (switch_tol > 0) ctl.switch_tol = switch_tol;
+    ctl.extra = extra;
+    KMeshControls kc;
+    kc.graph = graph != 0;
+    KMeshOperator<double> op64;
+    op64.build(c.sys, ks, c.stream);
+    std::unique_ptr<KMeshOperator<float>> op32;
+    if (ctl.mixed) {
+      op32.reset(new KMeshOperator<float>());
+      op32->build(c.sys, ks, c.stream);
+    }
+    std::vector<double> w;
+    SolveResult res;
+    solve_acc_kmesh(c.blas, op64, op32.get(), nbands, ctl, kc, w, res);
+    c.acc_history = res.history;
+    std::copy(w.begin(), w.end(), eigenvalues);
     if (stats) {
       stats->iterations = res.iterations;
       stats->fp64_iterations = res.fp64_iterations;

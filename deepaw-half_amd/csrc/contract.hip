@@ -37,6 +37,7 @@ struct GramArgs {
   cplx_t<R> *out[2];
   int64_t ldo[2];
   int64_t chunk_stride;  // distance between the partial tiles of consecutive chunks
+  int64_t bA, bB[2], bOut[2];  // batch strides (blockIdx.z)
   int n, ka, kb, chunk, tiles_a;
   int upper;             // grid enumerates only tiles with ta <= tb (tiles_a per side)
 };
@@ -70,9 +71,10 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs<R> g) {
   // loader: column lc of the tile, rows lr .. lr + L - 1 of the slab
   const int lc = tid / TPC, lr = (tid % TPC) * L;
   const bool ca = a0 + lc < g.ka, cb = b0 + lc < g.kb;
-  const V *pa = g.A + int64_t(min(a0 + lc, g.ka - 1)) * g.lda;
+  const int64_t bz = blockIdx.z;
+  const V *pa = g.A + bz * g.bA + int64_t(min(a0 + lc, g.ka - 1)) * g.lda;
   const V *pb[NB];
-  for (int j = 0; j < NB; ++j) pb[j] = g.B[j] + int64_t(min(b0 + lc, g.kb - 1)) * g.ldb[j];
+  for (int j = 0; j < NB; ++j) pb[j] = g.B[j] + bz * g.bB[j] + int64_t(min(b0 + lc, g.kb - 1)) * g.ldb[j];
   V ra[L], rb[NB][L];
   auto load = [&](int r0) {
     for (int i = 0; i < L; ++i) {
@@ -122,7 +124,9 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs<R> g) {
         if (cb2 >= g.kb) continue;
         for (int i = 0; i < 4; ++i) {
           const int ra2 = a0 + wa + 16 * t + mfma_row<R>(lane, i);
-          if (ra2 < g.ka) g.out[j][off + ra2 + int64_t(cb2) * g.ldo[j]] = mk<V>(acc[j][t][u][0][i], acc[j][t][u][1][i]);
+          if (ra2 < g.ka)
+            g.out[j][bz * g.bOut[j] + off + ra2 + int64_t(cb2) * g.ldo[j]] =
+                mk<V>(acc[j][t][u][0][i], acc[j][t][u][1][i]);
         }
       }
 }
@@ -130,9 +134,11 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs<R> g) {
 // C[p + b ldc] = sum over chunks c of partial[c stride + p + b ka], in chunk order
 template <typename V>
 __global__ void gram_reduce_kernel(const V *partial, int64_t stride, int nchunks, int ka, int kb, V *C,
-                                   int64_t ldc) {
+                                   int64_t ldc, int64_t sC) {
   const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (t >= int64_t(ka) * kb) return;
+  partial += blockIdx.y * stride * nchunks;  // batch entry blockIdx.y
+  C += blockIdx.y * sC;
   V acc = partial[t];
   for (int c = 1; c < nchunks; ++c) acc = cadd(acc, partial[c * stride + t]);
   const int b = static_cast<int>(t / ka), p = static_cast<int>(t - int64_t(b) * ka);
@@ -141,9 +147,10 @@ __global__ void gram_reduce_kernel(const V *partial, int64_t stride, int nchunks
 
 // C[j + i ldc] = conj(C[i + j ldc]) for i < j (lower triangle from the upper)
 template <typename V>
-__global__ void mirror_lower_kernel(V *C, int64_t ldc, int k) {
+__global__ void mirror_lower_kernel(V *C, int64_t ldc, int k, int64_t sC) {
   const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (t >= int64_t(k) * k) return;
+  C += blockIdx.y * sC;
   const int j = static_cast<int>(t / k), i = static_cast<int>(t - int64_t(j) * k);
   if (i <= j) return;  // (i, j) strictly lower: row i > column j
   const V u = C[j + int64_t(i) * ldc];
@@ -158,6 +165,7 @@ struct TallArgs {
   int64_t ldw[2];
   cplx_t<R> *C[2];
   int64_t ldc[2];
+  int64_t bA, bW[2], bC[2];  // batch strides (blockIdx.z)
   int n, m, k;
   R alpha, beta;
 };
@@ -180,17 +188,18 @@ __global__ __launch_bounds__(256) void tall_kernel(TallArgs<R> g) {
   const int akc = tid / TPA, ar0 = (tid % TPA) * L;  // A: inner column akc, rows ar0..
   const int wcol = tid / TPW, wk0 = (tid % TPW) * L;  // W: column wcol, inner rows wk0..
   const bool wcv = c0 + wcol < g.m;
+  const int64_t bz = blockIdx.z;
   V ra[L], rw[NB][L];
   auto load = [&](int k0) {
     const int kc = k0 + akc;
     const bool kv = kc < g.k;
-    const V *pa = g.A + int64_t(min(kc, g.k - 1)) * g.lda;
+    const V *pa = g.A + bz * g.bA + int64_t(min(kc, g.k - 1)) * g.lda;
     for (int i = 0; i < L; ++i) {
       const int r = g0 + ar0 + i;
       ra[i] = (kv && r < g.n) ? pa[r] : mk<V>(0, 0);
     }
     for (int j = 0; j < NB; ++j) {
-      const V *pw = g.W[j] + int64_t(min(c0 + wcol, g.m - 1)) * g.ldw[j];
+      const V *pw = g.W[j] + bz * g.bW[j] + int64_t(min(c0 + wcol, g.m - 1)) * g.ldw[j];
       for (int i = 0; i < L; ++i) {
         const int kr = k0 + wk0 + i;
         rw[j][i] = (wcv && kr < g.k) ? pw[kr] : mk<V>(0, 0);
@@ -237,7 +246,7 @@ __global__ __launch_bounds__(256) void tall_kernel(TallArgs<R> g) {
         for (int i = 0; i < 4; ++i) {
           const int r = g0 + wr + 16 * t + mfma_row<R>(lane, i);
           if (r >= g.n) continue;
-          V *c = g.C[j] + r + int64_t(col) * g.ldc[j];
+          V *c = g.C[j] + bz * g.bC[j] + r + int64_t(col) * g.ldc[j];
           R vr = g.alpha * acc[j][t][u][0][i], vi = g.alpha * acc[j][t][u][1][i];
           if (g.beta != R(0)) {
             const V o = *c;
@@ -262,16 +271,20 @@ void launch_tall(dim3 grid, const TallArgs<R> &a, hipStream_t s) {
 }  // namespace
 
 template <typename R>
-void gram(int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, const cplx_t<R> *B, int64_t ldb, cplx_t<R> *C,
-          int64_t ldc, const cplx_t<R> *B2, int64_t ldb2, cplx_t<R> *C2, int64_t ldc2,
-          DeviceArray<cplx_t<R>> &work, hipStream_t s, bool hermitian) {
+void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, int64_t sA, const cplx_t<R> *B,
+                  int64_t ldb, int64_t sB, cplx_t<R> *C, int64_t ldc, int64_t sC, const cplx_t<R> *B2, int64_t ldb2,
+                  int64_t sB2, cplx_t<R> *C2, int64_t ldc2, int64_t sC2, DeviceArray<cplx_t<R>> &work, hipStream_t s,
+                  bool hermitian) {
   using V = cplx_t<R>;
-  if (ka <= 0 || kb <= 0) return;
+  if (ka <= 0 || kb <= 0 || batch <= 0) return;
+  if (batch > 65535) throw Error(kInvalidArgument, "gram: batch too large");
   if (hermitian && ka != kb) throw Error(kInvalidArgument, "gram: a Hermitian result must be square");
   const int nb = B2 ? 2 : 1;
   if (n <= 0) {
-    HALF_HIP(hipMemset2DAsync(C, sizeof(V) * ldc, 0, sizeof(V) * ka, kb, s));
-    if (B2) HALF_HIP(hipMemset2DAsync(C2, sizeof(V) * ldc2, 0, sizeof(V) * ka, kb, s));
+    for (int b = 0; b < batch; ++b) {
+      HALF_HIP(hipMemset2DAsync(C + b * sC, sizeof(V) * ldc, 0, sizeof(V) * ka, kb, s));
+      if (B2) HALF_HIP(hipMemset2DAsync(C2 + b * sC2, sizeof(V) * ldc2, 0, sizeof(V) * ka, kb, s));
+    }
     return;
   }
   if (lda < n || ldb < n || ldc < ka || (B2 && (ldb2 < n || ldc2 < ka)))
@@ -280,18 +293,21 @@ void gram(int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, const cplx_t<R
   const int tw = 32 * wt;
   const int tiles_a = (ka + tw - 1) / tw;
   const int tiles = hermitian ? tiles_a * (tiles_a + 1) / 2 : tiles_a * ((kb + tw - 1) / tw);
-  // about a thousand workgroups, each chunk at least 64 rows
-  int nchunks = std::max(1, std::min({1024 / tiles, (n + 63) / 64, 64}));
+  // about a thousand workgroups over the batch, each chunk at least 64 rows
+  int nchunks = std::max(1, std::min({1024 / std::max(tiles * batch, 1), (n + 63) / 64, 64}));
   int chunk = (n + nchunks - 1) / nchunks;
   chunk = (chunk + KS - 1) / KS * KS;
   nchunks = (n + chunk - 1) / chunk;
   GramArgs<R> a{};
   a.A = A;
   a.lda = lda;
+  a.bA = sA;
   a.B[0] = B;
   a.ldb[0] = ldb;
+  a.bB[0] = sB;
   a.B[1] = B2 ? B2 : B;
   a.ldb[1] = B2 ? ldb2 : ldb;
+  a.bB[1] = B2 ? sB2 : sB;
   a.n = n;
   a.ka = ka;
   a.kb = kb;
@@ -302,69 +318,86 @@ void gram(int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, const cplx_t<R
   if (nchunks == 1) {
     a.out[0] = C;
     a.ldo[0] = ldc;
+    a.bOut[0] = sC;
     a.out[1] = C2;
     a.ldo[1] = ldc2;
+    a.bOut[1] = sC2;
     a.chunk_stride = 0;
   } else {
-    work.ensure(size_t(tile_elems) * nchunks * nb);
+    work.ensure(size_t(tile_elems) * nchunks * batch * nb);
     a.out[0] = work.get();
-    a.out[1] = work.get() + tile_elems * nchunks;
+    a.out[1] = work.get() + tile_elems * nchunks * batch;
     a.ldo[0] = a.ldo[1] = ka;
+    a.bOut[0] = a.bOut[1] = tile_elems * nchunks;
     a.chunk_stride = tile_elems;
   }
-  const dim3 grid(tiles, nchunks);
+  const dim3 grid(tiles, nchunks, batch);
   if (wt == 1)
     nb == 1 ? launch_gram<R, 1, 1>(grid, a, s) : launch_gram<R, 1, 2>(grid, a, s);
   else
     nb == 1 ? launch_gram<R, 2, 1>(grid, a, s) : launch_gram<R, 2, 2>(grid, a, s);
   HALF_HIP(hipGetLastError());
   if (nchunks > 1) {
-    const unsigned blocks = static_cast<unsigned>((tile_elems + 255) / 256);
-    hipLaunchKernelGGL((gram_reduce_kernel<V>), dim3(blocks), dim3(256), 0, s, a.out[0], tile_elems, nchunks, ka,
-                       kb, C, ldc);
+    const dim3 rg(static_cast<unsigned>((tile_elems + 255) / 256), batch);
+    hipLaunchKernelGGL((gram_reduce_kernel<V>), rg, dim3(256), 0, s, a.out[0], tile_elems, nchunks, ka, kb, C, ldc,
+                       sC);
     if (B2)
-      hipLaunchKernelGGL((gram_reduce_kernel<V>), dim3(blocks), dim3(256), 0, s, a.out[1], tile_elems, nchunks,
-                         ka, kb, C2, ldc2);
+      hipLaunchKernelGGL((gram_reduce_kernel<V>), rg, dim3(256), 0, s, a.out[1], tile_elems, nchunks, ka, kb, C2,
+                         ldc2, sC2);
     HALF_HIP(hipGetLastError());
   }
   if (hermitian) {
-    const unsigned blocks = static_cast<unsigned>((int64_t(ka) * ka + 255) / 256);
-    hipLaunchKernelGGL((mirror_lower_kernel<V>), dim3(blocks), dim3(256), 0, s, C, ldc, ka);
-    if (B2) hipLaunchKernelGGL((mirror_lower_kernel<V>), dim3(blocks), dim3(256), 0, s, C2, ldc2, ka);
+    const dim3 mg(static_cast<unsigned>((int64_t(ka) * ka + 255) / 256), batch);
+    hipLaunchKernelGGL((mirror_lower_kernel<V>), mg, dim3(256), 0, s, C, ldc, ka, sC);
+    if (B2) hipLaunchKernelGGL((mirror_lower_kernel<V>), mg, dim3(256), 0, s, C2, ldc2, ka, sC2);
     HALF_HIP(hipGetLastError());
   }
 }
 
 template <typename R>
-void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, const cplx_t<R> *W, int64_t ldw,
-               R beta, cplx_t<R> *C, int64_t ldc, const cplx_t<R> *W2, int64_t ldw2, cplx_t<R> *C2, int64_t ldc2,
-               hipStream_t s) {
-  if (n <= 0 || m <= 0) return;
+void gram(int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, const cplx_t<R> *B, int64_t ldb, cplx_t<R> *C,
+          int64_t ldc, const cplx_t<R> *B2, int64_t ldb2, cplx_t<R> *C2, int64_t ldc2,
+          DeviceArray<cplx_t<R>> &work, hipStream_t s, bool hermitian) {
+  gram_batched<R>(1, n, ka, kb, A, lda, 0, B, ldb, 0, C, ldc, 0, B2, ldb2, 0, C2, ldc2, 0, work, s, hermitian);
+}
+
+template <typename R>
+void gemm_tall_batched(int batch, int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, int64_t sA,
+                       const cplx_t<R> *W, int64_t ldw, int64_t sW, R beta, cplx_t<R> *C, int64_t ldc, int64_t sC,
+                       const cplx_t<R> *W2, int64_t ldw2, int64_t sW2, cplx_t<R> *C2, int64_t ldc2, int64_t sC2,
+                       hipStream_t s) {
+  if (n <= 0 || m <= 0 || batch <= 0) return;
   if (k <= 0) throw Error(kInvalidArgument, "gemm_tall: empty inner dimension");
+  if (batch > 65535) throw Error(kInvalidArgument, "gemm_tall: batch too large");
   if (lda < n || ldw < k || ldc < n || (W2 && (ldw2 < k || ldc2 < n)))
     throw Error(kInvalidArgument, "gemm_tall: leading dimension too small");
   const int nb = W2 ? 2 : 1;
   // 64-wide tiles when they still give a few hundred workgroups
-  const int64_t t64 = int64_t((n + 63) / 64) * ((m + 63) / 64);
+  const int64_t t64 = int64_t((n + 63) / 64) * ((m + 63) / 64) * batch;
   const int wt = t64 >= 512 ? 2 : 1;
   const int tw = 32 * wt;
   TallArgs<R> a{};
   a.A = A;
   a.lda = lda;
+  a.bA = sA;
   a.W[0] = W;
   a.ldw[0] = ldw;
+  a.bW[0] = sW;
   a.C[0] = C;
   a.ldc[0] = ldc;
+  a.bC[0] = sC;
   a.W[1] = W2 ? W2 : W;
   a.ldw[1] = W2 ? ldw2 : ldw;
+  a.bW[1] = W2 ? sW2 : sW;
   a.C[1] = W2 ? C2 : C;
   a.ldc[1] = W2 ? ldc2 : ldc;
+  a.bC[1] = W2 ? sC2 : sC;
   a.n = n;
   a.m = m;
   a.k = k;
   a.alpha = alpha;
   a.beta = beta;
-  const dim3 grid((n + tw - 1) / tw, (m + tw - 1) / tw);
+  const dim3 grid((n + tw - 1) / tw, (m + tw - 1) / tw, batch);
   if (grid.y > 65535) throw Error(kInvalidArgument, "gemm_tall: too many columns");
   if (wt == 1)
     nb == 1 ? launch_tall<R, 1, 1>(grid, a, s) : launch_tall<R, 1, 2>(grid, a, s);
@@ -373,12 +406,27 @@ void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, co
   HALF_HIP(hipGetLastError());
 }
 
+template <typename R>
+void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, const cplx_t<R> *W, int64_t ldw,
+               R beta, cplx_t<R> *C, int64_t ldc, const cplx_t<R> *W2, int64_t ldw2, cplx_t<R> *C2, int64_t ldc2,
+               hipStream_t s) {
+  gemm_tall_batched<R>(1, n, m, k, alpha, A, lda, 0, W, ldw, 0, beta, C, ldc, 0, W2, ldw2, 0, C2, ldc2, 0, s);
+}
+
 #define HALF_CONTRACT(R)                                                                                     \
   template void gram<R>(int, int, int, const cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *, \
                         int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *, int64_t, DeviceArray<cplx_t<R>> &, \
                         hipStream_t, bool);                                                                  \
+  template void gram_batched<R>(int, int, int, int, const cplx_t<R> *, int64_t, int64_t, const cplx_t<R> *,    \
+                                int64_t, int64_t, cplx_t<R> *, int64_t, int64_t, const cplx_t<R> *, int64_t,   \
+                                int64_t, cplx_t<R> *, int64_t, int64_t, DeviceArray<cplx_t<R>> &, hipStream_t, \
+                                bool);                                                                       \
   template void gemm_tall<R>(int, int, int, R, const cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, R,       \
-                             cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *, int64_t, hipStream_t);
+                             cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *, int64_t, hipStream_t); \
+  template void gemm_tall_batched<R>(int, int, int, int, R, const cplx_t<R> *, int64_t, int64_t,               \
+                                     const cplx_t<R> *, int64_t, int64_t, R, cplx_t<R> *, int64_t, int64_t,    \
+                                     const cplx_t<R> *, int64_t, int64_t, cplx_t<R> *, int64_t, int64_t,       \
+                                     hipStream_t);
 HALF_CONTRACT(double)
 HALF_CONTRACT(float)
 #undef HALF_CONTRACT

@@ -20,6 +20,14 @@ void gram(int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, const cplx_t<R
           int64_t ldc, const cplx_t<R> *B2, int64_t ldb2, cplx_t<R> *C2, int64_t ldc2,
           DeviceArray<cplx_t<R>> &work, hipStream_t s, bool hermitian = false);
 
+// The same for `batch` independent problems of equal shape, entry b's
+// operands at A + b sA, B + b sB, C + b sC (one launch for all of them).
+template <typename R>
+void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, int64_t sA, const cplx_t<R> *B,
+                  int64_t ldb, int64_t sB, cplx_t<R> *C, int64_t ldc, int64_t sC, const cplx_t<R> *B2, int64_t ldb2,
+                  int64_t sB2, cplx_t<R> *C2, int64_t ldc2, int64_t sC2, DeviceArray<cplx_t<R>> &work, hipStream_t s,
+                  bool hermitian = false);
+
 // C = beta C + alpha A W for A (n x k) and W (k x m); C is n x m.  When W2 is
 // given, also C2 = beta C2 + alpha A W2.  beta = 0 does not read C.  C must not
 // overlap A or W.
@@ -27,5 +35,12 @@ template <typename R>
 void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, const cplx_t<R> *W, int64_t ldw,
                R beta, cplx_t<R> *C, int64_t ldc, const cplx_t<R> *W2, int64_t ldw2, cplx_t<R> *C2, int64_t ldc2,
                hipStream_t s);
+
+// Batched form of gemm_tall (entry b's operands at base + b stride).
+template <typename R>
+void gemm_tall_batched(int batch, int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, int64_t sA,
+                       const cplx_t<R> *W, int64_t ldw, int64_t sW, R beta, cplx_t<R> *C, int64_t ldc, int64_t sC,
+                       const cplx_t<R> *W2, int64_t ldw2, int64_t sW2, cplx_t<R> *C2, int64_t ldc2, int64_t sC2,
+                       hipStream_t s);
 
 }  // namespace half

@@ -1,0 +1,545 @@
+// k-point mesh batching (reference behaviour per k point:
+// src/half_cuda_iterative_solver.cuf:79-207 for the solve,
+// src/half_cuda_assembly.cuf:468-519 for the operator).
+//
+// The reference solves k points one after another, each with its own
+// operator preparation and per-call cuBLAS/cuSOLVER handles.  Here all k
+// points of a mesh advance together:
+//   - the local stage is one launch per pass for every k point (per-k index
+//     tables in a device array, ragged line / plane counts retired per
+//     workgroup);
+//   - B^H psi and B (D c), B (Q c) are one batched contraction launch each,
+//     with every k point's projector matrix padded to the mesh's largest
+//     plane-wave count (zero rows), so ragged sizes need no special cases;
+//   - the Rayleigh-Ritz algebra of all k points is strided-batched: Grams on
+//     the batched MFMA contraction, rotations and updates with
+//     rocblas_zgemm_strided_batched, Cholesky factors and triangular solves
+//     with rocsolver_zpotrf_strided_batched / rocblas_ztrsm_strided_batched,
+//     eigensolves with rocsolver_zheevd_strided_batched;
+//   - the correction block's H.psi (the same launches every iteration) is
+//     captured once per precision into a hipGraph and replayed.
+// One host round trip per iteration carries the nk x m residual norms.
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "contract.hpp"
+#include "cplx.hpp"
+#include "kmesh.hpp"
+#include "trace.hpp"
+
+namespace half {
+
+namespace {
+
+constexpr int kThreads = 256;
+inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
+
+#define KM_BLAS(call)                                                                                    \
+  do {                                                                                                   \
+    rocblas_status st_ = (call);                                                                         \
+    if (st_ != rocblas_status_success)                                                                   \
+      throw Error(kInternal, std::string("rocBLAS/rocSOLVER status ") + std::to_string(int(st_)) + " in " #call); \
+  } while (0)
+
+inline rocblas_double_complex *rz(double2 *p) { return reinterpret_cast<rocblas_double_complex *>(p); }
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Per-k batched helpers: grid.y (or the column index) selects the k point.
+
+// x_k[g, b] = (g == idx[k m + b]), N rows per column, k stride zk
+__global__ void init_states_kernel(double2 *x, int64_t ld, int64_t zk, const int *idx, int N, int m) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(N) * m) return;
+  const int k = blockIdx.y;
+  const int b = static_cast<int>(t / N), g = static_cast<int>(t - int64_t(b) * N);
+  x[k * zk + g + b * ld] = make_double2(g == idx[k * m + b] ? 1.0 : 0.0, 0.0);
+}
+
+// R_k = HX_k - SX_k diag(w_k) (reference :29-40)
+__global__ void residual_kernel(const double2 *hx, const double2 *sx, int64_t zk, const double *w, int64_t wk,
+                                double2 *r, int64_t rk, int N, int m, int64_t ld) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(N) * m) return;
+  const int k = blockIdx.y;
+  const int b = static_cast<int>(t / N), g = static_cast<int>(t - int64_t(b) * N);
+  const int64_t o = k * zk + g + b * ld;
+  const double e = w[k * wk + b];
+  r[k * rk + g + b * ld] = make_double2(hx[o].x - e * sx[o].x, hx[o].y - e * sx[o].y);
+}
+
+// squared norms of ncols columns of length N (fixed-order tree per column)
+__global__ void column_norm2_kernel(const double2 *r, int64_t ld, int N, double *out) {
+  __shared__ double red[kThreads];
+  const double2 *c = r + blockIdx.x * ld;
+  double acc = 0;
+  for (int g = threadIdx.x; g < N; g += blockDim.x) acc += c[g].x * c[g].x + c[g].y * c[g].y;
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// P = -R/(|T - eps| + 1) or a unit seed vector (reference :42-64)
+__global__ void precondition_kernel(const double2 *r, int64_t rk, const double *kin, int64_t kk, const double *w,
+                                    int64_t wk, const int *seeds, double2 *p, int64_t zk, int N, int m, int64_t ld) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(N) * m) return;
+  const int k = blockIdx.y;
+  const int b = static_cast<int>(t / N), g = static_cast<int>(t - int64_t(b) * N);
+  const int64_t o = k * zk + g + b * ld;
+  const int seed = seeds[k * m + b];
+  if (seed >= 0) {
+    p[o] = make_double2(g == seed ? 1.0 : 0.0, 0.0);
+  } else {
+    const double2 v = r[k * rk + g + b * ld];
+    const double d = fabs(kin[k * kk + g] - w[k * wk + b]) + 1.0;
+    p[o] = make_double2(-v.x / d, -v.y / d);
+  }
+}
+
+// metric_k <- D metric_k D, D = diag(metric_k)^-1/2 (zero where not positive)
+__global__ void unit_diagonal_kernel(double2 *M, double *d, int m) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const int k = blockIdx.y;
+  const double v = M[int64_t(k) * m * m + j + int64_t(j) * m].x;
+  d[k * m + j] = v > 0 ? 1.0 / sqrt(v) : 0.0;
+}
+__global__ void scale_both_kernel(double2 *M, const double *d, int m) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(m) * m) return;
+  const int k = blockIdx.y;
+  const int j = static_cast<int>(t / m), i = static_cast<int>(t - int64_t(j) * m);
+  const double f = d[k * m + i] * d[k * m + j];
+  double2 &v = M[int64_t(k) * m * m + t];
+  v = make_double2(v.x * f, v.y * f);
+}
+// out[k (m + 1) + j] = U_jj^2, out[k (m + 1) + m] = potrf info of k
+__global__ void pivots_kernel(const double2 *U, const rocblas_int *info, double *out, int m) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (j < m) {
+    const double2 u = U[int64_t(k) * m * m + j + int64_t(j) * m];
+    out[k * (m + 1) + j] = u.x * u.x;
+  }
+  if (j == 0) out[k * (m + 1) + m] = static_cast<double>(info[k]);
+}
+// U_k <- U_k D_k^-1
+__global__ void unscale_columns_kernel(double2 *U, const double *d, int m) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(m) * m) return;
+  const int k = blockIdx.y;
+  const int j = static_cast<int>(t / m);
+  const double dj = d[k * m + j];
+  const double f = dj > 0 ? 1.0 / dj : 0.0;
+  double2 &v = U[int64_t(k) * m * m + t];
+  v = make_double2(v.x * f, v.y * f);
+}
+// V_k <- V_k diag(max(mu_k, floor))^-1/2 (reference :66-77)
+__global__ void scale_columns_kernel(double2 *u, const double *mu, int m, double floor_value) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(m) * m) return;
+  const int k = blockIdx.y;
+  const int col = static_cast<int>(t / m);
+  const double f = 1.0 / sqrt(fmax(mu[k * m + col], floor_value));
+  double2 &v = u[int64_t(k) * m * m + t];
+  v = make_double2(v.x * f, v.y * f);
+}
+
+// strided precision conversion of m columns of N rows for every k point
+template <typename VI, typename VO>
+__global__ void convert_kernel(const VI *in, int64_t ldi, int64_t ski, VO *out, int64_t ldo, int64_t sko, int N,
+                               int m) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(N) * m) return;
+  const int k = blockIdx.y;
+  const int b = static_cast<int>(t / N), g = static_cast<int>(t - int64_t(b) * N);
+  const VI v = in[k * ski + g + b * ldi];
+  VO o;
+  o.x = v.x;
+  o.y = v.y;
+  out[k * sko + g + b * ldo] = o;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// operator
+// ---------------------------------------------------------------------------
+
+template <typename R>
+void KMeshOperator<R>::build(const System &s, const std::vector<std::array<double, 3>> &kpoints, hipStream_t st) {
+  using V = cplx_t<R>;
+  sys = &s;
+  nk = static_cast<int>(kpoints.size());
+  np = s.nproj;
+  if (nk < 1 || nk > 65535) throw Error(kInvalidArgument, "k-point count must be 1..65535");
+  npw.assign(nk, 0);
+  nmax = 0;
+  for (int j = 0; j < nk; ++j) {
+    npw[j] = static_cast<int>(build_basis(s.crystal, s.charge.shape, s.encut, kpoints[j].data()).npw());
+    nmax = std::max(nmax, npw[j]);
+  }
+  B.resize(size_t(nk) * std::max(np, 1) * nmax);
+  kin.resize(size_t(nk) * nmax);
+  HALF_HIP(hipMemsetAsync(B.get(), 0, sizeof(V) * B.size(), st));
+  HALF_HIP(hipMemsetAsync(kin.get(), 0, sizeof(R) * kin.size(), st));
+  maps.clear();
+  maps.reserve(nk);
+  kinetic.assign(nk, {});
+  std::vector<LocalTables> tabs(nk);
+  lb = LocalBatch{};
+  lb.nk = nk;
+  lb.contiguous = true;
+  for (int j = 0; j < nk; ++j) {
+    KOperator<R> op;
+    op.build(s, kpoints[j].data(), st);
+    if (np > 0)
+      HALF_HIP(hipMemcpy2DAsync(B.get() + size_t(j) * np * nmax, sizeof(V) * nmax, op.B.get(), sizeof(V) * op.ldb,
+                                sizeof(V) * op.n, np, hipMemcpyDeviceToDevice, st));
+    HALF_HIP(hipMemcpyAsync(kin.get() + size_t(j) * nmax, op.kinetic.get(), sizeof(R) * op.n,
+                            hipMemcpyDeviceToDevice, st));
+    kinetic[j] = op.basis.kinetic;
+    lb.max_lines = std::max(lb.max_lines, op.lmap.nlines);
+    lb.max_planes = std::max(lb.max_planes, op.lmap.nplanes);
+    lb.contiguous = lb.contiguous && op.lmap.planes_contiguous;
+    maps.push_back(std::move(op.lmap));
+    tabs[j] = maps.back().tables();
+    HALF_HIP(hipStreamSynchronize(st));  // the copies read op's arrays
+  }
+  dtabs.upload(tabs, st);
+  lb.tabs = dtabs.get();
+  lb.kin_k = nmax;
+  HALF_HIP(hipStreamSynchronize(st));
+}
+
+template <typename R>
+void KMeshOperator<R>::apply(int nb, const cplx_t<R> *psi, int64_t ld, int64_t sk_in, cplx_t<R> *hpsi,
+                             cplx_t<R> *spsi, int64_t ldo, int64_t sk_out, hipStream_t st) {
+  if (nb <= 0) return;
+  if (ld < nmax || ldo < nmax) throw Error(kInvalidArgument, "k-mesh states need nmax rows");
+  const int64_t nr = sys->fft.nr;
+  boxes.ensure(size_t(nk) * nb * nr);
+  LocalBatch kb = lb;
+  kb.psi_k = sk_in;
+  kb.out_k = sk_out;
+  kb.box_k = int64_t(nb) * nr;
+  const R *veff;
+  const cplx_t<R> *vx;
+  if constexpr (std::is_same<R, double>::value) {
+    veff = sys->veff.get();
+    vx = sys->veff_x.get();
+  } else {
+    veff = sys->veff32.get();
+    vx = sys->veff_x32.get();
+  }
+  local_apply_batch<R>(sys->fft, kb, veff, vx, kin.get(), nb, psi, ld, hpsi, spsi, ldo, boxes.get(), st);
+  if (np > 0) {
+    const int64_t cnt = int64_t(np) * nb;
+    coeff.ensure(size_t(cnt) * nk);
+    dcoeff.ensure(size_t(cnt) * nk);
+    qcoeff.ensure(size_t(cnt) * nk);
+    trace_mark("proj_coeff_mfma", st);
+    gram_batched<R>(nk, nmax, np, nb, B.get(), nmax, int64_t(nmax) * np, psi, ld, sk_in, coeff.get(), np, cnt,
+                    nullptr, 0, 0, nullptr, 0, 0, work, st);
+    trace_mark("proj_mix", st);
+    mix_projector_coefficients<R>(*sys, nk * nb, coeff.get(), dcoeff.get(), qcoeff.get(), st);
+    trace_mark("proj_expand_mfma", st);
+    gemm_tall_batched<R>(nk, nmax, nb, np, R(1), B.get(), nmax, int64_t(nmax) * np, dcoeff.get(), np, cnt, R(1), hpsi,
+                         ldo, sk_out, qcoeff.get(), np, cnt, spsi, ldo, sk_out, st);
+  }
+  trace_close(st);
+}
+
+template struct KMeshOperator<double>;
+template struct KMeshOperator<float>;
+
+// ---------------------------------------------------------------------------
+// solver
+// ---------------------------------------------------------------------------
+
+void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<float> *op32, int nbands,
+                     const AccControls &ctl, const KMeshControls &kc, std::vector<double> &eigenvalues,
+                     SolveResult &res) {
+  hipStream_t s = blas.stream;
+  const int nk = op64.nk, N = op64.nmax;
+  const int mreq = nbands;
+  const int m = nbands + std::max(0, ctl.extra);
+  const int m2 = 2 * m;
+  if (mreq < 1) throw Error(kInvalidArgument, "invalid requested band count");
+  for (int j = 0; j < nk; ++j)
+    if (2 * m > op64.npw[j]) throw Error(kInvalidArgument, "ACC solver requires NBANDS <= NPL/2 at every k point");
+  if (!(ctl.tol > 0) || ctl.maxiter < 1) throw Error(kInvalidArgument, "invalid ACC solver controls");
+  if (ctl.mixed && (!op32 || op32->nk != nk || op32->nmax != N))
+    throw Error(kInternal, "mixed k-mesh solve needs the matching FP32 operator");
+  const double t0 = now();
+  const int64_t ld = N, zk = int64_t(N) * m2, rk = int64_t(N) * m;
+  const int64_t gk = int64_t(m2) * m2;  // Gram stride per k point
+  DeviceArray<double2> Z(size_t(nk) * zk), HZ(size_t(nk) * zk), SZ(size_t(nk) * zk), Rr(size_t(nk) * rk),
+      tmp(size_t(nk) * rk), cross(size_t(nk) * m * m), metric(size_t(nk) * m * m), hsub(size_t(nk) * gk),
+      ssub(size_t(nk) * gk), gwork;
+  DeviceArray<double> w(size_t(nk) * m2), E(size_t(nk) * m2), mu(size_t(nk) * m), res2(size_t(nk) * m),
+      dscale(size_t(nk) * m), piv(size_t(nk) * (m + 1));
+  DeviceArray<int> dseeds(size_t(nk) * m);
+  DeviceArray<rocblas_int> info(size_t(2) * nk);
+  DeviceArray<float2> p32, hp32, sp32;
+  for (auto *a : {&Z, &HZ, &SZ, &Rr})
+    HALF_HIP(hipMemsetAsync(a->get(), 0, sizeof(double2) * a->size(), s));
+  if (ctl.mixed) {
+    p32.resize(size_t(nk) * rk);
+    hp32.resize(size_t(nk) * rk);
+    sp32.resize(size_t(nk) * rk);
+    for (auto *a : {&p32, &hp32, &sp32}) HALF_HIP(hipMemsetAsync(a->get(), 0, sizeof(float2) * a->size(), s));
+  }
+  double2 *X = Z.get(), *P = Z.get() + rk;
+  double2 *HX = HZ.get(), *HP = HZ.get() + rk;
+  double2 *SX = SZ.get(), *SP = SZ.get() + rk;
+  const rocblas_double_complex one{1, 0}, zero{0, 0}, mone{-1, 0};
+  const dim3 gNm(blocks_for(int64_t(N) * m), nk), gmm(blocks_for(int64_t(m) * m), nk), gm(blocks_for(m), nk);
+  bool fp64_phase = !ctl.mixed;
+  res.applies = 0;
+  res.metric_fallbacks = 0;
+  res.history.clear();
+
+  auto apply = [&](double2 *in, double2 *hout, double2 *sout) {
+    if (fp64_phase) {
+      op64.apply(m, in, ld, zk, hout, sout, ld, zk, s);
+    } else {
+      hipLaunchKernelGGL((convert_kernel<double2, float2>), gNm, dim3(kThreads), 0, s, in, ld, zk, p32.get(), ld, rk,
+                         N, m);
+      op32->apply(m, p32.get(), ld, rk, hp32.get(), sp32.get(), ld, rk, s);
+      hipLaunchKernelGGL((convert_kernel<float2, double2>), gNm, dim3(kThreads), 0, s, hp32.get(), ld, rk, hout, ld,
+                         zk, N, m);
+      hipLaunchKernelGGL((convert_kernel<float2, double2>), gNm, dim3(kThreads), 0, s, sp32.get(), ld, rk, sout, ld,
+                         zk, N, m);
+      HALF_HIP(hipGetLastError());
+    }
+    res.applies += int64_t(m) * nk;
+  };
+  // the correction block's H.psi: eager once per precision, then captured
+  // and replayed (same launches, same pointers every iteration)
+  const bool use_graph = kc.graph && !trace_enabled();
+  hipGraph_t graphs[2] = {nullptr, nullptr};
+  hipGraphExec_t execs[2] = {nullptr, nullptr};
+  struct GraphGuard {
+    hipGraph_t *g;
+    hipGraphExec_t *e;
+    ~GraphGuard() {
+      for (int i = 0; i < 2; ++i) {
+        if (e[i]) (void)hipGraphExecDestroy(e[i]);
+        if (g[i]) (void)hipGraphDestroy(g[i]);
+      }
+    }
+  } guard{graphs, execs};
+  auto apply_p = [&]() {
+    const int ph = fp64_phase ? 1 : 0;
+    if (!use_graph || execs[ph]) {
+      if (execs[ph]) {
+        HALF_HIP(hipGraphLaunch(execs[ph], s));
+        res.applies += int64_t(m) * nk;
+      } else {
+        apply(P, HP, SP);
+      }
+      return;
+    }
+    apply(P, HP, SP);  // this iteration's work; sizes every workspace
+    HALF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int64_t applies0 = res.applies;
+    try {
+      apply(P, HP, SP);
+    } catch (...) {
+      hipGraph_t broken = nullptr;  // leave the stream usable
+      if (hipStreamEndCapture(s, &broken) == hipSuccess && broken) (void)hipGraphDestroy(broken);
+      throw;
+    }
+    res.applies = applies0;
+    HALF_HIP(hipStreamEndCapture(s, &graphs[ph]));
+    HALF_HIP(hipGraphInstantiate(&execs[ph], graphs[ph], nullptr, nullptr, 0));
+  };
+  // k x k generalized problems (A_j, B_j) at stride gstride: B = U^H U,
+  // A <- U^-H A U^-1, eigenpairs, A <- U^-1 V; values to w (stride m2)
+  auto generalized = [&](int k, int64_t gstride) {
+    KM_BLAS(rocsolver_zpotrf_strided_batched(blas.h, rocblas_fill_upper, k, rz(ssub.get()), k, gstride, info.get(),
+                                             nk));
+    KM_BLAS(rocblas_ztrsm_strided_batched(blas.h, rocblas_side_left, rocblas_fill_upper,
+                                          rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, k, k, &one,
+                                          rz(ssub.get()), k, gstride, rz(hsub.get()), k, gstride, nk));
+    KM_BLAS(rocblas_ztrsm_strided_batched(blas.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                          rocblas_diagonal_non_unit, k, k, &one, rz(ssub.get()), k, gstride,
+                                          rz(hsub.get()), k, gstride, nk));
+    KM_BLAS(rocsolver_zheevd_strided_batched(blas.h, rocblas_evect_original, rocblas_fill_upper, k, rz(hsub.get()), k,
+                                             gstride, w.get(), m2, E.get(), m2, info.get() + nk, nk));
+    KM_BLAS(rocblas_ztrsm_strided_batched(blas.h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                          rocblas_diagonal_non_unit, k, k, &one, rz(ssub.get()), k, gstride,
+                                          rz(hsub.get()), k, gstride, nk));
+    std::vector<rocblas_int> h(2 * nk);
+    HALF_HIP(hipMemcpyAsync(h.data(), info.get(), sizeof(rocblas_int) * h.size(), hipMemcpyDeviceToHost, s));
+    HALF_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < nk; ++j) {
+      if (h[j] != 0)
+        throw Error(kInternal, "k-mesh ACC subspace solve failed at k point " + std::to_string(j) +
+                                   ": overlap not positive definite");
+      if (h[nk + j] != 0) throw Error(kInternal, "k-mesh ACC subspace eigensolve failed at k point " + std::to_string(j));
+    }
+  };
+  // X, HX, SX <- first-k-columns of (Z, HZ, SZ) times hsub(:, :m)
+  auto rotate = [&](int k, int64_t gstride) {
+    for (double2 *blk : {Z.get(), HZ.get(), SZ.get()}) {
+      KM_BLAS(rocblas_zgemm_strided_batched(blas.h, rocblas_operation_none, rocblas_operation_none, N, m, k, &one,
+                                            rz(blk), ld, zk, rz(hsub.get()), k, gstride, &zero, rz(tmp.get()), ld, rk,
+                                            nk));
+      HALF_HIP(hipMemcpy2DAsync(blk, sizeof(double2) * zk, tmp.get(), sizeof(double2) * rk, sizeof(double2) * rk, nk,
+                                hipMemcpyDeviceToDevice, s));
+    }
+  };
+  auto rayleigh_ritz_x = [&]() {
+    gram_batched<double>(nk, N, m, m, X, ld, zk, HX, ld, zk, hsub.get(), m, int64_t(m) * m, SX, ld, zk, ssub.get(), m,
+                         int64_t(m) * m, gwork, s, true);
+    generalized(m, int64_t(m) * m);
+    rotate(m, int64_t(m) * m);
+  };
+
+  // initial subspace: unit vectors at each k point's m lowest kinetic energies
+  {
+    std::vector<int> idx(size_t(nk) * m);
+    for (int j = 0; j < nk; ++j) {
+      auto order = kinetic_order(op64.kinetic[j]);
+      std::copy(order.begin(), order.begin() + m, idx.begin() + size_t(j) * m);
+    }
+    DeviceArray<int> d_idx;
+    d_idx.upload(idx, s);
+    hipLaunchKernelGGL(init_states_kernel, gNm, dim3(kThreads), 0, s, X, ld, zk, d_idx.get(), N, m);
+    HALF_HIP(hipGetLastError());
+    apply(X, HX, SX);
+    rayleigh_ritz_x();
+    HALF_HIP(hipStreamSynchronize(s));
+  }
+  std::vector<double> r2(size_t(nk) * m), pivh(size_t(nk) * (m + 1));
+  std::vector<int> seeds(size_t(nk) * m);
+  double max_res = HUGE_VAL;
+  int iter = 0, fp64_iters = 0;
+  for (iter = 1; iter <= ctl.maxiter; ++iter) {
+    trace_mark("acc_residual", s);
+    hipLaunchKernelGGL(residual_kernel, gNm, dim3(kThreads), 0, s, HX, SX, zk, w.get(), int64_t(m2), Rr.get(), rk, N,
+                       m, ld);
+    hipLaunchKernelGGL(column_norm2_kernel, dim3(nk * m), dim3(kThreads), 0, s, Rr.get(), ld, N, res2.get());
+    HALF_HIP(hipGetLastError());
+    trace_mark("acc_norms_to_host", s);
+    HALF_HIP(hipMemcpyAsync(r2.data(), res2.get(), sizeof(double) * r2.size(), hipMemcpyDeviceToHost, s));
+    HALF_HIP(hipStreamSynchronize(s));
+    max_res = 0;
+    for (int j = 0; j < nk; ++j)
+      for (int b = 0; b < m; ++b) {
+        const double r = std::sqrt(std::max(0.0, r2[size_t(j) * m + b]));
+        if (b < mreq) max_res = std::max(max_res, r);
+        seeds[size_t(j) * m + b] = r < std::max(ctl.tol * 1e-2, 1e-12) ? (m + b) % op64.npw[j] : -1;
+      }
+    res.history.push_back(max_res);
+    if (fp64_phase) ++fp64_iters;
+    if (!fp64_phase && max_res <= std::max(ctl.switch_tol, ctl.tol)) {
+      fp64_phase = true;
+      apply(X, HX, SX);
+      rayleigh_ritz_x();
+      --iter;
+      continue;
+    }
+    if (fp64_phase && max_res <= ctl.tol) break;
+    trace_mark("acc_precondition", s);
+    dseeds.upload(seeds, s);
+    hipLaunchKernelGGL(precondition_kernel, gNm, dim3(kThreads), 0, s, Rr.get(), rk, op64.kin.get(),
+                       int64_t(op64.nmax), w.get(), int64_t(m2), dseeds.get(), P, zk, N, m, ld);
+    HALF_HIP(hipGetLastError());
+    apply_p();
+    // S-orthogonalise against X
+    trace_mark("acc_orthogonalise", s);
+    gram_batched<double>(nk, N, m, m, X, ld, zk, SP, ld, zk, cross.get(), m, int64_t(m) * m, nullptr, 0, 0, nullptr, 0,
+                         0, gwork, s);
+    for (auto pr : {std::make_pair(X, P), std::make_pair(HX, HP), std::make_pair(SX, SP)})
+      KM_BLAS(rocblas_zgemm_strided_batched(blas.h, rocblas_operation_none, rocblas_operation_none, N, m, m, &mone,
+                                            rz(pr.first), ld, zk, rz(cross.get()), m, int64_t(m) * m, &one,
+                                            rz(pr.second), ld, zk, nk));
+    // normalise in the S metric (unit-diagonal Cholesky; eigen-decomposition
+    // form when any k point's factor is ill-conditioned)
+    trace_mark("acc_metric_cholesky", s);
+    gram_batched<double>(nk, N, m, m, P, ld, zk, SP, ld, zk, metric.get(), m, int64_t(m) * m, nullptr, 0, 0, nullptr,
+                         0, 0, gwork, s, true);
+    hipLaunchKernelGGL(unit_diagonal_kernel, gm, dim3(kThreads), 0, s, metric.get(), dscale.get(), m);
+    hipLaunchKernelGGL(scale_both_kernel, gmm, dim3(kThreads), 0, s, metric.get(), dscale.get(), m);
+    HALF_HIP(hipGetLastError());
+    KM_BLAS(rocsolver_zpotrf_strided_batched(blas.h, rocblas_fill_upper, m, rz(metric.get()), m, int64_t(m) * m,
+                                             info.get(), nk));
+    hipLaunchKernelGGL(pivots_kernel, gm, dim3(kThreads), 0, s, metric.get(), info.get(), piv.get(), m);
+    HALF_HIP(hipGetLastError());
+    HALF_HIP(hipMemcpyAsync(pivh.data(), piv.get(), sizeof(double) * pivh.size(), hipMemcpyDeviceToHost, s));
+    HALF_HIP(hipStreamSynchronize(s));
+    bool chol_ok = true;
+    for (int j = 0; j < nk && chol_ok; ++j) {
+      const double *pj = pivh.data() + size_t(j) * (m + 1);
+      chol_ok = pj[m] == 0.0;
+      for (int b = 0; b < m && chol_ok; ++b) chol_ok = std::isfinite(pj[b]) && pj[b] > 1e-10;
+    }
+    if (chol_ok) {
+      hipLaunchKernelGGL(unscale_columns_kernel, gmm, dim3(kThreads), 0, s, metric.get(), dscale.get(), m);
+      HALF_HIP(hipGetLastError());
+      trace_mark("acc_normalise_trsm", s);
+      for (double2 *blk : {P, HP, SP})
+        KM_BLAS(rocblas_ztrsm_strided_batched(blas.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                              rocblas_diagonal_non_unit, N, m, &one, rz(metric.get()), m,
+                                              int64_t(m) * m, rz(blk), ld, zk, nk));
+    } else {
+      ++res.metric_fallbacks;
+      gram_batched<double>(nk, N, m, m, P, ld, zk, SP, ld, zk, metric.get(), m, int64_t(m) * m, nullptr, 0, 0,
+                           nullptr, 0, 0, gwork, s, true);
+      trace_mark("acc_metric_zheevd", s);
+      KM_BLAS(rocsolver_zheevd_strided_batched(blas.h, rocblas_evect_original, rocblas_fill_upper, m,
+                                               rz(metric.get()), m, int64_t(m) * m, mu.get(), m, E.get(), m2,
+                                               info.get(), nk));
+      std::vector<rocblas_int> h(nk);
+      HALF_HIP(hipMemcpyAsync(h.data(), info.get(), sizeof(rocblas_int) * nk, hipMemcpyDeviceToHost, s));
+      HALF_HIP(hipStreamSynchronize(s));
+      for (int j = 0; j < nk; ++j)
+        if (h[j] != 0) throw Error(kInternal, "k-mesh ACC metric eigensolve failed");
+      hipLaunchKernelGGL(scale_columns_kernel, gmm, dim3(kThreads), 0, s, metric.get(), mu.get(), m, 1e-14);
+      HALF_HIP(hipGetLastError());
+      for (double2 *blk : {P, HP, SP}) {
+        KM_BLAS(rocblas_zgemm_strided_batched(blas.h, rocblas_operation_none, rocblas_operation_none, N, m, m, &one,
+                                              rz(blk), ld, zk, rz(metric.get()), m, int64_t(m) * m, &zero,
+                                              rz(tmp.get()), ld, rk, nk));
+        HALF_HIP(hipMemcpy2DAsync(blk, sizeof(double2) * zk, tmp.get(), sizeof(double2) * rk, sizeof(double2) * rk,
+                                  nk, hipMemcpyDeviceToDevice, s));
+      }
+    }
+    // Rayleigh-Ritz in span[X P] for every k point
+    trace_mark("acc_projection_gram", s);
+    gram_batched<double>(nk, N, m2, m2, Z.get(), ld, zk, HZ.get(), ld, zk, hsub.get(), m2, gk, SZ.get(), ld, zk,
+                         ssub.get(), m2, gk, gwork, s, true);
+    trace_mark("acc_subspace_eigensolve", s);
+    generalized(m2, gk);
+    trace_mark("acc_rotate_gemm", s);
+    rotate(m2, gk);
+    trace_close(s);
+  }
+  res.iterations = std::min(iter, ctl.maxiter);
+  res.fp64_iterations = fp64_iters;
+  res.residual = max_res;
+  std::vector<double> wall(size_t(nk) * m2);
+  HALF_HIP(hipMemcpyAsync(wall.data(), w.get(), sizeof(double) * wall.size(), hipMemcpyDeviceToHost, s));
+  HALF_HIP(hipStreamSynchronize(s));
+  eigenvalues.resize(size_t(nk) * mreq);
+  for (int j = 0; j < nk; ++j)
+    std::copy(wall.begin() + size_t(j) * m2, wall.begin() + size_t(j) * m2 + mreq,
+              eigenvalues.begin() + size_t(j) * mreq);
+  res.overlap_min = res.overlap_max = -1.0;
+  res.seconds_solver = now() - t0;
+}
+
+}  // namespace half

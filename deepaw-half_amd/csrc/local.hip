@@ -3,7 +3,7 @@
 // Only the plane-wave sphere is populated in G space, so most 1D transforms
 // of a full 3D FFT act on zeros.  With the C-order box (i1 slowest, i3
 // contiguous) and the basis enumerated n1, n2, n3 ascending (so the plane
-// waves of one (i1, i2) line are contiguous), the transform is done in four
+// waves of one (i1, i2) line are contiguous), the transform is done in five
 // kernels that touch only what is needed:
 //
 //   z_bwd   active (i1,i2) lines: plane waves scattered into LDS straight
@@ -22,12 +22,12 @@
 //   z_fwd   active lines: forward transform along i3, gather the plane-wave
 //           coefficients and add T psi (S psi = psi is written here too)
 //
-// Where a whole n2 x n3 plane fits in LDS, z and y are fused per active
-// plane instead (plane_bwd: scatter, z transforms of the plane's active
-// lines, y transforms of all its columns; plane_fwd: the reverse, ending in
-// the gather), and only the active planes are stored, compactly in signed
-// order, so the local stage is three kernels and each active plane crosses
-// HBM four times per band application.
+// Every kernel serves a batch of k points that share the box (one launch per
+// stage for all of them): the k point of a workgroup is a grid coordinate,
+// its index tables come from a device array, and its states and boxes sit
+// at fixed strides.  Ragged sizes (lines and planes differ between k points)
+// are handled by launching for the largest and retiring workgroups past a k
+// point's own count before they touch anything.
 //
 // Box entries outside the active lines/planes are never read or written.
 // Numerically this is the same circular convolution as the reference's
@@ -51,24 +51,8 @@ struct Tuning {
   int tile[2][3] = {{1024, 1536, 768}, {1536, 1536, 768}};  // [fp64|fp32][x|y|z]
   int xgroup[2] = {4, 8};
   int xconv[2] = {1, 1};  // x pass as a per-column convolution when it fits
-  // lines per LDS batch of the plane-fused z/y passes; off by default: with
-  // one plane per workgroup its transform batches keep too few threads busy
-  // (measured slower than the separate passes on the 48^3 box)
-  int plane[2] = {0, 0};
 };
 Tuning g_tune;
-
-struct Tables {
-  int n1, n2, n3;
-  long long nr;
-  int nlines;
-  const int *line_i12, *line_start, *line_count, *line_of;
-  int nplanes;
-  const int *plane_i1, *plane_on;
-  const int *pw_i3;
-  int plane_s0;  // first signed plane of the contiguous run (x_conv_kernel)
-  int compact;   // planes stored compactly: plane q of a band at (band P + q) n2 n3
-};
 
 // Elements per thread of a tile (M * n <= tile budget <= 256 * KMAX).
 template <typename R>
@@ -76,17 +60,28 @@ struct Tile {
   static constexpr int KMAX = std::is_same<R, double>::value ? 8 : 16;
 };
 
+// k-point strides of the batch (elements)
+struct KStride {
+  long long psi, out, kin, box;
+};
+
 template <typename R>
 __global__ __launch_bounds__(256) void z_bwd_kernel(const cplx_t<R> *psi, long long ldpsi, cplx_t<R> *box,
-                                                    Tables t, FftArgs a) {
+                                                    const LocalTables *tabs, KStride ks, int nb, FftArgs a) {
   using V = cplx_t<R>;
   constexpr int K = Tile<R>::KMAX;
+  const int kp = blockIdx.y;
+  const LocalTables t = tabs[kp];
+  const long long lines_total = static_cast<long long>(nb) * t.nlines;
+  const int L0 = blockIdx.x * a.M;
+  if (L0 >= lines_total) return;  // past this k point's lines: no barrier reached
+  psi += kp * ks.psi;
+  box += kp * ks.box;
   extern __shared__ double2 smem_raw[];
   V *buf0 = reinterpret_cast<V *>(smem_raw);
   V *buf1 = buf0 + a.M * a.ld;
   V *tws = buf1 + a.M * a.ld;
   load_twiddles(tws, a);
-  const int L0 = blockIdx.x * a.M;
   const int MN = a.M * a.n;
   for (int e = threadIdx.x; e < a.M * a.ld; e += blockDim.x) buf0[e] = mk<V>(R(0), R(0));
   __syncthreads();
@@ -97,7 +92,7 @@ __global__ __launch_bounds__(256) void z_bwd_kernel(const cplx_t<R> *psi, long l
     if (e < MN) {
       const int l = e / a.n, j = e - l * a.n;
       const int L = L0 + l;
-      if (L < a.lines_total) {
+      if (L < lines_total) {
         const int band = L / t.nlines, line = L - band * t.nlines;
         if (j < t.line_count[line]) {
           const int g = t.line_start[line] + j;
@@ -114,7 +109,7 @@ __global__ __launch_bounds__(256) void z_bwd_kernel(const cplx_t<R> *psi, long l
     if (e >= MN) break;
     const int l = e / a.n, p = e - l * a.n;
     const int L = L0 + l;
-    if (L < a.lines_total) {
+    if (L < lines_total) {
       const int band = L / t.nlines, line = L - band * t.nlines;
       box[band * t.nr + static_cast<long long>(t.line_i12[line]) * t.n3 + p] = res[l * a.ld + p];
     }
@@ -125,9 +120,15 @@ __global__ __launch_bounds__(256) void z_bwd_kernel(const cplx_t<R> *psi, long l
 // the forward pass loads all and stores active lines only.  Each thread keeps
 // one line (M divides the 256 threads) and walks its points.
 template <typename R, bool BWD>
-__global__ __launch_bounds__(256) void y_kernel(cplx_t<R> *box, Tables t, FftArgs a) {
+__global__ __launch_bounds__(256) void y_kernel(cplx_t<R> *box, const LocalTables *tabs, KStride ks, int nb,
+                                                FftArgs a) {
   using V = cplx_t<R>;
   constexpr int K = Tile<R>::KMAX;
+  const int kp = blockIdx.y;
+  const LocalTables t = tabs[kp];
+  const long long lines_total = static_cast<long long>(nb) * t.nplanes * t.n3;
+  if (static_cast<long long>(blockIdx.x) * a.M >= lines_total) return;
+  box += kp * ks.box;
   extern __shared__ double2 smem_raw[];
   V *buf0 = reinterpret_cast<V *>(smem_raw);
   V *buf1 = buf0 + a.M * a.ld;
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(256) void y_kernel(cplx_t<R> *box, Tables t, FftArg
   load_twiddles(tws, a);
   const int l = threadIdx.x % a.M, p0 = threadIdx.x / a.M, pstep = blockDim.x / a.M;
   const int L = blockIdx.x * a.M + l;
-  const bool live = L < a.lines_total;
+  const bool live = L < lines_total;
   const int per_band = t.nplanes * t.n3;
   const int band = live ? L / per_band : 0;
   const int rem = L - band * per_band;
@@ -168,10 +169,13 @@ __global__ __launch_bounds__(256) void y_kernel(cplx_t<R> *box, Tables t, FftArg
 // takes Mc consecutive (i2,i3) columns for G bands (M = G * Mc lines); the
 // Veff tile of the Mc columns is read once into LDS and shared by the bands.
 template <typename R>
-__global__ __launch_bounds__(256) void x_veff_kernel(cplx_t<R> *box, const R *veff, R scale, Tables t, FftArgs a,
-                                                     int G, int nb) {
+__global__ __launch_bounds__(256) void x_veff_kernel(cplx_t<R> *box, const R *veff, R scale,
+                                                     const LocalTables *tabs, KStride ks, FftArgs a, int G, int nb) {
   using V = cplx_t<R>;
   constexpr int K = Tile<R>::KMAX;
+  const int kp = blockIdx.y;
+  const LocalTables t = tabs[kp];
+  box += kp * ks.box;
   extern __shared__ double2 smem_raw[];
   V *buf0 = reinterpret_cast<V *>(smem_raw);
   V *buf1 = buf0 + a.M * a.ld;
@@ -220,163 +224,34 @@ __global__ __launch_bounds__(256) void x_veff_kernel(cplx_t<R> *box, const R *ve
   }
 }
 
-// Plane-fused z and y passes.  Workgroup = one (band, active plane q).
-// LDS: the plane pb[n2][n3 + 1], two transform batches of M lines of
-// stride ldw = max(n2, n3) + 1, and both axes' twiddles.
-struct PlaneArgs {
-  const int *line0, *nline;  // lines of plane q (signed plane order)
-  int P, M, ldp, ldw;
-};
-
-template <typename R>
-__device__ inline void plane_lds(cplx_t<R> *smem, const PlaneArgs &pa, int n2, int n3, cplx_t<R> *&pb,
-                                 cplx_t<R> *&w0, cplx_t<R> *&w1, cplx_t<R> *&tz, cplx_t<R> *&ty) {
-  pb = smem;
-  w0 = pb + n2 * pa.ldp;
-  w1 = w0 + pa.M * pa.ldw;
-  tz = w1 + pa.M * pa.ldw;
-  ty = tz + n3;
-}
-
-// backward: psi -> plane q of `planes` (transform along i3 of the plane's
-// active lines, then along i2 of every column)
-template <typename R>
-__global__ __launch_bounds__(256) void plane_bwd_kernel(const cplx_t<R> *psi, long long ldpsi, cplx_t<R> *planes,
-                                                        Tables t, PlaneArgs pa, FftArgs az, FftArgs ay) {
-  using V = cplx_t<R>;
-  extern __shared__ double2 smem_raw[];
-  V *pb, *w0, *w1, *tz, *ty;
-  plane_lds<R>(reinterpret_cast<V *>(smem_raw), pa, t.n2, t.n3, pb, w0, w1, tz, ty);
-  load_twiddles(tz, az);
-  load_twiddles(ty, ay);
-  const int q = blockIdx.x % pa.P, band = blockIdx.x / pa.P;
-  const int n2 = t.n2, n3 = t.n3, M = pa.M;
-  for (int e = threadIdx.x; e < n2 * pa.ldp; e += blockDim.x) pb[e] = mk<V>(R(0), R(0));
-  const int l0 = pa.line0[q], nl = pa.nline[q];
-  const V *pband = psi + band * ldpsi;
-  for (int c0 = 0; c0 < nl; c0 += M) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < M * pa.ldw; e += blockDim.x) w0[e] = mk<V>(R(0), R(0));
-    __syncthreads();
-    for (int e = threadIdx.x; e < M * n3; e += blockDim.x) {
-      const int li = e / n3, j = e - li * n3;
-      if (c0 + li < nl) {
-        const int line = l0 + c0 + li;
-        if (j < t.line_count[line]) {
-          const int g = t.line_start[line] + j;
-          w0[li * pa.ldw + t.pw_i3[g]] = pband[g];
-        }
-      }
-    }
-    __syncthreads();
-    const V *res = run_stages(w0, w1, tz, az, +1);
-    for (int e = threadIdx.x; e < M * n3; e += blockDim.x) {
-      const int li = e / n3, j = e - li * n3;
-      if (c0 + li < nl) pb[(t.line_i12[l0 + c0 + li] % n2) * pa.ldp + j] = res[li * pa.ldw + j];
-    }
-  }
-  for (int j0 = 0; j0 < n3; j0 += M) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < M * n2; e += blockDim.x) {
-      const int ci = e / n2, i2 = e - ci * n2;
-      w0[ci * pa.ldw + i2] = j0 + ci < n3 ? pb[i2 * pa.ldp + j0 + ci] : mk<V>(R(0), R(0));
-    }
-    __syncthreads();
-    const V *res = run_stages(w0, w1, ty, ay, +1);
-    for (int e = threadIdx.x; e < M * n2; e += blockDim.x) {
-      const int ci = e / n2, i2 = e - ci * n2;
-      if (j0 + ci < n3) pb[i2 * pa.ldp + j0 + ci] = res[ci * pa.ldw + i2];
-    }
-  }
-  __syncthreads();
-  V *out = planes + (static_cast<long long>(band) * pa.P + q) * n2 * n3;
-  for (int e = threadIdx.x; e < n2 * n3; e += blockDim.x) {
-    const int i2 = e / n3, i3 = e - i2 * n3;
-    out[e] = pb[i2 * pa.ldp + i3];
-  }
-}
-
-// forward: plane q -> hpsi = (transform) + T psi, spsi = psi for the plane's
-// plane waves (transform along i2 of every column, then along i3 of the
-// active lines, then gather)
-template <typename R>
-__global__ __launch_bounds__(256) void plane_fwd_kernel(const cplx_t<R> *psi, long long ldpsi, const cplx_t<R> *planes,
-                                                        const R *kin, cplx_t<R> *hpsi, cplx_t<R> *spsi,
-                                                        long long ldout, Tables t, PlaneArgs pa, FftArgs az,
-                                                        FftArgs ay) {
-  using V = cplx_t<R>;
-  extern __shared__ double2 smem_raw[];
-  V *pb, *w0, *w1, *tz, *ty;
-  plane_lds<R>(reinterpret_cast<V *>(smem_raw), pa, t.n2, t.n3, pb, w0, w1, tz, ty);
-  load_twiddles(tz, az);
-  load_twiddles(ty, ay);
-  const int q = blockIdx.x % pa.P, band = blockIdx.x / pa.P;
-  const int n2 = t.n2, n3 = t.n3, M = pa.M;
-  const V *in = planes + (static_cast<long long>(band) * pa.P + q) * n2 * n3;
-  for (int e = threadIdx.x; e < n2 * n3; e += blockDim.x) {
-    const int i2 = e / n3, i3 = e - i2 * n3;
-    pb[i2 * pa.ldp + i3] = in[e];
-  }
-  for (int j0 = 0; j0 < n3; j0 += M) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < M * n2; e += blockDim.x) {
-      const int ci = e / n2, i2 = e - ci * n2;
-      w0[ci * pa.ldw + i2] = j0 + ci < n3 ? pb[i2 * pa.ldp + j0 + ci] : mk<V>(R(0), R(0));
-    }
-    __syncthreads();
-    const V *res = run_stages(w0, w1, ty, ay, -1);
-    for (int e = threadIdx.x; e < M * n2; e += blockDim.x) {
-      const int ci = e / n2, i2 = e - ci * n2;
-      if (j0 + ci < n3) pb[i2 * pa.ldp + j0 + ci] = res[ci * pa.ldw + i2];
-    }
-  }
-  const int l0 = pa.line0[q], nl = pa.nline[q];
-  const V *pband = psi + band * ldpsi;
-  for (int c0 = 0; c0 < nl; c0 += M) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < M * n3; e += blockDim.x) {
-      const int li = e / n3, j = e - li * n3;
-      w0[li * pa.ldw + j] = c0 + li < nl ? pb[(t.line_i12[l0 + c0 + li] % n2) * pa.ldp + j] : mk<V>(R(0), R(0));
-    }
-    __syncthreads();
-    const V *res = run_stages(w0, w1, tz, az, -1);
-    for (int e = threadIdx.x; e < M * n3; e += blockDim.x) {
-      const int li = e / n3, j = e - li * n3;
-      if (c0 + li >= nl) continue;
-      const int line = l0 + c0 + li;
-      if (j < t.line_count[line]) {
-        const int g = t.line_start[line] + j;
-        const V p = pband[g];
-        const V r = res[li * pa.ldw + t.pw_i3[g]];
-        hpsi[g + band * ldout] = mk<V>(r.x + kin[g] * p.x, r.y + kin[g] * p.y);
-        spsi[g + band * ldout] = p;
-      }
-    }
-  }
-}
-
 // x pass as a convolution over the active planes (see the header), for a
 // contiguous run of P planes s0 .. s0 + P - 1 (signed, modulo n1): with the
 // planes in that order, plane p' takes sum_p Vx[(p' - p) mod n1] in[p].  A
 // workgroup takes kXcCols consecutive (i2,i3) columns and kXcBands bands;
 // lanes 0-31 and 32-63 of a wave hold two bands of the same columns.  LDS
-// holds the 2P - 1 needed rows of Vx ordered by the difference p' - p + P - 1,
-// so every term of the sum is one LDS read at a fixed offset; each thread
-// keeps its column's P values in registers (PMAX >= P) and overwrites them.
+// holds the 2 pw - 1 rows of Vx around the difference 0, ordered by
+// p' - p + pw - 1 (pw = planes of the widest k point of the batch; narrower
+// k points use the middle rows), so every term of the sum is one LDS read at
+// a fixed offset; each thread keeps its column's P values in registers
+// (PMAX >= P) and overwrites them.
 constexpr int kXcCols = 32;
 constexpr int kXcBands = 8;
 
 template <typename R, int PMAX>
-__global__ __launch_bounds__(256) void x_conv_kernel(cplx_t<R> *box, const cplx_t<R> *vx, Tables t, int nb) {
+__global__ __launch_bounds__(256) void x_conv_kernel(cplx_t<R> *box, const cplx_t<R> *vx, const LocalTables *tabs,
+                                                     KStride ks, int nb, int pw) {
   using V = cplx_t<R>;
+  const int kp = blockIdx.z;
+  const LocalTables t = tabs[kp];
+  box += kp * ks.box;
   extern __shared__ double2 smem_raw[];
-  V *vs = reinterpret_cast<V *>(smem_raw);  // [2P - 1][kXcCols]
+  V *vs = reinterpret_cast<V *>(smem_raw);  // [2 pw - 1][kXcCols]
   const int C = t.n2 * t.n3;
   const int P = t.nplanes;
   const int c0 = blockIdx.x * kXcCols;
-  for (int e = threadIdx.x; e < (2 * P - 1) * kXcCols; e += blockDim.x) {
+  for (int e = threadIdx.x; e < (2 * pw - 1) * kXcCols; e += blockDim.x) {
     const int k = e / kXcCols, c = c0 + (e - k * kXcCols);
-    int d = (k - (P - 1)) % t.n1;
+    int d = (k - (pw - 1)) % t.n1;
     d += d < 0 ? t.n1 : 0;
     vs[e] = c < C ? vx[static_cast<long long>(d) * C + c] : mk<V>(R(0), R(0));
   }
@@ -386,18 +261,17 @@ __global__ __launch_bounds__(256) void x_conv_kernel(cplx_t<R> *box, const cplx_
   const int c = c0 + ci;
   const int band = blockIdx.y * kXcBands + wave * 2 + (lane >> 5);
   if (c >= C || band >= nb) return;  // no barrier follows
-  V *col = box + (t.compact ? static_cast<long long>(band) * P * C : static_cast<long long>(band) * t.nr) + c;
+  V *col = box + static_cast<long long>(band) * t.nr + c;
   const int s0 = t.plane_s0 < 0 ? t.plane_s0 + t.n1 : t.plane_s0;
   V in[PMAX];
 #pragma unroll
   for (int q = 0; q < PMAX; ++q) {
     int i1 = s0 + q;
     i1 -= i1 >= t.n1 ? t.n1 : 0;
-    if (t.compact) i1 = q;
     in[q] = q < P ? col[static_cast<long long>(i1) * C] : mk<V>(R(0), R(0));
   }
   for (int op = 0; op < P; ++op) {
-    const V *w = vs + (op + P - 1) * kXcCols + ci;  // w[-q kXcCols] = Vx[(op - q) mod n1]
+    const V *w = vs + (op + pw - 1) * kXcCols + ci;  // w[-q kXcCols] = Vx[(op - q) mod n1]
     R ar = 0, ai = 0;
 #pragma unroll
     for (int q = 0; q < PMAX; ++q) {
@@ -409,7 +283,6 @@ __global__ __launch_bounds__(256) void x_conv_kernel(cplx_t<R> *box, const cplx_
     }
     int i1 = s0 + op;
     i1 -= i1 >= t.n1 ? t.n1 : 0;
-    if (t.compact) i1 = op;
     col[static_cast<long long>(i1) * C] = mk<V>(ar, ai);
   }
 }
@@ -436,15 +309,25 @@ __global__ void veff_x_dft_kernel(const double *veff, double2 *vx, float2 *vx32,
 template <typename R>
 __global__ __launch_bounds__(256) void z_fwd_kernel(const cplx_t<R> *psi, long long ldpsi, const cplx_t<R> *box,
                                                     const R *kin, cplx_t<R> *hpsi, cplx_t<R> *spsi,
-                                                    long long ldout, Tables t, FftArgs a) {
+                                                    long long ldout, const LocalTables *tabs, KStride ks, int nb,
+                                                    FftArgs a) {
   using V = cplx_t<R>;
   constexpr int K = Tile<R>::KMAX;
+  const int kp = blockIdx.y;
+  const LocalTables t = tabs[kp];
+  const long long lines_total = static_cast<long long>(nb) * t.nlines;
+  const int L0 = blockIdx.x * a.M;
+  if (L0 >= lines_total) return;
+  psi += kp * ks.psi;
+  box += kp * ks.box;
+  kin += kp * ks.kin;
+  hpsi += kp * ks.out;
+  spsi += kp * ks.out;
   extern __shared__ double2 smem_raw[];
   V *buf0 = reinterpret_cast<V *>(smem_raw);
   V *buf1 = buf0 + a.M * a.ld;
   V *tws = buf1 + a.M * a.ld;
   load_twiddles(tws, a);
-  const int L0 = blockIdx.x * a.M;
   const int MN = a.M * a.n;
   V v[K];
 #pragma unroll
@@ -454,7 +337,7 @@ __global__ __launch_bounds__(256) void z_fwd_kernel(const cplx_t<R> *psi, long l
     if (e < MN) {
       const int l = e / a.n, p = e - l * a.n;
       const int L = L0 + l;
-      if (L < a.lines_total) {
+      if (L < lines_total) {
         const int band = L / t.nlines, line = L - band * t.nlines;
         v[k] = box[band * t.nr + static_cast<long long>(t.line_i12[line]) * t.n3 + p];
       }
@@ -476,7 +359,7 @@ __global__ __launch_bounds__(256) void z_fwd_kernel(const cplx_t<R> *psi, long l
     if (e >= MN) break;
     const int l = e / a.n, j = e - l * a.n;
     const int L = L0 + l;
-    if (L >= a.lines_total) continue;
+    if (L >= lines_total) continue;
     const int band = L / t.nlines, line = L - band * t.nlines;
     if (j < t.line_count[line]) {
       const int g = t.line_start[line] + j;
@@ -517,11 +400,28 @@ size_t lds_bytes(const FftArgs &a) {
 }
 
 template <typename R>
-void launch(const char *name, const void *fn, const FftArgs &a, hipStream_t s, void **args, long long blocks = -1,
+void launch(const char *name, const void *fn, const FftArgs &a, hipStream_t s, void **args, dim3 grid,
             size_t extra_lds = 0) {
   trace_mark(name, s);
-  if (blocks < 0) blocks = (a.lines_total + a.M - 1) / a.M;
-  HALF_HIP(hipLaunchKernel(fn, dim3(static_cast<unsigned>(blocks)), dim3(256), args, lds_bytes<R>(a) + extra_lds, s));
+  HALF_HIP(hipLaunchKernel(fn, grid, dim3(256), args, lds_bytes<R>(a) + extra_lds, s));
+}
+
+// register budget of x_conv_kernel per precision: the largest PMAX instantiated
+template <typename R>
+const void *x_conv_for(int planes, int &pmax) {
+  if constexpr (std::is_same<R, double>::value) {
+    if (planes <= 16) return pmax = 16, reinterpret_cast<const void *>(&x_conv_kernel<R, 16>);
+    if (planes <= 24) return pmax = 24, reinterpret_cast<const void *>(&x_conv_kernel<R, 24>);
+    if (planes <= 32) return pmax = 32, reinterpret_cast<const void *>(&x_conv_kernel<R, 32>);
+    if (planes <= 48) return pmax = 48, reinterpret_cast<const void *>(&x_conv_kernel<R, 48>);
+  } else {
+    if (planes <= 16) return pmax = 16, reinterpret_cast<const void *>(&x_conv_kernel<R, 16>);
+    if (planes <= 32) return pmax = 32, reinterpret_cast<const void *>(&x_conv_kernel<R, 32>);
+    if (planes <= 48) return pmax = 48, reinterpret_cast<const void *>(&x_conv_kernel<R, 48>);
+    if (planes <= 64) return pmax = 64, reinterpret_cast<const void *>(&x_conv_kernel<R, 64>);
+  }
+  pmax = 0;
+  return nullptr;
 }
 
 }  // namespace
@@ -552,32 +452,13 @@ void LocalMap::build(const Basis &b, hipStream_t s) {
     if (plane_on_h[i]) planes.push_back(i);
   // signed plane indices in [-n1/2, (n1-1)/2], the basis's enumeration
   // range: one contiguous run for a sphere
-  auto signed_of = [&](int i) { return i <= (n1 - 1) / 2 ? i : i - n1; };
   std::vector<int> signed_planes;
-  for (int i : planes) signed_planes.push_back(signed_of(i));
+  for (int i : planes) signed_planes.push_back(i <= (n1 - 1) / 2 ? i : i - n1);
   std::sort(signed_planes.begin(), signed_planes.end());
   planes_contiguous = !signed_planes.empty();
   for (size_t q = 1; q < signed_planes.size(); ++q)
     if (signed_planes[q] != signed_planes[0] + static_cast<int>(q)) planes_contiguous = false;
   plane_s0 = signed_planes.empty() ? 0 : signed_planes[0];
-  // lines of each plane in signed plane order; the basis enumerates the
-  // signed i1 outermost, so each plane's lines form one run
-  std::vector<int> q_of(n1, -1);
-  for (size_t q = 0; q < signed_planes.size(); ++q) q_of[(signed_planes[q] % n1 + n1) % n1] = static_cast<int>(q);
-  std::vector<int> qline0(signed_planes.size(), -1), qnline(signed_planes.size(), 0);
-  lines_by_plane = true;
-  int prev_q = -1;
-  for (size_t l = 0; l < lines.size(); ++l) {
-    const int q = q_of[lines[l] / n2];
-    if (q != prev_q) {
-      if (q < prev_q || qline0[q] >= 0) lines_by_plane = false;
-      qline0[q] = static_cast<int>(l);
-      prev_q = q;
-    }
-    ++qnline[q];
-  }
-  qplane_line0.upload(qline0, s);
-  qplane_nline.upload(qnline, s);
   nlines = static_cast<int>(lines.size());
   nplanes = static_cast<int>(planes.size());
   line_i12.upload(lines, s);
@@ -587,25 +468,28 @@ void LocalMap::build(const Basis &b, hipStream_t s) {
   plane_i1.upload(planes, s);
   plane_on.upload(plane_on_h, s);
   pw_i3.upload(i3, s);
+  const LocalTables t = tables();
+  dtab.upload(&t, 1, s);
   HALF_HIP(hipStreamSynchronize(s));
 }
 
-// register budget of x_conv_kernel per precision: the largest PMAX instantiated
-template <typename R>
-const void *x_conv_for(int planes, int &pmax) {
-  if constexpr (std::is_same<R, double>::value) {
-    if (planes <= 16) return pmax = 16, reinterpret_cast<const void *>(&x_conv_kernel<R, 16>);
-    if (planes <= 24) return pmax = 24, reinterpret_cast<const void *>(&x_conv_kernel<R, 24>);
-    if (planes <= 32) return pmax = 32, reinterpret_cast<const void *>(&x_conv_kernel<R, 32>);
-    if (planes <= 48) return pmax = 48, reinterpret_cast<const void *>(&x_conv_kernel<R, 48>);
-  } else {
-    if (planes <= 16) return pmax = 16, reinterpret_cast<const void *>(&x_conv_kernel<R, 16>);
-    if (planes <= 32) return pmax = 32, reinterpret_cast<const void *>(&x_conv_kernel<R, 32>);
-    if (planes <= 48) return pmax = 48, reinterpret_cast<const void *>(&x_conv_kernel<R, 48>);
-    if (planes <= 64) return pmax = 64, reinterpret_cast<const void *>(&x_conv_kernel<R, 64>);
-  }
-  pmax = 0;
-  return nullptr;
+LocalTables LocalMap::tables() const {
+  LocalTables t{};
+  t.n1 = n1;
+  t.n2 = n2;
+  t.n3 = n3;
+  t.nr = static_cast<long long>(n1) * n2 * n3;
+  t.nlines = nlines;
+  t.line_i12 = line_i12.get();
+  t.line_start = line_start.get();
+  t.line_count = line_count.get();
+  t.line_of = line_of.get();
+  t.nplanes = nplanes;
+  t.plane_i1 = plane_i1.get();
+  t.plane_on = plane_on.get();
+  t.pw_i3 = pw_i3.get();
+  t.plane_s0 = plane_s0;
+  return t;
 }
 
 void build_veff_x_dft(const double *veff, int n1, int n2, int n3, double2 *vx, float2 *vx32, hipStream_t s) {
@@ -617,100 +501,82 @@ void build_veff_x_dft(const double *veff, int n1, int n2, int n3, double2 *vx, f
 }
 
 template <typename R>
-void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const cplx_t<R> *vx, const R *kin, int nb,
-                 const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi, int64_t ldout,
-                 cplx_t<R> *boxes, hipStream_t s) {
+void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff, const cplx_t<R> *vx, const R *kin,
+                       int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
+                       int64_t ldout, cplx_t<R> *boxes, hipStream_t s) {
   using V = cplx_t<R>;
-  if (nb <= 0 || m.nlines == 0) return;
-  if (static_cast<long long>(nb) * (static_cast<long long>(m.n1) * m.n2 + static_cast<long long>(m.n2) * m.n3 +
-                                   static_cast<long long>(m.n1) * m.n3) >= (1ll << 31))
+  const int n1 = plan.n[0], n2 = plan.n[1], n3 = plan.n[2];
+  if (nb <= 0 || kb.nk <= 0 || kb.max_lines == 0) return;
+  if (kb.nk > 65535) throw Error(kInvalidArgument, "too many k points in one batch");
+  if (static_cast<long long>(nb) * (static_cast<long long>(n1) * n2 + static_cast<long long>(n2) * n3 +
+                                   static_cast<long long>(n1) * n3) >= (1ll << 31))
     throw Error(kInvalidArgument, "band batch too large for the local stage");
-  Tables t{m.n1, m.n2, m.n3, static_cast<long long>(m.n1) * m.n2 * m.n3, m.nlines, m.line_i12.get(),
-           m.line_start.get(), m.line_count.get(), m.line_of.get(), m.nplanes, m.plane_i1.get(), m.plane_on.get(),
-           m.pw_i3.get(), m.plane_s0, 0};
   long long lp = ldpsi, lo = ldout;
+  KStride ks{kb.psi_k, kb.out_k, kb.kin_k, kb.box_k};
+  const LocalTables *tabs = kb.tabs;
   const int prec = std::is_same<R, double>::value ? 0 : 1;
-  const int C = m.n2 * m.n3;
-
-  // x pass form: per-column convolution where the active planes are one run
-  // and fit the register budget, else transform / multiply / transform
+  const int C = n2 * n3;
+  const unsigned nk = static_cast<unsigned>(kb.nk);
+  FftArgs az = axis_args<R>(plan, 2, 0, 1);
+  FftArgs ay = axis_args<R>(plan, 1, 0, n3);
+  auto blocks = [](long long lines, int M) { return static_cast<unsigned>((lines + M - 1) / M); };
+  const dim3 gz(blocks(static_cast<long long>(nb) * kb.max_lines, az.M), nk);
+  const dim3 gy(blocks(static_cast<long long>(nb) * kb.max_planes * n3, ay.M), nk);
+  int nbi = nb;
+  {
+    void *args[] = {&psi, &lp, &boxes, &tabs, &ks, &nbi, &az};
+    launch<R>("local_z_bwd", reinterpret_cast<const void *>(&z_bwd_kernel<R>), az, s, args, gz);
+  }
+  {
+    void *args[] = {&boxes, &tabs, &ks, &nbi, &ay};
+    launch<R>("local_y_bwd", reinterpret_cast<const void *>(&y_kernel<R, true>), ay, s, args, gy);
+  }
+  // x pass: per-column convolution where every k point's active planes are
+  // one run and the widest fits the register budget, else transform-based
   int pmax = 0;
-  const size_t xconv_lds = size_t(2 * m.nplanes - 1) * kXcCols * sizeof(V);
-  const bool xconv_ok = vx && g_tune.xconv[prec] && m.planes_contiguous && m.nplanes <= m.n1 && xconv_lds <= 65536;
-  const void *xconv = xconv_ok ? x_conv_for<R>(m.nplanes, pmax) : nullptr;
-
-  // z and y fused per plane when a plane and two transform batches fit LDS
-  const int ldp = m.n3 + 1, ldw = std::max(m.n2, m.n3) + 1;
-  int Mp = g_tune.plane[prec];
-  auto plane_bytes = [&](int M) { return (size_t(m.n2) * ldp + 2 * size_t(M) * ldw + m.n2 + m.n3) * sizeof(V); };
-  while (Mp > 1 && plane_bytes(Mp) > 65536) Mp /= 2;
-  const bool fused = xconv && Mp >= 1 && m.lines_by_plane && plane_bytes(Mp) <= 65536;
-
-  if (fused) {
-    t.compact = 1;
-    FftArgs az = axis_args<R>(plan, 2, 0, 1), ay = axis_args<R>(plan, 1, 0, m.n3);
-    az.M = ay.M = Mp;
-    az.ld = ay.ld = ldw;
-    PlaneArgs pa{m.qplane_line0.get(), m.qplane_nline.get(), m.nplanes, Mp, ldp, ldw};
-    const dim3 grid(static_cast<unsigned>(int64_t(m.nplanes) * nb));
-    const size_t lds = plane_bytes(Mp);
-    {
-      void *args[] = {&psi, &lp, &boxes, &t, &pa, &az, &ay};
-      trace_mark("local_plane_bwd", s);
-      HALF_HIP(hipLaunchKernel(reinterpret_cast<const void *>(&plane_bwd_kernel<R>), grid, dim3(256), args, lds, s));
-    }
-    {
-      int nbi = nb;
-      void *args[] = {&boxes, &vx, &t, &nbi};
-      trace_mark("local_x_conv", s);
-      HALF_HIP(hipLaunchKernel(xconv, dim3((C + kXcCols - 1) / kXcCols, (nb + kXcBands - 1) / kXcBands), dim3(256),
-                               args, xconv_lds, s));
-    }
-    {
-      void *args[] = {&psi, &lp, &boxes, &kin, &hpsi, &spsi, &lo, &t, &pa, &az, &ay};
-      trace_mark("local_plane_fwd", s);
-      HALF_HIP(hipLaunchKernel(reinterpret_cast<const void *>(&plane_fwd_kernel<R>), grid, dim3(256), args, lds, s));
-    }
-    return;
-  }
-
-  FftArgs az = axis_args<R>(plan, 2, static_cast<long long>(nb) * m.nlines, 1);
-  FftArgs ay = axis_args<R>(plan, 1, static_cast<long long>(nb) * m.nplanes * m.n3, m.n3);
-  int G = std::max(1, std::min(g_tune.xgroup[prec], nb));
-  while (G > 1 && (G & (G - 1))) --G;  // power of two, divides M
-  FftArgs ax = axis_args<R>(plan, 0, static_cast<long long>(nb) * m.n2 * m.n3, m.n2 * m.n3, G);
-  if (ax.M % G) G = 1;
-  const int Mc = ax.M / G;
-  const long long xblocks = static_cast<long long>((m.n2 * m.n3 + Mc - 1) / Mc) * ((nb + G - 1) / G);
-  R scale = static_cast<R>(1.0 / double(t.nr));
-  {
-    void *args[] = {&psi, &lp, &boxes, &t, &az};
-    launch<R>("local_z_bwd", reinterpret_cast<const void *>(&z_bwd_kernel<R>), az, s, args);
-  }
-  {
-    void *args[] = {&boxes, &t, &ay};
-    launch<R>("local_y_bwd", reinterpret_cast<const void *>(&y_kernel<R, true>), ay, s, args);
-  }
+  int pw = kb.max_planes;
+  const size_t xconv_lds = size_t(2 * pw - 1) * kXcCols * sizeof(V);
+  const bool xconv_ok = vx && g_tune.xconv[prec] && kb.contiguous && pw <= n1 && xconv_lds <= 65536;
+  const void *xconv = xconv_ok ? x_conv_for<R>(pw, pmax) : nullptr;
   if (xconv) {
-    int nbi = nb;
-    void *args[] = {&boxes, &vx, &t, &nbi};
+    void *args[] = {&boxes, &vx, &tabs, &ks, &nbi, &pw};
     trace_mark("local_x_conv", s);
-    HALF_HIP(hipLaunchKernel(xconv, dim3((C + kXcCols - 1) / kXcCols, (nb + kXcBands - 1) / kXcBands), dim3(256),
-                             args, xconv_lds, s));
+    HALF_HIP(hipLaunchKernel(xconv, dim3((C + kXcCols - 1) / kXcCols, (nb + kXcBands - 1) / kXcBands, nk),
+                             dim3(256), args, xconv_lds, s));
   } else {
-    int nbi = nb;
-    void *args[] = {&boxes, &veff, &scale, &t, &ax, &G, &nbi};
-    launch<R>("local_x_veff", reinterpret_cast<const void *>(&x_veff_kernel<R>), ax, s, args, xblocks,
+    int G = std::max(1, std::min(g_tune.xgroup[prec], nb));
+    while (G > 1 && (G & (G - 1))) --G;  // power of two, divides M
+    FftArgs ax = axis_args<R>(plan, 0, 0, C, G);
+    if (ax.M % G) G = 1;
+    const int Mc = ax.M / G;
+    const dim3 gx(static_cast<unsigned>(((C + Mc - 1) / Mc) * ((nb + G - 1) / G)), nk);
+    R scale = static_cast<R>(1.0 / double(static_cast<long long>(n1) * C));
+    void *args[] = {&boxes, &veff, &scale, &tabs, &ks, &ax, &G, &nbi};
+    launch<R>("local_x_veff", reinterpret_cast<const void *>(&x_veff_kernel<R>), ax, s, args, gx,
               size_t(Mc) * ax.n * sizeof(R));
   }
   {
-    void *args[] = {&boxes, &t, &ay};
-    launch<R>("local_y_fwd", reinterpret_cast<const void *>(&y_kernel<R, false>), ay, s, args);
+    void *args[] = {&boxes, &tabs, &ks, &nbi, &ay};
+    launch<R>("local_y_fwd", reinterpret_cast<const void *>(&y_kernel<R, false>), ay, s, args, gy);
   }
   {
-    void *args[] = {&psi, &lp, &boxes, &kin, &hpsi, &spsi, &lo, &t, &az};
-    launch<R>("local_z_fwd", reinterpret_cast<const void *>(&z_fwd_kernel<R>), az, s, args);
+    void *args[] = {&psi, &lp, &boxes, &kin, &hpsi, &spsi, &lo, &tabs, &ks, &nbi, &az};
+    launch<R>("local_z_fwd", reinterpret_cast<const void *>(&z_fwd_kernel<R>), az, s, args, gz);
   }
+}
+
+template <typename R>
+void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const cplx_t<R> *vx, const R *kin, int nb,
+                 const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi, int64_t ldout,
+                 cplx_t<R> *boxes, hipStream_t s) {
+  if (m.nlines == 0) return;
+  LocalBatch kb;
+  kb.nk = 1;
+  kb.tabs = m.dtab.get();
+  kb.max_lines = m.nlines;
+  kb.max_planes = m.nplanes;
+  kb.contiguous = m.planes_contiguous;
+  local_apply_batch<R>(plan, kb, veff, vx, kin, nb, psi, ldpsi, hpsi, spsi, ldout, boxes, s);
 }
 
 bool set_local_tuning(const std::string &key, int value) {
@@ -731,11 +597,6 @@ bool set_local_tuning(const std::string &key, int value) {
     if (key == pre + ".xconv") {
       if (value < 0 || value > 1) return false;
       g_tune.xconv[p] = value;
-      return true;
-    }
-    if (key == pre + ".plane") {
-      if (value < 0 || value > 64 || (value & (value - 1))) return false;
-      g_tune.plane[p] = value;
       return true;
     }
   }
@@ -759,18 +620,19 @@ bool get_local_tuning(const std::string &key, int &value) {
       value = g_tune.xconv[p];
       return true;
     }
-    if (key == pre + ".plane") {
-      value = g_tune.plane[p];
-      return true;
-    }
   }
   return false;
 }
 
-template void local_apply<double>(const FftPlan &, const LocalMap &, const double *, const double2 *,
-                                  const double *, int, const double2 *, int64_t, double2 *, double2 *, int64_t,
-                                  double2 *, hipStream_t);
-template void local_apply<float>(const FftPlan &, const LocalMap &, const float *, const float2 *, const float *, int,
-                                 const float2 *, int64_t, float2 *, float2 *, int64_t, float2 *, hipStream_t);
+#define HALF_LOCAL(R)                                                                                              \
+  template void local_apply_batch<R>(const FftPlan &, const LocalBatch &, const R *, const cplx_t<R> *, const R *, \
+                                     int, const cplx_t<R> *, int64_t, cplx_t<R> *, cplx_t<R> *, int64_t,             \
+                                     cplx_t<R> *, hipStream_t);                                                    \
+  template void local_apply<R>(const FftPlan &, const LocalMap &, const R *, const cplx_t<R> *, const R *, int,     \
+                               const cplx_t<R> *, int64_t, cplx_t<R> *, cplx_t<R> *, int64_t, cplx_t<R> *,         \
+                               hipStream_t);
+HALF_LOCAL(double)
+HALF_LOCAL(float)
+#undef HALF_LOCAL
 
 }  // namespace half

@@ -7,6 +7,19 @@
 
 namespace half {
 
+// Index tables of one k point's plane-wave sphere on the FFT box, as the
+// kernels read them (device pointers into a LocalMap).
+struct LocalTables {
+  int n1, n2, n3;
+  long long nr;
+  int nlines;  // (i1, i2) lines holding plane waves
+  const int *line_i12, *line_start, *line_count, *line_of;
+  int nplanes;  // i1 planes holding plane waves
+  const int *plane_i1, *plane_on;
+  const int *pw_i3;
+  int plane_s0;  // first signed plane of the contiguous run
+};
+
 // Index tables of the plane-wave sphere on the FFT box at one k point.
 struct LocalMap {
   int n1 = 0, n2 = 0, n3 = 0;
@@ -16,17 +29,33 @@ struct LocalMap {
   // nplanes - 1 (modulo n1) when planes_contiguous (true for a sphere)
   int plane_s0 = 0;
   bool planes_contiguous = false;
-  // lines of each active plane in signed order q (plane q holds lines
-  // qplane_line0[q] .. + qplane_nline[q] - 1); valid when lines_by_plane
-  bool lines_by_plane = false;
   DeviceArray<int> line_i12, line_start, line_count, line_of, plane_i1, plane_on, pw_i3;
-  DeviceArray<int> qplane_line0, qplane_nline;
+  DeviceArray<LocalTables> dtab;  // one-entry device copy of tables()
   void build(const Basis &b, hipStream_t s);
+  LocalTables tables() const;
 };
 
-// hpsi = T psi + F[Veff F^-1[psi]]/N_r, spsi = psi for nb bands; boxes holds
-// nb full boxes of workspace.  vx is the i1-axis DFT of Veff/N_r from
-// build_veff_x_dft (null: transform-based x pass).
+// A batch of k points sharing the FFT box: k point j's tables are tabs[j]
+// (device array), and its states, outputs, kinetic energies and workspace
+// boxes sit at base + j * stride of the arrays passed to local_apply_batch.
+struct LocalBatch {
+  int nk = 1;
+  const LocalTables *tabs = nullptr;
+  int max_lines = 0, max_planes = 0;
+  bool contiguous = false;  // every k point's active planes form one run
+  long long psi_k = 0, out_k = 0, kin_k = 0, box_k = 0;
+};
+
+// hpsi = T psi + F[Veff F^-1[psi]]/N_r, spsi = psi for nb bands of every k
+// point of the batch; boxes holds nb full boxes per k point.  vx is the
+// i1-axis DFT of Veff/N_r from build_veff_x_dft (null: transform-based x
+// pass).
+template <typename R>
+void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff, const cplx_t<R> *vx, const R *kin,
+                       int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
+                       int64_t ldout, cplx_t<R> *boxes, hipStream_t s);
+
+// One k point.
 template <typename R>
 void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const cplx_t<R> *vx, const R *kin, int nb,
                  const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi, int64_t ldout,
@@ -38,12 +67,9 @@ void build_veff_x_dft(const double *veff, int n1, int n2, int n3, double2 *vx, f
 
 // Tunables of the local stage, keys "local.{fp64|fp32}.tile.{x|y|z}" (FFT
 // points per workgroup tile), "local.{fp64|fp32}.xgroup" (bands sharing a
-// Veff tile in the transform-based x pass), "local.{fp64|fp32}.xconv"
+// Veff tile in the transform-based x pass) and "local.{fp64|fp32}.xconv"
 // (1: x pass as a per-column convolution where the active planes fit, 0:
-// transform-based) and "local.{fp64|fp32}.plane" (0: separate z and y
-// passes over full boxes; M > 0: z and y fused per active plane with M
-// lines per LDS transform batch, planes stored compactly).  Return false for
-// an unknown key or an out-of-range value.
+// transform-based).  Return false for an unknown key or an out-of-range value.
 bool set_local_tuning(const std::string &key, int value);
 bool get_local_tuning(const std::string &key, int &value);
 

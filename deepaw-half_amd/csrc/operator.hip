@@ -486,6 +486,31 @@ void KOperator<R>::apply(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> 
 template struct KOperator<double>;
 template struct KOperator<float>;
 
+template <typename R>
+void mix_projector_coefficients(const System &sys, int ncols, const cplx_t<R> *c, cplx_t<R> *dc, cplx_t<R> *qc,
+                                hipStream_t st) {
+  using V = cplx_t<R>;
+  const int np = sys.nproj;
+  const int64_t count = int64_t(np) * ncols;
+  if (count == 0) return;
+  const R *db, *qb;
+  if constexpr (std::is_same<R, double>::value) {
+    db = sys.d_dblk.get();
+    qb = sys.d_qblk.get();
+  } else {
+    db = sys.d_dblk32.get();
+    qb = sys.d_qblk32.get();
+  }
+  hipLaunchKernelGGL((mix_kernel<V, R>), dim3(blocks_for(count)), dim3(kThreads), 0, st, c, dc, qc,
+                     sys.d_proj_atom.get(), sys.d_proj_lm.get(), sys.d_atom_first.get(), sys.d_atom_nlm.get(),
+                     sys.d_atom_boff.get(), db, qb, np, ncols);
+  HALF_HIP(hipGetLastError());
+}
+template void mix_projector_coefficients<double>(const System &, int, const double2 *, double2 *, double2 *,
+                                                 hipStream_t);
+template void mix_projector_coefficients<float>(const System &, int, const float2 *, float2 *, float2 *,
+                                                hipStream_t);
+
 void assemble_dense(const KOperator<double> &op, double2 *h, double2 *s, int64_t ld, hipStream_t st) {
   const System &sys = *op.sys;
   const int n = op.n, np = op.np;

@@ -88,6 +88,12 @@ struct KOperator {
   void project(int nb, const cplx_t<R> *psi, int64_t ldpsi, hipStream_t st);
 };
 
+// dc = D c, qc = Q c per atom block for ncols coefficient columns of length
+// np (column-major, leading dimension np), in the operator's precision.
+template <typename R>
+void mix_projector_coefficients(const System &sys, int ncols, const cplx_t<R> *c, cplx_t<R> *dc, cplx_t<R> *qc,
+                                hipStream_t st);
+
 // Dense H/S assembly on the device (FP64), column-major n x n with
 // leading dimension ld, hermitised (reference: src/half_dense_solver.F90:53-88).
 void assemble_dense(const KOperator<double> &op, double2 *h, double2 *s, int64_t ld,

@@ -42,6 +42,8 @@ void trace_mark(const char *name, hipStream_t s) {
 
 void trace_close(hipStream_t s) { trace_mark(nullptr, s); }
 
+bool trace_enabled() { return g_enabled; }
+
 }  // namespace half
 
 using namespace half;

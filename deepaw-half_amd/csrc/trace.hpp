@@ -10,5 +10,6 @@ namespace half {
 
 void trace_mark(const char *name, hipStream_t s);  // before a launch
 void trace_close(hipStream_t s);                   // after the last launch of a sequence
+bool trace_enabled();
 
 }  // namespace half

@@ -39,7 +39,7 @@ EXT_SYMBOLS = (
     "half_amd_solve_acc", "half_amd_set_solver", "half_amd_energy_terms",
     "half_amd_get_veff", "half_amd_nonlocal_forces", "half_amd_device_info",
     "half_amd_trace", "half_amd_trace_read", "half_amd_set_tuning", "half_amd_get_tuning",
-    "half_amd_set_projector_positions", "half_amd_solve_history",
+    "half_amd_set_projector_positions", "half_amd_solve_history", "half_amd_solve_kmesh",
 )
 
 
@@ -113,6 +113,7 @@ def load(path: str | None = None):
         "half_amd_get_tuning": [cp, C.POINTER(i), cp, i],
         "half_amd_set_projector_positions": [i64, i, pd, cp, i],
         "half_amd_solve_history": [i64, i, pd, pi32, cp, i],
+        "half_amd_solve_kmesh": [i64, i, pd, i, i, d, d, i, i, i, pd, C.POINTER(SolveStats), cp, i],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -325,6 +326,19 @@ class Context:
                                            v.ctypes.data if vectors else None, n, C.byref(st), e, 512), e)
         stats = {f: getattr(st, f) for f, _ in SolveStats._fields_}
         return w, (v.T.copy() if vectors else None), stats
+
+    def solve_kmesh(self, nbands, kpoints, mixed=False, tol=1e-4, switch_tol=0.0, maxiter=40, extra=0,
+                    graph=True):
+        """All-band solves at every k point of `kpoints` (nk x 3, fractional)
+        in lockstep; returns (eigenvalues nk x nbands, stats)."""
+        ks = np.ascontiguousarray(np.asarray(kpoints, np.float64).reshape(-1, 3))
+        w = np.zeros((len(ks), nbands))
+        st = SolveStats()
+        e = _err()
+        _check(self.lib.half_amd_solve_kmesh(self.handle, len(ks), _pd(ks), nbands, int(mixed), float(tol),
+                                             float(switch_tol), int(maxiter), int(extra), int(graph), _pd(w),
+                                             C.byref(st), e, 512), e)
+        return w, {f: getattr(st, f) for f, _ in SolveStats._fields_}
 
     def nonlocal_forces(self, eigenvalues, occupations, vectors, k=(0, 0, 0), kweight=1.0):
         vec = np.ascontiguousarray(np.asarray(vectors, np.complex128).T)

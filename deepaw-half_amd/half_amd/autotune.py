@@ -23,8 +23,7 @@ import numpy as np
 
 from . import BACKEND_CUDA, SOLVER_EVD, Context, get_tuning, hip, set_tuning, synth, trace, trace_read
 
-PASSES = {"x": ("local_x_veff", "local_x_conv"), "y": ("local_y_bwd", "local_y_fwd"), "z": ("local_z_bwd", "local_z_fwd"),
-          "plane": ("local_plane_bwd", "local_plane_fwd")}
+PASSES = {"x": ("local_x_veff", "local_x_conv"), "y": ("local_y_bwd", "local_y_fwd"), "z": ("local_z_bwd", "local_z_fwd")}
 
 
 def _pass_ms(kern, axis, steps):

@@ -72,6 +72,19 @@ int half_amd_solve_acc(half_handle handle, const double kpoint[3], int nbands, i
 int half_amd_solve_history(half_handle handle, int capacity, double *residuals_eV, int32_t *count,
                            char *error, int error_capacity);
 
+/* All-band solves at nk k points (fractional, nk x 3) of one context in
+ * lockstep: the H.psi of every k point is one launch per stage, the
+ * Rayleigh-Ritz algebra strided-batched over k, and the correction block's
+ * H.psi replayed from a captured hipGraph when graph != 0.  Controls as in
+ * half_amd_solve_acc; the solve stops when every k point's requested bands
+ * meet tol.  eigenvalues: nk x nbands, k-major.  stats: max_residual_eV is
+ * the largest over k, band_applications counts all k points.  The
+ * reference has no batched entry; it is the loop of half_solve_kpoint over
+ * k points (src/half_library.F90:120-150) at ACC (solver 4). */
+int half_amd_solve_kmesh(half_handle handle, int nk, const double *kpoints, int nbands, int mixed, double tol,
+                         double switch_tol, int maxiter, int extra, int graph, double *eigenvalues_eV,
+                         half_amd_solve_stats *stats, char *error, int error_capacity);
+
 /* Select the solver used by half_solve_kpoint for this context (1..4). */
 int half_amd_set_solver(half_handle handle, int solver, char *error, int error_capacity);
 

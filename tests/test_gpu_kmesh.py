@@ -1,0 +1,114 @@
+"""k-point mesh batching: all-band solves of every k point of a mesh in
+lockstep (one launch per operator stage for all k, strided-batched
+Rayleigh-Ritz, hipGraph-replayed correction block) against per-k dense
+solves and the reference's golden eigenvalues.
+
+Tolerances: FP64 <= 1e-9 eV per band, mixed <= 1e-8 eV per band (the
+per-k solver's bars); graph replay against eager launches bit-identical
+(same kernels, same order)."""
+import numpy as np
+import pytest
+from conftest import golden, require_same_inputs, workload_files
+
+import half_amd as H
+
+pytestmark = pytest.mark.gpu
+
+SI2 = ("tests/golden/inputs/si2.CHGCAR", "tests/golden/inputs/si2.POTCAR")
+# Gamma (golden si2_pbe), the golden si2_k point, and two mesh points with
+# fewer and more plane waves
+SI2_MESH = [(0.0, 0.0, 0.0), (0.1, 0.2, 0.3), (0.25, 0.0, 0.0), (0.5, 0.5, 0.0)]
+
+
+@pytest.fixture(scope="module")
+def si2ctx():
+    g = golden("si2_pbe")
+    print("[gpu] create si2 kmesh", flush=True)
+    ctx = H.Context(SI2[0], SI2[1], g["encut"], g["xc"], H.BACKEND_CUDA, H.SOLVER_EVD)
+    yield ctx
+    ctx.close()
+
+
+def _dense(ctx, nb, ks):
+    return np.array([ctx.solve(nb, k, vectors=False)[0] for k in ks])
+
+
+def test_kmesh_ragged_sizes(si2ctx):
+    sizes = [si2ctx.basis_size(k) for k in SI2_MESH]
+    assert len(set(sizes)) > 1, sizes  # the mesh exercises ragged plane-wave counts
+
+
+def test_kmesh_fp64_matches_dense_and_golden(si2ctx):
+    nb = 16
+    ref = _dense(si2ctx, nb, SI2_MESH)
+    assert np.abs(ref[0] - np.array(golden("si2_pbe")["eigenvalues"])).max() < 1e-9
+    assert np.abs(ref[1] - np.array(golden("si2_k")["eigenvalues"])).max() < 1e-9
+    w, st = si2ctx.solve_kmesh(nb, SI2_MESH, tol=1e-7, maxiter=200)
+    print("[gpu] si2 kmesh fp64", st, np.abs(w - ref).max(), flush=True)
+    assert w.shape == (len(SI2_MESH), nb)
+    assert st["max_residual_eV"] <= 1e-7
+    assert np.abs(w - ref).max() < 1e-9
+    hist = si2ctx.solve_history()
+    assert len(hist) == st["iterations"] + 1 or len(hist) == st["iterations"]
+
+
+def test_kmesh_mixed_matches_dense(si2ctx):
+    nb = 16
+    ref = _dense(si2ctx, nb, SI2_MESH)
+    w, st = si2ctx.solve_kmesh(nb, SI2_MESH, mixed=True, tol=1e-6, maxiter=200)
+    print("[gpu] si2 kmesh mixed", st, np.abs(w - ref).max(), flush=True)
+    assert st["fp64_iterations"] >= 1
+    assert np.abs(w - ref).max() < 1e-8
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_kmesh_graph_replay_identical(si2ctx, mixed):
+    nb = 12
+    wg, sg = si2ctx.solve_kmesh(nb, SI2_MESH, mixed=mixed, tol=1e-6, maxiter=200, extra=4, graph=True)
+    we, se = si2ctx.solve_kmesh(nb, SI2_MESH, mixed=mixed, tol=1e-6, maxiter=200, extra=4, graph=False)
+    assert sg["iterations"] == se["iterations"]
+    assert sg["band_applications"] == se["band_applications"]
+    assert np.array_equal(wg, we)
+
+
+def test_kmesh_single_point_matches_acc(si2ctx):
+    k = SI2_MESH[1]
+    w1, st1 = si2ctx.solve_kmesh(16, [k], tol=1e-7, maxiter=200)
+    w0, _, _ = si2ctx.solve_acc(16, k, tol=1e-7, maxiter=200)
+    assert np.abs(w1[0] - w0).max() < 1e-10
+
+
+def test_kmesh_argument_errors(si2ctx):
+    with pytest.raises(H.HalfError):
+        si2ctx.solve_kmesh(16, np.zeros((0, 3)))
+    with pytest.raises(H.HalfError):
+        si2ctx.solve_kmesh(400, SI2_MESH)  # 2 x 400 > plane waves
+    w, _ = si2ctx.solve_kmesh(4, SI2_MESH[:1], tol=1e-6, maxiter=100)  # context still usable
+    assert np.all(np.isfinite(w))
+
+
+def test_kmesh_si8k(tmp_path_factory):
+    """8-atom cubic cell, 40^3 box, about 5900 plane waves per k point: the
+    golden reference point k = (1/6, -1/3, 1/2) of the 6x6x6 mesh solved
+    together with two more mesh points."""
+    g = golden("si8k_k")
+    chg, pot = workload_files("si8k", str(tmp_path_factory.mktemp("si8kmesh")))
+    require_same_inputs(g, chg, pot)
+    print("[gpu] create si8k kmesh", flush=True)
+    ctx = H.Context(chg, pot, g["encut"], g["xc"], H.BACKEND_CUDA, H.SOLVER_EVD)
+    try:
+        nb = g["nbands"]
+        ks = [tuple(g["kpoint"]), (0.0, 0.0, 0.0), (1 / 6, 1 / 6, 1 / 6)]
+        ref0 = np.array(g["eigenvalues"])
+        w, st = ctx.solve_kmesh(nb, ks, tol=1e-6, maxiter=300, extra=8)
+        print("[gpu] si8k kmesh fp64", st, flush=True)
+        assert np.abs(w[0] - ref0).max() < 1e-9
+        for j in (1, 2):
+            wj, _, _ = ctx.solve_acc(nb, ks[j], tol=1e-6, maxiter=300, extra=8)
+            assert np.abs(w[j] - wj).max() < 1e-9
+        wm, st = ctx.solve_kmesh(nb, ks, mixed=True, tol=1e-6, maxiter=300, extra=8)
+        print("[gpu] si8k kmesh mixed", st, np.abs(wm - w).max(), flush=True)
+        assert st["fp64_iterations"] >= 1
+        assert np.abs(wm - w).max() < 1e-8
+    finally:
+        ctx.close()
