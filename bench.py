@@ -50,6 +50,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", type=int, default=8, help="buffer bands of the all-band solves")
     ap.add_argument("--tol", type=float, default=1e-5, help="residual tolerance (eV) of the all-band solves")
+    ap.add_argument("--kmesh", type=int, default=0, metavar="NK",
+                    help="only time the batched all-band solve of the workload's first NK mesh k points "
+                         "(-1: all) against per-k solves, and print that line")
+    ap.add_argument("--serial", type=int, default=-1, metavar="NK",
+                    help="with --kmesh: per-k comparison solves on the first NK points (-1: all)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the byte-counter calibration "
                          "for profiles) and print their rate")
@@ -75,6 +80,10 @@ def main():
         t1.record(stream)
         ms = t0.elapsed_ms(t1) / 5
         print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": ms, "GBps": 2 * nbytes / ms / 1e6}))
+        return
+
+    if args.kmesh:
+        kmesh_bench(args)
         return
 
     w = synth.workload(args.workload)
@@ -260,6 +269,79 @@ def kernel_roofline(kern, nr, grid, n, nproj, nb, steps, nlines, nplanes, fp64=T
                traffic=measured_traffic(workload, nb, fp64, name) if workload else None,
                bytes_per_launch=(b * steps / launches) if b else None)
     return out
+
+
+def kmesh_bench(args):
+    """All-band solves of every k point of the workload's mesh in one batched
+    call (operator stages one launch for all k, strided-batched
+    Rayleigh-Ritz, graph-replayed correction block), FP64 and mixed, against
+    the same solves done k point by k point.  seconds_solver excludes the
+    operator construction; seconds_wall includes it."""
+    import numpy as np
+    import half_amd as H
+    from half_amd import synth
+
+    w = synth.workload(args.workload)
+    nb = args.bands or w.nbands
+    ks = np.asarray(w.kpoints, float)
+    if args.kmesh > 0:
+        ks = ks[: args.kmesh]
+    nk = len(ks)
+    tmp = tempfile.mkdtemp(prefix="halfbench")
+    chg, pot, _ = synth.make_inputs(args.workload, tmp)
+    dev_name, cus = H.device_info()
+    print(f"bench: {dev_name} ({cus} CUs), workload {args.workload}, {nk} k points", file=sys.stderr, flush=True)
+    ctx = H.Context(chg, pot, w.encut, w.xc, H.BACKEND_CUDA, H.SOLVER_EVD)
+    sizes = [ctx.basis_size(tuple(k)) for k in ks]
+    for mixed in (False, True):  # library initialisation outside the timed solves
+        ctx.solve_kmesh(nb, ks[:2], mixed=mixed, tol=args.tol, maxiter=2, extra=args.extra)
+    out = {"metric": "k-mesh all-band solve seconds, batched vs per-k, 1 GPU", "unit": "s",
+           "higher_is_better": False, "n_gpus": 1, "data": "synthetic (seeded Gaussian density, synthetic PAW "
+           "datasets)", "config": {"workload": f"{args.workload}: {sum(w.counts)} atoms, {nk} k points, "
+                                               f"{min(sizes)}-{max(sizes)} plane waves, box {list(w.grid)}, "
+                                               f"{nb} bands per k",
+                                   "nk": nk, "bands": nb, "npw_min": min(sizes), "npw_max": max(sizes)},
+           "controls": {"tol_eV": args.tol, "extra_bands": args.extra, "maxiter": 400}}
+    mesh = {}
+    for mixed in (False, True):
+        key = "mixed" if mixed else "fp64"
+        t = time.perf_counter()
+        wv, st = ctx.solve_kmesh(nb, ks, mixed=mixed, tol=args.tol, maxiter=400, extra=args.extra)
+        el = time.perf_counter() - t
+        mesh[key] = (wv, st)
+        out[f"batched_{key}"] = {"seconds_wall": el, "seconds_solver": st["seconds"],
+                                 "iterations": st["iterations"], "fp64_iterations": st["fp64_iterations"],
+                                 "max_residual_eV": st["max_residual_eV"],
+                                 "band_applications": st["band_applications"],
+                                 "band_applications_per_s": st["band_applications"] / st["seconds"]}
+        print(f"bench: batched {key} {out[f'batched_{key}']}", file=sys.stderr, flush=True)
+    out["max_abs_diff_mixed_vs_fp64_eV"] = float(np.abs(mesh["mixed"][0] - mesh["fp64"][0]).max())
+    # one mixed solve on the launch-boundary timeline (eager launches): phases
+    H.trace(True)
+    ctx.solve_kmesh(nb, ks, mixed=True, tol=args.tol, maxiter=400, extra=args.extra, graph=False)
+    phases = H.trace_read()
+    H.trace(False)
+    out["batched_mixed_phase_ms"] = {p: round(v[0], 2) for p, v in sorted(phases.items(), key=lambda x: -x[1][0])}
+    ns = nk if args.serial < 0 else min(args.serial, nk)
+    if ns > 0:
+        for mixed in (False, True):
+            key = "mixed" if mixed else "fp64"
+            solver_s = 0.0
+            diff = 0.0
+            t = time.perf_counter()
+            for j in range(ns):
+                wj, _, st = ctx.solve_acc(nb, tuple(ks[j]), mixed=mixed, tol=args.tol, maxiter=400,
+                                          extra=args.extra)
+                solver_s += st["seconds"]
+                diff = max(diff, float(np.abs(wj - mesh[key][0][j]).max()))
+            el = time.perf_counter() - t
+            out[f"per_k_{key}"] = {"k_points": ns, "seconds_wall": el, "seconds_solver": solver_s,
+                                   "max_abs_diff_vs_batched_eV": diff}
+            out[f"speedup_batched_vs_per_k_{key}"] = (solver_s / ns) / (out[f"batched_{key}"]["seconds_solver"] / nk)
+            print(f"bench: per-k {key} {out[f'per_k_{key}']}", file=sys.stderr, flush=True)
+    out["value"] = out["batched_mixed"]["seconds_solver"]
+    ctx.close()
+    print(json.dumps(out))
 
 
 def cpu_baseline(chg, pot, w, k, nb):
