@@ -289,7 +289,13 @@ void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t 
   }
   if (lda < n || ldb < n || ldc < ka || (B2 && (ldb2 < n || ldc2 < ka)))
     throw Error(kInvalidArgument, "gram: leading dimension too small");
-  const int wt = std::max(ka, kb) <= 32 ? 1 : 2;
+  // 64-wide tiles; for the subspace Grams (Hermitian, matrix-core bound)
+  // 32-wide ones where they cover the result with less padding (80 columns:
+  // 96 against 128) below a few hundred columns, where the padding of the
+  // wider tiles stops mattering and their halved operand re-reads win
+  const int kmax = std::max(ka, kb);
+  const bool pad_less_32 = (kmax + 31) / 32 * 32 < (kmax + 63) / 64 * 64;
+  const int wt = (kmax <= 32 || (hermitian && pad_less_32 && kmax < 256)) ? 1 : 2;
   const int tw = 32 * wt;
   const int tiles_a = (ka + tw - 1) / tw;
   const int tiles = hermitian ? tiles_a * (tiles_a + 1) / 2 : tiles_a * ((kb + tw - 1) / tw);
@@ -372,9 +378,16 @@ void gemm_tall_batched(int batch, int n, int m, int k, R alpha, const cplx_t<R> 
   if (lda < n || ldw < k || ldc < n || (W2 && (ldw2 < k || ldc2 < n)))
     throw Error(kInvalidArgument, "gemm_tall: leading dimension too small");
   const int nb = W2 ? 2 : 1;
-  // 64-wide tiles when they still give a few hundred workgroups
+  const bool inplace = C == A || (W2 && C2 == A);
+  if (inplace) {
+    if (m > gemm_tall_inplace_max || (C == A && (ldc != lda || sC != sA)) ||
+        (W2 && C2 == A && (ldc2 != lda || sC2 != sA)))
+      throw Error(kInvalidArgument, "gemm_tall: in-place form needs C = A with the same layout and m <= 64");
+  }
+  // 64-wide tiles when they still give a few hundred workgroups (always in
+  // place: one column tile, so a workgroup owns whole rows)
   const int64_t t64 = int64_t((n + 63) / 64) * ((m + 63) / 64) * batch;
-  const int wt = t64 >= 512 ? 2 : 1;
+  const int wt = (inplace || t64 >= 512) ? 2 : 1;
   const int tw = 32 * wt;
   TallArgs<R> a{};
   a.A = A;

@@ -29,12 +29,17 @@ void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t 
                   bool hermitian = false);
 
 // C = beta C + alpha A W for A (n x k) and W (k x m); C is n x m.  When W2 is
-// given, also C2 = beta C2 + alpha A W2.  beta = 0 does not read C.  C must not
-// overlap A or W.
+// given, also C2 = beta C2 + alpha A W2.  beta = 0 does not read C.  C must
+// not overlap W, and either does not overlap A or is A itself (same pointer,
+// leading dimension and batch stride, m <= gemm_tall_inplace_max): the
+// in-place form gives each workgroup all output columns of its rows, which
+// it writes only after reading every inner column of those rows.
 template <typename R>
 void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, const cplx_t<R> *W, int64_t ldw,
                R beta, cplx_t<R> *C, int64_t ldc, const cplx_t<R> *W2, int64_t ldw2, cplx_t<R> *C2, int64_t ldc2,
                hipStream_t s);
+
+constexpr int gemm_tall_inplace_max = 64;
 
 // Batched form of gemm_tall (entry b's operands at base + b stride).
 template <typename R>

@@ -11,11 +11,13 @@
 //   - B^H psi and B (D c), B (Q c) are one batched contraction launch each,
 //     with every k point's projector matrix padded to the mesh's largest
 //     plane-wave count (zero rows), so ragged sizes need no special cases;
-//   - the Rayleigh-Ritz algebra of all k points is strided-batched: Grams on
-//     the batched MFMA contraction, rotations and updates with
-//     rocblas_zgemm_strided_batched, Cholesky factors and triangular solves
-//     with rocsolver_zpotrf_strided_batched / rocblas_ztrsm_strided_batched,
-//     eigensolves with rocsolver_zheevd_strided_batched;
+//   - the Rayleigh-Ritz algebra of all k points is batched: Grams, the
+//     S-orthogonalisation update, the metric normalisation and the Ritz
+//     rotations on the batched MFMA contractions (the tall right-multiplies
+//     in place), Cholesky factors and their explicit inverses with
+//     rocsolver_zpotrf/ztrtri_strided_batched, the small products with
+//     rocblas_zgemm_strided_batched, eigensolves with
+//     rocsolver_zheevd_strided_batched;
 //   - the correction block's H.psi (the same launches every iteration) is
 //     captured once per precision into a hipGraph and replayed.
 // One host round trip per iteration carries the nk x m residual norms.
@@ -143,6 +145,13 @@ __global__ void unscale_columns_kernel(double2 *U, const double *d, int m) {
   const double f = dj > 0 ? 1.0 / dj : 0.0;
   double2 &v = U[int64_t(k) * m * m + t];
   v = make_double2(v.x * f, v.y * f);
+}
+// zero the strictly lower triangle of k x k matrices at stride gstride
+__global__ void upper_only_kernel(double2 *M, int k, int64_t gstride) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(k) * k) return;
+  const int j = static_cast<int>(t / k), i = static_cast<int>(t - int64_t(j) * k);
+  if (i > j) M[blockIdx.y * gstride + t] = make_double2(0.0, 0.0);
 }
 // V_k <- V_k diag(max(mu_k, floor))^-1/2 (reference :66-77)
 __global__ void scale_columns_kernel(double2 *u, const double *mu, int m, double floor_value) {
@@ -286,11 +295,11 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
   const int64_t gk = int64_t(m2) * m2;  // Gram stride per k point
   DeviceArray<double2> Z(size_t(nk) * zk), HZ(size_t(nk) * zk), SZ(size_t(nk) * zk), Rr(size_t(nk) * rk),
       tmp(size_t(nk) * rk), cross(size_t(nk) * m * m), metric(size_t(nk) * m * m), hsub(size_t(nk) * gk),
-      ssub(size_t(nk) * gk), gwork;
+      ssub(size_t(nk) * gk), gvec(size_t(nk) * gk), gwork;
   DeviceArray<double> w(size_t(nk) * m2), E(size_t(nk) * m2), mu(size_t(nk) * m), res2(size_t(nk) * m),
       dscale(size_t(nk) * m), piv(size_t(nk) * (m + 1));
   DeviceArray<int> dseeds(size_t(nk) * m);
-  DeviceArray<rocblas_int> info(size_t(2) * nk);
+  DeviceArray<rocblas_int> info(size_t(3) * nk);
   DeviceArray<float2> p32, hp32, sp32;
   for (auto *a : {&Z, &HZ, &SZ, &Rr})
     HALF_HIP(hipMemsetAsync(a->get(), 0, sizeof(double2) * a->size(), s));
@@ -303,7 +312,7 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
   double2 *X = Z.get(), *P = Z.get() + rk;
   double2 *HX = HZ.get(), *HP = HZ.get() + rk;
   double2 *SX = SZ.get(), *SP = SZ.get() + rk;
-  const rocblas_double_complex one{1, 0}, zero{0, 0}, mone{-1, 0};
+  const rocblas_double_complex one{1, 0}, zero{0, 0};
   const dim3 gNm(blocks_for(int64_t(N) * m), nk), gmm(blocks_for(int64_t(m) * m), nk), gm(blocks_for(m), nk);
   bool fp64_phase = !ctl.mixed;
   res.applies = 0;
@@ -366,40 +375,55 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
     HALF_HIP(hipGraphInstantiate(&execs[ph], graphs[ph], nullptr, nullptr, 0));
   };
   // k x k generalized problems (A_j, B_j) at stride gstride: B = U^H U,
-  // A <- U^-H A U^-1, eigenpairs, A <- U^-1 V; values to w (stride m2)
+  // U^-1 formed explicitly (batched triangular solves are several times
+  // slower than trtri plus products at these sizes), A <- U^-H A U^-1,
+  // eigenpairs, eigenvectors U^-1 V into gvec; values to w (stride m2)
   auto generalized = [&](int k, int64_t gstride) {
     KM_BLAS(rocsolver_zpotrf_strided_batched(blas.h, rocblas_fill_upper, k, rz(ssub.get()), k, gstride, info.get(),
                                              nk));
-    KM_BLAS(rocblas_ztrsm_strided_batched(blas.h, rocblas_side_left, rocblas_fill_upper,
-                                          rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, k, k, &one,
-                                          rz(ssub.get()), k, gstride, rz(hsub.get()), k, gstride, nk));
-    KM_BLAS(rocblas_ztrsm_strided_batched(blas.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                                          rocblas_diagonal_non_unit, k, k, &one, rz(ssub.get()), k, gstride,
+    KM_BLAS(rocsolver_ztrtri_strided_batched(blas.h, rocblas_fill_upper, rocblas_diagonal_non_unit, k,
+                                             rz(ssub.get()), k, gstride, info.get() + nk, nk));
+    hipLaunchKernelGGL(upper_only_kernel, dim3(blocks_for(int64_t(k) * k), nk), dim3(kThreads), 0, s, ssub.get(), k,
+                       gstride);
+    HALF_HIP(hipGetLastError());
+    KM_BLAS(rocblas_zgemm_strided_batched(blas.h, rocblas_operation_none, rocblas_operation_none, k, k, k, &one,
+                                          rz(hsub.get()), k, gstride, rz(ssub.get()), k, gstride, &zero,
+                                          rz(gvec.get()), k, gstride, nk));
+    KM_BLAS(rocblas_zgemm_strided_batched(blas.h, rocblas_operation_conjugate_transpose, rocblas_operation_none, k, k,
+                                          k, &one, rz(ssub.get()), k, gstride, rz(gvec.get()), k, gstride, &zero,
                                           rz(hsub.get()), k, gstride, nk));
     KM_BLAS(rocsolver_zheevd_strided_batched(blas.h, rocblas_evect_original, rocblas_fill_upper, k, rz(hsub.get()), k,
-                                             gstride, w.get(), m2, E.get(), m2, info.get() + nk, nk));
-    KM_BLAS(rocblas_ztrsm_strided_batched(blas.h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
-                                          rocblas_diagonal_non_unit, k, k, &one, rz(ssub.get()), k, gstride,
-                                          rz(hsub.get()), k, gstride, nk));
-    std::vector<rocblas_int> h(2 * nk);
+                                             gstride, w.get(), m2, E.get(), m2, info.get() + 2 * nk, nk));
+    KM_BLAS(rocblas_zgemm_strided_batched(blas.h, rocblas_operation_none, rocblas_operation_none, k, m, k, &one,
+                                          rz(ssub.get()), k, gstride, rz(hsub.get()), k, gstride, &zero,
+                                          rz(gvec.get()), k, gstride, nk));
+    std::vector<rocblas_int> h(3 * nk);
     HALF_HIP(hipMemcpyAsync(h.data(), info.get(), sizeof(rocblas_int) * h.size(), hipMemcpyDeviceToHost, s));
     HALF_HIP(hipStreamSynchronize(s));
     for (int j = 0; j < nk; ++j) {
-      if (h[j] != 0)
+      if (h[j] != 0 || h[nk + j] != 0)
         throw Error(kInternal, "k-mesh ACC subspace solve failed at k point " + std::to_string(j) +
                                    ": overlap not positive definite");
-      if (h[nk + j] != 0) throw Error(kInternal, "k-mesh ACC subspace eigensolve failed at k point " + std::to_string(j));
+      if (h[2 * nk + j] != 0)
+        throw Error(kInternal, "k-mesh ACC subspace eigensolve failed at k point " + std::to_string(j));
     }
   };
-  // X, HX, SX <- first-k-columns of (Z, HZ, SZ) times hsub(:, :m)
-  auto rotate = [&](int k, int64_t gstride) {
-    for (double2 *blk : {Z.get(), HZ.get(), SZ.get()}) {
-      KM_BLAS(rocblas_zgemm_strided_batched(blas.h, rocblas_operation_none, rocblas_operation_none, N, m, k, &one,
-                                            rz(blk), ld, zk, rz(hsub.get()), k, gstride, &zero, rz(tmp.get()), ld, rk,
-                                            nk));
+  // blk(:, :m) <- blk(:, :k) W(:k, :m) for every k point: in place on the
+  // matrix cores when the m output columns fit one tile, else through tmp
+  auto right_multiply = [&](double2 *blk, const double2 *W, int k, int64_t ldw, int64_t sw) {
+    if (m <= gemm_tall_inplace_max) {
+      gemm_tall_batched<double>(nk, N, m, k, 1.0, blk, ld, zk, W, ldw, sw, 0.0, blk, ld, zk, nullptr, 0, 0, nullptr,
+                                0, 0, s);
+    } else {
+      gemm_tall_batched<double>(nk, N, m, k, 1.0, blk, ld, zk, W, ldw, sw, 0.0, tmp.get(), ld, rk, nullptr, 0, 0,
+                                nullptr, 0, 0, s);
       HALF_HIP(hipMemcpy2DAsync(blk, sizeof(double2) * zk, tmp.get(), sizeof(double2) * rk, sizeof(double2) * rk, nk,
                                 hipMemcpyDeviceToDevice, s));
     }
+  };
+  // X, HX, SX <- first k columns of (Z, HZ, SZ) times the lowest m Ritz vectors
+  auto rotate = [&](int k, int64_t gstride) {
+    for (double2 *blk : {Z.get(), HZ.get(), SZ.get()}) right_multiply(blk, gvec.get(), k, k, gstride);
   };
   auto rayleigh_ritz_x = [&]() {
     gram_batched<double>(nk, N, m, m, X, ld, zk, HX, ld, zk, hsub.get(), m, int64_t(m) * m, SX, ld, zk, ssub.get(), m,
@@ -464,9 +488,8 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
     gram_batched<double>(nk, N, m, m, X, ld, zk, SP, ld, zk, cross.get(), m, int64_t(m) * m, nullptr, 0, 0, nullptr, 0,
                          0, gwork, s);
     for (auto pr : {std::make_pair(X, P), std::make_pair(HX, HP), std::make_pair(SX, SP)})
-      KM_BLAS(rocblas_zgemm_strided_batched(blas.h, rocblas_operation_none, rocblas_operation_none, N, m, m, &mone,
-                                            rz(pr.first), ld, zk, rz(cross.get()), m, int64_t(m) * m, &one,
-                                            rz(pr.second), ld, zk, nk));
+      gemm_tall_batched<double>(nk, N, m, m, -1.0, pr.first, ld, zk, cross.get(), m, int64_t(m) * m, 1.0, pr.second,
+                                ld, zk, nullptr, 0, 0, nullptr, 0, 0, s);
     // normalise in the S metric (unit-diagonal Cholesky; eigen-decomposition
     // form when any k point's factor is ill-conditioned)
     trace_mark("acc_metric_cholesky", s);
@@ -490,11 +513,14 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
     if (chol_ok) {
       hipLaunchKernelGGL(unscale_columns_kernel, gmm, dim3(kThreads), 0, s, metric.get(), dscale.get(), m);
       HALF_HIP(hipGetLastError());
-      trace_mark("acc_normalise_trsm", s);
-      for (double2 *blk : {P, HP, SP})
-        KM_BLAS(rocblas_ztrsm_strided_batched(blas.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                                              rocblas_diagonal_non_unit, N, m, &one, rz(metric.get()), m,
-                                              int64_t(m) * m, rz(blk), ld, zk, nk));
+      trace_mark("acc_normalise", s);
+      // P <- P U^-1 with U^-1 explicit (a triangular factor whose scaled
+      // pivots all exceed 1e-10)
+      KM_BLAS(rocsolver_ztrtri_strided_batched(blas.h, rocblas_fill_upper, rocblas_diagonal_non_unit, m,
+                                               rz(metric.get()), m, int64_t(m) * m, info.get(), nk));
+      hipLaunchKernelGGL(upper_only_kernel, gmm, dim3(kThreads), 0, s, metric.get(), m, int64_t(m) * m);
+      HALF_HIP(hipGetLastError());
+      for (double2 *blk : {P, HP, SP}) right_multiply(blk, metric.get(), m, m, int64_t(m) * m);
     } else {
       ++res.metric_fallbacks;
       gram_batched<double>(nk, N, m, m, P, ld, zk, SP, ld, zk, metric.get(), m, int64_t(m) * m, nullptr, 0, 0,
@@ -510,13 +536,7 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
         if (h[j] != 0) throw Error(kInternal, "k-mesh ACC metric eigensolve failed");
       hipLaunchKernelGGL(scale_columns_kernel, gmm, dim3(kThreads), 0, s, metric.get(), mu.get(), m, 1e-14);
       HALF_HIP(hipGetLastError());
-      for (double2 *blk : {P, HP, SP}) {
-        KM_BLAS(rocblas_zgemm_strided_batched(blas.h, rocblas_operation_none, rocblas_operation_none, N, m, m, &one,
-                                              rz(blk), ld, zk, rz(metric.get()), m, int64_t(m) * m, &zero,
-                                              rz(tmp.get()), ld, rk, nk));
-        HALF_HIP(hipMemcpy2DAsync(blk, sizeof(double2) * zk, tmp.get(), sizeof(double2) * rk, sizeof(double2) * rk,
-                                  nk, hipMemcpyDeviceToDevice, s));
-      }
+      for (double2 *blk : {P, HP, SP}) right_multiply(blk, metric.get(), m, m, int64_t(m) * m);
     }
     // Rayleigh-Ritz in span[X P] for every k point
     trace_mark("acc_projection_gram", s);
