@@ -131,18 +131,40 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs<R> g) {
       }
 }
 
-// C[p + b ldc] = sum over chunks c of partial[c stride + p + b ka], in chunk order
+// C[p + b ldc] = sum over chunks c of partial[c stride + p + b ka].  A
+// workgroup takes 64 entries; its four waves sum the chunks c = q, q + 4,
+// q + 8, ... (q = wave) with eight loads in flight per step, and the four
+// partial sums are added in wave order: a fixed association, so the result
+// does not depend on scheduling.
 template <typename V>
-__global__ void gram_reduce_kernel(const V *partial, int64_t stride, int nchunks, int ka, int kb, V *C,
-                                   int64_t ldc, int64_t sC) {
-  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (t >= int64_t(ka) * kb) return;
+__global__ __launch_bounds__(256) void gram_reduce_kernel(const V *partial, int64_t stride, int nchunks, int ka,
+                                                          int kb, V *C, int64_t ldc, int64_t sC) {
+  __shared__ V red[4][64];
+  const int e = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int64_t t = blockIdx.x * int64_t(64) + e;
+  const bool live = t < int64_t(ka) * kb;
   partial += blockIdx.y * stride * nchunks;  // batch entry blockIdx.y
   C += blockIdx.y * sC;
-  V acc = partial[t];
-  for (int c = 1; c < nchunks; ++c) acc = cadd(acc, partial[c * stride + t]);
-  const int b = static_cast<int>(t / ka), p = static_cast<int>(t - int64_t(b) * ka);
-  C[p + int64_t(b) * ldc] = acc;
+  using R = typename scalar_of<V>::type;
+  V acc = mk<V>(R(0), R(0));
+  if (live) {
+    int c = q;
+    for (; c + 28 < nchunks; c += 32) {
+      V v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = partial[int64_t(c + 4 * i) * stride + t];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) acc = cadd(acc, v[i]);
+    }
+    for (; c < nchunks; c += 4) acc = cadd(acc, partial[int64_t(c) * stride + t]);
+  }
+  red[q][e] = acc;
+  __syncthreads();
+  if (q == 0 && live) {
+    const V s = cadd(cadd(red[0][e], red[1][e]), cadd(red[2][e], red[3][e]));
+    const int b = static_cast<int>(t / ka), p = static_cast<int>(t - int64_t(b) * ka);
+    C[p + int64_t(b) * ldc] = s;
+  }
 }
 
 // C[j + i ldc] = conj(C[i + j ldc]) for i < j (lower triangle from the upper)
@@ -238,7 +260,21 @@ __global__ __launch_bounds__(256) void tall_kernel(TallArgs<R> g) {
         }
     }
   }
-  for (int j = 0; j < NB; ++j)
+  // epilogue per output: with beta != 0 every old value of C this thread
+  // updates is loaded first (independent loads in flight together) and
+  // only then are the results stored
+  for (int j = 0; j < NB; ++j) {
+    V old[WT][WT][4];
+    if (g.beta != R(0)) {
+      for (int u = 0; u < WT; ++u)
+        for (int t = 0; t < WT; ++t)
+          for (int i = 0; i < 4; ++i) {
+            const int col = c0 + wc + 16 * u + row;
+            const int r = g0 + wr + 16 * t + mfma_row<R>(lane, i);
+            old[u][t][i] = (col < g.m && r < g.n) ? g.C[j][bz * g.bC[j] + r + int64_t(col) * g.ldc[j]]
+                                                   : mk<V>(R(0), R(0));
+          }
+    }
     for (int u = 0; u < WT; ++u) {
       const int col = c0 + wc + 16 * u + row;
       if (col >= g.m) continue;
@@ -246,16 +282,15 @@ __global__ __launch_bounds__(256) void tall_kernel(TallArgs<R> g) {
         for (int i = 0; i < 4; ++i) {
           const int r = g0 + wr + 16 * t + mfma_row<R>(lane, i);
           if (r >= g.n) continue;
-          V *c = g.C[j] + bz * g.bC[j] + r + int64_t(col) * g.ldc[j];
           R vr = g.alpha * acc[j][t][u][0][i], vi = g.alpha * acc[j][t][u][1][i];
           if (g.beta != R(0)) {
-            const V o = *c;
-            vr += g.beta * o.x;
-            vi += g.beta * o.y;
+            vr += g.beta * old[u][t][i].x;
+            vi += g.beta * old[u][t][i].y;
           }
-          *c = mk<V>(vr, vi);
+          g.C[j][bz * g.bC[j] + r + int64_t(col) * g.ldc[j]] = mk<V>(vr, vi);
         }
     }
+  }
 }
 
 template <typename R, int WT, int NB>
@@ -344,7 +379,7 @@ void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t 
     nb == 1 ? launch_gram<R, 2, 1>(grid, a, s) : launch_gram<R, 2, 2>(grid, a, s);
   HALF_HIP(hipGetLastError());
   if (nchunks > 1) {
-    const dim3 rg(static_cast<unsigned>((tile_elems + 255) / 256), batch);
+    const dim3 rg(static_cast<unsigned>((tile_elems + 63) / 64), batch);
     hipLaunchKernelGGL((gram_reduce_kernel<V>), rg, dim3(256), 0, s, a.out[0], tile_elems, nchunks, ka, kb, C, ldc,
                        sC);
     if (B2)

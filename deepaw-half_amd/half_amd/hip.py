@@ -1,6 +1,12 @@
-"""Minimal ctypes access to the HIP runtime that libhalf.so itself links
-(/opt/rocm/lib/libamdhip64.so.7): device buffers, streams and events for the
-device-level entry points, without a second HIP runtime in the process."""
+"""Minimal ctypes access to the HIP runtime that libhalf.so itself is bound
+to: device buffers, streams and events for the device-level entry points,
+without a second HIP runtime in the process.
+
+libhalf.so links libamdhip64.so.7 (/opt/rocm/lib).  When another library
+has already put a HIP runtime with that SONAME into the process (torch's
+bundled copy, if torch was imported first), the dynamic linker binds
+libhalf.so to that one, so the runtime used here is looked up among the
+loaded objects rather than opened by path."""
 from __future__ import annotations
 
 import ctypes as C
@@ -12,12 +18,25 @@ _hip = None
 H2D, D2H, D2D = 1, 2, 3
 
 
+def _loaded_runtime():
+    """Path of the HIP runtime mapped into this process (the /opt/rocm one
+    when two are mapped: libhalf.so, loaded first, is bound to it)."""
+    try:
+        with open("/proc/self/maps") as fh:
+            paths = {ln.split()[-1] for ln in fh if "libamdhip64" in ln and "/" in ln}
+    except OSError:
+        return None
+    paths = sorted(p for p in paths if os.path.basename(p).startswith("libamdhip64"))
+    rocm = [p for p in paths if p.startswith("/opt/rocm")]
+    return (rocm or paths or [None])[0]
+
+
 def lib():
     global _hip
     if _hip is None:
         from . import load
-        load()  # libhalf first: then this resolves to the runtime it linked
-        path = "/opt/rocm/lib/libamdhip64.so.7"
+        load()  # libhalf first: then the runtime it is bound to is loaded
+        path = _loaded_runtime() or "/opt/rocm/lib/libamdhip64.so.7"
         _hip = C.CDLL(path if os.path.exists(path) else "libamdhip64.so")
         _hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         _hip.hipFree.argtypes = [C.c_void_p]
