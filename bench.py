@@ -217,7 +217,9 @@ def kernel_bytes(name, nr, grid, n, nlines, nplanes, nb, sc, sr):
     """Bytes a kernel must move per step (nb bands) in the pruned scheme:
     each active line/plane read and written once, Veff read once per band
     (transform form of the x pass) or its i1-axis DFT once per launch
-    (convolution form), plane-wave vectors read/written once."""
+    (convolution forms), plane-wave vectors read/written once.  The fused
+    y-x-y kernel keeps the planes in LDS: its HBM bytes are the active
+    lines in and out."""
     n1, n2, n3 = grid
     line = nlines * n3 * sc
     plane = nplanes * n2 * n3 * sc
@@ -228,9 +230,11 @@ def kernel_bytes(name, nr, grid, n, nlines, nplanes, nb, sc, sr):
         "local_x_conv": 2 * plane,
         "local_y_fwd": plane + line,
         "local_z_fwd": line + n * (3 * sc + sr + 8),
+        # fused y-x-y pass: each active line's values read and written once
+        "local_yxy": 2 * line,
     }
-    # the convolution form reads the i1-axis DFT of Veff (nr complex) once per launch
-    once = {"local_x_conv": nr * sc}
+    # the convolution forms read the i1-axis DFT of Veff (nr complex) once per launch
+    once = {"local_x_conv": nr * sc, "local_yxy": nr * sc}
     return None if name not in table else nb * table[name] + once.get(name, 0)
 
 

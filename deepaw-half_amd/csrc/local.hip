@@ -51,6 +51,8 @@ struct Tuning {
   int tile[2][3] = {{1024, 1536, 768}, {1536, 1536, 768}};  // [fp64|fp32][x|y|z]
   int xgroup[2] = {4, 8};
   int xconv[2] = {1, 1};  // x pass as a per-column convolution when it fits
+  int fused[2] = {1, 1};  // y, x, y passes in one kernel where the convolution applies
+  int ygroup[2] = {1, 1}; // bands per workgroup of the fused kernel
 };
 Tuning g_tune;
 
@@ -65,53 +67,62 @@ struct KStride {
   long long psi, out, kin, box;
 };
 
-template <typename R>
+// Lines of one band per workgroup: blockIdx.x = block of a.M lines,
+// blockIdx.y = band, blockIdx.z = k point.  NN > 0: the transform length
+// n3 as a compile-time constant (runtime a.n otherwise).  a.M is a power of
+// two, so the element -> (line, point) splits below are shifts and masks.
+// TL: the transformed lines go to an i3-major line buffer, [band][i3][line]
+// (per band nlines n3 values), for the fused kernel; else to the box.
+template <typename R, bool TL, int NN>
 __global__ __launch_bounds__(256) void z_bwd_kernel(const cplx_t<R> *psi, long long ldpsi, cplx_t<R> *box,
                                                     const LocalTables *tabs, KStride ks, int nb, FftArgs a) {
   using V = cplx_t<R>;
   constexpr int K = Tile<R>::KMAX;
-  const int kp = blockIdx.y;
+  const int n = NN > 0 ? NN : a.n;
+  const int kp = blockIdx.z, band = blockIdx.y;
   const LocalTables t = tabs[kp];
-  const long long lines_total = static_cast<long long>(nb) * t.nlines;
-  const int L0 = blockIdx.x * a.M;
-  if (L0 >= lines_total) return;  // past this k point's lines: no barrier reached
-  psi += kp * ks.psi;
+  const int line0 = blockIdx.x * a.M;
+  if (line0 >= t.nlines) return;  // past this k point's lines: no barrier reached
+  psi += kp * ks.psi + band * ldpsi;
   box += kp * ks.box;
+  const int mshift = __ffs(a.M) - 1;
   extern __shared__ double2 smem_raw[];
   V *buf0 = reinterpret_cast<V *>(smem_raw);
   V *buf1 = buf0 + a.M * a.ld;
   V *tws = buf1 + a.M * a.ld;
   load_twiddles(tws, a);
-  const int MN = a.M * a.n;
+  const int MN = a.M * n;
   for (int e = threadIdx.x; e < a.M * a.ld; e += blockDim.x) buf0[e] = mk<V>(R(0), R(0));
   __syncthreads();
   // plane waves of each line, placed at their i3 (the rest of the line is 0)
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int e = threadIdx.x + k * 256;
-    if (e < MN) {
-      const int l = e / a.n, j = e - l * a.n;
-      const int L = L0 + l;
-      if (L < lines_total) {
-        const int band = L / t.nlines, line = L - band * t.nlines;
-        if (j < t.line_count[line]) {
-          const int g = t.line_start[line] + j;
-          buf0[l * a.ld + t.pw_i3[g]] = psi[g + band * ldpsi];
-        }
+    if (e < MN) {  // consecutive threads walk one line's plane waves (contiguous in psi)
+      const int l = e / n, j = e - l * n;
+      const int line = line0 + l;
+      if (line < t.nlines && j < t.line_count[line]) {
+        const int g = t.line_start[line] + j;
+        buf0[l * a.ld + t.pw_i3[g]] = psi[g];
       }
     }
   }
   __syncthreads();
-  V *res = run_stages(buf0, buf1, tws, a, +1);
+  V *res = stages<NN>(buf0, buf1, tws, a, +1);
+  const long long lstride = static_cast<long long>(t.nlines) * t.n3;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int e = threadIdx.x + k * 256;
     if (e >= MN) break;
-    const int l = e / a.n, p = e - l * a.n;
-    const int L = L0 + l;
-    if (L < lines_total) {
-      const int band = L / t.nlines, line = L - band * t.nlines;
-      box[band * t.nr + static_cast<long long>(t.line_i12[line]) * t.n3 + p] = res[l * a.ld + p];
+    if (TL) {  // consecutive threads take consecutive lines at one i3
+      const int l = e & (a.M - 1), p = e >> mshift;
+      const int line = line0 + l;
+      if (line < t.nlines) box[band * lstride + static_cast<long long>(p) * t.nlines + line] = res[l * a.ld + p];
+    } else {  // consecutive threads walk one line
+      const int l = e / n, p = e - l * n;
+      const int line = line0 + l;
+      if (line < t.nlines)
+        box[band * t.nr + static_cast<long long>(t.line_i12[line]) * t.n3 + p] = res[l * a.ld + p];
     }
   }
 }
@@ -119,7 +130,7 @@ __global__ __launch_bounds__(256) void z_bwd_kernel(const cplx_t<R> *psi, long l
 // Along i2 on the active planes; BWD loads active lines only and stores all,
 // the forward pass loads all and stores active lines only.  Each thread keeps
 // one line (M divides the 256 threads) and walks its points.
-template <typename R, bool BWD>
+template <typename R, bool BWD, int NN>
 __global__ __launch_bounds__(256) void y_kernel(cplx_t<R> *box, const LocalTables *tabs, KStride ks, int nb,
                                                 FftArgs a) {
   using V = cplx_t<R>;
@@ -157,7 +168,7 @@ __global__ __launch_bounds__(256) void y_kernel(cplx_t<R> *box, const LocalTable
     if (p < a.n) buf0[l * a.ld + p] = v[k];
   }
   __syncthreads();
-  V *res = run_stages(buf0, buf1, tws, a, BWD ? +1 : -1);
+  V *res = stages<NN>(buf0, buf1, tws, a, BWD ? +1 : -1);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int p = p0 + k * pstep;
@@ -287,9 +298,109 @@ __global__ __launch_bounds__(256) void x_conv_kernel(cplx_t<R> *box, const cplx_
   }
 }
 
+// Fused y, x and y passes on one i3 slice of G bands, reading and writing
+// the i3-major line buffer of z_bwd_kernel<R, true> in place:
+//   1. the slice's active lines go to LDS planes [g][q][i2] (q = signed
+//      plane - first signed plane; zeros elsewhere);
+//   2. backward transform along i2 for every (g, q);
+//   3. the x pass per (g, i2) column as the circular convolution
+//        out[q'] = sum_q Vx[(q' - q) mod n1][(i2, i3)] in[q],
+//      an item taking kYxyOut consecutive outputs with its window of
+//      PMAX + kYxyOut - 1 Vx values (read from the [i3][d][i2] copy, so the
+//      loads are contiguous along i2) in registers and the inputs from LDS;
+//   4. forward transform along i2;
+//   5. the slice's active lines stored back.
+// All band groups of one i3 fall on the same XCD (the grid is i3-fastest and
+// n3 is a multiple of the XCD count on the boxes used), so each XCD's L2
+// holds the Vx rows of its own i3 values.
+constexpr int kYxyOut = 4;
+
+template <typename R, int PMAX, int NN>
+__global__ __launch_bounds__(256) void yxy_kernel(cplx_t<R> *lines, const cplx_t<R> *vxT, const LocalTables *tabs,
+                                                  KStride ks, int nb, int G, FftArgs a) {
+  using V = cplx_t<R>;
+  const int kp = blockIdx.y;
+  const LocalTables t = tabs[kp];
+  const int P = t.nplanes;
+  const int n2 = NN > 0 ? NN : t.n2;
+  const int i3 = blockIdx.x % t.n3;
+  const int band0 = (blockIdx.x / t.n3) * G;
+  if (band0 >= nb || P == 0) return;  // uniform per workgroup: no barrier reached
+  lines += kp * ks.box;
+  FftArgs ay = a;
+  ay.M = G * P;  // y lines of the slice
+  const int ld = ay.ld;
+  extern __shared__ double2 smem_raw[];
+  V *A = reinterpret_cast<V *>(smem_raw);
+  V *B = A + ay.M * ld;
+  V *tws = B + ay.M * ld;
+  load_twiddles(tws, ay);
+  for (int e = threadIdx.x; e < ay.M * ld; e += blockDim.x) A[e] = mk<V>(R(0), R(0));
+  __syncthreads();
+  const long long lstride = static_cast<long long>(t.nlines) * t.n3;
+  const int half1 = (t.n1 - 1) / 2;
+  // line l of plane q at i2 sits at LDS offset (q ld + i2) of each band
+  auto slot = [&](int l) {
+    const int i12 = t.line_i12[l];
+    const int i1 = i12 / n2, i2 = i12 - i1 * n2;
+    return ((i1 <= half1 ? i1 : i1 - t.n1) - t.plane_s0) * ld + i2;
+  };
+  for (int g = 0; g < G && band0 + g < nb; ++g) {
+    const V *src = lines + (band0 + g) * lstride + static_cast<long long>(i3) * t.nlines;
+    for (int l = threadIdx.x; l < t.nlines; l += blockDim.x) A[g * P * ld + slot(l)] = src[l];
+  }
+  __syncthreads();
+  V *r1 = stages<NN>(A, B, tws, ay, +1);
+  V *o1 = r1 == A ? B : A;
+  const V *vcol = vxT + static_cast<long long>(i3) * t.n1 * n2;  // [d][i2] of this i3
+  const int nparts = (P + kYxyOut - 1) / kYxyOut;
+  for (int it = threadIdx.x; it < nparts * G * n2; it += blockDim.x) {
+    const int i2 = it % n2, rest = it / n2;
+    const int g = rest % G, qa = (rest / G) * kYxyOut;
+    // w[j] = Vx[(qa - (PMAX - 1) + j) mod n1]: output qa + o, input q uses
+    // w[o + PMAX - 1 - q] (a compile-time index)
+    // (-n1 <= qa - (PMAX - 1) + j < 2 n1: PMAX <= n1 + 1 is checked at launch)
+    V w[PMAX + kYxyOut - 1];
+#pragma unroll
+    for (int j = 0; j < PMAX + kYxyOut - 1; ++j) {
+      int d = qa - (PMAX - 1) + j;
+      d += d < 0 ? t.n1 : 0;
+      d -= d >= t.n1 ? t.n1 : 0;
+      w[j] = vcol[d * n2 + i2];
+    }
+    V acc[kYxyOut];
+#pragma unroll
+    for (int o = 0; o < kYxyOut; ++o) acc[o] = mk<V>(R(0), R(0));
+    const V *col = r1 + g * P * ld + i2;
+#pragma unroll
+    for (int q = 0; q < PMAX; ++q) {
+      if (q < P) {
+        const V v = col[q * ld];
+#pragma unroll
+        for (int o = 0; o < kYxyOut; ++o) {
+          const V c = w[o + PMAX - 1 - q];
+          acc[o].x += c.x * v.x - c.y * v.y;
+          acc[o].y += c.x * v.y + c.y * v.x;
+        }
+      }
+    }
+#pragma unroll
+    for (int o = 0; o < kYxyOut; ++o)
+      if (qa + o < P) o1[(g * P + qa + o) * ld + i2] = acc[o];
+  }
+  __syncthreads();
+  V *r2 = stages<NN>(o1, r1, tws, ay, -1);
+  for (int g = 0; g < G && band0 + g < nb; ++g) {
+    V *dst = lines + (band0 + g) * lstride + static_cast<long long>(i3) * t.nlines;
+    for (int l = threadIdx.x; l < t.nlines; l += blockDim.x) dst[l] = r2[g * P * ld + slot(l)];
+  }
+}
+
 // Vx[d][c] = scale sum_x veff[x][c] e^{-2 pi i d x / n1} for the C = n2 n3
 // columns of the C-order box, in FP64 and FP32
-__global__ void veff_x_dft_kernel(const double *veff, double2 *vx, float2 *vx32, int n1, int C, double scale) {
+__global__ void veff_x_dft_kernel(const double *veff, double2 *vx, float2 *vx32, int n1, int n2, int n3,
+                                  double scale) {
+  const int C = n2 * n3;
   const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
   if (t >= static_cast<long long>(n1) * C) return;
   const int d = static_cast<int>(t / C);
@@ -304,42 +415,52 @@ __global__ void veff_x_dft_kernel(const double *veff, double2 *vx, float2 *vx32,
   }
   vx[t] = make_double2(sr * scale, si * scale);
   vx32[t] = make_float2(static_cast<float>(sr * scale), static_cast<float>(si * scale));
+  // the same value in the [i3][d][i2] layout after the first N_r entries
+  const int i2 = c / n3, i3 = c - i2 * n3;
+  const long long u = static_cast<long long>(n1) * C + (static_cast<long long>(i3) * n1 + d) * n2 + i2;
+  vx[u] = make_double2(sr * scale, si * scale);
+  vx32[u] = make_float2(static_cast<float>(sr * scale), static_cast<float>(si * scale));
 }
 
-template <typename R>
+template <typename R, bool TL, int NN>
 __global__ __launch_bounds__(256) void z_fwd_kernel(const cplx_t<R> *psi, long long ldpsi, const cplx_t<R> *box,
                                                     const R *kin, cplx_t<R> *hpsi, cplx_t<R> *spsi,
                                                     long long ldout, const LocalTables *tabs, KStride ks, int nb,
                                                     FftArgs a) {
   using V = cplx_t<R>;
   constexpr int K = Tile<R>::KMAX;
-  const int kp = blockIdx.y;
+  const int n = NN > 0 ? NN : a.n;
+  const int kp = blockIdx.z, band = blockIdx.y;
   const LocalTables t = tabs[kp];
-  const long long lines_total = static_cast<long long>(nb) * t.nlines;
-  const int L0 = blockIdx.x * a.M;
-  if (L0 >= lines_total) return;
-  psi += kp * ks.psi;
+  const int line0 = blockIdx.x * a.M;
+  if (line0 >= t.nlines) return;
+  psi += kp * ks.psi + band * ldpsi;
   box += kp * ks.box;
   kin += kp * ks.kin;
-  hpsi += kp * ks.out;
-  spsi += kp * ks.out;
+  hpsi += kp * ks.out + band * ldout;
+  spsi += kp * ks.out + band * ldout;
+  const int mshift = __ffs(a.M) - 1;
   extern __shared__ double2 smem_raw[];
   V *buf0 = reinterpret_cast<V *>(smem_raw);
   V *buf1 = buf0 + a.M * a.ld;
   V *tws = buf1 + a.M * a.ld;
   load_twiddles(tws, a);
-  const int MN = a.M * a.n;
+  const int MN = a.M * n;
+  const long long lstride = static_cast<long long>(t.nlines) * t.n3;
   V v[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int e = threadIdx.x + k * 256;
     v[k] = mk<V>(R(0), R(0));
     if (e < MN) {
-      const int l = e / a.n, p = e - l * a.n;
-      const int L = L0 + l;
-      if (L < lines_total) {
-        const int band = L / t.nlines, line = L - band * t.nlines;
-        v[k] = box[band * t.nr + static_cast<long long>(t.line_i12[line]) * t.n3 + p];
+      if (TL) {
+        const int l = e & (a.M - 1), p = e >> mshift;
+        const int line = line0 + l;
+        if (line < t.nlines) v[k] = box[band * lstride + static_cast<long long>(p) * t.nlines + line];
+      } else {
+        const int l = e / n, p = e - l * n;
+        const int line = line0 + l;
+        if (line < t.nlines) v[k] = box[band * t.nr + static_cast<long long>(t.line_i12[line]) * t.n3 + p];
       }
     }
   }
@@ -347,26 +468,26 @@ __global__ __launch_bounds__(256) void z_fwd_kernel(const cplx_t<R> *psi, long l
   for (int k = 0; k < K; ++k) {
     const int e = threadIdx.x + k * 256;
     if (e < MN) {
-      const int l = e / a.n, p = e - l * a.n;
-      buf0[l * a.ld + p] = v[k];
+      if (TL)
+        buf0[(e & (a.M - 1)) * a.ld + (e >> mshift)] = v[k];
+      else
+        buf0[(e / n) * a.ld + (e - (e / n) * n)] = v[k];
     }
   }
   __syncthreads();
-  V *res = run_stages(buf0, buf1, tws, a, -1);
+  V *res = stages<NN>(buf0, buf1, tws, a, -1);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int e = threadIdx.x + k * 256;
     if (e >= MN) break;
-    const int l = e / a.n, j = e - l * a.n;
-    const int L = L0 + l;
-    if (L >= lines_total) continue;
-    const int band = L / t.nlines, line = L - band * t.nlines;
-    if (j < t.line_count[line]) {
+    const int l = e / n, j = e - l * n;  // consecutive threads walk one line's plane waves
+    const int line = line0 + l;
+    if (line < t.nlines && j < t.line_count[line]) {
       const int g = t.line_start[line] + j;
-      const V p = psi[g + band * ldpsi];
+      const V p = psi[g];
       const V r = res[l * a.ld + t.pw_i3[g]];
-      hpsi[g + band * ldout] = mk<V>(r.x + kin[g] * p.x, r.y + kin[g] * p.y);
-      spsi[g + band * ldout] = p;
+      hpsi[g] = mk<V>(r.x + kin[g] * p.x, r.y + kin[g] * p.y);
+      spsi[g] = p;
     }
   }
 }
@@ -422,6 +543,60 @@ const void *x_conv_for(int planes, int &pmax) {
   }
   pmax = 0;
   return nullptr;
+}
+
+// Transform lengths with compile-time kernels (the boxes of the workloads
+// in use); any other length runs the runtime-plan kernels.
+#define HALF_CT_LENGTHS(X) X(24) X(40) X(48) X(96)
+
+template <typename R, bool TL>
+const void *z_bwd_for(int n) {
+#define HALF_CASE(N) \
+  if (n == N) return reinterpret_cast<const void *>(&z_bwd_kernel<R, TL, N>);
+  HALF_CT_LENGTHS(HALF_CASE)
+#undef HALF_CASE
+  return reinterpret_cast<const void *>(&z_bwd_kernel<R, TL, 0>);
+}
+
+template <typename R, bool TL>
+const void *z_fwd_for(int n) {
+#define HALF_CASE(N) \
+  if (n == N) return reinterpret_cast<const void *>(&z_fwd_kernel<R, TL, N>);
+  HALF_CT_LENGTHS(HALF_CASE)
+#undef HALF_CASE
+  return reinterpret_cast<const void *>(&z_fwd_kernel<R, TL, 0>);
+}
+
+template <typename R, bool BWD>
+const void *y_for(int n) {
+#define HALF_CASE(N) \
+  if (n == N) return reinterpret_cast<const void *>(&y_kernel<R, BWD, N>);
+  HALF_CT_LENGTHS(HALF_CASE)
+#undef HALF_CASE
+  return reinterpret_cast<const void *>(&y_kernel<R, BWD, 0>);
+}
+
+template <typename R, int PMAX>
+const void *yxy_len(int n2) {
+#define HALF_CASE(N) \
+  if (n2 == N) return reinterpret_cast<const void *>(&yxy_kernel<R, PMAX, N>);
+  HALF_CT_LENGTHS(HALF_CASE)
+#undef HALF_CASE
+  return reinterpret_cast<const void *>(&yxy_kernel<R, PMAX, 0>);
+}
+
+// register budget of yxy_kernel per precision (window of PMAX + 3 values)
+template <typename R>
+const void *yxy_for(int planes, int n1, int n2) {
+  int pmax = planes <= 16 ? 16 : planes <= 24 ? 24 : planes <= 32 ? 32 : 48;
+  if (std::is_same<R, double>::value && pmax > 32) return nullptr;
+  if (planes > 48 || pmax > n1 + 1) return nullptr;  // window arithmetic assumes PMAX <= n1 + 1
+  switch (pmax) {
+    case 16: return yxy_len<R, 16>(n2);
+    case 24: return yxy_len<R, 24>(n2);
+    case 32: return yxy_len<R, 32>(n2);
+    default: return yxy_len<R, 48>(n2);
+  }
 }
 
 }  // namespace
@@ -496,7 +671,7 @@ void build_veff_x_dft(const double *veff, int n1, int n2, int n3, double2 *vx, f
   const long long total = static_cast<long long>(n1) * n2 * n3;
   const double scale = 1.0 / static_cast<double>(total);
   hipLaunchKernelGGL(veff_x_dft_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s, veff, vx,
-                     vx32, n1, n2 * n3, scale);
+                     vx32, n1, n2, n3, scale);
   HALF_HIP(hipGetLastError());
 }
 
@@ -520,16 +695,44 @@ void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff,
   FftArgs az = axis_args<R>(plan, 2, 0, 1);
   FftArgs ay = axis_args<R>(plan, 1, 0, n3);
   auto blocks = [](long long lines, int M) { return static_cast<unsigned>((lines + M - 1) / M); };
-  const dim3 gz(blocks(static_cast<long long>(nb) * kb.max_lines, az.M), nk);
+  if (nb > 65535) throw Error(kInvalidArgument, "band batch too large for the local stage");
+  const dim3 gz(blocks(kb.max_lines, az.M), static_cast<unsigned>(nb), nk);
   const dim3 gy(blocks(static_cast<long long>(nb) * kb.max_planes * n3, ay.M), nk);
   int nbi = nb;
+  // fused y-x-y pass: every k point's active planes one run, the widest
+  // within the register budget, the LDS of a band group within 64 KiB
+  {
+    const int pwf = kb.max_planes;
+    int G = std::max(1, std::min(g_tune.ygroup[prec], nb));
+    const size_t lds = (2 * size_t(G) * pwf * ay.ld + n2) * sizeof(V);
+    const void *yxy =
+        (vx && g_tune.fused[prec] && kb.contiguous && lds <= 65536) ? yxy_for<R>(pwf, n1, n2) : nullptr;
+    if (yxy) {
+      const cplx_t<R> *vxT = vx + static_cast<long long>(n1) * C;
+      {
+        void *args[] = {&psi, &lp, &boxes, &tabs, &ks, &nbi, &az};
+        launch<R>("local_z_bwd", z_bwd_for<R, true>(n3), az, s, args, gz);
+      }
+      {
+        void *args[] = {&boxes, &vxT, &tabs, &ks, &nbi, &G, &ay};
+        trace_mark("local_yxy", s);
+        HALF_HIP(hipLaunchKernel(yxy, dim3(static_cast<unsigned>(n3 * ((nb + G - 1) / G)), nk), dim3(256), args,
+                                 lds, s));
+      }
+      {
+        void *args[] = {&psi, &lp, &boxes, &kin, &hpsi, &spsi, &lo, &tabs, &ks, &nbi, &az};
+        launch<R>("local_z_fwd", z_fwd_for<R, true>(n3), az, s, args, gz);
+      }
+      return;
+    }
+  }
   {
     void *args[] = {&psi, &lp, &boxes, &tabs, &ks, &nbi, &az};
-    launch<R>("local_z_bwd", reinterpret_cast<const void *>(&z_bwd_kernel<R>), az, s, args, gz);
+    launch<R>("local_z_bwd", z_bwd_for<R, false>(n3), az, s, args, gz);
   }
   {
     void *args[] = {&boxes, &tabs, &ks, &nbi, &ay};
-    launch<R>("local_y_bwd", reinterpret_cast<const void *>(&y_kernel<R, true>), ay, s, args, gy);
+    launch<R>("local_y_bwd", y_for<R, true>(n2), ay, s, args, gy);
   }
   // x pass: per-column convolution where every k point's active planes are
   // one run and the widest fits the register budget, else transform-based
@@ -557,11 +760,11 @@ void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff,
   }
   {
     void *args[] = {&boxes, &tabs, &ks, &nbi, &ay};
-    launch<R>("local_y_fwd", reinterpret_cast<const void *>(&y_kernel<R, false>), ay, s, args, gy);
+    launch<R>("local_y_fwd", y_for<R, false>(n2), ay, s, args, gy);
   }
   {
     void *args[] = {&psi, &lp, &boxes, &kin, &hpsi, &spsi, &lo, &tabs, &ks, &nbi, &az};
-    launch<R>("local_z_fwd", reinterpret_cast<const void *>(&z_fwd_kernel<R>), az, s, args, gz);
+    launch<R>("local_z_fwd", z_fwd_for<R, false>(n3), az, s, args, gz);
   }
 }
 
@@ -599,6 +802,16 @@ bool set_local_tuning(const std::string &key, int value) {
       g_tune.xconv[p] = value;
       return true;
     }
+    if (key == pre + ".fused") {
+      if (value < 0 || value > 1) return false;
+      g_tune.fused[p] = value;
+      return true;
+    }
+    if (key == pre + ".ygroup") {
+      if (value < 1 || value > 8) return false;
+      g_tune.ygroup[p] = value;
+      return true;
+    }
   }
   return false;
 }
@@ -618,6 +831,14 @@ bool get_local_tuning(const std::string &key, int &value) {
     }
     if (key == pre + ".xconv") {
       value = g_tune.xconv[p];
+      return true;
+    }
+    if (key == pre + ".fused") {
+      value = g_tune.fused[p];
+      return true;
+    }
+    if (key == pre + ".ygroup") {
+      value = g_tune.ygroup[p];
       return true;
     }
   }

@@ -48,8 +48,8 @@ struct LocalBatch {
 
 // hpsi = T psi + F[Veff F^-1[psi]]/N_r, spsi = psi for nb bands of every k
 // point of the batch; boxes holds nb full boxes per k point.  vx is the
-// i1-axis DFT of Veff/N_r from build_veff_x_dft (null: transform-based x
-// pass).
+// i1-axis DFT of Veff/N_r in both layouts from build_veff_x_dft (2 N_r
+// values; null: transform-based x pass).
 template <typename R>
 void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff, const cplx_t<R> *vx, const R *kin,
                        int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
@@ -62,14 +62,18 @@ void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const cp
                  cplx_t<R> *boxes, hipStream_t s);
 
 // vx[d][c] = sum_x veff[x][c] e^{-2 pi i d x / n1} / (n1 n2 n3) for the
-// n2 n3 columns c of a C-order n1 x n2 x n3 box, FP64 and FP32 copies.
+// n2 n3 columns c = i2 n3 + i3 of a C-order n1 x n2 x n3 box, FP64 and FP32
+// copies, each 2 n1 n2 n3 long: [d][c] first, then [i3][d][i2].
 void build_veff_x_dft(const double *veff, int n1, int n2, int n3, double2 *vx, float2 *vx32, hipStream_t s);
 
 // Tunables of the local stage, keys "local.{fp64|fp32}.tile.{x|y|z}" (FFT
 // points per workgroup tile), "local.{fp64|fp32}.xgroup" (bands sharing a
-// Veff tile in the transform-based x pass) and "local.{fp64|fp32}.xconv"
+// Veff tile in the transform-based x pass), "local.{fp64|fp32}.xconv"
 // (1: x pass as a per-column convolution where the active planes fit, 0:
-// transform-based).  Return false for an unknown key or an out-of-range value.
+// transform-based), "local.{fp64|fp32}.fused" (1: y, x and y passes in one
+// kernel per (band, i3) slice where the convolution applies, 0: separate
+// passes) and "local.{fp64|fp32}.ygroup" (bands per workgroup of the fused
+// kernel).  Return false for an unknown key or an out-of-range value.
 bool set_local_tuning(const std::string &key, int value);
 bool get_local_tuning(const std::string &key, int &value);
 

@@ -380,8 +380,8 @@ void System::build(hipStream_t s) {
   build_veff_device(*this, s);
   if (use_qdep) build_qdep(*this, s);
   const int64_t nr = int64_t(fft.n[0]) * fft.n[1] * fft.n[2];
-  veff_x.resize(size_t(nr));
-  veff_x32.resize(size_t(nr));
+  veff_x.resize(size_t(2 * nr));  // [d][column], then [i3][d][i2]
+  veff_x32.resize(size_t(2 * nr));
   build_veff_x_dft(veff.get(), fft.n[0], fft.n[1], fft.n[2], veff_x.get(), veff_x32.get(), s);
   ready = true;
 }

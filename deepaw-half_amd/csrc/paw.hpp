@@ -30,7 +30,9 @@ struct System {
   DeviceArray<double> veff;    // C-order, eV
   DeviceArray<float> veff32;
   DeviceArray<double2> veff_g;  // forward FFT(veff) / N_r
-  DeviceArray<double2> veff_x;  // i1-axis DFT of veff / N_r per (i2,i3) column, [d][column]
+  // i1-axis DFT of veff / N_r per (i2,i3) column: N_r values as [d][column],
+  // then the same N_r values as [i3][d][i2] (build_veff_x_dft)
+  DeviceArray<double2> veff_x;
   DeviceArray<float2> veff_x32;
   double e_hartree = 0, e_xc = 0, e_xc_potential = 0;
   bool ready = false;

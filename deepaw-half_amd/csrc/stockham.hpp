@@ -158,6 +158,70 @@ __device__ V *run_stages(V *in, V *out, const V *tws, const FftArgs &a, int sign
 }
 
 
+// Compile-time form of run_stages for a transform length N known when the
+// kernel is built: the same radix order as the runtime plan (fft.hip's
+// factor: 8, 4, 5, 3, 2, 7 greedily), so the arithmetic is identical, but
+// every divisor and stride is a constant (the runtime form spends most of
+// its instructions on integer division).
+struct CtRadices {
+  int count;
+  int r[8];
+};
+
+constexpr CtRadices ct_factor(int n) {
+  CtRadices p{0, {0, 0, 0, 0, 0, 0, 0, 0}};
+  const int order[6] = {8, 4, 5, 3, 2, 7};
+  for (int i = 0; i < 6; ++i)
+    while (n % order[i] == 0 && n > 1 && p.count < 8) {
+      p.r[p.count++] = order[i];
+      n /= order[i];
+    }
+  if (n != 1) p.count = -1;
+  return p;
+}
+
+template <typename V, int N, int S, int NS>
+__device__ inline V *ct_stages(V *in, V *out, const V *tws, int M, int ld, int sign) {
+  constexpr CtRadices P = ct_factor(N);
+  static_assert(P.count > 0, "transform length without a plan");
+  if constexpr (S == P.count) {
+    return in;
+  } else {
+    constexpr int R = P.r[S];
+    constexpr int nr = N / R;
+    constexpr int tstep = N / (NS * R);
+    const int tasks = M * nr;
+    for (int t = threadIdx.x; t < tasks; t += blockDim.x) {
+      const int l = t / nr;
+      const int j = t - l * nr;
+      const int k = j % NS;
+      V v[8];
+      const V *src = in + l * ld + j;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = src[r * nr];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle(tws, k * r * tstep, sign));
+      }
+      butterfly(v, R, sign, tws, N);
+      V *dst = out + l * ld + (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[r * NS] = v[r];
+    }
+    __syncthreads();
+    return ct_stages<V, N, S + 1, NS * R>(out, in, tws, M, ld, sign);
+  }
+}
+
+// Stages of length NN when NN > 0 (compile-time), else of a.n (runtime).
+template <int NN, typename V>
+__device__ inline V *stages(V *in, V *out, const V *tws, const FftArgs &a, int sign) {
+  if constexpr (NN > 0)
+    return ct_stages<V, NN, 0, 1>(in, out, tws, a.M, a.ld, sign);
+  else
+    return run_stages(in, out, tws, a, sign);
+}
+
 // Twiddle table of one axis into LDS (converted to the working precision).
 template <typename V>
 __device__ inline void load_twiddles(V *tws, const FftArgs &a) {
