@@ -251,6 +251,34 @@ void KMeshOperator<R>::apply(int nb, const cplx_t<R> *psi, int64_t ld, int64_t s
     veff = sys->veff32.get();
     vx = sys->veff_x32.get();
   }
+  if (np > 0 && operator_overlap() && !trace_enabled()) {
+    // projector stage on the second stream, added by the local stage's last
+    // pass (as KOperator::apply does)
+    const int64_t cnt = int64_t(np) * nb;
+    coeff.ensure(size_t(cnt) * nk);
+    dcoeff.ensure(size_t(cnt) * nk);
+    qcoeff.ensure(size_t(cnt) * nk);
+    nl_h.ensure(size_t(nk) * nb * nmax);
+    nl_s.ensure(size_t(nk) * nb * nmax);
+    aux.ensure();
+    HALF_HIP(hipEventRecord(aux.fork, st));
+    HALF_HIP(hipStreamWaitEvent(aux.stream, aux.fork, 0));
+    gram_batched<R>(nk, nmax, np, nb, B.get(), nmax, int64_t(nmax) * np, psi, ld, sk_in, coeff.get(), np, cnt,
+                    nullptr, 0, 0, nullptr, 0, 0, work, aux.stream);
+    mix_projector_coefficients<R>(*sys, nk * nb, coeff.get(), dcoeff.get(), qcoeff.get(), aux.stream);
+    gemm_tall_batched<R>(nk, nmax, nb, np, R(1), B.get(), nmax, int64_t(nmax) * np, dcoeff.get(), np, cnt, R(0),
+                         nl_h.get(), nmax, int64_t(nb) * nmax, qcoeff.get(), np, cnt, nl_s.get(), nmax,
+                         int64_t(nb) * nmax, aux.stream);
+    HALF_HIP(hipEventRecord(aux.join, aux.stream));
+    LocalAddend<R> add;
+    add.h = nl_h.get();
+    add.s = nl_s.get();
+    add.ld = nmax;
+    add.kstride = int64_t(nb) * nmax;
+    add.wait = aux.join;
+    local_apply_batch<R>(sys->fft, kb, veff, vx, kin.get(), nb, psi, ld, hpsi, spsi, ldo, boxes.get(), st, add);
+    return;
+  }
   local_apply_batch<R>(sys->fft, kb, veff, vx, kin.get(), nb, psi, ld, hpsi, spsi, ldo, boxes.get(), st);
   if (np > 0) {
     const int64_t cnt = int64_t(np) * nb;

@@ -27,6 +27,8 @@ struct KMeshOperator {
   DeviceArray<cplx_t<R>> B;  // [k][np][nmax]
   DeviceArray<R> kin;        // [k][nmax]
   DeviceArray<cplx_t<R>> boxes, coeff, dcoeff, qcoeff, work;
+  DeviceArray<cplx_t<R>> nl_h, nl_s;  // projector-stage results of the overlapped apply, [k][band][nmax]
+  AuxStream aux;
 
   void build(const System &s, const std::vector<std::array<double, 3>> &kpoints, hipStream_t st);
   // H psi and S psi for nb states of every k point: state b of k point j is

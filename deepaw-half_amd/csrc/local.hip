@@ -422,11 +422,14 @@ __global__ void veff_x_dft_kernel(const double *veff, double2 *vx, float2 *vx32,
   vx32[u] = make_float2(static_cast<float>(sr * scale), static_cast<float>(si * scale));
 }
 
+// addh / adds (may be null): added to H psi / S psi (band b of k point j at
+// + j addk + b addld).
 template <typename R, bool TL, int NN>
 __global__ __launch_bounds__(256) void z_fwd_kernel(const cplx_t<R> *psi, long long ldpsi, const cplx_t<R> *box,
                                                     const R *kin, cplx_t<R> *hpsi, cplx_t<R> *spsi,
                                                     long long ldout, const LocalTables *tabs, KStride ks, int nb,
-                                                    FftArgs a) {
+                                                    FftArgs a, const cplx_t<R> *addh, const cplx_t<R> *adds,
+                                                    long long addld, long long addk) {
   using V = cplx_t<R>;
   constexpr int K = Tile<R>::KMAX;
   const int n = NN > 0 ? NN : a.n;
@@ -439,6 +442,10 @@ __global__ __launch_bounds__(256) void z_fwd_kernel(const cplx_t<R> *psi, long l
   kin += kp * ks.kin;
   hpsi += kp * ks.out + band * ldout;
   spsi += kp * ks.out + band * ldout;
+  if (addh) {
+    addh += kp * addk + band * addld;
+    adds += kp * addk + band * addld;
+  }
   const int mshift = __ffs(a.M) - 1;
   extern __shared__ double2 smem_raw[];
   V *buf0 = reinterpret_cast<V *>(smem_raw);
@@ -485,9 +492,14 @@ __global__ __launch_bounds__(256) void z_fwd_kernel(const cplx_t<R> *psi, long l
     if (line < t.nlines && j < t.line_count[line]) {
       const int g = t.line_start[line] + j;
       const V p = psi[g];
-      const V r = res[l * a.ld + t.pw_i3[g]];
+      V r = res[l * a.ld + t.pw_i3[g]];
+      V q = p;
+      if (addh) {
+        r = cadd(r, addh[g]);
+        q = cadd(q, adds[g]);
+      }
       hpsi[g] = mk<V>(r.x + kin[g] * p.x, r.y + kin[g] * p.y);
-      spsi[g] = p;
+      spsi[g] = q;
     }
   }
 }
@@ -678,7 +690,7 @@ void build_veff_x_dft(const double *veff, int n1, int n2, int n3, double2 *vx, f
 template <typename R>
 void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff, const cplx_t<R> *vx, const R *kin,
                        int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
-                       int64_t ldout, cplx_t<R> *boxes, hipStream_t s) {
+                       int64_t ldout, cplx_t<R> *boxes, hipStream_t s, const LocalAddend<R> &add) {
   using V = cplx_t<R>;
   const int n1 = plan.n[0], n2 = plan.n[1], n3 = plan.n[2];
   if (nb <= 0 || kb.nk <= 0 || kb.max_lines == 0) return;
@@ -720,7 +732,10 @@ void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff,
                                  lds, s));
       }
       {
-        void *args[] = {&psi, &lp, &boxes, &kin, &hpsi, &spsi, &lo, &tabs, &ks, &nbi, &az};
+        const cplx_t<R> *ah = add.h, *as = add.s;
+        long long ald = add.ld, akk = add.kstride;
+        void *args[] = {&psi, &lp, &boxes, &kin, &hpsi, &spsi, &lo, &tabs, &ks, &nbi, &az, &ah, &as, &ald, &akk};
+        if (add.wait) HALF_HIP(hipStreamWaitEvent(s, add.wait, 0));
         launch<R>("local_z_fwd", z_fwd_for<R, true>(n3), az, s, args, gz);
       }
       return;
@@ -763,7 +778,10 @@ void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff,
     launch<R>("local_y_fwd", y_for<R, false>(n2), ay, s, args, gy);
   }
   {
-    void *args[] = {&psi, &lp, &boxes, &kin, &hpsi, &spsi, &lo, &tabs, &ks, &nbi, &az};
+    const cplx_t<R> *ah = add.h, *as = add.s;
+    long long ald = add.ld, akk = add.kstride;
+    void *args[] = {&psi, &lp, &boxes, &kin, &hpsi, &spsi, &lo, &tabs, &ks, &nbi, &az, &ah, &as, &ald, &akk};
+    if (add.wait) HALF_HIP(hipStreamWaitEvent(s, add.wait, 0));
     launch<R>("local_z_fwd", z_fwd_for<R, false>(n3), az, s, args, gz);
   }
 }
@@ -771,7 +789,7 @@ void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff,
 template <typename R>
 void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const cplx_t<R> *vx, const R *kin, int nb,
                  const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi, int64_t ldout,
-                 cplx_t<R> *boxes, hipStream_t s) {
+                 cplx_t<R> *boxes, hipStream_t s, const LocalAddend<R> &add) {
   if (m.nlines == 0) return;
   LocalBatch kb;
   kb.nk = 1;
@@ -779,7 +797,7 @@ void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const cp
   kb.max_lines = m.nlines;
   kb.max_planes = m.nplanes;
   kb.contiguous = m.planes_contiguous;
-  local_apply_batch<R>(plan, kb, veff, vx, kin, nb, psi, ldpsi, hpsi, spsi, ldout, boxes, s);
+  local_apply_batch<R>(plan, kb, veff, vx, kin, nb, psi, ldpsi, hpsi, spsi, ldout, boxes, s, add);
 }
 
 bool set_local_tuning(const std::string &key, int value) {
@@ -848,10 +866,10 @@ bool get_local_tuning(const std::string &key, int &value) {
 #define HALF_LOCAL(R)                                                                                              \
   template void local_apply_batch<R>(const FftPlan &, const LocalBatch &, const R *, const cplx_t<R> *, const R *, \
                                      int, const cplx_t<R> *, int64_t, cplx_t<R> *, cplx_t<R> *, int64_t,             \
-                                     cplx_t<R> *, hipStream_t);                                                    \
+                                     cplx_t<R> *, hipStream_t, const LocalAddend<R> &);                            \
   template void local_apply<R>(const FftPlan &, const LocalMap &, const R *, const cplx_t<R> *, const R *, int,     \
                                const cplx_t<R> *, int64_t, cplx_t<R> *, cplx_t<R> *, int64_t, cplx_t<R> *,         \
-                               hipStream_t);
+                               hipStream_t, const LocalAddend<R> &);
 HALF_LOCAL(double)
 HALF_LOCAL(float)
 #undef HALF_LOCAL

@@ -46,6 +46,16 @@ struct LocalBatch {
   long long psi_k = 0, out_k = 0, kin_k = 0, box_k = 0;
 };
 
+// Terms added to H psi and S psi as the local stage writes them: band b of
+// k point j at h + j kstride + b ld (likewise s); the stream waits on
+// `wait` (if set) before the final pass reads them.
+template <typename R>
+struct LocalAddend {
+  const cplx_t<R> *h = nullptr, *s = nullptr;
+  long long ld = 0, kstride = 0;
+  hipEvent_t wait = nullptr;
+};
+
 // hpsi = T psi + F[Veff F^-1[psi]]/N_r, spsi = psi for nb bands of every k
 // point of the batch; boxes holds nb full boxes per k point.  vx is the
 // i1-axis DFT of Veff/N_r in both layouts from build_veff_x_dft (2 N_r
@@ -53,13 +63,13 @@ struct LocalBatch {
 template <typename R>
 void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff, const cplx_t<R> *vx, const R *kin,
                        int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
-                       int64_t ldout, cplx_t<R> *boxes, hipStream_t s);
+                       int64_t ldout, cplx_t<R> *boxes, hipStream_t s, const LocalAddend<R> &add = {});
 
 // One k point.
 template <typename R>
 void local_apply(const FftPlan &plan, const LocalMap &m, const R *veff, const cplx_t<R> *vx, const R *kin, int nb,
                  const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi, int64_t ldout,
-                 cplx_t<R> *boxes, hipStream_t s);
+                 cplx_t<R> *boxes, hipStream_t s, const LocalAddend<R> &add = {});
 
 // vx[d][c] = sum_x veff[x][c] e^{-2 pi i d x / n1} / (n1 n2 n3) for the
 // n2 n3 columns c = i2 n3 + i3 of a C-order n1 x n2 x n3 box, FP64 and FP32

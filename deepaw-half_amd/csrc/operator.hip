@@ -415,9 +415,34 @@ void KOperator<R>::build(const System &s, const double kpoint[3], hipStream_t st
   HALF_HIP(hipStreamSynchronize(st));
 }
 
+namespace {
+bool g_overlap = true;
+}
+
+bool operator_overlap() { return g_overlap; }
+
+bool set_operator_tuning(const std::string &key, int value) {
+  if (key != "operator.overlap" || value < 0 || value > 1) return false;
+  g_overlap = value != 0;
+  return true;
+}
+
+bool get_operator_tuning(const std::string &key, int &value) {
+  if (key != "operator.overlap") return false;
+  value = g_overlap ? 1 : 0;
+  return true;
+}
+
 template <typename R>
 void KOperator<R>::apply_local(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
                                int64_t ldout, hipStream_t st) {
+  apply_local_added(nb, psi, ldpsi, hpsi, spsi, ldout, st, nullptr, nullptr, nullptr);
+}
+
+template <typename R>
+void KOperator<R>::apply_local_added(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi,
+                                     cplx_t<R> *spsi, int64_t ldout, hipStream_t st, const cplx_t<R> *addh,
+                                     const cplx_t<R> *adds, hipEvent_t wait) {
   const int64_t nr = sys->fft.nr;
   // bands per launch: all of them unless the boxes would exceed 2 GiB
   int batch = box_batch > 0 ? box_batch
@@ -435,8 +460,15 @@ void KOperator<R>::apply_local(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx
   }
   for (int b0 = 0; b0 < nb; b0 += batch) {
     const int cnt = std::min(batch, nb - b0);
+    LocalAddend<R> add;
+    if (addh) {
+      add.h = addh + b0 * ldout;
+      add.s = adds + b0 * ldout;
+      add.ld = ldout;
+      add.wait = b0 == 0 ? wait : nullptr;
+    }
     local_apply<R>(sys->fft, lmap, veff, vx, kinetic.get(), cnt, psi + b0 * ldpsi, ldpsi, hpsi + b0 * ldout,
-                   spsi + b0 * ldout, ldout, boxes.get(), st);
+                   spsi + b0 * ldout, ldout, boxes.get(), st, add);
   }
 }
 
@@ -474,13 +506,44 @@ void KOperator<R>::apply_projectors(int nb, const cplx_t<R> *psi, int64_t ldpsi,
                st);
 }
 
+// Projector stage with beta = 0 into hout / sout (ld ldout) on stream st.
+template <typename R>
+void KOperator<R>::projector_terms(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hout, cplx_t<R> *sout,
+                                   int64_t ldout, hipStream_t st) {
+  project(nb, psi, ldpsi, st);
+  const int64_t count = int64_t(np) * nb;
+  dcoeff.ensure(count);
+  qcoeff.ensure(count);
+  mix_projector_coefficients<R>(*sys, nb, coeff.get(), dcoeff.get(), qcoeff.get(), st);
+  gemm_tall<R>(n, nb, np, R(1), B.get(), ldb, dcoeff.get(), np, R(0), hout, ldout, qcoeff.get(), np, sout, ldout,
+               st);
+}
+
+// H psi, S psi.  With overlap on (and no launch timeline being recorded),
+// the projector stage, whose contractions fill the chip poorly at tens of
+// bands, runs on the second stream while the local stage runs here; the
+// local stage's last pass waits for it and adds its result as it writes
+// H psi and S psi.  Otherwise the two stages run in order on `st`.
 template <typename R>
 void KOperator<R>::apply(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
                          int64_t ldout, hipStream_t st) {
   if (nb <= 0) return;
-  apply_local(nb, psi, ldpsi, hpsi, spsi, ldout, st);
-  apply_projectors(nb, psi, ldpsi, hpsi, spsi, ldout, st);
-  trace_close(st);
+  if (np == 0 || !operator_overlap() || trace_enabled()) {
+    apply_local(nb, psi, ldpsi, hpsi, spsi, ldout, st);
+    apply_projectors(nb, psi, ldpsi, hpsi, spsi, ldout, st);
+    trace_close(st);
+    return;
+  }
+  aux.ensure();
+  nl_h.ensure(size_t(ldout) * nb);
+  nl_s.ensure(size_t(ldout) * nb);
+  // the fork follows everything queued on st (psi is ready), and a later
+  // call's fork follows this call's last pass (nl_h / nl_s are consumed)
+  HALF_HIP(hipEventRecord(aux.fork, st));
+  HALF_HIP(hipStreamWaitEvent(aux.stream, aux.fork, 0));
+  projector_terms(nb, psi, ldpsi, nl_h.get(), nl_s.get(), ldout, aux.stream);
+  HALF_HIP(hipEventRecord(aux.join, aux.stream));
+  apply_local_added(nb, psi, ldpsi, hpsi, spsi, ldout, st, nl_h.get(), nl_s.get(), aux.join);
 }
 
 template struct KOperator<double>;

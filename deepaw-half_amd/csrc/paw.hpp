@@ -60,6 +60,43 @@ struct System {
   int max_nlm() const;
 };
 
+// A second stream (default priority) and a fork/join event pair, created on
+// first use: the projector stage runs there while the local stage runs on
+// the caller's stream.  Destroying it drains the stream first.
+struct AuxStream {
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  AuxStream() = default;
+  AuxStream(const AuxStream &) = delete;
+  AuxStream &operator=(const AuxStream &) = delete;
+  AuxStream(AuxStream &&o) noexcept : stream(o.stream), fork(o.fork), join(o.join) {
+    o.stream = nullptr;
+    o.fork = o.join = nullptr;
+  }
+  ~AuxStream() { release(); }
+  void ensure() {
+    if (stream) return;
+    HALF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HALF_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    HALF_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+  }
+  void release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+    fork = join = nullptr;
+  }
+};
+
+// Whether H.psi overlaps its projector stage (second stream) with the local
+// stage; off, everything runs in order on the caller's stream.  Key
+// "operator.overlap" (0 / 1) through half_amd_set_tuning.
+bool operator_overlap();
+bool set_operator_tuning(const std::string &key, int value);
+bool get_operator_tuning(const std::string &key, int &value);
+
 template <typename R>
 struct KOperator {
   const System *sys = nullptr;
@@ -76,6 +113,8 @@ struct KOperator {
   int box_batch = 0;  // bands per FFT launch; 0 = all bands within a 2 GiB workspace
   DeviceArray<cplx_t<R>> boxes;
   DeviceArray<cplx_t<R>> coeff, dcoeff, qcoeff, partial;
+  DeviceArray<cplx_t<R>> nl_h, nl_s;  // projector-stage results of the overlapped apply
+  AuxStream aux;
 
   void build(const System &s, const double kpoint[3], hipStream_t st);
   // H psi and S psi for nb states (column-major, leading dimensions ld*).
@@ -84,6 +123,14 @@ struct KOperator {
   // Stages, exposed for the local-only / projector-only checks and timing.
   void apply_local(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
                    int64_t ldout, hipStream_t st);
+  // local stage that adds addh / adds (layout of hpsi / spsi; null: none)
+  // after waiting on `wait` (may be null)
+  void apply_local_added(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
+                         int64_t ldout, hipStream_t st, const cplx_t<R> *addh, const cplx_t<R> *adds,
+                         hipEvent_t wait);
+  // B D B^H psi and B Q B^H psi written (not added) to hout / sout
+  void projector_terms(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hout, cplx_t<R> *sout,
+                       int64_t ldout, hipStream_t st);
   void apply_projectors(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi,
                         cplx_t<R> *spsi, int64_t ldout, hipStream_t st);
   // c = B^H psi  (np x nb, leading dimension np) into `coeff`

@@ -162,7 +162,7 @@ __device__ V *run_stages(V *in, V *out, const V *tws, const FftArgs &a, int sign
 // kernel is built: the same radix order as the runtime plan (fft.hip's
 // factor: 8, 4, 5, 3, 2, 7 greedily), so the arithmetic is identical, but
 // every divisor and stride is a constant (the runtime form spends most of
-// its instructions on integer division).
+// its instructions on integer division).  M (lines) must be below 2^16.
 struct CtRadices {
   int count;
   int r[8];
@@ -191,9 +191,15 @@ __device__ inline V *ct_stages(V *in, V *out, const V *tws, int M, int ld, int s
     constexpr int nr = N / R;
     constexpr int tstep = N / (NS * R);
     const int tasks = M * nr;
+    // line-fastest task order: the lanes of a 16-lane LDS cycle take one
+    // point of 16 consecutive lines, whose odd stride ld spreads them over
+    // distinct banks, and share one twiddle (a broadcast).  j = t / M in
+    // single precision is exact here: (t + 0.5) / M lies at least 0.5 / M
+    // from an integer and t, M < 2^16.
+    const float inv_m = 1.0f / static_cast<float>(M);
     for (int t = threadIdx.x; t < tasks; t += blockDim.x) {
-      const int l = t / nr;
-      const int j = t - l * nr;
+      const int j = __float2int_rz((static_cast<float>(t) + 0.5f) * inv_m);
+      const int l = t - j * M;
       const int k = j % NS;
       V v[8];
       const V *src = in + l * ld + j;

@@ -71,6 +71,18 @@ def test_kmesh_graph_replay_identical(si2ctx, mixed):
     assert np.array_equal(wg, we)
 
 
+def test_kmesh_overlap_matches_in_order(si2ctx):
+    """The mesh operator with its projector stage on the second stream gives
+    the eigenvalues of the in-order form."""
+    try:
+        H.set_tuning("operator.overlap", 0)
+        w0, s0 = si2ctx.solve_kmesh(12, SI2_MESH, mixed=True, tol=1e-7, maxiter=200, extra=4)
+    finally:
+        H.set_tuning("operator.overlap", 1)
+    w1, s1 = si2ctx.solve_kmesh(12, SI2_MESH, mixed=True, tol=1e-7, maxiter=200, extra=4)
+    assert np.abs(w0 - w1).max() < 1e-10
+
+
 def test_kmesh_single_point_matches_acc(si2ctx):
     k = SI2_MESH[1]
     w1, st1 = si2ctx.solve_kmesh(16, [k], tol=1e-7, maxiter=200)
