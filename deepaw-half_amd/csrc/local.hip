@@ -48,11 +48,11 @@ namespace {
 // half_amd_set_tuning, chosen by the autotuner in half_amd/autotune.py.
 struct Tuning {
   // defaults from the autotuner on the 48^3 HfO2-like box, 64 bands
-  int tile[2][3] = {{1024, 1536, 768}, {1536, 1536, 768}};  // [fp64|fp32][x|y|z]
-  int xgroup[2] = {4, 8};
+  int tile[2][3] = {{1536, 768, 768}, {3072, 1536, 1024}};  // [fp64|fp32][x|y|z]
+  int xgroup[2] = {1, 8};
   int xconv[2] = {1, 1};  // x pass as a per-column convolution when it fits
   int fused[2] = {1, 1};  // y, x, y passes in one kernel where the convolution applies
-  int ygroup[2] = {1, 1}; // bands per workgroup of the fused kernel
+  int ygroup[2] = {2, 2}; // bands per workgroup of the fused kernel (fewer where LDS is short)
 };
 Tuning g_tune;
 
@@ -716,7 +716,9 @@ void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff,
   {
     const int pwf = kb.max_planes;
     int G = std::max(1, std::min(g_tune.ygroup[prec], nb));
-    const size_t lds = (2 * size_t(G) * pwf * ay.ld + n2) * sizeof(V);
+    auto lds_of = [&](int g) { return (2 * size_t(g) * pwf * ay.ld + n2) * sizeof(V); };
+    while (G > 1 && lds_of(G) > 65536) --G;
+    const size_t lds = lds_of(G);
     const void *yxy =
         (vx && g_tune.fused[prec] && kb.contiguous && lds <= 65536) ? yxy_for<R>(pwf, n1, n2) : nullptr;
     if (yxy) {

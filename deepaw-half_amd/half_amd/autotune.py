@@ -1,12 +1,18 @@
-"""Autotuner for the local-stage tile shapes of the H.psi operator.
+"""Autotuner for the local-stage tile shapes and forms of the H.psi operator.
 
 For one workload it times the FP64 and FP32 operators on a resident band
 block while sweeping, per precision:
   local.<p>.tile.{x,y,z}   FFT points per workgroup tile of each pass
   local.<p>.xgroup         bands sharing one Veff tile in the x pass
-Each candidate is applied to all three passes at once and every pass keeps the
-value that minimised its own traced time (the passes are independent
-kernels); xgroup is swept afterwards with the chosen tiles.
+  local.<p>.fused          y, x, y passes fused per (band, i3) slice or not
+  local.<p>.ygroup         bands per workgroup of the fused kernel
+  operator.overlap         projector stage on a second stream or in order
+The tiles are swept with the separate passes (fused off, so every pass runs):
+each candidate is applied to all three passes at once and every pass keeps
+the value that minimised its own traced time (the passes are independent
+kernels); xgroup is swept afterwards with the chosen tiles.  The forms are
+then chosen by the whole operator's event-timed step (no launch timeline,
+so the overlap is in effect).
 
     python -m half_amd.autotune --workload hfo2 --bands 64 --out tuning.json
 
@@ -50,6 +56,17 @@ def tune(workload="hfo2", bands=0, steps=10, verbose=True):
         x = hip.Buffer.from_numpy((rng.standard_normal((nb, n)) + 1j * rng.standard_normal((nb, n))).astype(dt))
         hx, sx = hip.Buffer(x.shape, dt), hip.Buffer(x.shape, dt)
 
+        def step_ms():
+            for _ in range(2):
+                op.apply_device(nb, x.ptr, hx.ptr, sx.ptr, stream=stream.ptr)
+            stream.synchronize()
+            t0, t1 = hip.Event(), hip.Event()
+            t0.record(stream)
+            for _ in range(steps):
+                op.apply_device(nb, x.ptr, hx.ptr, sx.ptr, stream=stream.ptr)
+            t1.record(stream)
+            return t0.elapsed_ms(t1) / steps
+
         def run():
             for _ in range(2):
                 op.apply_device(nb, x.ptr, hx.ptr, sx.ptr, stream=stream.ptr)
@@ -62,6 +79,7 @@ def tune(workload="hfo2", bands=0, steps=10, verbose=True):
             trace(False)
             return kern
 
+        set_tuning(f"local.{p}.fused", 0)
         best = {a: (None, float("inf")) for a in "xyz"}
         for c in cands:
             for a in "xyz":
@@ -89,8 +107,24 @@ def tune(workload="hfo2", bands=0, steps=10, verbose=True):
                 gbest = (g, ms)
         set_tuning(f"local.{p}.xgroup", gbest[0])
         result["best"][f"local.{p}.xgroup"] = gbest[0]
-        kern = run()
-        result["timings"][f"{p}.tuned_step_ms"] = sum(v[0] for v in kern.values()) / steps
+        forms = [(0, 1, ov) for ov in (0, 1)] + [(1, g, ov) for g in (1, 2) for ov in (0, 1)]
+        fbest = (None, float("inf"))
+        for fused, g, ov in forms:
+            set_tuning(f"local.{p}.fused", fused)
+            set_tuning(f"local.{p}.ygroup", g)
+            set_tuning("operator.overlap", ov)
+            ms = step_ms()
+            result["timings"][f"{p}.fused={fused},ygroup={g},overlap={ov}"] = ms
+            if ms < fbest[1]:
+                fbest = ((fused, g, ov), ms)
+        (fused, g, ov) = fbest[0]
+        set_tuning(f"local.{p}.fused", fused)
+        set_tuning(f"local.{p}.ygroup", g)
+        set_tuning("operator.overlap", ov)
+        result["best"][f"local.{p}.fused"] = fused
+        result["best"][f"local.{p}.ygroup"] = g
+        result["best"][f"operator.overlap.{p}"] = ov
+        result["timings"][f"{p}.tuned_step_ms"] = fbest[1]
         if verbose:
             print(f"autotune {p}: {result['best']} step {result['timings'][f'{p}.tuned_step_ms']:.4f} ms",
                   file=sys.stderr, flush=True)
