@@ -291,10 +291,20 @@ def partial_waves(sp: Species, seed: int):
     return r, wps, wae
 
 
-def write_potcar(path: str, w: Workload, with_partial_waves: bool = False) -> None:
+def core_table(zcore: float = 1.0, rc: float = 0.7) -> np.ndarray:
+    """Partial core charge table on the local-part grid q_i = gmax i / 1000:
+    zcore exp(-q^2 rc^2 / 4), the transform of a normalised Gaussian core
+    holding zcore electrons (the table's q = 0 value is the core electron
+    count in the CHGCAR convention)."""
+    q = PSP_GMAX * np.arange(1000) / 1000.0
+    return zcore * np.exp(-q * q * rc * rc / 4.0)
+
+
+def write_potcar(path: str, w: Workload, with_partial_waves: bool = False, partial_core: bool = False) -> None:
     """POTCAR-format datasets; with_partial_waves adds the radial grid and the
     pseudo / all-electron partial waves the potential-dependent D correction
-    (QDEP) integrates, otherwise the PAW section carries an 8-point stub."""
+    (QDEP) integrates, otherwise the PAW section carries an 8-point stub;
+    partial_core adds a Gaussian partial core charge (core_table)."""
     chunks = []
     for it, s in enumerate(w.species):
         sp = SPECIES[s]
@@ -315,6 +325,9 @@ def write_potcar(path: str, w: Workload, with_partial_waves: bool = False) -> No
         c.append(_fmt_block(loc))
         c.append(" gradient corrections used for XC")
         c.append("         4")
+        if partial_core:
+            c.append(" core charge-density (partial)")
+            c.append(_fmt_block(core_table()))
         c.append(" atomic pseudo charge-density")
         c.append(_fmt_block(rho_atom))
         c.append(f"  {PSPNL_GMAX:.10f}  T")
@@ -372,21 +385,22 @@ def sha256(path: str) -> str:
     return h.hexdigest()
 
 
-def make_inputs(name: str, outdir: str, partial_waves: bool = False):
+def make_inputs(name: str, outdir: str, partial_waves: bool = False, partial_core: bool = False):
     """Write CHGCAR/POTCAR for a workload into outdir (cached by content).
     partial_waves selects the datasets with PAW partial waves (file
-    <name>.paw.POTCAR), needed by the potential-dependent D correction."""
+    <name>.paw.POTCAR), needed by the potential-dependent D correction;
+    partial_core adds a partial core charge (<name>[.paw].core.POTCAR)."""
     w = workload(name)
     os.makedirs(outdir, exist_ok=True)
     tag = w.extra.get("inputs", w.name)
     chg = os.path.join(outdir, f"{tag}.CHGCAR")
-    pot = os.path.join(outdir, f"{tag}.paw.POTCAR" if partial_waves else f"{tag}.POTCAR")
+    pot = os.path.join(outdir, tag + (".paw" if partial_waves else "") + (".core" if partial_core else "") + ".POTCAR")
     if not os.path.exists(chg):
         tmp = chg + f".tmp{os.getpid()}"
         write_chgcar(tmp, w)
         os.replace(tmp, chg)
     if not os.path.exists(pot):
         tmp = pot + f".tmp{os.getpid()}"
-        write_potcar(tmp, w, partial_waves)
+        write_potcar(tmp, w, partial_waves, partial_core)
         os.replace(tmp, pot)
     return chg, pot, w

@@ -768,9 +768,10 @@ def reorder_chgcar_f_to_c(values, grid):
     return np.asarray(values).reshape(grid[2], grid[1], grid[0]).transpose(2, 1, 0).ravel()
 
 
-def build_veff(values, grid, potcars, crystal, use_pbe):
+def build_veff(values, grid, potcars, crystal, use_pbe, with_vxc=False):
     """src/half_potential.F90:175-238.  Returns (veff C-order, e_H, e_xc,
-    e_xc_potential)."""
+    e_xc_potential), and the XC potential (C order, eV) as a fifth value when
+    with_vxc (the reference's vxc_out)."""
     n = grid[0] * grid[1] * grid[2]
     vol = crystal.volume
     charge_c = reorder_chgcar_f_to_c(values, grid)
@@ -808,6 +809,8 @@ def build_veff(values, grid, potcars, crystal, use_pbe):
     else:
         vxc, exc = perdew_zunger_xc(charge_c + core_r, vol)
     exv = float(np.sum(charge_c * vxc) / n)
+    if with_vxc:
+        return local_r + vxc, e_h, exc, exv, vxc
     return local_r + vxc, e_h, exc, exv
 
 
@@ -912,6 +915,161 @@ def nonlocal_forces(basis_q, paw, eigenvalues, occupations, kweight, vectors, ni
                         forces[ion0 + iat, al] += -kweight * f * float(np.vdot(c[:, ib], dd[iat, :, :, al] @ c[:, ib]).real)
         ion0 += sp["natoms"]
     return forces
+
+
+# --------------------------------------------------------------------------
+# analytic force set (app/half_cli.F90:921-962, src/half_local_forces.F90,
+# src/half_uspp.F90:16-111)
+# --------------------------------------------------------------------------
+
+def reorder_c_to_f(values, grid):
+    """C-order box (i1 slowest) to the CHGCAR order (x fastest)."""
+    return np.asarray(values).reshape(grid).transpose(2, 1, 0).ravel()
+
+
+def smooth_density(basis, vectors, occupations, kweight, grid):
+    """src/half_local_forces.F90 accumulate_smooth_density: sum_n w f_n
+    |psi_n(r)|^2 with psi(r) the unnormalised backward transform of the
+    plane-wave coefficients (so the grid mean is the electron count, the
+    CHGCAR convention); returned in CHGCAR order."""
+    n = grid[0] * grid[1] * grid[2]
+    dens = np.zeros(n)
+    for ib, f in enumerate(occupations):
+        if f == 0.0:
+            continue
+        box = np.zeros(n, complex)
+        box[basis.fft_index] = vectors[:, ib]
+        dens += kweight * f * np.abs(fft3_backward(box, grid)) ** 2
+    return reorder_c_to_f(dens, grid)
+
+
+def augmentation_occupancy(paw, vectors, occupations, kweight):
+    """src/half_uspp.F90:26-45: rho_ab^I = sum_n w f_n Re[conj(c_a) c_b],
+    c = B^H psi; one nlm x nlm matrix per atom, species order."""
+    out = []
+    for sp in paw:
+        for iat in range(sp["natoms"]):
+            c = sp["beta"][iat].conj() @ vectors  # nlm x nb
+            m = np.zeros((sp["nlm"], sp["nlm"]))
+            for ib, f in enumerate(occupations):
+                if f != 0.0:
+                    m += kweight * f * np.real(np.outer(c[:, ib].conj(), c[:, ib]))
+            out.append(m)
+    return out
+
+
+def augmentation_density(potcars, crystal, grid, occupancy):
+    """src/half_uspp.F90:47-111 (add_augmentation_density): inside each
+    atom's PAW sphere, Omega sum_k q_k g_l(r) Y_k(r_hat) with
+    q_k = sum_ab rho_ab M_ab,k (M: Gaunt x (AE - PS) partial-wave moment,
+    g_l: two-Bessel compensation shape).  Returns the CHGCAR-order field
+    and its grid mean (the augmentation charge)."""
+    n1, n2, n3 = grid
+    out = np.zeros(n1 * n2 * n3)  # CHGCAR order: index i1 + n1 (i2 + n2 i3)
+    i3, i2, i1 = np.meshgrid(np.arange(n3), np.arange(n2), np.arange(n1), indexing="ij")
+    frac = np.stack([i1.ravel() / n1, i2.ravel() / n2, i3.ravel() / n3], 1)
+    ion0 = 0
+    k_at = 0
+    for it, pc in enumerate(potcars):
+        lm = [(ich, l, m) for ich, l in enumerate(pc["lps"]) for m in range(-l, l + 1)]
+        nlm = len(lm)
+        lmax = 2 * max(pc["lps"])
+        kl = [(l, m) for l in range(lmax + 1) for m in range(-l, l + 1)]
+        r = pc["rgrid"]
+        rc = pc["paw_rmax"]
+        rw = radial_weights(r)
+        mult = np.zeros((nlm, nlm, len(kl)))
+        for a, (cha, la, ma) in enumerate(lm):
+            for bb, (chb, lb, mb) in enumerate(lm):
+                for k, (l, m) in enumerate(kl):
+                    if abs(la - lb) <= l <= la + lb and (la + lb + l) % 2 == 0:
+                        mom = np.sum(rw * (pc["wae"][:, cha] * pc["wae"][:, chb] - pc["wps"][:, cha] * pc["wps"][:, chb])
+                                     * r ** l)
+                        mult[a, bb, k] = gaunt_numeric(la, ma, lb, mb, l, m) * mom
+        roots = [(bessel_root(l, 1), bessel_root(l, 2)) for l in range(lmax + 1)]
+        cc = [compensation_coefficients(l, rc, *roots[l]) for l in range(lmax + 1)]
+        for iat in range(crystal.counts[it]):
+            q = np.einsum("ab,abk->k", occupancy[k_at], mult)
+            k_at += 1
+            df = frac - crystal.positions[ion0 + iat]
+            df -= np.rint(df)
+            cart = df @ crystal.lattice
+            rr = np.sqrt((cart * cart).sum(1))
+            inside = rr <= rc
+            if not inside.any():
+                continue
+            ci, ri = cart[inside], rr[inside]
+            big = ri > 1e-14
+            theta = np.where(big, np.arccos(np.clip(ci[:, 2] / np.where(big, ri, 1.0), -1, 1)), 0.0)
+            phi = np.where(big, np.arctan2(ci[:, 1], ci[:, 0]), 0.0)
+            val = np.zeros(ri.size)
+            for k, (l, m) in enumerate(kl):
+                if q[k] == 0.0:
+                    continue
+                z1, z2 = roots[l]
+                c1, c2 = cc[l]
+                g = c1 * np.array([sph_bessel(l, z1 * x / rc) for x in ri]) + \
+                    c2 * np.array([sph_bessel(l, z2 * x / rc) for x in ri])
+                val += q[k] * g * real_ylm(l, m, theta, phi)
+            out[np.nonzero(inside)[0]] += crystal.volume * val
+        ion0 += crystal.counts[it]
+    return out, float(out.sum() / out.size)
+
+
+def _radial_table_forces(field_g, grid, potcars, crystal, table_key, cut, scale_by_volume, subtract_coulomb):
+    """F_I = -sum_G q t(|G|) Im[conj(field_G) e^{-i G.R_I}] over the box
+    (src/half_local_forces.F90:11-150); t is the spline of the species'
+    table (minus 4 pi Z felect / G^2 for the local part), zero at G = 0 and
+    beyond gmax (1 - cut / n_table); divided by the cell volume for the local
+    part."""
+    g = _gvec_grid(grid, crystal)
+    g2 = np.einsum("ij,ij->i", g, g)
+    gn = np.sqrt(g2)
+    f = np.zeros((crystal.nions, 3))
+    ion0 = 0
+    for it, p in enumerate(potcars):
+        tab = p.get(table_key)
+        if tab is None or (table_key == "pspcor" and not p["has_core"]):
+            ion0 += crystal.counts[it]
+            continue
+        m2 = spline_second(tab, p["psp_gmax"])
+        ok = (g2 > 1e-12) & ~(gn > p["psp_gmax"] - cut * p["psp_gmax"] / len(tab))
+        rad = np.where(ok, spline_eval(tab, m2, p["psp_gmax"], gn), 0.0)
+        if subtract_coulomb:
+            rad = np.where(ok, rad - 4 * PI * p["zval"] * FELECT / np.where(ok, g2, 1.0), 0.0)
+        if scale_by_volume:
+            rad = rad / crystal.volume
+        for iat in range(crystal.counts[it]):
+            rpos = crystal.positions[ion0 + iat] @ crystal.lattice
+            ph = g @ rpos
+            term = rad * np.imag(np.conj(field_g) * (np.cos(ph) - 1j * np.sin(ph)))
+            f[ion0 + iat] = -(g * term[:, None]).sum(0)
+        ion0 += crystal.counts[it]
+    return f
+
+
+def local_ionic_forces(rho_f, grid, potcars, crystal):
+    """src/half_local_forces.F90:11-52 on a CHGCAR-order density."""
+    n = grid[0] * grid[1] * grid[2]
+    rho_g = fft3_forward(reorder_chgcar_f_to_c(rho_f, grid) + 0j, grid) / n
+    return _radial_table_forces(rho_g, grid, potcars, crystal, "psp_local", 1, True, True)
+
+
+def nlcc_forces(vxc_c, grid, potcars, crystal):
+    """src/half_local_forces.F90:54-98 on a C-order XC potential."""
+    n = grid[0] * grid[1] * grid[2]
+    v_g = fft3_forward(np.asarray(vxc_c, float) + 0j, grid) / n
+    return _radial_table_forces(v_g, grid, potcars, crystal, "pspcor", 3, False, False)
+
+
+def harris_correction_forces(rho_in_f, rho_out_f, grid, potcars, crystal, use_pbe, t=1e-4):
+    """src/half_local_forces.F90:124-158: the XC kernel on (n_out - n_in) by
+    central differences of v_xc, contracted with the core density
+    derivative."""
+    delta = np.asarray(rho_out_f) - np.asarray(rho_in_f)
+    vp = build_veff(np.asarray(rho_in_f) + t * delta, grid, potcars, crystal, use_pbe, with_vxc=True)[4]
+    vm = build_veff(np.asarray(rho_in_f) - t * delta, grid, potcars, crystal, use_pbe, with_vxc=True)[4]
+    return nlcc_forces((vp - vm) / (2 * t), grid, potcars, crystal)
 
 
 def compute_occupations(eig, weights, nelect, sigma, ismear=0):

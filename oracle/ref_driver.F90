@@ -9,6 +9,7 @@
 ! Usage:
 !   half_ref_driver solve CHGCAR POTCAR ENCUT lda|pbe NBANDS KX KY KZ VEFF_OUT [uspp]
 !   half_ref_driver kat
+!   half_ref_driver forces CHGCAR POTCAR ENCUT lda|pbe NBANDS KX KY KZ [uspp]
 !
 ! With "uspp" the potential-dependent D correction of src/half_uspp.F90
 ! (build_uspp_dij_cpu, QDEP) is applied after the projectors are built, as
@@ -20,6 +21,12 @@
 ! app/half_cli.F90:896-920 and the non-local force of src/half_forces.F90:9-73.
 ! "kat" evaluates the reference routines on the known-answer inputs of
 ! app/half_energy_check.F90:19-72.
+! "forces" mirrors the analytic force assembly of app/half_cli.F90:921-962
+! for one k point of weight 1 with zero-temperature occupations: smooth
+! output density (src/half_local_forces.F90 accumulate_smooth_density),
+! augmentation occupancy and density (src/half_uspp.F90:26-111), Ewald,
+! local, non-local (projector + dD/dR parts), NLCC and Harris-correction
+! forces.
 program half_ref_driver
   use half_kinds, only: dp, i32, i64
   use half_types, only: crystal_t, charge_grid_t, plane_wave_basis_t, potcar_t
@@ -32,6 +39,10 @@ program half_ref_driver
   use half_forces, only: add_nonlocal_paw_forces
   use half_uspp, only: build_uspp_dij_cpu
   use half_energy, only: compute_occupations, ewald_energy, ewald_forces
+  use half_uspp, only: augmentation_occupancy_t, initialize_augmentation_occupancy, &
+    accumulate_augmentation_occupancy, add_augmentation_density
+  use half_local_forces, only: local_ionic_forces, nlcc_forces, accumulate_smooth_density, &
+    harris_correction_forces
   implicit none
   character(len=1024) :: mode
   call get_command_argument(1, mode)
@@ -40,6 +51,8 @@ program half_ref_driver
     call run_solve()
   case ('kat')
     call run_kat()
+  case ('forces')
+    call run_forces()
   case default
     write(0,'(A)') 'usage: half_ref_driver solve|kat ...'
     stop 2
@@ -177,6 +190,103 @@ contains
       write(*,'(A,I5,3ES25.17)') 'force_nl ', i, forces(i, :)
     end do
   end subroutine run_solve
+
+  subroutine run_forces()
+    character(len=1024) :: chg_path, pot_path, arg
+    type(crystal_t) :: crystal
+    type(charge_grid_t) :: charge, rho_out
+    type(potcar_t), allocatable :: potcars(:)
+    type(plane_wave_basis_t) :: basis
+    type(paw_species_t), allocatable :: paw(:)
+    type(augmentation_occupancy_t), allocatable :: occupancy(:)
+    real(dp), allocatable :: veff(:), vxc(:), values(:), occ(:,:), eig2(:,:), charges(:)
+    real(dp), allocatable :: f_ewald(:,:), f_local(:,:), f_nl(:,:), f_proj(:,:), f_aug(:,:), f_nlcc(:,:), f_harris(:,:)
+    complex(dp), allocatable :: vectors(:,:)
+    real(dp) :: encut, kpt(3), eh, exc, exv, smin, smax, mu, band, entropy, nelect, aug_charge, smooth_sum
+    real(dp) :: weights(1)
+    integer :: nbands, i, it, ion, iat, a, b
+    logical :: pbe, uspp
+    call get_command_argument(2, chg_path); call get_command_argument(3, pot_path)
+    call get_command_argument(4, arg); read(arg, *) encut
+    call get_command_argument(5, arg); pbe = trim(arg) == 'pbe'
+    call get_command_argument(6, arg); read(arg, *) nbands
+    do i = 1, 3
+      call get_command_argument(6 + i, arg); read(arg, *) kpt(i)
+    end do
+    call get_command_argument(10, arg); uspp = trim(arg) == 'uspp'
+    call read_chgcar(trim(chg_path), crystal, charge)
+    call read_potcar(trim(pot_path), potcars)
+    call validate_potcar_structure(potcars, crystal)
+    if (pbe) then
+      call build_veff_pbe(charge, potcars, crystal, veff, eh, exc, exv, vxc)
+    else
+      call build_veff_lda(charge, potcars, crystal, veff, eh, exc, exv, vxc)
+    end if
+    call build_plane_wave_basis(crystal, charge%shape, encut, kpt, basis)
+    call build_paw_operators(potcars, crystal, basis, paw)
+    if (uspp) call build_uspp_dij_cpu(veff, charge%shape, potcars, crystal, paw)
+    call solve_dense_gamma(veff, basis, paw, values, smin, smax, vectors)
+    nelect = 0
+    do it = 1, size(potcars)
+      nelect = nelect + potcars(it)%zval*real(crystal%counts(it), dp)
+    end do
+    weights = 1.0_dp
+    allocate(eig2(1, nbands)); eig2(1,:) = values(:nbands)
+    call compute_occupations(eig2, weights, nelect, 0.0_dp, occ, mu, band, entropy)
+    write(*,'(A,I0)') 'npw ', basis%npw
+    do i = 1, nbands
+      write(*,'(A,I5,2ES25.17)') 'eig_occ ', i, values(i), occ(1, i)
+    end do
+    ! output density: smooth part from the bands, then the augmentation part
+    rho_out%shape = charge%shape
+    allocate(rho_out%values(size(charge%values))); rho_out%values = 0
+    call accumulate_smooth_density(basis, vectors(:, :nbands), occ(1,:), 1.0_dp, charge%shape, rho_out%values)
+    smooth_sum = sum(rho_out%values)
+    call initialize_augmentation_occupancy(paw, occupancy)
+    call accumulate_augmentation_occupancy(paw, vectors(:, :nbands), occ(1,:), 1.0_dp, occupancy)
+    call add_augmentation_density(potcars, crystal, charge%shape, occupancy, rho_out%values, aug_charge)
+    write(*,'(A,ES25.17)') 'smooth_sum ', smooth_sum
+    write(*,'(A,ES25.17)') 'aug_charge ', aug_charge
+    write(*,'(A,ES25.17)') 'rho_out_sum ', sum(rho_out%values)
+    do i = 1, size(rho_out%values), 29
+      write(*,'(A,I9,ES25.17)') 'rho_out ', i, rho_out%values(i)
+    end do
+    ion = 0
+    do it = 1, size(paw)
+      do iat = 1, paw(it)%natoms
+        ion = ion + 1
+        do b = 1, paw(it)%nlm
+          do a = 1, paw(it)%nlm
+            write(*,'(A,3I5,ES25.17)') 'aug_occ ', ion, a, b, occupancy(it)%matrix(iat, a, b)
+          end do
+        end do
+      end do
+    end do
+    allocate(charges(crystal%nions)); ion = 0
+    do it = 1, size(potcars)
+      do iat = 1, crystal%counts(it); ion = ion + 1; charges(ion) = potcars(it)%zval; end do
+    end do
+    allocate(f_ewald(crystal%nions,3), f_local(crystal%nions,3), f_nl(crystal%nions,3), f_proj(crystal%nions,3), &
+      f_aug(crystal%nions,3), f_nlcc(crystal%nions,3), f_harris(crystal%nions,3))
+    f_nl = 0; f_proj = 0; f_aug = 0
+    call add_nonlocal_paw_forces(basis, paw, values(:nbands), occ(1,:), 1.0_dp, vectors(:, :nbands), f_nl, &
+      f_proj, f_aug)
+    call ewald_forces(crystal, charges, f_ewald)
+    call local_ionic_forces(rho_out, potcars, crystal, f_local)
+    call nlcc_forces(vxc, potcars, crystal, charge%shape, f_nlcc)
+    call harris_correction_forces(charge, rho_out, potcars, crystal, pbe, f_harris)
+    do i = 1, crystal%nions
+      write(*,'(A,I5,3ES25.17)') 'force_ewald ', i, f_ewald(i, :)
+      write(*,'(A,I5,3ES25.17)') 'force_local ', i, f_local(i, :)
+      write(*,'(A,I5,3ES25.17)') 'force_nonlocal ', i, f_nl(i, :)
+      write(*,'(A,I5,3ES25.17)') 'force_projector ', i, f_proj(i, :)
+      write(*,'(A,I5,3ES25.17)') 'force_augmentation ', i, f_aug(i, :)
+      write(*,'(A,I5,3ES25.17)') 'force_nlcc ', i, f_nlcc(i, :)
+      write(*,'(A,I5,3ES25.17)') 'force_harris ', i, f_harris(i, :)
+      write(*,'(A,I5,3ES25.17)') 'force_total ', i, f_ewald(i, :) + f_local(i, :) + f_nl(i, :) + f_nlcc(i, :) + &
+        f_harris(i, :)
+    end do
+  end subroutine run_forces
 
   subroutine run_kat()
     ! Known-answer inputs of app/half_energy_check.F90:19-72.

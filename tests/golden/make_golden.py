@@ -11,6 +11,10 @@ Cases: si2_pbe, si2_lda, si2_k (PBE at k = (0.1, 0.2, 0.3)), hfo2_pbe, si8k_k
 (8-atom cell at k = (1/6, -1/3, 1/2) of the 6x6x6 mesh), kat, and the
 potential-dependent D correction (QDEP, src/half_uspp.F90) on datasets with
 PAW partial waves: si2_qdep (k = (0.1, 0.2, 0.3)) and hfo2_qdep (Gamma).
+Force cases (the analytic force assembly of app/half_cli.F90:921-962) on the
+si2 cell with atom 0 displaced by (0.06, -0.03, 0.02) A (si2d.CHGCAR):
+si2d_forces (plain datasets, PBE) and si2d_core_lda / si2d_core_pbe
+(datasets with PAW partial waves and a partial core, QDEP on).
 The si2 input files are committed next to this script; the hfo2 inputs are
 regenerated by the seeded generator and their sha256 is recorded so a test can
 refuse to compare against goldens made from different inputs.
@@ -40,6 +44,13 @@ CASES = {  # workload, xc, k point, bands, QDEP
     "si2_qdep": ("si2", "pbe", (0.1, 0.2, 0.3), 16, True),
     "hfo2_qdep": ("hfo2", "pbe", (0.0, 0.0, 0.0), 64, True),
 }
+
+FORCE_CASES = {  # POTCAR variant, xc, k point, bands, QDEP
+    "si2d_forces": ("si2.POTCAR", "pbe", (0.1, 0.2, 0.3), 8, False),
+    "si2d_core_lda": ("si2.paw.core.POTCAR", "lda", (0.1, 0.2, 0.3), 8, True),
+    "si2d_core_pbe": ("si2.paw.core.POTCAR", "pbe", (0.1, 0.2, 0.3), 8, True),
+}
+SI2D_SHIFT = (0.06, -0.03, 0.02)  # Cartesian displacement of atom 0 (A)
 
 _FLOAT = re.compile(r"^([-+]?\d+\.\d+)([-+]\d{3})$")
 
@@ -123,6 +134,48 @@ def run_case(name):
     print(name, "npw", g["npw"], "eig0", g["eigenvalues"][0])
 
 
+def si2d_inputs():
+    """si2 cell with atom 0 displaced (no symmetry zeroes the forces), the
+    density generated for the displaced atoms; POTCAR variants of si2."""
+    import dataclasses
+    chg = os.path.join(INPUTS, "si2d.CHGCAR")
+    if not os.path.exists(chg):
+        w = synth.workload("si2")
+        pos = w.positions.copy()
+        pos[0] += np.asarray(SI2D_SHIFT) @ np.linalg.inv(w.lattice)
+        synth.write_chgcar(chg, dataclasses.replace(w, positions=pos))
+    core = os.path.join(INPUTS, "si2.paw.core.POTCAR")
+    if not os.path.exists(core):
+        synth.write_potcar(core, synth.workload("si2"), True, True)
+    return chg
+
+
+def run_force_case(name):
+    potname, xc, k, nb, qdep = FORCE_CASES[name]
+    chg = si2d_inputs()
+    pot = os.path.join(INPUTS, potname)
+    w = synth.workload("si2")
+    cmd = [DRIVER, "forces", chg, pot, repr(w.encut), xc, str(nb), *map(repr, k)]
+    if qdep:
+        cmd.append("uspp")
+    d = parse(subprocess.run(cmd, check=True, capture_output=True, text=True).stdout)
+    g = {"workload": "si2d", "xc": xc, "kpoint": list(k), "nbands": nb, "encut": w.encut, "qdep": qdep,
+         "potcar": potname, "chgcar_sha256": synth.sha256(chg), "potcar_sha256": synth.sha256(pot),
+         "npw": int(d["npw"][0][0]),
+         "eigenvalues": [r[1] for r in d["eig_occ"]], "occupations": [r[2] for r in d["eig_occ"]],
+         "smooth_sum": d["smooth_sum"][0][0], "aug_charge": d["aug_charge"][0][0],
+         "rho_out_sum": d["rho_out_sum"][0][0],
+         # one-based Fortran-order grid index (x fastest), value (cell volume x density)
+         "rho_out_samples": [[int(r[0]), r[1]] for r in d["rho_out"]],
+         # one-based ion, a, b, occupancy
+         "aug_occupancy": [[int(r[0]), int(r[1]), int(r[2]), r[3]] for r in d["aug_occ"]]}
+    for key in ("ewald", "local", "nonlocal", "projector", "augmentation", "nlcc", "harris", "total"):
+        g["force_" + key] = [r[1:] for r in d["force_" + key]]
+    with open(os.path.join(HERE, f"{name}.json"), "w") as fh:
+        json.dump(g, fh, indent=1)
+    print(name, "total force atom 1", g["force_total"][0])
+
+
 def run_kat():
     text = subprocess.run([DRIVER, "kat"], check=True, capture_output=True, text=True).stdout
     d = parse(text)
@@ -138,9 +191,11 @@ def run_kat():
 
 
 if __name__ == "__main__":
-    todo = sys.argv[1:] or list(CASES) + ["kat"]
+    todo = sys.argv[1:] or list(CASES) + ["kat"] + list(FORCE_CASES)
     for c in todo:
         if c == "kat":
             run_kat()
+        elif c in FORCE_CASES:
+            run_force_case(c)
         else:
             run_case(c)
