@@ -21,6 +21,7 @@
 #include "../../include/half_amd.h"
 #include "common.hpp"
 #include "forces.hpp"
+#include "fset.hpp"
 #include "kmesh.hpp"
 #include "paw.hpp"
 #include "qdep.hpp"
@@ -686,6 +687,49 @@ int half_amd_solve_kmesh(half_handle handle, int nk, const double *kpoints, int 
       stats->band_applications = res.applies;
       stats->metric_fallbacks = res.metric_fallbacks;
       stats->reserved = 0;
+    }
+    return int(kSuccess);
+  });
+}
+
+int half_amd_force_set(half_handle handle, int nk, const double *kpoints, const double *kweights, int nbands,
+                       const double *eigenvalues, const double *occupations, const HALF_COMPLEX *const *vectors,
+                       const int64_t *ld, double *forces, double *components, double *density, double *charges,
+                       char *error, int capacity) {
+  clear_error(error, capacity);
+  const int slot = context_slot(handle);
+  if (!slot) return bad_handle(error, capacity);
+  if (nk < 1 || nbands < 1 || !kpoints || !kweights || !eigenvalues || !occupations || !vectors || !ld || !forces)
+    return export_error(kInvalidArgument, "k points, bands, eigenvalues, occupations, vectors and forces are required",
+                        error, capacity);
+  return guarded(error, capacity, [&]() {
+    Context &c = *g_contexts[slot];
+    for (int j = 0; j < nk; ++j) {
+      if (!vectors[j]) return export_error(kInvalidArgument, "eigenvectors are required", error, capacity);
+      if (ld[j] < make_basis(c, kpoints + 3 * j).npw())
+        return export_error(kCapacity, "eigenvector leading dimension is too small", error, capacity);
+    }
+    ForceSetState st;
+    st.reset(c.sys, c.stream);
+    for (int j = 0; j < nk; ++j) {
+      std::vector<double> eps(eigenvalues + size_t(j) * nbands, eigenvalues + size_t(j + 1) * nbands);
+      std::vector<double> occ(occupations + size_t(j) * nbands, occupations + size_t(j + 1) * nbands);
+      KOperator<double> &op = c.op64(kpoints + 3 * j);
+      force_set_add_k(c.sys, op, nbands, reinterpret_cast<const double2 *>(vectors[j]), ld[j], eps, occ, kweights[j],
+                      st, c.stream);
+    }
+    ForceSetResult r;
+    force_set_finish(c.sys, st, density != nullptr, r, c.stream);
+    std::copy(r.total.begin(), r.total.end(), forces);
+    if (components) {
+      const std::vector<double> *parts[7] = {&r.ewald, &r.local, &r.nonlocal, &r.projector,
+                                             &r.augmentation, &r.nlcc, &r.harris};
+      for (int i = 0; i < 7; ++i) std::copy(parts[i]->begin(), parts[i]->end(), components + i * r.total.size());
+    }
+    if (density) std::copy(r.rho_out.begin(), r.rho_out.end(), density);
+    if (charges) {
+      charges[0] = r.smooth_charge;
+      charges[1] = r.aug_charge;
     }
     return int(kSuccess);
   });

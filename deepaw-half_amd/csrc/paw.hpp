@@ -151,6 +151,21 @@ void assemble_dense(const KOperator<double> &op, double2 *h, double2 *s, int64_t
 // Device Veff construction (reference: src/half_potential.F90:175-238).
 void build_veff_device(System &sys, hipStream_t s);
 
+// V_xc (eV, C order) of a C-order charge (cell volume x density, device)
+// plus the datasets' partial core (reference: the vxc_out of
+// src/half_potential.F90:31-94).
+void xc_potential_device(const System &sys, const double *charge_c, double *vxc, hipStream_t s);
+
+// f_G = FFT(f) / N_r of a real C-order field (device).
+void field_transform(const System &sys, const double *field_c, double2 *field_g, hipStream_t s);
+
+// F_I = -sum_G q t(|G|) Im[conj(f_G) e^{-i q.R_I}] for every atom (natoms x 3,
+// written): t the local part over the cell volume (core = false; f = the
+// output density) or the partial core table (core = true; f = V_xc or the
+// XC response) (reference: src/half_local_forces.F90:11-98, 160-190).
+void table_forces(const System &sys, const double2 *field_g, bool core, std::vector<double> &forces,
+                  hipStream_t s);
+
 // Conversions between precisions of state blocks.
 void convert_states(const double2 *in, int64_t ldin, float2 *out, int64_t ldout, int n, int nb,
                     hipStream_t s);

@@ -438,6 +438,125 @@ void invert3(const double m[3][3], double inv[9]) {
   inv[8] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det;
 }
 
+// ---------------------------------------------------------------------------
+// augmentation density (reference: src/half_uspp.F90:47-111)
+// ---------------------------------------------------------------------------
+
+constexpr int kAugLmax = 8;  // compensation shapes up to l = 8 (partial waves up to f)
+
+// j_l by the host sph_bessel's recipe: power series below x = l + 2,
+// upward recurrence from j_0, j_1 above
+__device__ inline double sph_bessel_d(int l, double x) {
+  if (fabs(x) < l + 2) {
+    double den = 1;
+    for (int i = 1; i <= l; ++i) den *= 2 * i + 1;
+    const double x2 = x * x;
+    double term = pow(x, l) / den, v = term;
+    for (int k = 0; k <= 30; ++k) {
+      term = -term * x2 / (2.0 * (k + 1) * (2 * l + 2 * k + 3));
+      v += term;
+      if (fabs(term) < 1e-16 * fmax(1.0, fabs(v))) break;
+    }
+    return v;
+  }
+  double jm = sin(x) / x;
+  if (l == 0) return jm;
+  double j = sin(x) / (x * x) - cos(x) / x;
+  for (int n = 1; n < l; ++n) {
+    const double jn = (2 * n + 1) * j / x - jm;
+    jm = j;
+    j = jn;
+  }
+  return j;
+}
+
+// real Ylm at x = cos(theta) with the host ylm's conventions
+__device__ inline double ylm_d(int l, int m, double x, double phi) {
+  const int am = m < 0 ? -m : m;
+  double p0 = 1, f = 1;
+  for (int i = 1; i <= am; ++i) {
+    p0 = -p0 * f * sqrt(fmax(0.0, 1 - x * x));
+    f += 2;
+  }
+  double p = p0;
+  if (l > am) {
+    double p1 = x * (2 * am + 1) * p0;
+    for (int n = am + 2; n <= l; ++n) {
+      const double p2 = ((2 * n - 1) * x * p1 - (n + am - 1) * p0) / (n - am);
+      p0 = p1;
+      p1 = p2;
+    }
+    p = p1;
+  }
+  double fl = 1, fh = 1;
+  for (int i = 2; i <= l - am; ++i) fl *= i;
+  for (int i = 2; i <= l + am; ++i) fh *= i;
+  const double norm = sqrt((2 * l + 1) / (4 * kPi) * fl / fh);
+  const double sgn = (am & 1) ? -1.0 : 1.0;
+  if (m > 0) return sqrt(2.0) * sgn * norm * p * cos(am * phi);
+  if (m < 0) return sqrt(2.0) * sgn * norm * p * sin(am * phi);
+  return norm * p;
+}
+
+struct AugSpecies {
+  double rc;
+  int lmax;
+  double c1[kAugLmax + 1], c2[kAugLmax + 1], z1[kAugLmax + 1], z2[kAugLmax + 1];
+};
+
+struct AugArgs {
+  int n1, n2, n3, natoms, naug_max;
+  double lattice[9];      // rows are lattice vectors
+  double volume;
+  const double *frac;     // natoms x 3 fractional positions
+  const int *species;     // per atom
+  const double *q;        // natoms x naug_max: sum_ab rho_ab M_ab,k
+  const AugSpecies *sp;
+};
+
+// one thread per grid point (C order): Omega sum over the atoms whose PAW
+// sphere holds the point of sum_k q_k g_l(r) Y_k(r_hat), the displacement
+// wrapped to the nearest image like the reference (rounding half away
+// from zero)
+__global__ void aug_density_kernel(AugArgs a, double *aug) {
+  const int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  const int64_t nr = int64_t(a.n1) * a.n2 * a.n3;
+  if (idx >= nr) return;
+  const int i3 = static_cast<int>(idx % a.n3);
+  const int i2 = static_cast<int>((idx / a.n3) % a.n2);
+  const int i1 = static_cast<int>(idx / (int64_t(a.n2) * a.n3));
+  const double fr[3] = {double(i1) / a.n1, double(i2) / a.n2, double(i3) / a.n3};
+  double total = 0;
+  for (int at = 0; at < a.natoms; ++at) {
+    const AugSpecies &sp = a.sp[a.species[at]];
+    double df[3];
+    for (int j = 0; j < 3; ++j) {
+      df[j] = fr[j] - a.frac[3 * at + j];
+      df[j] -= round(df[j]);
+    }
+    double c[3];
+    for (int j = 0; j < 3; ++j) c[j] = df[0] * a.lattice[j] + df[1] * a.lattice[3 + j] + df[2] * a.lattice[6 + j];
+    const double r = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    if (r > sp.rc) continue;
+    double x = 1, phi = 0;
+    if (r > 1e-14) {
+      x = fmin(1.0, fmax(-1.0, c[2] / r));
+      phi = atan2(c[1], c[0]);
+    }
+    // the reference takes theta = acos(x) and Y_lm(cos(theta), phi)
+    x = cos(acos(x));
+    double value = 0;
+    const double *q = a.q + int64_t(at) * a.naug_max;
+    for (int l = 0, k = 0; l <= sp.lmax; ++l) {
+      const double g = sp.c1[l] * sph_bessel_d(l, sp.z1[l] * r / sp.rc) + sp.c2[l] * sph_bessel_d(l, sp.z2[l] * r / sp.rc);
+      for (int m = -l; m <= l; ++m, ++k)
+        if (q[k] != 0.0) value += q[k] * g * ylm_d(l, m, x, phi);
+    }
+    total += a.volume * value;
+  }
+  aug[idx] = total;
+}
+
 }  // namespace
 
 void build_qdep(System &sys, hipStream_t s) {
@@ -514,6 +633,74 @@ void build_qdep(System &sys, hipStream_t s) {
   std::vector<float> d32(sys.dblk.begin(), sys.dblk.end());
   sys.d_dblk32.upload(d32, s);
   HALF_HIP(hipStreamSynchronize(s));
+}
+
+double augmentation_density(const System &sys, const std::vector<double> &occupancy, double *aug, hipStream_t s) {
+  const int n1 = sys.fft.n[0], n2 = sys.fft.n[1], n3 = sys.fft.n[2];
+  const int64_t nr = int64_t(n1) * n2 * n3;
+  const int nsp = static_cast<int>(sys.sets.size());
+  std::vector<AugSpecies> sps(nsp);
+  std::vector<SpeciesTables> tabs;
+  int naug_max = 1;
+  for (int it = 0; it < nsp; ++it) {
+    const Dataset &d = sys.sets[it];
+    tabs.push_back(species_tables(d));
+    const SpeciesTables &t = tabs.back();
+    if (t.lmax > kAugLmax) throw Error(kUnavailable, "augmentation: angular momentum above 8");
+    AugSpecies &a = sps[it];
+    a.rc = d.paw_rmax;
+    a.lmax = t.lmax;
+    for (int l = 0; l <= t.lmax; ++l) {
+      a.z1[l] = bessel_root(l, 1);
+      a.z2[l] = bessel_root(l, 2);
+      compensation(l, a.rc, a.z1[l], a.z2[l], a.c1[l], a.c2[l]);
+    }
+    naug_max = std::max(naug_max, t.naug);
+  }
+  // q_k of every atom: sum_ab rho_ab M_ab,k (rho at the atom's D-block offset)
+  std::vector<double> q(size_t(sys.natoms) * naug_max, 0.0), frac(size_t(sys.natoms) * 3);
+  std::vector<int> species(sys.natoms);
+  for (int at = 0; at < sys.natoms; ++at) {
+    const int it = sys.atom_species[at];
+    const SpeciesTables &t = tabs[it];
+    species[at] = it;
+    const double *rho = occupancy.data() + sys.atom_boff[at];
+    for (int k = 0; k < t.naug; ++k) {
+      double v = 0;
+      for (int b = 0; b < t.nlm; ++b)
+        for (int aa = 0; aa < t.nlm; ++aa) v += rho[aa * t.nlm + b] * t.mult[(size_t(aa) * t.nlm + b) * t.naug + k];
+      q[size_t(at) * naug_max + k] = v;
+    }
+    for (int j = 0; j < 3; ++j) frac[3 * at + j] = sys.crystal.positions[3 * at + j];
+  }
+  DeviceArray<double> d_q, d_frac;
+  DeviceArray<int> d_sp;
+  DeviceArray<AugSpecies> d_species;
+  d_q.upload(q, s);
+  d_frac.upload(frac, s);
+  d_sp.upload(species, s);
+  d_species.upload(sps, s);
+  AugArgs a{};
+  a.n1 = n1;
+  a.n2 = n2;
+  a.n3 = n3;
+  a.natoms = sys.natoms;
+  a.naug_max = naug_max;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) a.lattice[3 * i + j] = sys.crystal.lattice[i][j];
+  a.volume = sys.crystal.volume;
+  a.frac = d_frac.get();
+  a.species = d_sp.get();
+  a.q = d_q.get();
+  a.sp = d_species.get();
+  hipLaunchKernelGGL(aug_density_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, a, aug);
+  HALF_HIP(hipGetLastError());
+  std::vector<double> h(nr);
+  HALF_HIP(hipMemcpyAsync(h.data(), aug, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
+  HALF_HIP(hipStreamSynchronize(s));
+  double sum = 0;
+  for (double v : h) sum += v;
+  return sum / double(nr);
 }
 
 }  // namespace half

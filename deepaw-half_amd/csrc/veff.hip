@@ -300,15 +300,23 @@ double device_sum(const double *x, int64_t n, hipStream_t s) {
 
 }  // namespace
 
-void build_veff_device(System &sys, hipStream_t s) {
-  const int *sh = sys.charge.shape;
-  const int64_t nr = sys.charge.size();
-  Geometry geo{sh[0], sh[1], sh[2], {}, sys.crystal.volume};
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) geo.b[i][j] = sys.crystal.reciprocal[i][j];
-  const int nsp = static_cast<int>(sys.sets.size());
+namespace {
 
-  // species tables
+// Device species tables of the local and partial-core parts plus the atom
+// positions, and the box geometry.
+struct SpeciesStore {
+  DeviceArray<double> d_loc, d_locm2, d_core, d_corem2, d_gmax, d_zval, d_pos;
+  DeviceArray<int> d_has, d_asp;
+  SpeciesTabs tabs{};
+  Geometry geo{};
+};
+
+void make_species(const System &sys, SpeciesStore &st, hipStream_t s) {
+  const int *sh = sys.charge.shape;
+  st.geo = Geometry{sh[0], sh[1], sh[2], {}, sys.crystal.volume};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) st.geo.b[i][j] = sys.crystal.reciprocal[i][j];
+  const int nsp = static_cast<int>(sys.sets.size());
   std::vector<double> loc, locm2, core, corem2, gmax, zval;
   std::vector<int> has_core;
   for (auto &d : sys.sets) {
@@ -341,22 +349,119 @@ void build_veff_device(System &sys, hipStream_t s) {
                         f[2] * sys.crystal.lattice[2][j]);
       }
   }
-  DeviceArray<double> d_loc, d_locm2, d_core, d_corem2, d_gmax, d_zval, d_pos;
-  DeviceArray<int> d_has, d_asp;
-  d_loc.upload(loc, s);
-  d_locm2.upload(locm2, s);
-  d_core.upload(core, s);
-  d_corem2.upload(corem2, s);
-  d_gmax.upload(gmax, s);
-  d_zval.upload(zval, s);
-  d_pos.upload(pos, s);
-  d_has.upload(has_core, s);
-  d_asp.upload(atom_species, s);
-  SpeciesTabs tabs{d_loc.get(), d_locm2.get(), d_core.get(), d_corem2.get(), d_gmax.get(), d_zval.get(),
-                   d_has.get(), d_asp.get(), d_pos.get(), nsp, static_cast<int>(atom_species.size())};
+  st.d_loc.upload(loc, s);
+  st.d_locm2.upload(locm2, s);
+  st.d_core.upload(core, s);
+  st.d_corem2.upload(corem2, s);
+  st.d_gmax.upload(gmax, s);
+  st.d_zval.upload(zval, s);
+  st.d_pos.upload(pos, s);
+  st.d_has.upload(has_core, s);
+  st.d_asp.upload(atom_species, s);
+  st.tabs = SpeciesTabs{st.d_loc.get(), st.d_locm2.get(), st.d_core.get(), st.d_corem2.get(), st.d_gmax.get(),
+                        st.d_zval.get(), st.d_has.get(), st.d_asp.get(), st.d_pos.get(), nsp,
+                        static_cast<int>(atom_species.size())};
+  HALF_HIP(hipStreamSynchronize(s));  // the uploads read host vectors local to this call
+}
 
+// V_xc[charge + core] (eV, C order) into vxc; part holds per-point energy
+// terms; returns e_xc (reference: src/half_potential.F90:229-235)
+double xc_stage(const System &sys, const Geometry &geo, const double *charge_c, const double2 *core_r,
+                double *vxc, double *part, hipStream_t s) {
+  const int64_t nr = int64_t(geo.n1) * geo.n2 * geo.n3;
+  double exc = 0;
+  if (sys.use_pbe) {
+    DeviceArray<double2> work(3 * nr), freq(nr), div(3 * nr);
+    hipLaunchKernelGGL(pbe_prepare_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, charge_c, core_r, freq.get(),
+                       nr, 1.0 / sys.crystal.volume);
+    fft3<double>(sys.fft, freq.get(), 1, -1, s);
+    hipLaunchKernelGGL(spectral_derivative_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, freq.get(),
+                       int64_t(0), work.get(), geo);
+    fft3<double>(sys.fft, work.get(), 3, +1, s);  // gradients (unscaled) in .x
+    hipLaunchKernelGGL(pbe_point_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, charge_c, core_r, work.get(), vxc,
+                       part, nr, sys.crystal.volume);
+    exc = device_sum(part, nr, s);
+    fft3<double>(sys.fft, work.get(), 3, -1, s);  // fluxes to G space
+    hipLaunchKernelGGL(spectral_derivative_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, work.get(), nr,
+                       div.get(), geo);
+    // only the alpha-th derivative of the alpha-th flux is needed
+    fft3<double>(sys.fft, div.get(), 3, +1, s);
+    hipLaunchKernelGGL(pbe_finish_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, charge_c, core_r, div.get(), vxc,
+                       nr, sys.crystal.volume);
+    HALF_HIP(hipStreamSynchronize(s));  // the work arrays are released here
+  } else {
+    hipLaunchKernelGGL(lda_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, charge_c, core_r, vxc, part, nr,
+                       sys.crystal.volume);
+    exc = device_sum(part, nr, s);
+  }
+  HALF_HIP(hipGetLastError());
+  return exc;
+}
+
+// F_I -= q t(|G|) Im[conj(f_G) e^{-i q.R_I}] over the box for atom blockIdx.y,
+// one partial 3-vector per workgroup (fixed-order tree); t is the local part
+// (minus the Coulomb tail, over the cell volume) or, with use_core, the
+// partial core table (reference: src/half_local_forces.F90:11-98)
+__global__ __launch_bounds__(256) void table_force_kernel(const double2 *field_g, Geometry geo, SpeciesTabs sp,
+                                                          int use_core, double *partial) {
+  __shared__ double red[3][256];
+  const int a = blockIdx.y;
+  const int it = sp.atom_species[a];
+  const int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  const int64_t nr = int64_t(geo.n1) * geo.n2 * geo.n3;
+  double f[3] = {0, 0, 0};
+  if (idx < nr) {
+    double q[3];
+    grid_q(geo, idx, q);
+    const double g2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    const double gn = sqrt(g2);
+    const double gmax = sp.gmax[it];
+    bool ok = g2 > 1e-12;
+    double rad = 0;
+    if (!use_core) {
+      ok = ok && !(gn > gmax - gmax / 1000);
+      if (ok)
+        rad = (spline_eval_d(sp.loc + it * 1000, sp.loc_m2 + it * 1000, 1000, gmax, gn) -
+               4 * kPi * sp.zval[it] * kFelect / g2) /
+              geo.volume;
+    } else {
+      ok = ok && sp.has_core[it] && !(gn > gmax - 3 * gmax / 1000);
+      if (ok) rad = spline_eval_d(sp.core + it * 1000, sp.core_m2 + it * 1000, 1000, gmax, gn);
+    }
+    if (ok) {
+      const double ph = q[0] * sp.pos[3 * a] + q[1] * sp.pos[3 * a + 1] + q[2] * sp.pos[3 * a + 2];
+      double sn, cs;
+      sincos(ph, &sn, &cs);
+      const double2 v = field_g[idx];
+      const double term = rad * (-v.x * sn - v.y * cs);
+      for (int j = 0; j < 3; ++j) f[j] = -q[j] * term;
+    }
+  }
+  for (int j = 0; j < 3; ++j) red[j][threadIdx.x] = f[j];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w)
+      for (int j = 0; j < 3; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int j = 0; j < 3; ++j) partial[(int64_t(a) * gridDim.x + blockIdx.x) * 3 + j] = red[j][0];
+}
+
+__global__ void to_complex_kernel(const double *v, double2 *z, int64_t n) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < n) z[i] = make_double2(v[i], 0.0);
+}
+
+}  // namespace
+
+void build_veff_device(System &sys, hipStream_t s) {
+  const int *sh = sys.charge.shape;
+  const int64_t nr = sys.charge.size();
+  SpeciesStore st;
+  make_species(sys, st, s);
   DeviceArray<double> raw, charge_c, vxc, part;
-  DeviceArray<double2> rho_g, local_g, core_g, work;
+  DeviceArray<double2> rho_g, local_g, core_g;
   raw.upload(sys.charge.values, s);
   charge_c.resize(nr);
   rho_g.resize(nr);
@@ -369,38 +474,12 @@ void build_veff_device(System &sys, hipStream_t s) {
   HALF_HIP(hipGetLastError());
   fft3<double>(sys.fft, rho_g.get(), 1, -1, s);
   hipLaunchKernelGGL(reciprocal_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, rho_g.get(),
-                     local_g.get(), core_g.get(), part.get(), geo, tabs, 1.0 / double(nr));
+                     local_g.get(), core_g.get(), part.get(), st.geo, st.tabs, 1.0 / double(nr));
   HALF_HIP(hipGetLastError());
   sys.e_hartree = device_sum(part.get(), nr, s);
   fft3<double>(sys.fft, local_g.get(), 1, +1, s);  // local_r in .x
   fft3<double>(sys.fft, core_g.get(), 1, +1, s);   // core_r in .x
-
-  if (sys.use_pbe) {
-    work.resize(3 * nr);
-    DeviceArray<double2> freq(nr);
-    hipLaunchKernelGGL(pbe_prepare_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, charge_c.get(),
-                       core_g.get(), freq.get(), nr, 1.0 / sys.crystal.volume);
-    fft3<double>(sys.fft, freq.get(), 1, -1, s);
-    hipLaunchKernelGGL(spectral_derivative_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, freq.get(),
-                       int64_t(0), work.get(), geo);
-    fft3<double>(sys.fft, work.get(), 3, +1, s);  // gradients (unscaled) in .x
-    hipLaunchKernelGGL(pbe_point_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, charge_c.get(),
-                       core_g.get(), work.get(), vxc.get(), part.get(), nr, sys.crystal.volume);
-    sys.e_xc = device_sum(part.get(), nr, s);
-    fft3<double>(sys.fft, work.get(), 3, -1, s);  // fluxes to G space
-    DeviceArray<double2> div(3 * nr);
-    hipLaunchKernelGGL(spectral_derivative_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, work.get(),
-                       nr, div.get(), geo);
-    // only the alpha-th derivative of the alpha-th flux is needed
-    fft3<double>(sys.fft, div.get(), 3, +1, s);
-    hipLaunchKernelGGL(pbe_finish_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, charge_c.get(),
-                       core_g.get(), div.get(), vxc.get(), nr, sys.crystal.volume);
-  } else {
-    hipLaunchKernelGGL(lda_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, charge_c.get(), core_g.get(),
-                       vxc.get(), part.get(), nr, sys.crystal.volume);
-    sys.e_xc = device_sum(part.get(), nr, s);
-  }
-  HALF_HIP(hipGetLastError());
+  sys.e_xc = xc_stage(sys, st.geo, charge_c.get(), core_g.get(), vxc.get(), part.get(), s);
   sys.veff.resize(nr);
   sys.veff32.resize(nr);
   sys.veff_g.resize(nr);
@@ -413,6 +492,52 @@ void build_veff_device(System &sys, hipStream_t s) {
                      1.0 / double(nr));
   HALF_HIP(hipGetLastError());
   HALF_HIP(hipStreamSynchronize(s));
+}
+
+void xc_potential_device(const System &sys, const double *charge_c, double *vxc, hipStream_t s) {
+  const int64_t nr = sys.charge.size();
+  SpeciesStore st;
+  make_species(sys, st, s);
+  DeviceArray<double2> rho_g(nr), local_g(nr), core_g(nr);
+  DeviceArray<double> part(nr);
+  hipLaunchKernelGGL(to_complex_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, charge_c, rho_g.get(), nr);
+  HALF_HIP(hipGetLastError());
+  fft3<double>(sys.fft, rho_g.get(), 1, -1, s);
+  hipLaunchKernelGGL(reciprocal_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, rho_g.get(), local_g.get(),
+                     core_g.get(), part.get(), st.geo, st.tabs, 1.0 / double(nr));
+  HALF_HIP(hipGetLastError());
+  fft3<double>(sys.fft, core_g.get(), 1, +1, s);  // core_r in .x
+  (void)xc_stage(sys, st.geo, charge_c, core_g.get(), vxc, part.get(), s);
+  HALF_HIP(hipStreamSynchronize(s));
+}
+
+void field_transform(const System &sys, const double *field_c, double2 *field_g, hipStream_t s) {
+  const int64_t nr = sys.charge.size();
+  hipLaunchKernelGGL(to_complex_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, field_c, field_g, nr);
+  HALF_HIP(hipGetLastError());
+  fft3<double>(sys.fft, field_g, 1, -1, s);
+  hipLaunchKernelGGL(scale_complex_kernel, dim3(blocks_for(nr)), dim3(kThreads), 0, s, field_g, nr,
+                     1.0 / double(nr));
+  HALF_HIP(hipGetLastError());
+}
+
+void table_forces(const System &sys, const double2 *field_g, bool core, std::vector<double> &forces, hipStream_t s) {
+  const int64_t nr = sys.charge.size();
+  SpeciesStore st;
+  make_species(sys, st, s);
+  const unsigned nblk = blocks_for(nr);
+  const int nat = st.tabs.natoms;
+  DeviceArray<double> partial(size_t(nat) * nblk * 3);
+  hipLaunchKernelGGL(table_force_kernel, dim3(nblk, nat), dim3(kThreads), 0, s, field_g, st.geo, st.tabs,
+                     core ? 1 : 0, partial.get());
+  HALF_HIP(hipGetLastError());
+  std::vector<double> h(partial.size());
+  HALF_HIP(hipMemcpyAsync(h.data(), partial.get(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+  HALF_HIP(hipStreamSynchronize(s));
+  forces.assign(size_t(nat) * 3, 0.0);
+  for (int a = 0; a < nat; ++a)
+    for (unsigned b = 0; b < nblk; ++b)
+      for (int j = 0; j < 3; ++j) forces[size_t(a) * 3 + j] += h[(size_t(a) * nblk + b) * 3 + j];
 }
 
 }  // namespace half

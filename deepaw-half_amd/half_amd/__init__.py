@@ -40,6 +40,7 @@ EXT_SYMBOLS = (
     "half_amd_get_veff", "half_amd_nonlocal_forces", "half_amd_device_info",
     "half_amd_trace", "half_amd_trace_read", "half_amd_set_tuning", "half_amd_get_tuning",
     "half_amd_set_projector_positions", "half_amd_solve_history", "half_amd_solve_kmesh",
+    "half_amd_force_set",
 )
 
 
@@ -114,6 +115,8 @@ def load(path: str | None = None):
         "half_amd_set_projector_positions": [i64, i, pd, cp, i],
         "half_amd_solve_history": [i64, i, pd, pi32, cp, i],
         "half_amd_solve_kmesh": [i64, i, pd, i, i, d, d, i, i, i, pd, C.POINTER(SolveStats), cp, i],
+        "half_amd_force_set": [i64, i, pd, pd, i, pd, pd, C.POINTER(C.c_void_p), C.POINTER(C.c_int64), pd, pd,
+                               pd, pd, cp, i],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -351,6 +354,38 @@ class Context:
         _check(self.lib.half_amd_nonlocal_forces(self.handle, _kp(k), nb, _pd(eps), _pd(occ), float(kweight),
                                                  vec.ctypes.data, n, _pd(f), e, 512), e)
         return f
+
+    FORCE_PARTS = ("ewald", "local", "nonlocal", "projector", "augmentation", "nlcc", "harris")
+
+    def force_set(self, kpoints, kweights, eigenvalues, occupations, vectors, density=False):
+        """Analytic force set (half_amd_force_set) for the bands of each k
+        point: vectors[j] is npw_j x nbands.  Returns a dict with "total"
+        and the FORCE_PARTS (natoms x 3 each), "smooth_charge",
+        "aug_charge" and, with density, "density" (CHGCAR order)."""
+        ks = np.ascontiguousarray(np.asarray(kpoints, float).reshape(-1, 3))
+        nk = len(ks)
+        kw = np.ascontiguousarray(np.asarray(kweights, float).reshape(nk))
+        eps = np.ascontiguousarray(np.asarray(eigenvalues, float).reshape(nk, -1))
+        nb = eps.shape[1]
+        occ = np.ascontiguousarray(np.asarray(occupations, float).reshape(nk, nb))
+        cols = [np.asfortranarray(np.asarray(v, np.complex128)[:, :nb]) for v in vectors]
+        ptrs = (C.c_void_p * nk)(*[c.ctypes.data for c in cols])
+        lds = (C.c_int64 * nk)(*[c.shape[0] for c in cols])
+        nat = self.system_info()["nions"]
+        total = np.zeros((nat, 3))
+        parts = np.zeros((7, nat, 3))
+        charges = np.zeros(2)
+        grid = self.system_info()["grid"]
+        dens = np.zeros(int(np.prod(grid))) if density else None
+        e = _err()
+        _check(self.lib.half_amd_force_set(self.handle, nk, _pd(ks), _pd(kw), nb, _pd(eps), _pd(occ), ptrs, lds,
+                                           _pd(total), _pd(parts), _pd(dens) if density else None, _pd(charges),
+                                           e, 512), e)
+        out = {"total": total, "smooth_charge": charges[0], "aug_charge": charges[1]}
+        out.update({k: parts[i] for i, k in enumerate(self.FORCE_PARTS)})
+        if density:
+            out["density"] = dens
+        return out
 
     def solve_history(self):
         """Max residual (eV) per iteration of the last solve_acc."""

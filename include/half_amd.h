@@ -85,6 +85,25 @@ int half_amd_solve_kmesh(half_handle handle, int nk, const double *kpoints, int 
                          double switch_tol, int maxiter, int extra, int graph, double *eigenvalues_eV,
                          half_amd_solve_stats *stats, char *error, int error_capacity);
 
+/* Full analytic force set (reference: app/half_cli.F90:921-962) for the
+ * bands of nk k points (fractional kpoints nk x 3, weights kweights):
+ * eigenvalues / occupations nk x nbands (k-major), vectors[j] the host
+ * eigenvectors of k point j (npw_j x nbands, leading dimension ld[j]).
+ * Output density = sum_k w sum_n f_n |psi|^2 + augmentation density of
+ * rho_ab = sum w f Re[conj(c_a) c_b]; forces (natoms x 3, eV/A, written) =
+ * Ewald + local (output density) + non-local (projector + dD/dR) + NLCC
+ * (V_xc of the context's density) + Harris correction (XC response to the
+ * output - input density on the partial core, central difference t = 1e-4).
+ * components (may be null): 7 x natoms x 3, in the order Ewald, local,
+ * non-local, its projector part, its dD/dR part, NLCC, Harris.  density (may
+ * be null): the output density, CHGCAR order (x fastest), cell volume x
+ * density.  charges (may be null): grid means of the smooth and the
+ * augmentation parts. */
+int half_amd_force_set(half_handle handle, int nk, const double *kpoints, const double *kweights, int nbands,
+                       const double *eigenvalues, const double *occupations, const HALF_COMPLEX *const *vectors,
+                       const int64_t *ld, double *forces, double *components, double *density, double *charges,
+                       char *error, int error_capacity);
+
 /* Select the solver used by half_solve_kpoint for this context (1..4). */
 int half_amd_set_solver(half_handle handle, int solver, char *error, int error_capacity);
 
