@@ -207,6 +207,86 @@ void copy_vectors_out(const DeviceArray<double2> &v, int n, int nb, double *out,
   HALF_HIP(hipStreamSynchronize(s));
 }
 
+// Makes a request geometry the context's crystal.  The density file fixes
+// the grid and the composition, so the request must have the file's grid,
+// one type per potential dataset and the file's ion count per type; its
+// lattice and fractional positions (any ion order, regrouped by type, stable)
+// replace the file's.  Veff, the projector tables, the atom positions and,
+// with the potential-dependent correction, D are rebuilt on it; cached
+// operators and the context's operator handles are dropped, since their
+// basis and projectors belong to the old cell.  On a failed rebuild the
+// file's crystal is restored.  This is the fix the reference records in
+// docs/validation/HFO2_NWAVE_BOUNDARY_MISMATCH.md:43-50: the reference stores
+// the geometry (src/half_library.F90:227-247) but keeps building from the
+// density file (the same document, :33-36).
+void apply_request_geometry(Context &c, int64_t handle, const half_request_geometry_v1 &g) {
+  System &s = c.sys;
+  const int ntypes = static_cast<int>(s.sets.size());
+  if (g.ntypes != ntypes)
+    throw Error(kInvalidArgument, "request geometry has " + std::to_string(g.ntypes) +
+                                      " types but the potential file has " + std::to_string(ntypes) + " datasets");
+  for (int d = 0; d < 3; ++d)
+    if (g.grid[d] != s.charge.shape[d])
+      throw Error(kInvalidArgument, "request grid " + std::to_string(g.grid[0]) + "x" + std::to_string(g.grid[1]) +
+                                        "x" + std::to_string(g.grid[2]) + " differs from the density grid " +
+                                        std::to_string(s.charge.shape[0]) + "x" + std::to_string(s.charge.shape[1]) +
+                                        "x" + std::to_string(s.charge.shape[2]));
+  std::vector<int> counts(ntypes, 0);
+  for (int i = 0; i < g.nions; ++i) ++counts[g.species[i] - 1];
+  if (counts != s.crystal.counts)
+    throw Error(kInvalidArgument, "request geometry ion count per type differs from the density file");
+  Crystal cell = s.crystal;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) cell.lattice[i][j] = g.lattice[3 * i + j];
+  cell.positions.clear();
+  for (int t = 1; t <= ntypes; ++t)
+    for (int i = 0; i < g.nions; ++i)
+      if (g.species[i] == t)
+        cell.positions.insert(cell.positions.end(), g.positions_fractional + 3 * i, g.positions_fractional + 3 * i + 3);
+  cell.update_geometry();
+  HALF_HIP(hipStreamSynchronize(c.stream));
+  for (auto it = g_ops.begin(); it != g_ops.end();)
+    it = it->second.owner == handle ? g_ops.erase(it) : std::next(it);
+  c.ops64.clear();
+  c.ops32.clear();
+  const Crystal file_cell = s.crystal;
+  s.crystal = cell;
+  try {
+    s.build(c.stream);
+    HALF_HIP(hipStreamSynchronize(c.stream));
+  } catch (...) {
+    s.crystal = file_cell;
+    s.build(c.stream);
+    throw;
+  }
+}
+
+// First G vector that lies in only one of the two sets, as "(x, y, z) in the
+// host basis only" / "... in the HALF basis only"; empty when the sets agree.
+std::string first_differing_g(const Basis &b, int64_t host_npw, const int32_t *host_gvec) {
+  auto key = [](int32_t x, int32_t y, int32_t z) {
+    return (uint64_t(uint32_t(x) & 0x1fffff) << 42) | (uint64_t(uint32_t(y) & 0x1fffff) << 21) |
+           uint64_t(uint32_t(z) & 0x1fffff);
+  };
+  auto text = [](const int32_t *g, const char *where) {
+    return "(" + std::to_string(g[0]) + ", " + std::to_string(g[1]) + ", " + std::to_string(g[2]) + ") in the " +
+           where + " basis only";
+  };
+  std::unordered_map<uint64_t, int64_t> host, own;
+  host.reserve(size_t(host_npw) * 2);
+  own.reserve(size_t(b.npw()) * 2);
+  for (int64_t i = 0; i < host_npw; ++i) host.emplace(key(host_gvec[3 * i], host_gvec[3 * i + 1], host_gvec[3 * i + 2]), i);
+  for (int64_t i = 0; i < b.npw(); ++i) {
+    const int32_t *g = &b.gvec[3 * i];
+    own.emplace(key(g[0], g[1], g[2]), i);
+    if (!host.count(key(g[0], g[1], g[2]))) return text(g, "HALF");
+  }
+  for (int64_t i = 0; i < host_npw; ++i)
+    if (!own.count(key(host_gvec[3 * i], host_gvec[3 * i + 1], host_gvec[3 * i + 2])))
+      return text(host_gvec + 3 * i, "host");
+  return std::string();
+}
+
 void run_solve(Context &c, const double *k, int nbands, bool want_vectors, DeviceArray<double2> &vec,
                SolveResult &res) {
   KOperator<double> &op = c.op64(k);
@@ -299,22 +379,27 @@ int half_set_request_geometry(half_handle handle, const half_request_geometry_v1
   for (int i = 0; i < g->nions; ++i)
     if (g->species[i] < 1 || g->species[i] > g->ntypes)
       return export_error(kInvalidArgument, "invalid request geometry values", error, capacity);
+  // the reference rejects NaN; the geometry is used here, so infinities too
   for (int i = 0; i < 9; ++i)
-    if (g->lattice[i] != g->lattice[i]) return export_error(kInvalidArgument, "invalid request geometry values", error, capacity);
+    if (!std::isfinite(g->lattice[i]))
+      return export_error(kInvalidArgument, "invalid request geometry values", error, capacity);
   for (int i = 0; i < 3 * g->nions; ++i)
-    if (g->positions_fractional[i] != g->positions_fractional[i])
+    if (!std::isfinite(g->positions_fractional[i]))
       return export_error(kInvalidArgument, "invalid request geometry values", error, capacity);
   if (std::fabs(det9(g->lattice)) < 1e-12)
     return export_error(kInvalidArgument, "request lattice is singular", error, capacity);
-  Context &c = *g_contexts[slot];
-  c.req_nions = g->nions;
-  c.req_ntypes = g->ntypes;
-  for (int i = 0; i < 3; ++i) c.req_grid[i] = g->grid[i];
-  std::memcpy(c.req_lattice, g->lattice, sizeof(c.req_lattice));
-  c.req_species.assign(g->species, g->species + g->nions);
-  c.req_positions.assign(g->positions_fractional, g->positions_fractional + 3 * g->nions);
-  c.has_request = true;
-  return int(kSuccess);
+  return guarded(error, capacity, [&]() {
+    Context &c = *g_contexts[slot];
+    apply_request_geometry(c, handle, *g);
+    c.req_nions = g->nions;
+    c.req_ntypes = g->ntypes;
+    for (int i = 0; i < 3; ++i) c.req_grid[i] = g->grid[i];
+    std::memcpy(c.req_lattice, g->lattice, sizeof(c.req_lattice));
+    c.req_species.assign(g->species, g->species + g->nions);
+    c.req_positions.assign(g->positions_fractional, g->positions_fractional + 3 * g->nions);
+    c.has_request = true;
+    return int(kSuccess);
+  });
 }
 
 int half_get_request_geometry(half_handle handle, int32_t *nions, int32_t *ntypes, int32_t grid[3],
@@ -486,11 +571,15 @@ int half_solve_kpoint_mapped(half_handle handle, const double kpoint[3], int nba
     if (nbands > n) return export_error(kInvalidArgument, "invalid requested band count", error, capacity);
     if (c.solver == kSolverAcc && 2 * int64_t(nbands) > n)
       return export_error(kInvalidArgument, "ACC solver requires NBANDS <= NPL/2", error, capacity);
-    DeviceArray<double2> vec;
-    SolveResult res;
-    run_solve(c, kpoint, nbands, true, vec, res);
+    // The bases are compared before the solve (the reference solves first,
+    // src/half_c_api.F90:543-551); a mismatch names both sizes and the first
+    // G vector found in one basis only.
     if (host_npw != n)
-      return export_error(kInvalidArgument, "host and HALF plane-wave bases have different sizes", error, capacity);
+      return export_error(kInvalidArgument,
+                          "host and HALF plane-wave bases have different sizes (host " + std::to_string(host_npw) +
+                              ", HALF " + std::to_string(n) + "); first differing G vector " +
+                              first_differing_g(b, host_npw, host_gvec),
+                          error, capacity);
     // G-set permutation (reference: src/half_c_api.F90:553-585)
     std::unordered_map<uint64_t, int64_t> table;
     table.reserve(size_t(n) * 2);
@@ -502,19 +591,27 @@ int half_solve_kpoint_mapped(half_handle handle, const double kpoint[3], int nba
       if (!table.emplace(key(host_gvec[3 * i], host_gvec[3 * i + 1], host_gvec[3 * i + 2]), i).second)
         return export_error(kInvalidArgument, "host G-vector list contains a duplicate", error, capacity);
     }
+    std::vector<int64_t> slot_of(n);
+    for (int64_t i = 0; i < n; ++i) {
+      auto it = table.find(key(b.gvec[3 * i], b.gvec[3 * i + 1], b.gvec[3 * i + 2]));
+      if (it == table.end())
+        return export_error(kInvalidArgument,
+                            "host and HALF plane-wave bases contain different G vectors; first differing G vector " +
+                                first_differing_g(b, host_npw, host_gvec),
+                            error, capacity);
+      slot_of[i] = it->second;
+    }
+    DeviceArray<double2> vec;
+    SolveResult res;
+    run_solve(c, kpoint, nbands, true, vec, res);
     std::vector<double2> hv(size_t(n) * nbands);
     HALF_HIP(hipMemcpyAsync(hv.data(), vec.get(), hv.size() * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
     HALF_HIP(hipStreamSynchronize(c.stream));
     double2 *out = reinterpret_cast<double2 *>(eigenvectors);
     for (int bnd = 0; bnd < nbands; ++bnd)
       for (int64_t i = 0; i < ld; ++i) out[i + bnd * ld] = make_double2(0, 0);
-    for (int64_t i = 0; i < n; ++i) {
-      auto it = table.find(key(b.gvec[3 * i], b.gvec[3 * i + 1], b.gvec[3 * i + 2]));
-      if (it == table.end())
-        return export_error(kInvalidArgument, "host and HALF plane-wave bases contain different G vectors", error,
-                            capacity);
-      for (int bnd = 0; bnd < nbands; ++bnd) out[it->second + bnd * ld] = hv[i + size_t(bnd) * n];
-    }
+    for (int64_t i = 0; i < n; ++i)
+      for (int bnd = 0; bnd < nbands; ++bnd) out[slot_of[i] + bnd * ld] = hv[i + size_t(bnd) * n];
     std::copy(res.eigenvalues.begin(), res.eigenvalues.end(), eigenvalues);
     *overlap_min = res.overlap_min;
     *overlap_max = res.overlap_max;

@@ -406,14 +406,17 @@ class Context:
         return Operator(self, k, precision)
 
     # -- request geometry --------------------------------------------------
-    def set_request_geometry(self, lattice, species, positions, grid):
+    def set_request_geometry(self, lattice, species, positions, grid, ntypes=None):
+        """The host's cell (rows = lattice vectors, A), one-based type of
+        every ion, fractional positions and FFT grid; the context then builds
+        everything on it (half_set_request_geometry)."""
         sp = np.ascontiguousarray(species, np.int32)
         pos = np.ascontiguousarray(positions, float).ravel()
         g = RequestGeometry()
         g.struct_size = C.sizeof(RequestGeometry)
         g.flags = 0
         g.nions = sp.size
-        g.ntypes = int(sp.max()) if sp.size else 0
+        g.ntypes = int(ntypes) if ntypes is not None else (int(sp.max()) if sp.size else 0)
         g.grid[:] = list(grid)
         g.lattice[:] = list(np.asarray(lattice, float).ravel())
         g.species = sp.ctypes.data_as(C.POINTER(C.c_int32))

@@ -201,16 +201,17 @@ def _fmt_block(values, per_line=5, fmt="%.12E"):
     return "\n".join(out)
 
 
-def write_chgcar(path: str, w: Workload) -> None:
+def write_chgcar(path: str, w: Workload, header_fmt: str = "%.12f") -> None:
+    """header_fmt formats the lattice and position numbers of the header."""
     rho = smooth_density(w)
     lines = [f"synthetic {w.name}", "1.0"]
     for v in w.lattice:
-        lines.append("  " + "  ".join("%.12f" % x for x in v))
+        lines.append("  " + "  ".join(header_fmt % x for x in v))
     lines.append("  " + "  ".join(w.species))
     lines.append("  " + "  ".join(str(c) for c in w.counts))
     lines.append("Direct")
     for p in w.positions:
-        lines.append("  " + "  ".join("%.12f" % x for x in p))
+        lines.append("  " + "  ".join(header_fmt % x for x in p))
     lines.append("")
     lines.append(" %d %d %d" % w.grid)
     # x-fastest order, as CHGCAR stores the grid
@@ -404,3 +405,83 @@ def make_inputs(name: str, outdir: str, partial_waves: bool = False, partial_cor
         write_potcar(tmp, w, partial_waves, partial_core)
         os.replace(tmp, pot)
     return chg, pot, w
+
+
+# --------------------------------------------------------------------------
+# lattice-precision boundary case (request geometry)
+# --------------------------------------------------------------------------
+
+def _kinetic(lattice, grid, kpoint):
+    """G vectors of the box (n_i in [-n/2, (n-1)/2], n1 outermost) and
+    hsqdtm |(G + k) Brec|^2 on `lattice` (reference: src/half_basis.F90:9-55)."""
+    r = [np.arange(-n // 2, (n - 1) // 2 + 1) for n in grid]
+    a, b, c = np.meshgrid(*r, indexing="ij")
+    g = np.stack([a.ravel(), b.ravel(), c.ravel()], 1)
+    q = (g + np.asarray(kpoint, float)) @ reciprocal(lattice)
+    return g, HSQDTM * np.einsum("ij,ij->i", q, q)
+
+
+def boundary_lattice(lattice, grid, encut, kpoint, digits=6, delta=1e-9):
+    """A uniformly scaled copy s * lattice, kept at full double precision, on
+    which one plane wave sits just above the cutoff (hsqdtm |k+G|^2 =
+    encut (1 + delta), so the strict test excludes it) while the same lattice
+    rounded to `digits` decimals, the precision of a CHGCAR header, puts it
+    below: the rounded cell's basis holds extra G vectors at the unchanged
+    ENCUT.  Every other vector stays at least delta/2 (full) and delta/10
+    (rounded) away from the cutoff, relative, so both counts are exact in any
+    evaluation order.  Returns (full lattice, rounded lattice, count on the
+    full lattice, count on the rounded lattice)."""
+    g, e1 = _kinetic(np.asarray(lattice, float), grid, kpoint)
+    order = np.argsort(np.abs(e1 - encut))
+    for c in order[:256]:
+        s = math.sqrt(e1[c] / (encut * (1.0 + delta)))
+        full = s * np.asarray(lattice, float)
+        rounded = np.round(full, digits)
+        _, ef = _kinetic(full, grid, kpoint)
+        _, er = _kinetic(rounded, grid, kpoint)
+        rel_f = np.abs(ef / encut - 1.0)
+        rel_f[c] = np.inf
+        if rel_f.min() < 0.5 * delta or np.abs(er / encut - 1.0).min() < 0.1 * delta:
+            continue
+        nf, nr = int((ef < encut).sum()), int((er < encut).sum())
+        if nr > nf:
+            return full, rounded, nf, nr
+    raise ValueError("no boundary scaling found")
+
+
+def write_boundary_inputs(outdir: str, name: str = "hfo2", encut: float = 520.0, kpoint=(0.0, 0.5, 0.5),
+                          digits: int = 6):
+    """A cell whose CHGCAR header precision changes its plane-wave basis.
+
+    The workload's lattice is scaled by boundary_lattice and its fractional
+    positions displaced by up to 1e-7 (seeded), both kept at full precision.
+    Two CHGCAR files hold the same density values: <name>.round.CHGCAR with
+    the lattice and positions written to `digits` decimals (as a DeePAW
+    density file stores them) and <name>.full.CHGCAR with them written to 17
+    significant digits (they read back as the same doubles).  The POTCAR is
+    the workload's.  Returns a dict with both files, the full-precision
+    lattice and positions, the one-based type of every ion, the grid, ENCUT,
+    k point and the basis sizes on the full and rounded lattices."""
+    w = workload(name)
+    full, rounded, nf, nr = boundary_lattice(w.lattice, w.grid, encut, kpoint, digits)
+    rng = np.random.default_rng(w.seed + 17)
+    pos = w.positions + rng.uniform(-1e-7, 1e-7, w.positions.shape)
+    wf = Workload(w.name, full, w.species, w.counts, pos, w.grid, encut, np.array([kpoint], float), w.nbands,
+                  w.seed, w.xc, dict(w.extra))
+    os.makedirs(outdir, exist_ok=True)
+    paths = {}
+    for tag, fmt in (("round", "%%.%df" % digits), ("full", "%.17g")):
+        p = os.path.join(outdir, f"{name}.{tag}.CHGCAR")
+        tmp = p + f".tmp{os.getpid()}"
+        write_chgcar(tmp, wf, header_fmt=fmt)
+        os.replace(tmp, p)
+        paths[tag] = p
+    pot = os.path.join(outdir, f"{name}.boundary.POTCAR")
+    if not os.path.exists(pot):
+        tmp = pot + f".tmp{os.getpid()}"
+        write_potcar(tmp, w)
+        os.replace(tmp, pot)
+    types = np.concatenate([np.full(c, i + 1, np.int32) for i, c in enumerate(w.counts)])
+    return {"chgcar_round": paths["round"], "chgcar_full": paths["full"], "potcar": pot, "lattice": full,
+            "lattice_round": rounded, "positions": pos, "species": types, "grid": w.grid, "encut": encut,
+            "kpoint": tuple(kpoint), "npw_full": nf, "npw_round": nr}

@@ -141,7 +141,14 @@ int half_create_from_escn(const char *base_url, const char *potential_path,
 /* replaces src/half_c_api.F90:279-287 */
 int half_destroy(half_handle handle, char *error, int error_capacity);
 
-/* replaces src/half_c_api.F90:289-335 */
+/* replaces src/half_c_api.F90:289-335.  Same validation and messages; in
+ * addition the geometry becomes the context's cell (the fix recorded in the
+ * reference's docs/validation/HFO2_NWAVE_BOUNDARY_MISMATCH.md:43-50): the
+ * request must have the density file's grid, one type per potential dataset
+ * and the file's ion count per type (else HALF_ERROR_INVALID_ARGUMENT, the
+ * context unchanged); its lattice and positions then replace the file's for
+ * the basis, projectors, Veff and D of every later call, and the context's
+ * device operator handles become invalid. */
 int half_set_request_geometry(half_handle handle, const half_request_geometry_v1 *geometry,
                               char *error, int error_capacity);
 /* replaces src/half_c_api.F90:337-386 */
@@ -149,7 +156,8 @@ int half_get_request_geometry(half_handle handle, int32_t *nions, int32_t *ntype
                               double lattice[9], int64_t species_capacity, int32_t *species,
                               int64_t positions_capacity, double *positions_fractional,
                               char *error, int error_capacity);
-/* replaces src/half_c_api.F90:388-401 */
+/* replaces src/half_c_api.F90:388-401; reports the request geometry's
+ * lattice and volume once one is set. */
 int half_get_system_info(half_handle handle, int *nions, int *ntypes, int grid[3],
                          double lattice[9], double *volume_A3, char *error, int error_capacity);
 
@@ -173,7 +181,9 @@ int half_apply_hs(half_handle handle, const double kpoint[3], int nstates, int64
 int half_solve_kpoint(half_handle handle, const double kpoint[3], int nbands,
                       double *eigenvalues_eV, HALF_COMPLEX *eigenvectors, int64_t ld,
                       double *overlap_min, double *overlap_max, char *error, int error_capacity);
-/* replaces src/half_c_api.F90:521-588 */
+/* replaces src/half_c_api.F90:521-588.  The bases are compared before the
+ * solve; a mismatch message keeps the reference's text and adds both sizes
+ * and the first G vector found in one basis only. */
 int half_solve_kpoint_mapped(half_handle handle, const double kpoint[3], int nbands,
                              int64_t host_npw, const int32_t *host_gvec, double *eigenvalues_eV,
                              HALF_COMPLEX *eigenvectors, int64_t ld, double *overlap_min,
