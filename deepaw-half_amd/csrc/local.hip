@@ -53,8 +53,21 @@ struct Tuning {
   int xconv[2] = {1, 1};  // x pass as a per-column convolution when it fits
   int fused[2] = {1, 1};  // y, x, y passes in one kernel where the convolution applies
   int ygroup[2] = {2, 2}; // bands per workgroup of the fused kernel (fewer where LDS is short)
+  int vxlds[2] = {1, 1};  // fused kernel with the slice's Vx rows in LDS where it applies
 };
 Tuning g_tune;
+
+// acc += c v as four fused multiply-adds (the contraction of
+// acc + (c.x v.x - c.y v.y) would leave a multiply and an add per part)
+__device__ inline float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <typename V>
+__device__ inline void cmac(V &acc, const V &c, const V &v) {
+  acc.x = fma_r(c.x, v.x, acc.x);
+  acc.x = fma_r(-c.y, v.y, acc.x);
+  acc.y = fma_r(c.x, v.y, acc.y);
+  acc.y = fma_r(c.y, v.x, acc.y);
+}
 
 // Elements per thread of a tile (M * n <= tile budget <= 256 * KMAX).
 template <typename R>
@@ -283,18 +296,13 @@ __global__ __launch_bounds__(256) void x_conv_kernel(cplx_t<R> *box, const cplx_
   }
   for (int op = 0; op < P; ++op) {
     const V *w = vs + (op + pw - 1) * kXcCols + ci;  // w[-q kXcCols] = Vx[(op - q) mod n1]
-    R ar = 0, ai = 0;
+    V acc = mk<V>(R(0), R(0));
 #pragma unroll
-    for (int q = 0; q < PMAX; ++q) {
-      if (q < P) {
-        const V v = w[-q * kXcCols];
-        ar += v.x * in[q].x - v.y * in[q].y;
-        ai += v.x * in[q].y + v.y * in[q].x;
-      }
-    }
+    for (int q = 0; q < PMAX; ++q)
+      if (q < P) cmac(acc, w[-q * kXcCols], in[q]);
     int i1 = s0 + op;
     i1 -= i1 >= t.n1 ? t.n1 : 0;
-    col[static_cast<long long>(i1) * C] = mk<V>(ar, ai);
+    col[static_cast<long long>(i1) * C] = acc;
   }
 }
 
@@ -375,14 +383,11 @@ __global__ __launch_bounds__(256) void yxy_kernel(cplx_t<R> *lines, const cplx_t
 #pragma unroll
     for (int q = 0; q < PMAX; ++q) {
       if (q < P) {
-        const V v = col[q * ld];
+        const V v = *col;
 #pragma unroll
-        for (int o = 0; o < kYxyOut; ++o) {
-          const V c = w[o + PMAX - 1 - q];
-          acc[o].x += c.x * v.x - c.y * v.y;
-          acc[o].y += c.x * v.y + c.y * v.x;
-        }
+        for (int o = 0; o < kYxyOut; ++o) cmac(acc[o], w[o + PMAX - 1 - q], v);
       }
+      col += ld;
     }
 #pragma unroll
     for (int o = 0; o < kYxyOut; ++o)
@@ -390,6 +395,131 @@ __global__ __launch_bounds__(256) void yxy_kernel(cplx_t<R> *lines, const cplx_t
   }
   __syncthreads();
   V *r2 = stages<NN>(o1, r1, tws, ay, -1);
+  for (int g = 0; g < G && band0 + g < nb; ++g) {
+    V *dst = lines + (band0 + g) * lstride + static_cast<long long>(i3) * t.nlines;
+    for (int l = threadIdx.x; l < t.nlines; l += blockDim.x) dst[l] = r2[g * P * ld + slot(l)];
+  }
+}
+
+// yxy_kernel with the slice's Vx rows in LDS, for eight waves per
+// workgroup.  The convolution needs Vx[d][i2] for d = -(P-1) .. P-1 only,
+// 2P - 1 rows: they are copied to the second plane buffer once the forward
+// y transform has freed it (one bulk load, every thread's share in flight
+// together).  The convolution keeps a sliding window of kYxyOut values (one
+// LDS load per input, instead of PMAX + kYxyOut - 1 values held per item),
+// so the kernel fits 128 registers and two workgroups of eight waves share
+// a CU (the plane buffers of two bands, 60 KB in FP64 at 48^3, allow two).
+// It writes in place: all parts of one column are lanes of one wave in one
+// round, part-major so a wave's reads stay on consecutive i2, and a wave's
+// LDS reads of a round precede its writes.  Needs NN > 0 and G >= 2
+// ((2P - 1) n2 values fit in the G P ld of a plane buffer).
+constexpr int kYxyLdsThreads = 512;
+
+template <typename R, int PMAX, int NN>
+__global__ __attribute__((amdgpu_flat_work_group_size(kYxyLdsThreads, kYxyLdsThreads), amdgpu_waves_per_eu(4)))
+void yxy_lds_kernel(cplx_t<R> *lines, const cplx_t<R> *vxT, const LocalTables *tabs, KStride ks, int nb, int G,
+                    FftArgs a) {
+  using V = cplx_t<R>;
+  static_assert(NN > 0, "compile-time y length");
+  constexpr int kVxPer = ((2 * PMAX - 1) * NN + kYxyLdsThreads - 1) / kYxyLdsThreads;
+  constexpr int kWaves = kYxyLdsThreads / 64;
+  const int kp = blockIdx.y;
+  const LocalTables t = tabs[kp];
+  const int P = t.nplanes;
+  const int i3 = blockIdx.x % t.n3;
+  const int band0 = (blockIdx.x / t.n3) * G;
+  if (band0 >= nb || P == 0) return;  // uniform per workgroup: no barrier reached
+  lines += kp * ks.box;
+  FftArgs ay = a;
+  ay.M = G * P;
+  constexpr int ld = NN + 1;  // the y axis' LDS line stride (checked at launch)
+  extern __shared__ double2 smem_raw[];
+  V *A = reinterpret_cast<V *>(smem_raw);
+  V *B = A + ay.M * ld;
+  V *tws = B + ay.M * ld;
+  load_twiddles(tws, ay);
+  for (int e = threadIdx.x; e < ay.M * ld; e += blockDim.x) A[e] = mk<V>(R(0), R(0));
+  __syncthreads();
+  const long long lstride = static_cast<long long>(t.nlines) * t.n3;
+  const int half1 = (t.n1 - 1) / 2;
+  auto slot = [&](int l) {
+    const int i12 = t.line_i12[l];
+    const int i1 = i12 / NN, i2 = i12 - i1 * NN;
+    return ((i1 <= half1 ? i1 : i1 - t.n1) - t.plane_s0) * ld + i2;
+  };
+  for (int g = 0; g < G && band0 + g < nb; ++g) {
+    const V *src = lines + (band0 + g) * lstride + static_cast<long long>(i3) * t.nlines;
+    for (int l = threadIdx.x; l < t.nlines; l += blockDim.x) A[g * P * ld + slot(l)] = src[l];
+  }
+  __syncthreads();
+  V *r1 = stages<NN>(A, B, tws, ay, +1);  // ends with a barrier
+  V *o1 = r1 == A ? B : A;
+  // row r of the staged block is Vx[(r - (P - 1)) mod n1][i2] of this i3
+  const V *vcol = vxT + static_cast<long long>(i3) * t.n1 * NN;
+#pragma unroll
+  for (int k = 0; k < kVxPer; ++k) {
+    const int e = threadIdx.x + k * kYxyLdsThreads;
+    if (e < (2 * P - 1) * NN) {
+      const int r = e / NN, i2 = e - r * NN;
+      const int d = r - (P - 1);
+      o1[e] = vcol[(d < 0 ? d + t.n1 : d) * NN + i2];
+    }
+  }
+  __syncthreads();
+  const int nparts = (P + kYxyOut - 1) / kYxyOut;
+  const int cpw = 64 / nparts;  // columns per wave and round
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ncol = G * NN;
+  if (lane < cpw * nparts) {
+    const int part = lane / cpw, cl = lane - part * cpw;
+    const int qa = part * kYxyOut;
+    // the round loop depends on the wave only
+    for (int c0 = wave * cpw; c0 < ncol; c0 += kWaves * cpw) {
+      const int col = c0 + cl;
+      if (col >= ncol) continue;
+      const int g = col / NN, i2 = col - g * NN;
+      // Output qa + o takes input q with row qa + o - q + P - 1, in
+      // [0, 2P - 2] for every kept pair (qa + o < P, q < P).  wv[o] holds
+      // that row for the current q; the next q needs the previous entry
+      // (wv[o - 1]) and one new row, qa + P - 2 - q.  Row offsets are
+      // clamped into [0, 2P - 2] (a median of three); the clamp only
+      // touches values no kept output uses.
+      // The loads and the shift run for every q < PMAX (inputs past P
+      // read row 0 of the column and are not used), so the window stays in
+      // the same registers with no branch joins.
+      const int lo = i2, hi = (2 * P - 2) * NN + i2;
+      const int top = (qa + P - 2) * NN + i2;  // offset of row qa + P - 2
+      auto clamp_off = [&](int off) { return off < lo ? lo : (off > hi ? hi : off); };
+      V wv[kYxyOut];
+#pragma unroll
+      for (int o = 0; o < kYxyOut; ++o) wv[o] = o1[clamp_off(top + (o + 1) * NN)];
+      V acc[kYxyOut];
+#pragma unroll
+      for (int o = 0; o < kYxyOut; ++o) acc[o] = mk<V>(R(0), R(0));
+      V *col_p = r1 + g * P * ld + i2;
+      const V *pin = col_p;
+#pragma unroll
+      for (int q = 0; q < PMAX; ++q) {
+        const V v = *(q < P ? pin : col_p);
+        if (q < P) {
+#pragma unroll
+          for (int o = 0; o < kYxyOut; ++o) cmac(acc[o], wv[o], v);
+        }
+#pragma unroll
+        for (int o = kYxyOut - 1; o > 0; --o) wv[o] = wv[o - 1];
+        wv[0] = o1[clamp_off(top - q * NN)];
+        pin += ld;
+      }
+      V *pout = col_p + qa * ld;
+#pragma unroll
+      for (int o = 0; o < kYxyOut; ++o) {
+        if (qa + o < P) *pout = acc[o];
+        pout += ld;
+      }
+    }
+  }
+  __syncthreads();
+  V *r2 = stages<NN>(r1, o1, tws, ay, -1);
   for (int g = 0; g < G && band0 + g < nb; ++g) {
     V *dst = lines + (band0 + g) * lstride + static_cast<long long>(i3) * t.nlines;
     for (int l = threadIdx.x; l < t.nlines; l += blockDim.x) dst[l] = r2[g * P * ld + slot(l)];
@@ -588,26 +718,39 @@ const void *y_for(int n) {
   return reinterpret_cast<const void *>(&y_kernel<R, BWD, 0>);
 }
 
+// A fused kernel and its workgroup size.
+struct YxyKernel {
+  const void *fn = nullptr;
+  int threads = 256;
+};
+
+// Vx rows staged in LDS (yxy_lds_kernel): compile-time y length, a band
+// group of two or more, and at most 16 staged values per thread
 template <typename R, int PMAX>
-const void *yxy_len(int n2) {
-#define HALF_CASE(N) \
-  if (n2 == N) return reinterpret_cast<const void *>(&yxy_kernel<R, PMAX, N>);
+YxyKernel yxy_len(int n2, bool vx_lds) {
+#define HALF_CASE(N)                                                                             \
+  if (n2 == N) {                                                                                 \
+    if (vx_lds && (2 * PMAX - 1) * N <= 16 * kYxyLdsThreads)                                     \
+      return {reinterpret_cast<const void *>(&yxy_lds_kernel<R, PMAX, N>), kYxyLdsThreads};      \
+    return {reinterpret_cast<const void *>(&yxy_kernel<R, PMAX, N>), 256};                       \
+  }
   HALF_CT_LENGTHS(HALF_CASE)
 #undef HALF_CASE
-  return reinterpret_cast<const void *>(&yxy_kernel<R, PMAX, 0>);
+  return {reinterpret_cast<const void *>(&yxy_kernel<R, PMAX, 0>), 256};
 }
 
 // register budget of yxy_kernel per precision (window of PMAX + 3 values)
 template <typename R>
-const void *yxy_for(int planes, int n1, int n2) {
+YxyKernel yxy_for(int planes, int n1, int n2, int G, int ld) {
   int pmax = planes <= 16 ? 16 : planes <= 24 ? 24 : planes <= 32 ? 32 : 48;
-  if (std::is_same<R, double>::value && pmax > 32) return nullptr;
-  if (planes > 48 || pmax > n1 + 1) return nullptr;  // window arithmetic assumes PMAX <= n1 + 1
+  if (std::is_same<R, double>::value && pmax > 32) return {};
+  if (planes > 48 || pmax > n1 + 1) return {};  // window arithmetic assumes PMAX <= n1 + 1
+  const bool vx_lds = g_tune.vxlds[std::is_same<R, double>::value ? 0 : 1] && G >= 2 && ld == n2 + 1;
   switch (pmax) {
-    case 16: return yxy_len<R, 16>(n2);
-    case 24: return yxy_len<R, 24>(n2);
-    case 32: return yxy_len<R, 32>(n2);
-    default: return yxy_len<R, 48>(n2);
+    case 16: return yxy_len<R, 16>(n2, vx_lds);
+    case 24: return yxy_len<R, 24>(n2, vx_lds);
+    case 32: return yxy_len<R, 32>(n2, vx_lds);
+    default: return yxy_len<R, 48>(n2, vx_lds);
   }
 }
 
@@ -719,9 +862,9 @@ void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff,
     auto lds_of = [&](int g) { return (2 * size_t(g) * pwf * ay.ld + n2) * sizeof(V); };
     while (G > 1 && lds_of(G) > 65536) --G;
     const size_t lds = lds_of(G);
-    const void *yxy =
-        (vx && g_tune.fused[prec] && kb.contiguous && lds <= 65536) ? yxy_for<R>(pwf, n1, n2) : nullptr;
-    if (yxy) {
+    const YxyKernel yxy =
+        (vx && g_tune.fused[prec] && kb.contiguous && lds <= 65536) ? yxy_for<R>(pwf, n1, n2, G, ay.ld) : YxyKernel{};
+    if (yxy.fn) {
       const cplx_t<R> *vxT = vx + static_cast<long long>(n1) * C;
       {
         void *args[] = {&psi, &lp, &boxes, &tabs, &ks, &nbi, &az};
@@ -730,8 +873,8 @@ void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff,
       {
         void *args[] = {&boxes, &vxT, &tabs, &ks, &nbi, &G, &ay};
         trace_mark("local_yxy", s);
-        HALF_HIP(hipLaunchKernel(yxy, dim3(static_cast<unsigned>(n3 * ((nb + G - 1) / G)), nk), dim3(256), args,
-                                 lds, s));
+        HALF_HIP(hipLaunchKernel(yxy.fn, dim3(static_cast<unsigned>(n3 * ((nb + G - 1) / G)), nk),
+                                 dim3(static_cast<unsigned>(yxy.threads)), args, lds, s));
       }
       {
         const cplx_t<R> *ah = add.h, *as = add.s;
@@ -832,6 +975,11 @@ bool set_local_tuning(const std::string &key, int value) {
       g_tune.ygroup[p] = value;
       return true;
     }
+    if (key == pre + ".vxlds") {
+      if (value < 0 || value > 1) return false;
+      g_tune.vxlds[p] = value;
+      return true;
+    }
   }
   return false;
 }
@@ -859,6 +1007,10 @@ bool get_local_tuning(const std::string &key, int &value) {
     }
     if (key == pre + ".ygroup") {
       value = g_tune.ygroup[p];
+      return true;
+    }
+    if (key == pre + ".vxlds") {
+      value = g_tune.vxlds[p];
       return true;
     }
   }

@@ -82,8 +82,10 @@ void build_veff_x_dft(const double *veff, int n1, int n2, int n3, double2 *vx, f
 // (1: x pass as a per-column convolution where the active planes fit, 0:
 // transform-based), "local.{fp64|fp32}.fused" (1: y, x and y passes in one
 // kernel per (band, i3) slice where the convolution applies, 0: separate
-// passes) and "local.{fp64|fp32}.ygroup" (bands per workgroup of the fused
-// kernel).  Return false for an unknown key or an out-of-range value.
+// passes), "local.{fp64|fp32}.ygroup" (bands per workgroup of the fused
+// kernel) and "local.{fp64|fp32}.vxlds" (1: the fused kernel stages the
+// slice's Vx rows in LDS where its band group is two or more, 0: it reads
+// them from L2).  Return false for an unknown key or an out-of-range value.
 bool set_local_tuning(const std::string &key, int value);
 bool get_local_tuning(const std::string &key, int &value);
 

@@ -6,6 +6,7 @@ block while sweeping, per precision:
   local.<p>.xgroup         bands sharing one Veff tile in the x pass
   local.<p>.fused          y, x, y passes fused per (band, i3) slice or not
   local.<p>.ygroup         bands per workgroup of the fused kernel
+  local.<p>.vxlds          fused kernel with the Vx rows staged in LDS or read from L2
   operator.overlap         projector stage on a second stream or in order
 The tiles are swept with the separate passes (fused off, so every pass runs):
 each candidate is applied to all three passes at once and every pass keeps
@@ -107,22 +108,27 @@ def tune(workload="hfo2", bands=0, steps=10, verbose=True):
                 gbest = (g, ms)
         set_tuning(f"local.{p}.xgroup", gbest[0])
         result["best"][f"local.{p}.xgroup"] = gbest[0]
-        forms = [(0, 1, ov) for ov in (0, 1)] + [(1, g, ov) for g in (1, 2) for ov in (0, 1)]
+        # the Vx rows go to LDS only with band groups of two or more
+        forms = ([(0, 1, 0, ov) for ov in (0, 1)] + [(1, 1, 0, ov) for ov in (0, 1)]
+                 + [(1, g, v, ov) for g in (2, 4) for v in (0, 1) for ov in (0, 1)])
         fbest = (None, float("inf"))
-        for fused, g, ov in forms:
+        for fused, g, v, ov in forms:
             set_tuning(f"local.{p}.fused", fused)
             set_tuning(f"local.{p}.ygroup", g)
+            set_tuning(f"local.{p}.vxlds", v)
             set_tuning("operator.overlap", ov)
             ms = step_ms()
-            result["timings"][f"{p}.fused={fused},ygroup={g},overlap={ov}"] = ms
+            result["timings"][f"{p}.fused={fused},ygroup={g},vxlds={v},overlap={ov}"] = ms
             if ms < fbest[1]:
-                fbest = ((fused, g, ov), ms)
-        (fused, g, ov) = fbest[0]
+                fbest = ((fused, g, v, ov), ms)
+        (fused, g, v, ov) = fbest[0]
         set_tuning(f"local.{p}.fused", fused)
         set_tuning(f"local.{p}.ygroup", g)
+        set_tuning(f"local.{p}.vxlds", v)
         set_tuning("operator.overlap", ov)
         result["best"][f"local.{p}.fused"] = fused
         result["best"][f"local.{p}.ygroup"] = g
+        result["best"][f"local.{p}.vxlds"] = v
         result["best"][f"operator.overlap.{p}"] = ov
         result["timings"][f"{p}.tuned_step_ms"] = fbest[1]
         if verbose:

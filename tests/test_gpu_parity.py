@@ -226,26 +226,31 @@ def test_abi_errors_on_live_context(si2):
 def test_local_x_pass_forms(si2):
     """The local stage's forms agree and each reproduces the reference's
     H.psi: y, x, y passes fused in one kernel per (band, i3) slice (default;
-    band groups of 1, 2 and 4 over 5 bands, so the last group is partial),
-    separate passes with the x pass as a per-column convolution, and
-    transform / multiply / transform."""
+    band groups of 1, 2 and 4 over 5 bands, so the last group is partial;
+    groups of two or more with the Vx rows staged in LDS, and 2 also with
+    them read from L2), separate passes with the x pass as a per-column
+    convolution, and transform / multiply / transform."""
     from half_amd import hip
     name, g, ctx = si2
     n = g["npw"]
     nb = 5
     psi = psi_block(n, nb)
     ref = np.load(f"tests/golden/{name}.apply.npy")
-    forms = {"fused1": (1, 1, 1), "fused2": (1, 1, 2), "fused4": (1, 1, 4), "conv": (0, 1, 1), "fft": (0, 0, 1)}
+    forms = {"fused1": (1, 1, 1, 1), "fused2": (1, 1, 2, 1), "fused4": (1, 1, 4, 1), "fused2_l2": (1, 1, 2, 0),
+             "conv": (0, 1, 1, 1), "fft": (0, 0, 1, 1)}
+    saved = {f"local.fp{p}.{k}": H.get_tuning(f"local.fp{p}.{k}") for p in (64, 32)
+             for k in ("fused", "xconv", "ygroup", "vxlds")}
     stream = hip.Stream()
     try:
         for prec, dt in ((64, np.complex128), (32, np.complex64)):
             op = ctx.operator(g["kpoint"], prec)
             x = hip.Buffer.from_numpy(np.ascontiguousarray(psi.T).astype(dt))
             out = {}
-            for form, (fused, xconv, group) in forms.items():
+            for form, (fused, xconv, group, vxlds) in forms.items():
                 H.set_tuning(f"local.fp{prec}.fused", fused)
                 H.set_tuning(f"local.fp{prec}.xconv", xconv)
                 H.set_tuning(f"local.fp{prec}.ygroup", group)
+                H.set_tuning(f"local.fp{prec}.vxlds", vxlds)
                 hp, sp = hip.Buffer(x.shape, dt), hip.Buffer(x.shape, dt)
                 op.apply_device(nb, x.ptr, hp.ptr, sp.ptr, stream=stream.ptr)
                 stream.synchronize()
@@ -258,10 +263,8 @@ def test_local_x_pass_forms(si2):
                 assert np.abs(out[form][0][:, :3] - ref[0]).max() / scale < (1e-11 if prec == 64 else 2e-6), (prec, form)
                 assert np.abs(out[form][1][:, :3] - ref[1]).max() / np.abs(ref[1]).max() < (1e-14 if prec == 64 else 1e-6)
     finally:
-        for prec in (64, 32):
-            H.set_tuning(f"local.fp{prec}.fused", 1)
-            H.set_tuning(f"local.fp{prec}.xconv", 1)
-            H.set_tuning(f"local.fp{prec}.ygroup", 1)
+        for key, value in saved.items():
+            H.set_tuning(key, value)
 
 
 def test_operator_overlap_matches_in_order(si2):
@@ -274,6 +277,8 @@ def test_operator_overlap_matches_in_order(si2):
     n = g["npw"]
     nb = 7
     psi = psi_block(n, nb)
+    saved = {f"local.fp{p}.{k}": H.get_tuning(f"local.fp{p}.{k}") for p in (64, 32)
+             for k in ("fused", "xconv", "ygroup", "vxlds")}
     stream = hip.Stream()
     try:
         for prec, dt in ((64, np.complex128), (32, np.complex64)):
