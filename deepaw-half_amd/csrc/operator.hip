@@ -22,6 +22,7 @@
 #include "cplx.hpp"
 #include "mfma.hpp"
 #include "paw.hpp"
+#include "projector.hpp"
 #include "qdep.hpp"
 #include "trace.hpp"
 
@@ -176,9 +177,12 @@ __global__ void mix_kernel(const V *coeff, V *dc, V *qc, const int *p_atom, cons
 // ---------------------------------------------------------------------------
 
 // E[G,p] = sum_q B[G,q] D(q,p), F[G,p] = sum_q B[G,q] Q(q,p) within an atom
-__global__ void row_mix_kernel(const double2 *B, int64_t ldb, double2 *E, double2 *F, const int *p_atom,
-                               const int *p_lm, const int *atom_first, const int *atom_nlm,
-                               const int *atom_boff, const double *dblk, const double *qblk, int n, int np) {
+// (leading dimension ldb for all three), summed in FP64
+template <typename V>
+__global__ void row_mix_kernel(const V *B, int64_t ldb, V *E, V *F, const int *p_atom, const int *p_lm,
+                               const int *atom_first, const int *atom_nlm, const int *atom_boff, const double *dblk,
+                               const double *qblk, int n, int np) {
+  using S = typename scalar_of<V>::type;
   const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (t >= int64_t(n) * np) return;
   const int p = static_cast<int>(t / n);
@@ -187,17 +191,17 @@ __global__ void row_mix_kernel(const double2 *B, int64_t ldb, double2 *E, double
   const int first = atom_first[at], nlm = atom_nlm[at];
   const double *dcol = dblk + atom_boff[at];
   const double *qcol = qblk + atom_boff[at];
-  double2 e = make_double2(0, 0), f = make_double2(0, 0);
+  double ex = 0, ey = 0, fx = 0, fy = 0;
   for (int a = 0; a < nlm; ++a) {
-    const double2 v = B[g + int64_t(first + a) * ldb];
+    const V v = B[g + int64_t(first + a) * ldb];
     const double d = dcol[a * nlm + b], q = qcol[a * nlm + b];
-    e.x += v.x * d;
-    e.y += v.y * d;
-    f.x += v.x * q;
-    f.y += v.y * q;
+    ex += double(v.x) * d;
+    ey += double(v.y) * d;
+    fx += double(v.x) * q;
+    fy += double(v.y) * q;
   }
-  E[g + int64_t(p) * n] = e;
-  F[g + int64_t(p) * n] = f;
+  E[g + int64_t(p) * ldb] = mk<V>(static_cast<S>(ex), static_cast<S>(ey));
+  F[g + int64_t(p) * ldb] = mk<V>(static_cast<S>(fx), static_cast<S>(fy));
 }
 
 // H[i,j] = Vg((Gi-Gj) mod n) + delta_ij T_i + sum_p E[i,p] conj(B[j,p]);
@@ -220,7 +224,7 @@ __global__ __launch_bounds__(256) void dense_kernel(const double2 *E, const doub
     double er = 0, ei = 0, fr = 0, fi = 0, br = 0, bi = 0;
     if (p < np) {
       if (iv) {
-        const double2 e = E[i + int64_t(p) * n], f = F[i + int64_t(p) * n];
+        const double2 e = E[i + int64_t(p) * ldb], f = F[i + int64_t(p) * ldb];
         er = e.x;
         ei = e.y;
         fr = f.x;
@@ -417,19 +421,31 @@ void KOperator<R>::build(const System &s, const double kpoint[3], hipStream_t st
 
 namespace {
 bool g_overlap = true;
+// projector stage on the skinny kernels of projector.hip (B D and B Q formed
+// at build); off: the generic contractions with a separate D / Q mix
+bool g_projector = true;
 }
 
 bool operator_overlap() { return g_overlap; }
 
 bool set_operator_tuning(const std::string &key, int value) {
-  if (key != "operator.overlap" || value < 0 || value > 1) return false;
-  g_overlap = value != 0;
+  if (value < 0 || value > 1) return false;
+  if (key == "operator.overlap")
+    g_overlap = value != 0;
+  else if (key == "operator.projector")
+    g_projector = value != 0;
+  else
+    return false;
   return true;
 }
 
 bool get_operator_tuning(const std::string &key, int &value) {
-  if (key != "operator.overlap") return false;
-  value = g_overlap ? 1 : 0;
+  if (key == "operator.overlap")
+    value = g_overlap ? 1 : 0;
+  else if (key == "operator.projector")
+    value = g_projector ? 1 : 0;
+  else
+    return false;
   return true;
 }
 
@@ -473,10 +489,32 @@ void KOperator<R>::apply_local_added(int nb, const cplx_t<R> *psi, int64_t ldpsi
 }
 
 template <typename R>
+void KOperator<R>::mixed_blocks(hipStream_t st) {
+  if (np <= 0 || n <= 0 || BD.size()) return;
+  BD.resize(size_t(ldb) * np);
+  BQ.resize(size_t(ldb) * np);
+  const System &s = *sys;
+  hipLaunchKernelGGL((row_mix_kernel<cplx_t<R>>), dim3(blocks_for(int64_t(n) * np)), dim3(kThreads), 0, st, B.get(),
+                     ldb, BD.get(), BQ.get(), s.d_proj_atom.get(), s.d_proj_lm.get(), s.d_atom_first.get(),
+                     s.d_atom_nlm.get(), s.d_atom_boff.get(), s.d_dblk.get(), s.d_qblk.get(), n, np);
+  HALF_HIP(hipGetLastError());
+}
+
+template <typename R>
+bool KOperator<R>::skinny(int nb) const {
+  // the 16-wide tiles re-read B once per 16 bands and psi once per 16
+  // projectors; past these sizes the 64-wide contractions move less data
+  return g_projector && np <= kSkinnyMaxProjectors && nb <= kSkinnyMaxBands;
+}
+
+template <typename R>
 void KOperator<R>::project(int nb, const cplx_t<R> *psi, int64_t ldpsi, hipStream_t st) {
   coeff.ensure(int64_t(np) * nb);
   trace_mark("proj_coeff_mfma", st);
-  gram<R>(n, np, nb, B.get(), ldb, psi, ldpsi, coeff.get(), np, nullptr, 0, nullptr, 0, partial, st);
+  if (skinny(nb))
+    projector_coefficients<R>(n, np, nb, B.get(), ldb, psi, ldpsi, coeff.get(), np, cwork, st);
+  else
+    gram<R>(n, np, nb, B.get(), ldb, psi, ldpsi, coeff.get(), np, nullptr, 0, nullptr, 0, partial, st);
 }
 
 template <typename R>
@@ -485,6 +523,12 @@ void KOperator<R>::apply_projectors(int nb, const cplx_t<R> *psi, int64_t ldpsi,
   using V = cplx_t<R>;
   if (np == 0) return;
   project(nb, psi, ldpsi, st);
+  if (skinny(nb)) {
+    mixed_blocks(st);
+    trace_mark("proj_expand_mfma", st);
+    projector_expand<R>(n, np, nb, BD.get(), BQ.get(), ldb, coeff.get(), np, true, hpsi, spsi, ldout, st);
+    return;
+  }
   const int64_t count = int64_t(np) * nb;
   dcoeff.ensure(count);
   qcoeff.ensure(count);
@@ -511,6 +555,12 @@ template <typename R>
 void KOperator<R>::projector_terms(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hout, cplx_t<R> *sout,
                                    int64_t ldout, hipStream_t st) {
   project(nb, psi, ldpsi, st);
+  if (skinny(nb)) {
+    mixed_blocks(st);
+    trace_mark("proj_expand_mfma", st);
+    projector_expand<R>(n, np, nb, BD.get(), BQ.get(), ldb, coeff.get(), np, false, hout, sout, ldout, st);
+    return;
+  }
   const int64_t count = int64_t(np) * nb;
   dcoeff.ensure(count);
   qcoeff.ensure(count);
@@ -574,25 +624,19 @@ template void mix_projector_coefficients<double>(const System &, int, const doub
 template void mix_projector_coefficients<float>(const System &, int, const float2 *, float2 *, float2 *,
                                                 hipStream_t);
 
-void assemble_dense(const KOperator<double> &op, double2 *h, double2 *s, int64_t ld, hipStream_t st) {
+void assemble_dense(KOperator<double> &op, double2 *h, double2 *s, int64_t ld, hipStream_t st) {
   const System &sys = *op.sys;
+  op.mixed_blocks(st);
   const int n = op.n, np = op.np;
-  DeviceArray<double2> E(size_t(n) * std::max(np, 1)), F(size_t(n) * std::max(np, 1));
   DeviceArray<int32_t> gvec;
   DeviceArray<double> kin;
   gvec.upload(op.basis.gvec, st);
   kin.upload(op.basis.kinetic, st);
-  if (np > 0) {
-    hipLaunchKernelGGL(row_mix_kernel, dim3(blocks_for(int64_t(n) * np)), dim3(kThreads), 0, st, op.B.get(),
-                       op.ldb, E.get(), F.get(), sys.d_proj_atom.get(), sys.d_proj_lm.get(),
-                       sys.d_atom_first.get(), sys.d_atom_nlm.get(), sys.d_atom_boff.get(), sys.d_dblk.get(),
-                       sys.d_qblk.get(), n, np);
-    HALF_HIP(hipGetLastError());
-  }
+  // B D and B Q were formed with the operator (leading dimension ldb = n)
   const int tiles = (n + 15) / 16;
   const int64_t ntiles = int64_t(tiles) * tiles;
-  hipLaunchKernelGGL(dense_kernel, dim3(static_cast<unsigned>((ntiles + 3) / 4)), dim3(256), 0, st, E.get(),
-                     F.get(), op.B.get(), op.ldb, sys.veff_g.get(), gvec.get(), kin.get(), h, s, ld, n, np, tiles,
+  hipLaunchKernelGGL(dense_kernel, dim3(static_cast<unsigned>((ntiles + 3) / 4)), dim3(256), 0, st, op.BD.get(),
+                     op.BQ.get(), op.B.get(), op.ldb, sys.veff_g.get(), gvec.get(), kin.get(), h, s, ld, n, np, tiles,
                      sys.charge.shape[0], sys.charge.shape[1], sys.charge.shape[2]);
   HALF_HIP(hipGetLastError());
   hipLaunchKernelGGL(hermitize_kernel, dim3(tiles, tiles), dim3(16, 16), 0, st, h, s, ld, n);

@@ -15,6 +15,7 @@
 #include "common.hpp"
 #include "fft.hpp"
 #include "local.hpp"
+#include "projector.hpp"
 
 namespace half {
 
@@ -92,7 +93,10 @@ struct AuxStream {
 
 // Whether H.psi overlaps its projector stage (second stream) with the local
 // stage; off, everything runs in order on the caller's stream.  Key
-// "operator.overlap" (0 / 1) through half_amd_set_tuning.
+// "operator.overlap" (0 / 1) through half_amd_set_tuning.  Key
+// "operator.projector" (0 / 1): the projector stage on the skinny kernels of
+// projector.hip (default) or on the generic contractions of contract.hip
+// with a separate D / Q mix.
 bool operator_overlap();
 bool set_operator_tuning(const std::string &key, int value);
 bool get_operator_tuning(const std::string &key, int &value);
@@ -108,11 +112,13 @@ struct KOperator {
   DeviceArray<R> kinetic;
   DeviceArray<double> qvec;  // npw x 3 (for the force adjoint)
   DeviceArray<cplx_t<R>> B;
+  DeviceArray<cplx_t<R>> BD, BQ;  // B D and B Q (per-atom blocks), leading dimension ldb; formed on first use
   LocalMap lmap;  // plane-wave sphere on the box (pruned transforms)
   // workspace
   int box_batch = 0;  // bands per FFT launch; 0 = all bands within a 2 GiB workspace
   DeviceArray<cplx_t<R>> boxes;
   DeviceArray<cplx_t<R>> coeff, dcoeff, qcoeff, partial;
+  CoeffWork<R> cwork;
   DeviceArray<cplx_t<R>> nl_h, nl_s;  // projector-stage results of the overlapped apply
   AuxStream aux;
 
@@ -135,7 +141,15 @@ struct KOperator {
                         cplx_t<R> *spsi, int64_t ldout, hipStream_t st);
   // c = B^H psi  (np x nb, leading dimension np) into `coeff`
   void project(int nb, const cplx_t<R> *psi, int64_t ldpsi, hipStream_t st);
+  // BD and BQ, once
+  void mixed_blocks(hipStream_t st);
+  // projector stage on the skinny kernels for nb bands
+  bool skinny(int nb) const;
 };
+
+// Sizes up to which the projector stage uses the skinny kernels.
+constexpr int kSkinnyMaxProjectors = 512;
+constexpr int kSkinnyMaxBands = 256;
 
 // dc = D c, qc = Q c per atom block for ncols coefficient columns of length
 // np (column-major, leading dimension np), in the operator's precision.
@@ -145,8 +159,7 @@ void mix_projector_coefficients(const System &sys, int ncols, const cplx_t<R> *c
 
 // Dense H/S assembly on the device (FP64), column-major n x n with
 // leading dimension ld, hermitised (reference: src/half_dense_solver.F90:53-88).
-void assemble_dense(const KOperator<double> &op, double2 *h, double2 *s, int64_t ld,
-                    hipStream_t st);
+void assemble_dense(KOperator<double> &op, double2 *h, double2 *s, int64_t ld, hipStream_t st);
 
 // Device Veff construction (reference: src/half_potential.F90:175-238).
 void build_veff_device(System &sys, hipStream_t s);

@@ -228,7 +228,7 @@ std::vector<int> kinetic_order(const std::vector<double> &kin) {
   return out;
 }
 
-void solve_dense(Blas &blas, const KOperator<double> &op, int solver, int nbands, bool want_vectors,
+void solve_dense(Blas &blas, KOperator<double> &op, int solver, int nbands, bool want_vectors,
                  DeviceArray<double2> &vectors, SolveResult &res) {
   const int n = op.n;
   hipStream_t s = blas.stream;

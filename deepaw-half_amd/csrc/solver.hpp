@@ -42,7 +42,7 @@ struct AccControls {
 
 // Dense solve of the generalized problem on the device.  Eigenvectors (if
 // requested) are left on the device in `vectors` (n x nbands, ld = n).
-void solve_dense(Blas &blas, const KOperator<double> &op, int solver, int nbands, bool want_vectors,
+void solve_dense(Blas &blas, KOperator<double> &op, int solver, int nbands, bool want_vectors,
                  DeviceArray<double2> &vectors, SolveResult &res);
 
 // All-band solve (reference: src/half_cuda_iterative_solver.cuf:79-207).
