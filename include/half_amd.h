@@ -149,9 +149,14 @@ int half_amd_trace_read(int max_kernels, char *names, double *total_ms, int64_t 
 
 /* Kernel tunables, e.g. "local.fp64.tile.x" (FFT points per workgroup tile
  * of the x/y/z local-stage passes), "local.fp64.xgroup" (bands sharing a
- * Veff tile in the transform-based x pass) and "local.fp64.xconv" (1: x pass
- * as a per-column convolution, 0: transform-based); "fp32" keys for the
- * FP32 operator.  Process-wide. */
+ * Veff tile in the transform-based x pass), "local.fp64.xconv" (1: x pass
+ * as a per-column convolution, 0: transform-based), "local.fp64.fused" and
+ * "local.fp64.ygroup" (y, x, y passes in one kernel per slice of ygroup
+ * bands), "local.fp64.vxlds" (1: that kernel stages the slice's Vx rows in
+ * LDS); "fp32" keys for the FP32 operator; "operator.overlap" (projector
+ * stage on a second stream) and "operator.projector" (1: projector stage on
+ * the skinny 16x16-tile kernels, 0: on the 64-wide contractions with a
+ * separate D/Q mix).  Process-wide. */
 int half_amd_set_tuning(const char *key, int value, char *error, int error_capacity);
 int half_amd_get_tuning(const char *key, int *value, char *error, int error_capacity);
 
