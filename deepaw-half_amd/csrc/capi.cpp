@@ -22,6 +22,7 @@
 #include "common.hpp"
 #include "forces.hpp"
 #include "fset.hpp"
+#include "h5io.hpp"
 #include "kmesh.hpp"
 #include "paw.hpp"
 #include "qdep.hpp"
@@ -828,6 +829,58 @@ int half_amd_force_set(half_handle handle, int nk, const double *kpoints, const 
       charges[0] = r.smooth_charge;
       charges[1] = r.aug_charge;
     }
+    return int(kSuccess);
+  });
+}
+
+int half_amd_write_vaspwave(half_handle handle, const char *path, int nk, const double *kpoints, int nbands,
+                            const double *eigenvalues, const double *occupations, double fermi,
+                            const HALF_COMPLEX *const *vectors, const int64_t *ld, char *error, int capacity) {
+  clear_error(error, capacity);
+  const int slot = context_slot(handle);
+  if (!slot) return bad_handle(error, capacity);
+  if (!path || !*path || nk < 1 || nbands < 1 || !kpoints || !eigenvalues || !occupations || !vectors || !ld)
+    return export_error(kInvalidArgument, "path, k points, bands, eigenvalues, occupations and vectors are required",
+                        error, capacity);
+  // reference: src/half_vaspwave.F90:110-118
+  for (int64_t i = 0; i < int64_t(nk) * nbands; ++i)
+    if (!(occupations[i] >= -1e-10 && occupations[i] <= 2.0 + 1e-10))
+      return export_error(kInvalidArgument, "invalid HDF5 occupations", error, capacity);
+  if (!std::isfinite(fermi)) return export_error(kInvalidArgument, "Fermi level must be finite", error, capacity);
+  return guarded(error, capacity, [&]() {
+    Context &c = *g_contexts[slot];
+    WaveExport w;
+    w.system = c.sys.crystal.system;
+    w.crystal = &c.sys.crystal;
+    w.charge = &c.sys.charge;
+    w.encut = c.sys.encut;
+    w.fermi = fermi;
+    w.nk = nk;
+    w.nb = nbands;
+    w.kpoints = kpoints;
+    w.eigenvalues = eigenvalues;
+    w.occupations = occupations;
+    for (int j = 0; j < nk; ++j) {
+      const Basis b = make_basis(c, kpoints + 3 * j);
+      const int64_t n = b.npw();
+      if (nbands > n) return export_error(kInvalidArgument, "invalid requested band count", error, capacity);
+      if (!vectors[j]) return export_error(kInvalidArgument, "eigenvectors are required", error, capacity);
+      if (ld[j] < n) return export_error(kCapacity, "eigenvector leading dimension is too small", error, capacity);
+      // coefficients in VASP's plane-wave order, single precision
+      // (reference: src/half_vaspwave.F90:121-130)
+      const std::vector<int64_t> order = vasp_order(b);
+      const double2 *v = reinterpret_cast<const double2 *>(vectors[j]);
+      std::vector<float> packed(size_t(2) * n * nbands);
+      for (int band = 0; band < nbands; ++band)
+        for (int64_t g = 0; g < n; ++g) {
+          const double2 x = v[order[g] + band * ld[j]];
+          packed[2 * (band * n + g)] = static_cast<float>(x.x);
+          packed[2 * (band * n + g) + 1] = static_cast<float>(x.y);
+        }
+      w.npw.push_back(static_cast<int>(n));
+      w.coefficients.push_back(std::move(packed));
+    }
+    write_vaspwave(path, w);
     return int(kSuccess);
   });
 }

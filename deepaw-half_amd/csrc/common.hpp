@@ -50,6 +50,7 @@ constexpr double kEdeps = 4.0 * kPi * kFelect;
 constexpr double kHsqdtm = kRytoev * kAutoa * kAutoa;
 
 struct Crystal {
+  std::string system;            // first line of the density file
   double lattice[3][3] = {};     // rows are lattice vectors, Angstrom
   double reciprocal[3][3] = {};  // rows b_i = 2 pi (A^-1)^T
   double volume = 0;

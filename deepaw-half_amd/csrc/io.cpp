@@ -9,6 +9,7 @@
 #include <sstream>
 
 #include "common.hpp"
+#include "h5io.hpp"
 
 namespace half {
 
@@ -62,6 +63,12 @@ struct Cursor {
   explicit Cursor(const std::string &path, const std::string &kind) : what(kind) {
     std::ifstream in(path);
     if (!in) throw Error(kIo, "cannot open " + kind + ": " + path);
+    std::string l;
+    while (std::getline(in, l)) lines.push_back(l);
+  }
+  // over text already in memory
+  Cursor(const std::string &text, const std::string &kind, bool) : what(kind) {
+    std::istringstream in(text);
     std::string l;
     while (std::getline(in, l)) lines.push_back(l);
   }
@@ -153,8 +160,17 @@ void Crystal::update_geometry() {
 }
 
 void read_chgcar(const std::string &path, Crystal &crystal, ChargeGrid &charge) {
+  // a VASP HDF5 file in place of the CHGCAR (reference: src/half_chgcar.F90:25)
+  if (is_hdf5(path)) {
+    read_h5_chgcar(path, crystal, charge);
+    return;
+  }
   Cursor c(path, "CHGCAR");
-  c.line();  // system name
+  {
+    const std::string &name = c.line();
+    const size_t b = name.find_first_not_of(" \t"), e = name.find_last_not_of(" \t\r");
+    crystal.system = b == std::string::npos ? "" : name.substr(b, e - b + 1);
+  }
   auto sw = words(c.line());
   double scale;
   if (sw.empty() || !parse_double(sw[0], scale)) throw Error(kIo, "failed to read lattice scale");
@@ -351,7 +367,9 @@ void read_dataset(Cursor &c, const std::string &header, Dataset &p) {
 }  // namespace
 
 std::vector<Dataset> read_potcar(const std::string &path) {
-  Cursor c(path, "POTCAR");
+  // a VASP HDF5 file carries the POTCAR text at /input/potcar/content
+  // (reference: src/half_potcar.F90:34)
+  Cursor c = is_hdf5(path) ? Cursor(read_h5_potcar(path), "POTCAR", true) : Cursor(path, "POTCAR");
   std::vector<Dataset> out;
   while (true) {
     while (!c.eof() && blank(c.lines[c.pos])) ++c.pos;

@@ -71,7 +71,7 @@ __global__ __launch_bounds__(64 * kCoeffWaves) void coeff_kernel(const cplx_t<R>
   const int ch = slab * kCoeffWaves + wave;
   const int r0 = ch * chunk, r1 = min(n, r0 + chunk);
   A4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-#pragma unroll 4
+  // (MFMAs are convergent: the compiler keeps this runtime-bounded loop rolled)
   for (int k = r0; k < r1; k += 4) {
     const int kk = k + kq;
     const int kc = min(kk, n - 1);
@@ -133,7 +133,6 @@ __global__ __launch_bounds__(64 * kExpandWaves) void expand_kernel(const cplx_t<
   const V *pc = c + int64_t(min(b0 + i, nb - 1)) * ldc;   // B columns: likewise
   const int pbeg = wave * pchunk, pend = min(np, pbeg + pchunk);
   A4 hr = {0, 0, 0, 0}, hi = {0, 0, 0, 0}, sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
-#pragma unroll 3
   for (int p = pbeg; p < pend; p += 4) {
     const int pp = p + kq;
     const int64_t pcl = min(pp, np - 1);

@@ -40,7 +40,7 @@ EXT_SYMBOLS = (
     "half_amd_get_veff", "half_amd_nonlocal_forces", "half_amd_device_info",
     "half_amd_trace", "half_amd_trace_read", "half_amd_set_tuning", "half_amd_get_tuning",
     "half_amd_set_projector_positions", "half_amd_solve_history", "half_amd_solve_kmesh",
-    "half_amd_force_set",
+    "half_amd_force_set", "half_amd_write_vaspwave",
 )
 
 
@@ -117,6 +117,8 @@ def load(path: str | None = None):
         "half_amd_solve_kmesh": [i64, i, pd, i, i, d, d, i, i, i, pd, C.POINTER(SolveStats), cp, i],
         "half_amd_force_set": [i64, i, pd, pd, i, pd, pd, C.POINTER(C.c_void_p), C.POINTER(C.c_int64), pd, pd,
                                pd, pd, cp, i],
+        "half_amd_write_vaspwave": [i64, cp, i, pd, i, pd, pd, d, C.POINTER(C.c_void_p), C.POINTER(C.c_int64),
+                                    cp, i],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -386,6 +388,21 @@ class Context:
         if density:
             out["density"] = dens
         return out
+
+    def write_vaspwave(self, path, kpoints, eigenvalues, occupations, vectors, fermi):
+        """vaspwave.h5 export (half_amd_write_vaspwave): vectors[j] is
+        npw_j x nbands for k point j."""
+        ks = np.ascontiguousarray(np.asarray(kpoints, float).reshape(-1, 3))
+        nk = len(ks)
+        eps = np.ascontiguousarray(np.asarray(eigenvalues, float).reshape(nk, -1))
+        nb = eps.shape[1]
+        occ = np.ascontiguousarray(np.asarray(occupations, float).reshape(nk, nb))
+        cols = [np.asfortranarray(np.asarray(v, np.complex128)[:, :nb]) for v in vectors]
+        ptrs = (C.c_void_p * nk)(*[c.ctypes.data for c in cols])
+        lds = (C.c_int64 * nk)(*[c.shape[0] for c in cols])
+        e = _err()
+        _check(self.lib.half_amd_write_vaspwave(self.handle, str(path).encode(), nk, _pd(ks), nb, _pd(eps),
+                                                _pd(occ), float(fermi), ptrs, lds, e, 512), e)
 
     def solve_history(self):
         """Max residual (eV) per iteration of the last solve_acc."""

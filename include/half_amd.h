@@ -104,6 +104,24 @@ int half_amd_force_set(half_handle handle, int nk, const double *kpoints, const 
                        const int64_t *ld, double *forces, double *components, double *density, double *charges,
                        char *error, int error_capacity);
 
+/* vaspwave.h5 export of solved bands (reference: src/half_vaspwave.F90:94-155
+ * and src/half_hdf5_bridge.c:77-130, called from app/half_cli.F90:601,899):
+ * the context's structure and input charge, ENCUT, the Fermi level, and per
+ * k point its eigenvalues (celtot), occupations / 2 (fertot), plane-wave
+ * count, k point and the nbands eigenvectors in VASP's plane-wave order as
+ * single precision (wave, nbands x npw x 2).  Arguments as
+ * half_amd_force_set; occupations must lie in [0, 2] (HALF_ERROR_INVALID_ARGUMENT
+ * otherwise, where the reference stops).  The HDF5 1.10 C library
+ * (libhdf5.so.103, or the path in HALF_HDF5_LIBRARY) is loaded on first use;
+ * without it the call returns HALF_ERROR_UNAVAILABLE.  The same library
+ * lets half_create_from_files read a VASP HDF5 file (vaspout.h5 or a
+ * vaspwave.h5) given as the charge path and/or the potential path, as the
+ * reference's readers do (src/half_chgcar.F90:25, src/half_potcar.F90:34). */
+int half_amd_write_vaspwave(half_handle handle, const char *path, int nk, const double *kpoints, int nbands,
+                            const double *eigenvalues, const double *occupations, double fermi,
+                            const HALF_COMPLEX *const *vectors, const int64_t *ld, char *error,
+                            int error_capacity);
+
 /* Select the solver used by half_solve_kpoint for this context (1..4). */
 int half_amd_set_solver(half_handle handle, int solver, char *error, int error_capacity);
 

@@ -263,6 +263,18 @@ def build_basis(crystal, grid, encut, kpoint):
     return Basis(tuple(grid), g.astype(np.int32), q[keep], kin[keep], idx.astype(np.int64))
 
 
+def vasp_order(basis):
+    """src/half_vaspwave.F90:142-155: basis indices in the order VASP stores
+    plane-wave coefficients: a box-sized lookup (C-order index -> basis
+    index) walked with the z box coordinate slowest and x fastest."""
+    n1, n2, n3 = basis.grid
+    lookup = np.full(n1 * n2 * n3, -1, np.int64)
+    lookup[basis.fft_index] = np.arange(len(basis.fft_index))
+    gz, gy, gx = np.meshgrid(np.arange(n3), np.arange(n2), np.arange(n1), indexing="ij")
+    walk = lookup[(gx * n2 * n3 + gy * n3 + gz).ravel()]
+    return walk[walk >= 0]
+
+
 # --------------------------------------------------------------------------
 # splines and real spherical harmonics (src/half_paw.F90:78-137)
 # --------------------------------------------------------------------------
