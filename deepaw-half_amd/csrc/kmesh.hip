@@ -30,6 +30,7 @@
 #include "contract.hpp"
 #include "cplx.hpp"
 #include "kmesh.hpp"
+#include "projector.hpp"
 #include "trace.hpp"
 
 namespace half {
@@ -202,6 +203,13 @@ void KMeshOperator<R>::build(const System &s, const std::vector<std::array<doubl
   kin.resize(size_t(nk) * nmax);
   HALF_HIP(hipMemsetAsync(B.get(), 0, sizeof(V) * B.size(), st));
   HALF_HIP(hipMemsetAsync(kin.get(), 0, sizeof(R) * kin.size(), st));
+  const bool mixed = np > 0 && np <= kSkinnyMaxProjectors;
+  if (mixed) {
+    BD.resize(B.size());
+    BQ.resize(B.size());
+    HALF_HIP(hipMemsetAsync(BD.get(), 0, sizeof(V) * BD.size(), st));
+    HALF_HIP(hipMemsetAsync(BQ.get(), 0, sizeof(V) * BQ.size(), st));
+  }
   maps.clear();
   maps.reserve(nk);
   kinetic.assign(nk, {});
@@ -215,6 +223,13 @@ void KMeshOperator<R>::build(const System &s, const std::vector<std::array<doubl
     if (np > 0)
       HALF_HIP(hipMemcpy2DAsync(B.get() + size_t(j) * np * nmax, sizeof(V) * nmax, op.B.get(), sizeof(V) * op.ldb,
                                 sizeof(V) * op.n, np, hipMemcpyDeviceToDevice, st));
+    if (mixed) {
+      op.mixed_blocks(st);
+      HALF_HIP(hipMemcpy2DAsync(BD.get() + size_t(j) * np * nmax, sizeof(V) * nmax, op.BD.get(),
+                                sizeof(V) * op.ldb, sizeof(V) * op.n, np, hipMemcpyDeviceToDevice, st));
+      HALF_HIP(hipMemcpy2DAsync(BQ.get() + size_t(j) * np * nmax, sizeof(V) * nmax, op.BQ.get(),
+                                sizeof(V) * op.ldb, sizeof(V) * op.n, np, hipMemcpyDeviceToDevice, st));
+    }
     HALF_HIP(hipMemcpyAsync(kin.get() + size_t(j) * nmax, op.kinetic.get(), sizeof(R) * op.n,
                             hipMemcpyDeviceToDevice, st));
     kinetic[j] = op.basis.kinetic;
@@ -263,12 +278,19 @@ void KMeshOperator<R>::apply(int nb, const cplx_t<R> *psi, int64_t ld, int64_t s
     aux.ensure();
     HALF_HIP(hipEventRecord(aux.fork, st));
     HALF_HIP(hipStreamWaitEvent(aux.stream, aux.fork, 0));
-    gram_batched<R>(nk, nmax, np, nb, B.get(), nmax, int64_t(nmax) * np, psi, ld, sk_in, coeff.get(), np, cnt,
-                    nullptr, 0, 0, nullptr, 0, 0, work, aux.stream);
-    mix_projector_coefficients<R>(*sys, nk * nb, coeff.get(), dcoeff.get(), qcoeff.get(), aux.stream);
-    gemm_tall_batched<R>(nk, nmax, nb, np, R(1), B.get(), nmax, int64_t(nmax) * np, dcoeff.get(), np, cnt, R(0),
-                         nl_h.get(), nmax, int64_t(nb) * nmax, qcoeff.get(), np, cnt, nl_s.get(), nmax,
-                         int64_t(nb) * nmax, aux.stream);
+    if (projector_skinny(np, nb) && BD.size()) {
+      projector_coefficients_batched<R>(nk, nmax, np, nb, B.get(), nmax, int64_t(nmax) * np, psi, ld, sk_in,
+                                        coeff.get(), np, cnt, cwork, aux.stream);
+      projector_expand_batched<R>(nk, nmax, np, nb, BD.get(), BQ.get(), nmax, int64_t(nmax) * np, coeff.get(), np,
+                                  cnt, false, nl_h.get(), nl_s.get(), nmax, int64_t(nb) * nmax, aux.stream);
+    } else {
+      gram_batched<R>(nk, nmax, np, nb, B.get(), nmax, int64_t(nmax) * np, psi, ld, sk_in, coeff.get(), np, cnt,
+                      nullptr, 0, 0, nullptr, 0, 0, work, aux.stream);
+      mix_projector_coefficients<R>(*sys, nk * nb, coeff.get(), dcoeff.get(), qcoeff.get(), aux.stream);
+      gemm_tall_batched<R>(nk, nmax, nb, np, R(1), B.get(), nmax, int64_t(nmax) * np, dcoeff.get(), np, cnt, R(0),
+                           nl_h.get(), nmax, int64_t(nb) * nmax, qcoeff.get(), np, cnt, nl_s.get(), nmax,
+                           int64_t(nb) * nmax, aux.stream);
+    }
     HALF_HIP(hipEventRecord(aux.join, aux.stream));
     LocalAddend<R> add;
     add.h = nl_h.get();
@@ -285,6 +307,16 @@ void KMeshOperator<R>::apply(int nb, const cplx_t<R> *psi, int64_t ld, int64_t s
     coeff.ensure(size_t(cnt) * nk);
     dcoeff.ensure(size_t(cnt) * nk);
     qcoeff.ensure(size_t(cnt) * nk);
+    if (projector_skinny(np, nb) && BD.size()) {
+      trace_mark("proj_coeff_mfma", st);
+      projector_coefficients_batched<R>(nk, nmax, np, nb, B.get(), nmax, int64_t(nmax) * np, psi, ld, sk_in,
+                                        coeff.get(), np, cnt, cwork, st);
+      trace_mark("proj_expand_mfma", st);
+      projector_expand_batched<R>(nk, nmax, np, nb, BD.get(), BQ.get(), nmax, int64_t(nmax) * np, coeff.get(), np,
+                                  cnt, true, hpsi, spsi, ldo, sk_out, st);
+      trace_close(st);
+      return;
+    }
     trace_mark("proj_coeff_mfma", st);
     gram_batched<R>(nk, nmax, np, nb, B.get(), nmax, int64_t(nmax) * np, psi, ld, sk_in, coeff.get(), np, cnt,
                     nullptr, 0, 0, nullptr, 0, 0, work, st);

@@ -24,7 +24,9 @@ struct KMeshOperator {
   std::vector<LocalMap> maps;
   DeviceArray<LocalTables> dtabs;
   LocalBatch lb;
-  DeviceArray<cplx_t<R>> B;  // [k][np][nmax]
+  DeviceArray<cplx_t<R>> B;       // [k][np][nmax]
+  DeviceArray<cplx_t<R>> BD, BQ;  // B D and B Q, [k][np][nmax] (for the skinny projector kernels)
+  CoeffWork<R> cwork;
   DeviceArray<R> kin;        // [k][nmax]
   DeviceArray<cplx_t<R>> boxes, coeff, dcoeff, qcoeff, work;
   DeviceArray<cplx_t<R>> nl_h, nl_s;  // projector-stage results of the overlapped apply, [k][band][nmax]

@@ -500,11 +500,15 @@ void KOperator<R>::mixed_blocks(hipStream_t st) {
   HALF_HIP(hipGetLastError());
 }
 
-template <typename R>
-bool KOperator<R>::skinny(int nb) const {
+bool projector_skinny(int np, int nb) {
   // the 16-wide tiles re-read B once per 16 bands and psi once per 16
   // projectors; past these sizes the 64-wide contractions move less data
-  return g_projector && np <= kSkinnyMaxProjectors && nb <= kSkinnyMaxBands;
+  return g_projector && np > 0 && np <= kSkinnyMaxProjectors && nb <= kSkinnyMaxBands;
+}
+
+template <typename R>
+bool KOperator<R>::skinny(int nb) const {
+  return projector_skinny(np, nb);
 }
 
 template <typename R>

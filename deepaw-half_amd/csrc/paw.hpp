@@ -150,6 +150,9 @@ struct KOperator {
 // Sizes up to which the projector stage uses the skinny kernels.
 constexpr int kSkinnyMaxProjectors = 512;
 constexpr int kSkinnyMaxBands = 256;
+// Whether np projectors and nb bands take the skinny kernels (and the
+// "operator.projector" key allows them).
+bool projector_skinny(int np, int nb);
 
 // dc = D c, qc = Q c per atom block for ncols coefficient columns of length
 // np (column-major, leading dimension np), in the operator's precision.

@@ -46,6 +46,11 @@ __device__ inline void tile_slot(int row, int col, int &lane, int &r) {
 // turn: id % kXcds is the XCD of workgroup id.
 constexpr int kXcds = 8;
 
+// batch strides (elements) of the operands of one launch
+struct Strides3 {
+  int64_t a, b, c;
+};
+
 // Workgroup (tile, slab): wave w sums rows [r0, r1) of chunk
 // slab * kCoeffWaves + w.  One-dimensional grid of tiles * slabs workgroups,
 // slabs a multiple of kXcds: slab = id % kXcds + kXcds ((id / kXcds) / tiles),
@@ -55,11 +60,16 @@ template <typename R>
 __global__ __launch_bounds__(64 * kCoeffWaves) void coeff_kernel(const cplx_t<R> *B, int64_t ldb,
                                                                  const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *c,
                                                                  int64_t ldc, cplx_t<R> *partial, int n, int np,
-                                                                 int nb, int chunk, int ptiles, int tiles) {
+                                                                 int nb, int chunk, int ptiles, int tiles,
+                                                                 Strides3 bs) {
   using V = cplx_t<R>;
   using A4 = typename Acc<R>::type;
   __shared__ V red[kCoeffWaves][4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t bz = blockIdx.y;  // batch entry
+  B += bz * bs.a;
+  psi += bz * bs.b;
+  partial += bz * int64_t(gridDim.x) * 256;
   const int slot = blockIdx.x / kXcds;
   const int tile = slot % tiles, slab = blockIdx.x % kXcds + kXcds * (slot / tiles);
   const int p0 = (tile % ptiles) * 16, b0 = (tile / ptiles) * 16;
@@ -97,9 +107,11 @@ __global__ __launch_bounds__(64 * kCoeffWaves) void coeff_kernel(const cplx_t<R>
 // entry, consecutive threads down a column of c
 template <typename V>
 __global__ __launch_bounds__(256) void coeff_reduce_kernel(const V *partial, int slabs, int tiles, int ptiles,
-                                                           V *c, int64_t ldc, int np, int nb) {
+                                                           V *c, int64_t ldc, int np, int nb, int64_t sc) {
   const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (t >= int64_t(np) * nb) return;
+  partial += blockIdx.y * int64_t(slabs) * tiles * 256;
+  c += blockIdx.y * sc;
   const int b = static_cast<int>(t / np), p = static_cast<int>(t - int64_t(b) * np);
   const int tile = (b >> 4) * ptiles + (p >> 4);
   const V *src = partial + int64_t(tile) * 256 + (p & 15) + 16 * (b & 15);
@@ -119,11 +131,17 @@ __global__ __launch_bounds__(64 * kExpandWaves) void expand_kernel(const cplx_t<
                                                                    int64_t ldb, const cplx_t<R> *c, int64_t ldc,
                                                                    cplx_t<R> *H, cplx_t<R> *S, int64_t ldo, int n,
                                                                    int np, int nb, int add, int pchunk, int gtiles,
-                                                                   int btiles) {
+                                                                   int btiles, Strides3 bs) {
   using V = cplx_t<R>;
   using A4 = typename Acc<R>::type;
   __shared__ V red[2][kExpandWaves][4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t bz = blockIdx.y;  // batch entry
+  BD += bz * bs.a;
+  BQ += bz * bs.a;
+  c += bz * bs.b;
+  H += bz * bs.c;
+  S += bz * bs.c;
   const int slot = blockIdx.x / kXcds;
   const int gt = (slot / btiles) * kXcds + blockIdx.x % kXcds, bt = slot % btiles;
   if (gt >= gtiles) return;  // uniform per workgroup: no barrier reached
@@ -168,15 +186,23 @@ __global__ __launch_bounds__(64 * kExpandWaves) void expand_kernel(const cplx_t<
 template <typename R>
 void projector_coefficients(int n, int np, int nb, const cplx_t<R> *B, int64_t ldb, const cplx_t<R> *psi,
                             int64_t ldpsi, cplx_t<R> *c, int64_t ldc, CoeffWork<R> &work, hipStream_t s) {
-  if (np <= 0 || nb <= 0) return;
+  projector_coefficients_batched<R>(1, n, np, nb, B, ldb, 0, psi, ldpsi, 0, c, ldc, 0, work, s);
+}
+
+template <typename R>
+void projector_coefficients_batched(int batch, int n, int np, int nb, const cplx_t<R> *B, int64_t ldb, int64_t sB,
+                                    const cplx_t<R> *psi, int64_t ldpsi, int64_t sPsi, cplx_t<R> *c, int64_t ldc,
+                                    int64_t sC, CoeffWork<R> &work, hipStream_t s) {
+  if (np <= 0 || nb <= 0 || batch <= 0) return;
+  if (batch > 65535) throw Error(kInvalidArgument, "projector coefficients: batch too large");
   if (n <= 0) throw Error(kInvalidArgument, "projector coefficients: empty basis");
   if (ldb < n || ldpsi < n || ldc < np) throw Error(kInvalidArgument, "projector coefficients: leading dimension");
   const int ptiles = (np + 15) / 16, btiles = (nb + 15) / 16;
   const int64_t tiles = int64_t(ptiles) * btiles;
   if (tiles > (1ll << 31) - 1) throw Error(kInvalidArgument, "projector coefficients: too many tiles");
-  // about two thousand waves, each at least 32 rows (8 MFMA steps), in
-  // slabs of kCoeffWaves chunks, a multiple of kXcds slabs
-  const int64_t target = std::max<int64_t>(1, 2048 / tiles);
+  // about two thousand waves (over the batch), each at least 32 rows (8 MFMA
+  // steps), in slabs of kCoeffWaves chunks, a multiple of kXcds slabs
+  const int64_t target = std::max<int64_t>(1, 2048 / (tiles * batch));
   int chunk = static_cast<int>((n + target - 1) / target);
   chunk = std::max(32, (chunk + 3) / 4 * 4);
   int slabs = (n + chunk * kCoeffWaves - 1) / (chunk * kCoeffWaves);
@@ -184,14 +210,15 @@ void projector_coefficients(int n, int np, int nb, const cplx_t<R> *B, int64_t l
   chunk = (n + slabs * kCoeffWaves - 1) / (slabs * kCoeffWaves);
   chunk = (chunk + 3) / 4 * 4;
   if (int64_t(slabs) * tiles > (1ll << 31) - 1) throw Error(kInvalidArgument, "projector coefficients: grid");
-  work.partial.ensure(size_t(tiles) * slabs * 256);
-  hipLaunchKernelGGL((coeff_kernel<R>), dim3(static_cast<unsigned>(tiles * slabs)), dim3(64 * kCoeffWaves), 0, s,
-                     B, ldb, psi, ldpsi, c, ldc, work.partial.get(), n, np, nb, chunk, ptiles,
-                     static_cast<int>(tiles));
+  work.partial.ensure(size_t(tiles) * slabs * 256 * batch);
+  hipLaunchKernelGGL((coeff_kernel<R>), dim3(static_cast<unsigned>(tiles * slabs), static_cast<unsigned>(batch)),
+                     dim3(64 * kCoeffWaves), 0, s, B, ldb, psi, ldpsi, c, ldc, work.partial.get(), n, np, nb, chunk,
+                     ptiles, static_cast<int>(tiles), Strides3{sB, sPsi, 0});
   HALF_HIP(hipGetLastError());
   const int64_t entries = int64_t(np) * nb;
-  hipLaunchKernelGGL((coeff_reduce_kernel<cplx_t<R>>), dim3(static_cast<unsigned>((entries + 255) / 256)),
-                     dim3(256), 0, s, work.partial.get(), slabs, static_cast<int>(tiles), ptiles, c, ldc, np, nb);
+  hipLaunchKernelGGL((coeff_reduce_kernel<cplx_t<R>>),
+                     dim3(static_cast<unsigned>((entries + 255) / 256), static_cast<unsigned>(batch)), dim3(256), 0,
+                     s, work.partial.get(), slabs, static_cast<int>(tiles), ptiles, c, ldc, np, nb, sC);
   HALF_HIP(hipGetLastError());
 }
 
@@ -199,7 +226,15 @@ template <typename R>
 void projector_expand(int n, int np, int nb, const cplx_t<R> *BD, const cplx_t<R> *BQ, int64_t ldb,
                       const cplx_t<R> *c, int64_t ldc, bool add, cplx_t<R> *H, cplx_t<R> *S, int64_t ldo,
                       hipStream_t s) {
-  if (n <= 0 || nb <= 0) return;
+  projector_expand_batched<R>(1, n, np, nb, BD, BQ, ldb, 0, c, ldc, 0, add, H, S, ldo, 0, s);
+}
+
+template <typename R>
+void projector_expand_batched(int batch, int n, int np, int nb, const cplx_t<R> *BD, const cplx_t<R> *BQ,
+                              int64_t ldb, int64_t sB, const cplx_t<R> *c, int64_t ldc, int64_t sC, bool add,
+                              cplx_t<R> *H, cplx_t<R> *S, int64_t ldo, int64_t sO, hipStream_t s) {
+  if (n <= 0 || nb <= 0 || batch <= 0) return;
+  if (batch > 65535) throw Error(kInvalidArgument, "projector expansion: batch too large");
   if (np <= 0) throw Error(kInvalidArgument, "projector expansion: no projectors");
   if (ldb < n || ldc < np || ldo < n) throw Error(kInvalidArgument, "projector expansion: leading dimension");
   const int gt = (n + 15) / 16, bt = (nb + 15) / 16;
@@ -207,17 +242,24 @@ void projector_expand(int n, int np, int nb, const cplx_t<R> *BD, const cplx_t<R
   if (groups > (1ll << 31) - 1) throw Error(kInvalidArgument, "projector expansion: grid");
   int pchunk = (np + kExpandWaves - 1) / kExpandWaves;
   pchunk = (pchunk + 3) / 4 * 4;
-  hipLaunchKernelGGL((expand_kernel<R>), dim3(static_cast<unsigned>(groups)), dim3(64 * kExpandWaves), 0, s, BD, BQ,
-                     ldb, c, ldc, H, S, ldo, n, np, nb, add ? 1 : 0, pchunk, gt, bt);
+  hipLaunchKernelGGL((expand_kernel<R>), dim3(static_cast<unsigned>(groups), static_cast<unsigned>(batch)),
+                     dim3(64 * kExpandWaves), 0, s, BD, BQ, ldb, c, ldc, H, S, ldo, n, np, nb, add ? 1 : 0, pchunk,
+                     gt, bt, Strides3{sB, sC, sO});
   HALF_HIP(hipGetLastError());
 }
 
-#define HALF_PROJECTOR(R)                                                                                         \
-  template void projector_coefficients<R>(int, int, int, const cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, \
-                                          cplx_t<R> *, int64_t, CoeffWork<R> &, hipStream_t);                    \
-  template void projector_expand<R>(int, int, int, const cplx_t<R> *, const cplx_t<R> *, int64_t,               \
-                                    const cplx_t<R> *, int64_t, bool, cplx_t<R> *, cplx_t<R> *, int64_t,         \
-                                    hipStream_t);
+#define HALF_PROJECTOR(R)                                                                                          \
+  template void projector_coefficients<R>(int, int, int, const cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t,  \
+                                          cplx_t<R> *, int64_t, CoeffWork<R> &, hipStream_t);                     \
+  template void projector_coefficients_batched<R>(int, int, int, int, const cplx_t<R> *, int64_t, int64_t,        \
+                                                  const cplx_t<R> *, int64_t, int64_t, cplx_t<R> *, int64_t,      \
+                                                  int64_t, CoeffWork<R> &, hipStream_t);                          \
+  template void projector_expand<R>(int, int, int, const cplx_t<R> *, const cplx_t<R> *, int64_t,                \
+                                    const cplx_t<R> *, int64_t, bool, cplx_t<R> *, cplx_t<R> *, int64_t,          \
+                                    hipStream_t);                                                                 \
+  template void projector_expand_batched<R>(int, int, int, int, const cplx_t<R> *, const cplx_t<R> *, int64_t,   \
+                                            int64_t, const cplx_t<R> *, int64_t, int64_t, bool, cplx_t<R> *,      \
+                                            cplx_t<R> *, int64_t, int64_t, hipStream_t);
 HALF_PROJECTOR(double)
 HALF_PROJECTOR(float)
 
