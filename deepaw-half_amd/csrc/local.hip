@@ -48,11 +48,11 @@ namespace {
 // half_amd_set_tuning, chosen by the autotuner in half_amd/autotune.py.
 struct Tuning {
   // defaults from the autotuner on the 48^3 HfO2-like box, 64 bands
-  int tile[2][3] = {{1536, 768, 768}, {3072, 1536, 1024}};  // [fp64|fp32][x|y|z]
-  int xgroup[2] = {1, 8};
+  int tile[2][3] = {{1536, 1024, 768}, {3072, 2048, 1024}};  // [fp64|fp32][x|y|z]
+  int xgroup[2] = {8, 8};
   int xconv[2] = {1, 1};  // x pass as a per-column convolution when it fits
   int fused[2] = {1, 1};  // y, x, y passes in one kernel where the convolution applies
-  int ygroup[2] = {2, 2}; // bands per workgroup of the fused kernel (fewer where LDS is short)
+  int ygroup[2] = {4, 4}; // bands per workgroup of the fused kernel (fewer where LDS is short)
   int vxlds[2] = {1, 1};  // fused kernel with the slice's Vx rows in LDS where it applies
 };
 Tuning g_tune;
