@@ -403,9 +403,9 @@ __global__ __launch_bounds__(256) void yxy_kernel(cplx_t<R> *lines, const cplx_t
 
 // yxy_kernel with the slice's Vx rows in LDS, for eight waves per
 // workgroup.  The convolution needs Vx[d][i2] for d = -(P-1) .. P-1 only,
-// 2P - 1 rows: they are copied to the second plane buffer once the forward
-// y transform has freed it (one bulk load, every thread's share in flight
-// together).  The convolution keeps a sliding window of kYxyOut values (one
+// 2P - 1 rows: every thread loads its share into registers at the start (the
+// latency runs alongside the line loads and the forward y transform) and
+// writes it to the second plane buffer once that transform has freed it.  The convolution keeps a sliding window of kYxyOut values (one
 // LDS load per input, instead of PMAX + kYxyOut - 1 values held per item),
 // so the kernel fits 128 registers and two workgroups of eight waves share
 // a CU (the plane buffers of two bands, 60 KB in FP64 at 48^3, allow two).
@@ -429,6 +429,21 @@ void yxy_lds_kernel(cplx_t<R> *lines, const cplx_t<R> *vxT, const LocalTables *t
   const int i3 = blockIdx.x % t.n3;
   const int band0 = (blockIdx.x / t.n3) * G;
   if (band0 >= nb || P == 0) return;  // uniform per workgroup: no barrier reached
+  // row r of the staged block is Vx[(r - (P - 1)) mod n1][i2] of this i3; the
+  // loads are issued here so their latency overlaps the line loads and the
+  // forward transform, and the values go to LDS once that transform is done
+  const V *vcol = vxT + static_cast<long long>(i3) * t.n1 * NN;
+  V vreg[kVxPer];
+#pragma unroll
+  for (int k = 0; k < kVxPer; ++k) {
+    const int e = threadIdx.x + k * kYxyLdsThreads;
+    vreg[k] = mk<V>(R(0), R(0));
+    if (e < (2 * P - 1) * NN) {
+      const int r = e / NN, i2 = e - r * NN;
+      const int d = r - (P - 1);
+      vreg[k] = vcol[(d < 0 ? d + t.n1 : d) * NN + i2];
+    }
+  }
   lines += kp * ks.box;
   FftArgs ay = a;
   ay.M = G * P;
@@ -454,16 +469,10 @@ void yxy_lds_kernel(cplx_t<R> *lines, const cplx_t<R> *vxT, const LocalTables *t
   __syncthreads();
   V *r1 = stages<NN>(A, B, tws, ay, +1);  // ends with a barrier
   V *o1 = r1 == A ? B : A;
-  // row r of the staged block is Vx[(r - (P - 1)) mod n1][i2] of this i3
-  const V *vcol = vxT + static_cast<long long>(i3) * t.n1 * NN;
 #pragma unroll
   for (int k = 0; k < kVxPer; ++k) {
     const int e = threadIdx.x + k * kYxyLdsThreads;
-    if (e < (2 * P - 1) * NN) {
-      const int r = e / NN, i2 = e - r * NN;
-      const int d = r - (P - 1);
-      o1[e] = vcol[(d < 0 ? d + t.n1 : d) * NN + i2];
-    }
+    if (e < (2 * P - 1) * NN) o1[e] = vreg[k];
   }
   __syncthreads();
   const int nparts = (P + kYxyOut - 1) / kYxyOut;
