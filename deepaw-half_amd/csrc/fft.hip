@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void fft_lines_kernel(cplx_t<R> *data, FftArgs
 
 bool factor(int n, int *radix, int &count) {
   count = 0;
-  const int order[] = {8, 4, 5, 3, 2, 7};
+  const int order[] = {8, 6, 4, 5, 3, 2, 7};
   for (int r : order)
     while (n % r == 0 && n > 1) {
       if (count == 8) return false;

@@ -1,7 +1,7 @@
 // Batched 3D complex FFT on C-order boxes, written for gfx950.
 //
 // Each 1D pass stages a tile of M lines in LDS and runs mixed-radix Stockham
-// autosort stages (radices 8, 4, 5, 3, 2, generic 7) between two LDS
+// autosort stages (radices 8, 6, 4, 5, 3, 2, generic 7) between two LDS
 // buffers, so one pass is one HBM read and one HBM write of the tile.  Tiles
 // of the strided axes take M lines that are adjacent in the contiguous
 // direction, so every global access is a coalesced run.  The Veff multiply

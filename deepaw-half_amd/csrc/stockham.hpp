@@ -75,6 +75,40 @@ __device__ inline void butterfly(V *v, int R, int sign, const V *tws, int n) {
       v[3] = csub(r2, i2);
       break;
     }
+    case 6: {
+      // pairs at distance 3 (radix 2), then radix 3 on the sums (even
+      // outputs) and on the twiddled differences (odd outputs):
+      // X[2k] = DFT3(v_j + v_j+3)[k], X[2k+1] = DFT3((v_j - v_j+3) w6^j)[k]
+      const S c = S(-0.5), s3 = sg * S(0.86602540378443864676372317075294);
+      V a[3], b[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        a[j] = cadd(v[j], v[j + 3]);
+        b[j] = csub(v[j], v[j + 3]);
+      }
+      b[1] = cmul(b[1], mk<V>(S(0.5), s3));  // w6 = exp(sign i pi / 3)
+      b[2] = cmul(b[2], mk<V>(S(-0.5), s3));  // w6^2
+      V x[3], y[3];
+      {
+        V a1 = cadd(a[1], a[2]), t = cadd(a[0], cscale(a1, c)), u = cmul_i(csub(a[1], a[2]), s3);
+        x[0] = cadd(a[0], a1);
+        x[1] = cadd(t, u);
+        x[2] = csub(t, u);
+      }
+      {
+        V a1 = cadd(b[1], b[2]), t = cadd(b[0], cscale(a1, c)), u = cmul_i(csub(b[1], b[2]), s3);
+        y[0] = cadd(b[0], a1);
+        y[1] = cadd(t, u);
+        y[2] = csub(t, u);
+      }
+      v[0] = x[0];
+      v[2] = x[1];
+      v[4] = x[2];
+      v[1] = y[0];
+      v[3] = y[1];
+      v[5] = y[2];
+      break;
+    }
     case 8: {
       const S h = S(0.70710678118654752440084436210485);
       // radix-2 split into two radix-4 halves
@@ -114,7 +148,7 @@ __device__ inline void butterfly(V *v, int R, int sign, const V *tws, int n) {
       break;
     }
     default:
-      break;  // the plan only produces radices 8, 7, 5, 4, 3, 2
+      break;  // the plan only produces radices 8, 7, 6, 5, 4, 3, 2
   }
 }
 
@@ -160,7 +194,7 @@ __device__ V *run_stages(V *in, V *out, const V *tws, const FftArgs &a, int sign
 
 // Compile-time form of run_stages for a transform length N known when the
 // kernel is built: the same radix order as the runtime plan (fft.hip's
-// factor: 8, 4, 5, 3, 2, 7 greedily), so the arithmetic is identical, but
+// factor: 8, 6, 4, 5, 3, 2, 7 greedily), so the arithmetic is identical, but
 // every divisor and stride is a constant (the runtime form spends most of
 // its instructions on integer division).  M (lines) must be below 2^16.
 struct CtRadices {
@@ -170,8 +204,8 @@ struct CtRadices {
 
 constexpr CtRadices ct_factor(int n) {
   CtRadices p{0, {0, 0, 0, 0, 0, 0, 0, 0}};
-  const int order[6] = {8, 4, 5, 3, 2, 7};
-  for (int i = 0; i < 6; ++i)
+  const int order[7] = {8, 6, 4, 5, 3, 2, 7};
+  for (int i = 0; i < 7; ++i)
     while (n % order[i] == 0 && n > 1 && p.count < 8) {
       p.r[p.count++] = order[i];
       n /= order[i];
