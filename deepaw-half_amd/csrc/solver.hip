@@ -24,6 +24,7 @@
 #include "cplx.hpp"
 #include "solver.hpp"
 #include "trace.hpp"
+#include "tridiag.hpp"
 
 namespace half {
 
@@ -150,6 +151,29 @@ void check_info(const DeviceArray<rocblas_int> &info, hipStream_t s, const char 
   if (h != 0) throw Error(kInternal, std::string(what) + " failed (info " + std::to_string(h) + ")");
 }
 
+bool g_hetrd = true;
+
+// zheevd with vectors (upper triangle of A referenced; eigenvectors replace
+// A, eigenvalues ascending in w; e is workspace of n values).  Orders up to
+// kTridiagMaxOrder reduce in one workgroup (hetrd_upper), then rocSOLVER
+// solves the tridiagonal problem by divide and conquer (zstedc) and applies
+// the reflectors (zunmtr): the reduction is the part of rocSOLVER's zheevd
+// that is a chain of launches per column at these orders.
+void heevd_vectors(Blas &blas, int k, double2 *A, double *w, double *e, rocblas_int *info) {
+  if (!g_hetrd || k > kTridiagMaxOrder) {
+    HALF_BLAS(rocsolver_zheevd(blas.h, rocblas_evect_original, rocblas_fill_upper, k, rz(A), k, w, e, info));
+    return;
+  }
+  blas.eig_tau.ensure(size_t(k));
+  blas.eig_z.ensure(size_t(k) * k);
+  double2 *Z = blas.eig_z.get(), *tau = blas.eig_tau.get();
+  hetrd_upper(k, 1, A, k, 0, w, 0, e, 0, tau, 0, blas.stream);
+  HALF_BLAS(rocsolver_zstedc(blas.h, rocblas_evect_tridiagonal, k, w, e, rz(Z), k, info));
+  HALF_BLAS(rocsolver_zunmtr(blas.h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, k, k, rz(A), k,
+                             rz(tau), rz(Z), k));
+  HALF_HIP(hipMemcpyAsync(A, Z, sizeof(double2) * size_t(k) * k, hipMemcpyDeviceToDevice, blas.stream));
+}
+
 // Type-1 generalized Hermitian eigenproblem A y = e B y of order k, upper
 // triangles referenced (LAPACK zhegvd semantics): B = U^H U, C = U^-H A U^-1,
 // C v = e v, y = U^-1 v.  Eigenvectors replace A, eigenvalues go to w
@@ -164,8 +188,10 @@ void generalized_eigh(Blas &blas, int k, double2 *A, double2 *B, double *w, doub
                           rocblas_diagonal_non_unit, k, k, &one, rz(B), k, rz(A), k));
   HALF_BLAS(rocblas_ztrsm(blas.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                           rocblas_diagonal_non_unit, k, k, &one, rz(B), k, rz(A), k));
-  HALF_BLAS(rocsolver_zheevd(blas.h, want_vectors ? rocblas_evect_original : rocblas_evect_none,
-                             rocblas_fill_upper, k, rz(A), k, w, e, info + 1));
+  if (want_vectors)
+    heevd_vectors(blas, k, A, w, e, info + 1);
+  else
+    HALF_BLAS(rocsolver_zheevd(blas.h, rocblas_evect_none, rocblas_fill_upper, k, rz(A), k, w, e, info + 1));
   if (want_vectors)
     HALF_BLAS(rocblas_ztrsm(blas.h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
                             rocblas_diagonal_non_unit, k, k, &one, rz(B), k, rz(A), k));
@@ -183,6 +209,18 @@ double now() {
 }
 
 }  // namespace
+
+bool set_solver_tuning(const std::string &key, int value) {
+  if (key != "solver.hetrd" || value < 0 || value > 1) return false;
+  g_hetrd = value != 0;
+  return true;
+}
+
+bool get_solver_tuning(const std::string &key, int &value) {
+  if (key != "solver.hetrd") return false;
+  value = g_hetrd ? 1 : 0;
+  return true;
+}
 
 void Blas::init(hipStream_t s) {
   stream = s;

@@ -14,6 +14,7 @@ enum SolverKind { kSolverEvd = 1, kSolverEvj = 2, kSolverEvx = 3, kSolverAcc = 4
 struct Blas {
   rocblas_handle h = nullptr;
   hipStream_t stream = nullptr;
+  DeviceArray<double2> eig_z, eig_tau;  // workspace of the small eigensolver
   void init(hipStream_t s);
   void release();
   ~Blas();
@@ -50,6 +51,13 @@ void solve_dense(Blas &blas, KOperator<double> &op, int solver, int nbands, bool
 // left in `vectors` when requested.
 void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int nbands,
                const AccControls &ctl, bool want_vectors, DeviceArray<double2> &vectors, SolveResult &res);
+
+// Key "solver.hetrd" (0 / 1): Hermitian eigensolves with vectors of order
+// up to kTridiagMaxOrder reduce to tridiagonal form in one workgroup
+// (tridiag.hip) before rocSOLVER's zstedc and zunmtr (default), or run
+// rocSOLVER's zheevd throughout.
+bool set_solver_tuning(const std::string &key, int value);
+bool get_solver_tuning(const std::string &key, int &value);
 
 // Heap-sort order of plane waves by kinetic energy, ties as in the reference
 // (reference: src/half_cuda_iterative_solver.cuf:277-310).  Zero-based.

@@ -1,0 +1,232 @@
+// Householder reduction of a Hermitian matrix to real symmetric tridiagonal
+// form, LAPACK zhetrd 'U' conventions (the unblocked zhetd2 sequence:
+// for i = n-2 .. 0, a reflector annihilates A(0:i-1, i+1), then
+// A(0:i, 0:i) -= v w^H + w v^H with w = tau A v - tau/2 (tau A v)^H v v).
+//
+// The Rayleigh-Ritz problems of the all-band solve are small (2m = 80-300)
+// and one matrix at a time, where a library reduction is a chain of a few
+// launches per column and takes milliseconds.  Here one workgroup does the
+// whole reduction: 512 threads, the matrix in global memory (it stays in
+// the L2 of the XCD that runs the workgroup), the vectors in LDS.  Each
+// column step is one pass over the active block that applies the previous
+// step's rank-2 update and forms this step's product A v in the same read:
+// the reflector only needs the one column the update changes first, so
+// that column is updated ahead of the pass.  Lanes own rows (column-major
+// loads are coalesced), waves own columns; the eight per-wave partial
+// products are added in a fixed order, so results do not depend on timing.
+#include "cplx.hpp"
+#include "tridiag.hpp"
+
+#include "common.hpp"
+
+namespace half {
+
+namespace {
+
+constexpr int kTrdThreads = 512;
+constexpr int kTrdWaves = kTrdThreads / 64;
+constexpr int kTrdGroup = 4;  // columns per load group in the pass
+
+__device__ inline double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
+// sum of the per-wave values red[0..kTrdWaves) in a fixed order
+__device__ inline double wave_total(const double *red) {
+  double s = 0;
+#pragma unroll
+  for (int k = 0; k < kTrdWaves; ++k) s += red[k];
+  return s;
+}
+
+template <int NMAX>
+__global__ __launch_bounds__(kTrdThreads) void hetrd_upper_kernel(int n, double2 *A, int64_t lda, int64_t sA,
+                                                                   double *D, int64_t sD, double *E, int64_t sE,
+                                                                   double2 *Tau, int64_t sT) {
+  constexpr int RC = NMAX / 64;  // row chunks per lane
+  static_assert(NMAX <= kTrdThreads && NMAX % 64 == 0, "one thread per row in the vector phases");
+  __shared__ double2 vb[2][NMAX], wb[2][NMAX];  // v, w of this step and of the previous one
+  __shared__ double2 part[kTrdWaves][NMAX];     // per-wave partial rows of A v
+  __shared__ double2 nextcol[NMAX];             // column i of the block after the pass of step i
+  __shared__ double red[kTrdWaves];             // per-wave partial v^H A v
+  __shared__ double2 s_alpha, s_tau;
+
+  A += blockIdx.x * sA;
+  D += blockIdx.x * sD;
+  E += blockIdx.x * sE;
+  Tau += blockIdx.x * sT;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  int cur = 0;
+  bool prev = false;  // whether the previous step left a rank-2 update to apply
+
+  for (int i = n - 2; i >= 0; --i) {
+    const int c = i + 1, m = i + 1;
+    const double2 *vp = vb[cur ^ 1], *wp = wb[cur ^ 1];
+    double2 *v = vb[cur], *w = wb[cur];
+
+    // wave 0: column c (rows 0..c) with the previous step's update, then
+    // the reflector (zlarfg) that annihilates rows 0..i-1
+    if (wid == 0) {
+      double2 x[RC];
+      double nrm = 0;
+      const double2 cwc = prev ? wp[c] : mk<double2>(0, 0), cvc = prev ? vp[c] : mk<double2>(0, 0);
+#pragma unroll
+      for (int k = 0; k < RC; ++k) {
+        const int r = lane + 64 * k;
+        x[k] = mk<double2>(0, 0);
+        if (r <= c) {
+          x[k] = (i == n - 2) ? A[r + c * lda] : nextcol[r];
+          if (prev) x[k] = csub(x[k], cadd(cmulc(vp[r], cwc), cmulc(wp[r], cvc)));
+          if (r < i) nrm += x[k].x * x[k].x + x[k].y * x[k].y;
+          if (r == i) s_alpha = x[k];
+          if (r == c) D[c] = x[k].x;
+        }
+      }
+      const double xn2 = wave_sum(nrm);
+      const double2 alpha = s_alpha;
+      const bool refl = !(xn2 == 0.0 && alpha.y == 0.0);
+      double beta = alpha.x;
+      double2 taui = mk<double2>(0, 0), scal = mk<double2>(1, 0);
+      if (refl) {
+        beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
+        taui = mk<double2>((beta - alpha.x) / beta, -alpha.y / beta);
+        const double zr = alpha.x - beta, zi = alpha.y, q = 1.0 / (zr * zr + zi * zi);
+        scal = mk<double2>(zr * q, -zi * q);  // 1 / (alpha - beta)
+      }
+#pragma unroll
+      for (int k = 0; k < RC; ++k) {
+        const int r = lane + 64 * k;
+        if (r < i) {
+          const double2 y = cmul(x[k], scal);
+          A[r + c * lda] = y;
+          v[r] = y;
+        } else if (r == i) {
+          A[i + c * lda] = mk<double2>(beta, 0);
+          v[i] = mk<double2>(1, 0);
+        }
+      }
+      if (lane == 0) {
+        E[i] = beta;
+        Tau[i] = taui;
+        s_tau = taui;
+      }
+    }
+    __syncthreads();
+
+    // one pass over A(0:i, 0:i): the previous update, then the partial A v.
+    // A wave's columns j = wid + 8 s go in groups of kTrdGroup; the loads of
+    // the next group are issued before the current one is updated and
+    // stored (distinct columns, so the order is safe, but the compiler
+    // cannot see that and would wait out the L2 latency once per column).
+    // Loads are clamped into the block instead of masked, so they need no
+    // branches; what a clamped lane computes is never stored or summed.
+    double2 acc[RC], vpr[RC], wpr[RC];
+#pragma unroll
+    for (int k = 0; k < RC; ++k) {
+      const int r = min(lane + 64 * k, m - 1);
+      acc[k] = mk<double2>(0, 0);
+      vpr[k] = prev ? vp[r] : mk<double2>(0, 0);
+      wpr[k] = prev ? wp[r] : mk<double2>(0, 0);
+    }
+    const int ng = (m > wid) ? ((m - wid + kTrdWaves - 1) / kTrdWaves + kTrdGroup - 1) / kTrdGroup : 0;
+    auto load = [&](int g, double2 (&b)[kTrdGroup][RC]) {
+#pragma unroll
+      for (int u = 0; u < kTrdGroup; ++u) {
+        const int64_t j = min(wid + kTrdWaves * (g * kTrdGroup + u), m - 1);
+#pragma unroll
+        for (int k = 0; k < RC; ++k) b[u][k] = A[min(lane + 64 * k, m - 1) + j * lda];
+      }
+    };
+    auto process = [&](int g, const double2 (&b)[kTrdGroup][RC]) {
+#pragma unroll
+      for (int u = 0; u < kTrdGroup; ++u) {
+        const int j = wid + kTrdWaves * (g * kTrdGroup + u);
+        if (j < m) {  // uniform over the wave
+          const double2 vj = v[j];
+          const double2 cw = prev ? wp[j] : mk<double2>(0, 0), cv = prev ? vp[j] : mk<double2>(0, 0);
+#pragma unroll
+          for (int k = 0; k < RC; ++k) {
+            const int r = lane + 64 * k;
+            const double2 a = csub(b[u][k], cadd(cmulc(vpr[k], cw), cmulc(wpr[k], cv)));
+            if (r < m) {
+              A[r + int64_t(j) * lda] = a;
+              if (j == i) nextcol[r] = a;  // the next step's Householder column
+            }
+            acc[k] = cadd(acc[k], cmul(a, vj));
+          }
+        }
+      }
+    };
+    {
+      double2 ba[kTrdGroup][RC], bb[kTrdGroup][RC];
+      if (ng > 0) load(0, ba);
+      for (int g = 0; g < ng; g += 2) {
+        if (g + 1 < ng) load(g + 1, bb);
+        process(g, ba);
+        if (g + 1 < ng) {
+          if (g + 2 < ng) load(g + 2, ba);
+          process(g + 1, bb);
+        }
+      }
+    }
+    double s = 0;  // this wave's part of v^H A v
+#pragma unroll
+    for (int k = 0; k < RC; ++k) {
+      const int r = lane + 64 * k;
+      if (r < m) {
+        part[wid][r] = acc[k];
+        const double2 vr = v[r];
+        s += vr.x * acc[k].x + vr.y * acc[k].y;
+      }
+    }
+    s = wave_sum(s);
+    if (lane == 0) red[wid] = s;
+    __syncthreads();
+
+    // w = tau A v - tau/2 ((tau A v)^H v) v = tau A v - |tau|^2/2 (v^H A v) v
+    // (v^H A v is real for Hermitian A)
+    if (t < m) {
+      double2 y = part[0][t];
+#pragma unroll
+      for (int k = 1; k < kTrdWaves; ++k) y = cadd(y, part[k][t]);
+      const double2 taui = s_tau;
+      const double a2 = -0.5 * (taui.x * taui.x + taui.y * taui.y) * wave_total(red);
+      w[t] = cadd(cmul(taui, y), cscale(v[t], a2));
+    }
+    prev = !(s_tau.x == 0.0 && s_tau.y == 0.0);
+    cur ^= 1;
+    __syncthreads();
+  }
+
+  if (t == 0) {
+    double2 a = A[0];
+    if (prev) {
+      const double2 v0 = vb[cur ^ 1][0], w0 = wb[cur ^ 1][0];
+      a = csub(a, cadd(cmulc(v0, w0), cmulc(w0, v0)));
+    }
+    D[0] = a.x;
+  }
+}
+
+}  // namespace
+
+void hetrd_upper(int n, int batch, double2 *A, int64_t lda, int64_t sA, double *d, int64_t sd, double *e,
+                 int64_t se, double2 *tau, int64_t stau, hipStream_t s) {
+  if (n < 1 || n > kTridiagMaxOrder || lda < n || batch < 1)
+    throw Error(kInvalidArgument, "hetrd_upper: order " + std::to_string(n) + " outside 1.." +
+                                      std::to_string(kTridiagMaxOrder) + " or bad leading dimension / batch");
+  const dim3 grid(batch), block(kTrdThreads);
+  if (n <= 64)
+    hipLaunchKernelGGL(hetrd_upper_kernel<64>, grid, block, 0, s, n, A, lda, sA, d, sd, e, se, tau, stau);
+  else if (n <= 128)
+    hipLaunchKernelGGL(hetrd_upper_kernel<128>, grid, block, 0, s, n, A, lda, sA, d, sd, e, se, tau, stau);
+  else if (n <= 192)
+    hipLaunchKernelGGL(hetrd_upper_kernel<192>, grid, block, 0, s, n, A, lda, sA, d, sd, e, se, tau, stau);
+  else
+    hipLaunchKernelGGL(hetrd_upper_kernel<256>, grid, block, 0, s, n, A, lda, sA, d, sd, e, se, tau, stau);
+  HALF_HIP(hipGetLastError());
+}
+
+}  // namespace half

@@ -1,0 +1,24 @@
+// Householder reduction of small Hermitian matrices to real symmetric
+// tridiagonal form, one workgroup per matrix (see tridiag.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace half {
+
+// Largest order the one-workgroup reduction takes.
+constexpr int kTridiagMaxOrder = 256;
+
+// LAPACK zhetrd with uplo = 'U' on `batch` matrices of order n <= kTridiagMaxOrder
+// (column-major, leading dimension lda, matrix b at A + b * sA): on return
+// d (n) and e (n - 1) hold the tridiagonal matrix T = Q^H A Q, tau (n - 1)
+// and the strictly upper triangle of A the reflectors
+// Q = H(n-2) ... H(0), H(i) = I - tau_i v v^H, v(i) = 1, v(0:i-1) stored in
+// A(0:i-1, i+1), v(i+1:) = 0; the rest of A is overwritten.  The output
+// feeds rocSOLVER's zstedc / dsterf (d, e) and zunmtr (A, tau) unchanged.
+void hetrd_upper(int n, int batch, double2 *A, int64_t lda, int64_t sA, double *d, int64_t sd, double *e,
+                 int64_t se, double2 *tau, int64_t stau, hipStream_t s);
+
+}  // namespace half

@@ -83,6 +83,39 @@ def test_dense_eigensolvers(si2):
     ctx.set_solver(H.SOLVER_EVD)
 
 
+def test_small_eigensolver_matches_library(si2):
+    """Dense generalized solves of order <= 256 (the si2 cell at reduced
+    cutoffs) with the one-workgroup tridiagonal reduction against rocSOLVER's
+    zheevd throughout: the same eigenvalues, and eigenvectors that satisfy
+    H v = e S v and are S-orthonormal."""
+    name, g, _ = si2
+    if name != "si2_pbe":
+        pytest.skip("one dataset suffices")
+    for frac in (0.3, 0.45):
+        ctx = H.Context(*SI2, frac * g["encut"], g["xc"], H.BACKEND_CUDA, H.SOLVER_EVD)
+        try:
+            h, s = ctx.assemble_hs(g["kpoint"])
+            n = h.shape[0]
+            assert n <= 256
+            nb = min(16, n)
+            out = {}
+            for on in (1, 0):
+                H.set_tuning("solver.hetrd", on)
+                try:
+                    out[on] = ctx.solve(nb, g["kpoint"], vectors=True)
+                finally:
+                    H.set_tuning("solver.hetrd", 1)
+        finally:
+            ctx.close()
+        w1, v1 = out[1][0], out[1][1]
+        print(f"[gpu] order {n}: |dw| {np.abs(w1 - out[0][0]).max():.2e}", flush=True)
+        assert np.abs(w1 - out[0][0]).max() < 1e-10
+        assert np.abs(w1 - np.linalg.eigvalsh(np.linalg.solve(np.linalg.cholesky(s), h) @
+                                              np.linalg.inv(np.linalg.cholesky(s)).conj().T)[:nb]).max() < 1e-9
+        assert np.abs(h @ v1 - (s @ v1) * w1[None, :]).max() < 1e-9
+        assert np.abs(v1.conj().T @ s @ v1 - np.eye(nb)).max() < 1e-11
+
+
 def test_acc_abi_defaults(si2):
     """ACC through half_solve_kpoint uses the reference's fixed controls
     (tol 1e-4 eV, 40 iterations); its eigenvalue error is second order in
@@ -335,12 +368,21 @@ def test_hfo2_forces_finite_difference(hfo2):
 def test_hfo2_acc_fp64_and_mixed(hfo2):
     g, ctx = hfo2
     ref = np.array(g["eigenvalues"])
-    w, _, st = ctx.solve_acc(64, tol=1e-6, maxiter=300, extra=8)
-    print("[gpu] hfo2 fp64 acc", st, np.abs(w - ref).max(), flush=True)
-    assert np.abs(w - ref).max() < 1e-9
+    w64, _, st64 = ctx.solve_acc(64, tol=1e-6, maxiter=300, extra=8)
+    print("[gpu] hfo2 fp64 acc", st64, np.abs(w64 - ref).max(), flush=True)
+    assert np.abs(w64 - ref).max() < 1e-9
     w, _, st = ctx.solve_acc(64, mixed=True, tol=1e-6, maxiter=300, extra=8)
     print("[gpu] hfo2 mixed acc", st, np.abs(w - ref).max(), flush=True)
     assert np.abs(w - ref).max() < 1e-8
+    # the 144-wide Rayleigh-Ritz problems on rocSOLVER's zheevd throughout
+    H.set_tuning("solver.hetrd", 0)
+    try:
+        w0, _, st0 = ctx.solve_acc(64, tol=1e-6, maxiter=300, extra=8)
+    finally:
+        H.set_tuning("solver.hetrd", 1)
+    print("[gpu] hfo2 fp64 acc, library eigensolver", st0, flush=True)
+    assert np.abs(w0 - ref).max() < 1e-9
+    assert np.abs(w0 - w64).max() < 1e-10
 
 
 # --------------------------------------------------------------------------
