@@ -174,7 +174,10 @@ int half_amd_trace_read(int max_kernels, char *names, double *total_ms, int64_t 
  * LDS); "fp32" keys for the FP32 operator; "operator.overlap" (projector
  * stage on a second stream) and "operator.projector" (1: projector stage on
  * the skinny 16x16-tile kernels, 0: on the 64-wide contractions with a
- * separate D/Q mix).  Process-wide. */
+ * separate D/Q mix); "solver.hetrd" (1: Hermitian eigensolves with vectors
+ * of order <= 256 reduce to tridiagonal form in one workgroup before
+ * rocSOLVER's zstedc / zunmtr, 0: rocSOLVER's zheevd throughout).
+ * Process-wide. */
 int half_amd_set_tuning(const char *key, int value, char *error, int error_capacity);
 int half_amd_get_tuning(const char *key, int *value, char *error, int error_capacity);
 
