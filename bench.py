@@ -176,6 +176,16 @@ def main():
                                args.workload)
     roofline32 = kernel_roofline(kernels[32], nr, info["grid"], n, nproj, nb, args.steps, nlines, nplanes, False,
                                  args.workload)
+    # every operator kernel against its own bound (MFMA for the projector
+    # contractions, HBM for the local stage)
+    per_kernel = {}
+    for prec in kernels:
+        per_kernel[str(prec)] = {}
+        for kname in kernels[prec]:
+            r = kernel_roofline(kernels[prec], nr, info["grid"], n, nproj, nb, args.steps, nlines, nplanes,
+                                prec == 64, args.workload, name=kname)
+            per_kernel[str(prec)][kname] = {q: r.get(q) for q in ("bound", "avg_launch_ms", "achieved", "unit", "frac",
+                                                              "traffic")}
     b_alg = algorithmic_bytes(nr, n, True)
     f_proj = 24.0 * n * nproj
     t_band = per_step[64] * 1e-3 / nb
@@ -200,6 +210,7 @@ def main():
         "roofline": roofline,
         "roofline_fp32": roofline32,
         "kernels_ms_per_step": {str(p): {k: v[0] / args.steps for k, v in kernels[p].items()} for p in kernels},
+        "kernel_rooflines": per_kernel,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -252,12 +263,14 @@ def measured_traffic(workload, nb, fp64, name):
     return t["bytes_per_launch"].get("fp64" if fp64 else "fp32", {}).get(name)
 
 
-def kernel_roofline(kern, nr, grid, n, nproj, nb, steps, nlines, nplanes, fp64=True, workload=None):
-    """Roofline of the kernel with the largest share of the traced time:
-    its algorithmic bytes (or flops) over its summed launch durations."""
+def kernel_roofline(kern, nr, grid, n, nproj, nb, steps, nlines, nplanes, fp64=True, workload=None, name=None):
+    """Roofline of one traced kernel (default: the one with the largest share
+    of the traced time): its algorithmic bytes (or flops) over its summed
+    launch durations."""
     sc, sr = (16, 8) if fp64 else (8, 4)
     per_step_flops = {"proj_coeff_mfma": 8.0 * n * nproj * nb, "proj_expand_mfma": 16.0 * n * nproj * nb}
-    name = max(kern, key=lambda k: kern[k][0])
+    if name is None:
+        name = max(kern, key=lambda k: kern[k][0])
     total_ms, launches = kern[name]
     t = total_ms * 1e-3
     out = {"kernel": name, "avg_launch_ms": total_ms / launches, "launches_per_step": launches / steps}
