@@ -84,10 +84,11 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs<R> g) {
       for (int j = 0; j < NB; ++j) rb[j][i] = (rv && cb) ? pb[j][r] : mk<V>(0, 0);
     }
   };
-  A4 acc[NB][WT][WT][2];
+  A4 acc[NB][WT][WT][kCmacParts<R>];
   for (int j = 0; j < NB; ++j)
     for (int t = 0; t < WT; ++t)
-      for (int u = 0; u < WT; ++u) acc[j][t][u][0] = acc[j][t][u][1] = A4{0, 0, 0, 0};
+      for (int u = 0; u < WT; ++u)
+        for (int q = 0; q < kCmacParts<R>; ++q) acc[j][t][u][q] = A4{0, 0, 0, 0};
   if (r_begin < r_end) load(r_begin);
   for (int r0 = r_begin; r0 < r_end; r0 += KS) {
     __syncthreads();
@@ -112,7 +113,7 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs<R> g) {
       for (int j = 0; j < NB; ++j)
         for (int u = 0; u < WT; ++u) {
           const R br = sb[j][0][kk][wb + 16 * u + row], bi = sb[j][1][kk][wb + 16 * u + row];
-          for (int t = 0; t < WT; ++t) cmfma<R>(ar[t], ai[t], br, bi, acc[j][t][u][0], acc[j][t][u][1]);
+          for (int t = 0; t < WT; ++t) cmac<R>(ar[t], ai[t], br, bi, acc[j][t][u]);
         }
     }
   }
@@ -124,9 +125,11 @@ __global__ __launch_bounds__(256) void gram_kernel(GramArgs<R> g) {
         if (cb2 >= g.kb) continue;
         for (int i = 0; i < 4; ++i) {
           const int ra2 = a0 + wa + 16 * t + mfma_row<R>(lane, i);
-          if (ra2 < g.ka)
-            g.out[j][bz * g.bOut[j] + off + ra2 + int64_t(cb2) * g.ldo[j]] =
-                mk<V>(acc[j][t][u][0][i], acc[j][t][u][1][i]);
+          if (ra2 < g.ka) {
+            R re, im;
+            cmac_result<R>(acc[j][t][u], i, re, im);
+            g.out[j][bz * g.bOut[j] + off + ra2 + int64_t(cb2) * g.ldo[j]] = mk<V>(re, im);
+          }
         }
       }
 }
@@ -228,10 +231,11 @@ __global__ __launch_bounds__(256) void tall_kernel(TallArgs<R> g) {
       }
     }
   };
-  A4 acc[NB][WT][WT][2];
+  A4 acc[NB][WT][WT][kCmacParts<R>];
   for (int j = 0; j < NB; ++j)
     for (int t = 0; t < WT; ++t)
-      for (int u = 0; u < WT; ++u) acc[j][t][u][0] = acc[j][t][u][1] = A4{0, 0, 0, 0};
+      for (int u = 0; u < WT; ++u)
+        for (int q = 0; q < kCmacParts<R>; ++q) acc[j][t][u][q] = A4{0, 0, 0, 0};
   load(0);
   for (int k0 = 0; k0 < g.k; k0 += KS) {
     __syncthreads();
@@ -256,7 +260,7 @@ __global__ __launch_bounds__(256) void tall_kernel(TallArgs<R> g) {
       for (int j = 0; j < NB; ++j)
         for (int u = 0; u < WT; ++u) {
           const R yr = sw[j][0][kk][wc + 16 * u + row], yi = sw[j][1][kk][wc + 16 * u + row];
-          for (int t = 0; t < WT; ++t) cmfma<R>(xr[t], xi[t], yr, yi, acc[j][t][u][0], acc[j][t][u][1]);
+          for (int t = 0; t < WT; ++t) cmac<R>(xr[t], xi[t], yr, yi, acc[j][t][u]);
         }
     }
   }
@@ -282,7 +286,10 @@ __global__ __launch_bounds__(256) void tall_kernel(TallArgs<R> g) {
         for (int i = 0; i < 4; ++i) {
           const int r = g0 + wr + 16 * t + mfma_row<R>(lane, i);
           if (r >= g.n) continue;
-          R vr = g.alpha * acc[j][t][u][0][i], vi = g.alpha * acc[j][t][u][1][i];
+          R vr, vi;
+          cmac_result<R>(acc[j][t][u], i, vr, vi);
+          vr *= g.alpha;
+          vi *= g.alpha;
           if (g.beta != R(0)) {
             vr += g.beta * old[u][t][i].x;
             vi += g.beta * old[u][t][i].y;
