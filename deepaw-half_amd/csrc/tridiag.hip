@@ -14,6 +14,11 @@
 // that column is updated ahead of the pass.  Lanes own rows (column-major
 // loads are coalesced), waves own columns; the eight per-wave partial
 // products are added in a fixed order, so results do not depend on timing.
+// A step has three barriers: wave 0 forms the reflector from the column the
+// previous pass left in LDS, all waves make the pass, every thread forms
+// its row of w.  The pass streams the block through one CU (about 1 ms at
+// order 144, against 2 ms for rocSOLVER's reduction); orders above 256 are
+// left to rocSOLVER.
 #include "cplx.hpp"
 #include "tridiag.hpp"
 
