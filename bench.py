@@ -160,6 +160,7 @@ def main():
             H.trace(False)
             solve[key] = {"seconds": el, "iterations": st["iterations"], "fp64_iterations": st["fp64_iterations"],
                           "max_residual_eV": st["max_residual_eV"], "band_applications": st["band_applications"],
+                          "rr_order_mean": st["rr_order_mean"],
                           "phase_ms": {p: round(v[0], 3) for p, v in sorted(phases.items(), key=lambda x: -x[1][0])}}
             if g is not None and len(g["eigenvalues"]) >= nb:
                 solve[key]["max_abs_err_vs_reference_eV"] = float(np.abs(wv - np.array(g["eigenvalues"][:nb])).max())

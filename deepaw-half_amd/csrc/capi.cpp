@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -733,7 +734,7 @@ int half_amd_solve_acc(half_handle handle, const double kpoint[3], int nbands, i
       stats->seconds = res.seconds_solver;
       stats->band_applications = res.applies;
       stats->metric_fallbacks = res.metric_fallbacks;
-      stats->reserved = 0;
+      stats->rr_order_mean = static_cast<int32_t>(std::lround(res.rr_order_mean));
     }
     return int(kSuccess);
   });
@@ -784,7 +785,7 @@ int half_amd_solve_kmesh(half_handle handle, int nk, const double *kpoints, int 
       stats->seconds = res.seconds_solver;
       stats->band_applications = res.applies;
       stats->metric_fallbacks = res.metric_fallbacks;
-      stats->reserved = 0;
+      stats->rr_order_mean = static_cast<int32_t>(std::lround(res.rr_order_mean));
     }
     return int(kSuccess);
   });

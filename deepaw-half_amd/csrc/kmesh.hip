@@ -611,6 +611,7 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
   res.iterations = std::min(iter, ctl.maxiter);
   res.fp64_iterations = fp64_iters;
   res.residual = max_res;
+  res.rr_order_mean = m2;  // every k point keeps its full [X P] subspace
   std::vector<double> wall(size_t(nk) * m2);
   HALF_HIP(hipMemcpyAsync(wall.data(), w.get(), sizeof(double) * wall.size(), hipMemcpyDeviceToHost, s));
   HALF_HIP(hipStreamSynchronize(s));

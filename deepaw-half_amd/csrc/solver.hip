@@ -90,18 +90,21 @@ __global__ void column_norm2_kernel(const double2 *r, int64_t ld, int n, double 
 
 // P = -R/(|T - eps|+1), or a unit seed vector for converged bands
 // (reference: src/half_cuda_iterative_solver.cuf:42-64)
-__global__ void precondition_kernel(const double2 *r, const double *kin, const double *w, const int *seeds,
-                                    double2 *p, int n, int m, int64_t ld) {
+// Column c of P is the correction of band cols[c] (c < p): a unit vector at
+// plane wave seeds[c] where that is set, else the preconditioned residual.
+__global__ void precondition_kernel(const double2 *r, const double *kin, const double *w, const int *cols,
+                                    const int *seeds, double2 *p, int n, int np, int64_t ld) {
   const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (t >= int64_t(n) * m) return;
-  const int b = static_cast<int>(t / n);
-  const int g = static_cast<int>(t - int64_t(b) * n);
-  const int64_t o = g + b * ld;
-  if (seeds[b] >= 0) {
-    p[o] = make_double2(g == seeds[b] ? 1.0 : 0.0, 0.0);
+  if (t >= int64_t(n) * np) return;
+  const int c = static_cast<int>(t / n);
+  const int g = static_cast<int>(t - int64_t(c) * n);
+  const int b = cols[c];
+  const int64_t o = g + c * ld, ro = g + b * ld;
+  if (seeds[c] >= 0) {
+    p[o] = make_double2(g == seeds[c] ? 1.0 : 0.0, 0.0);
   } else {
     const double d = fabs(kin[g] - w[b]) + 1.0;
-    p[o] = make_double2(-r[o].x / d, -r[o].y / d);
+    p[o] = make_double2(-r[ro].x / d, -r[ro].y / d);
   }
 }
 
@@ -152,6 +155,11 @@ void check_info(const DeviceArray<rocblas_int> &info, hipStream_t s, const char 
 }
 
 bool g_hetrd = true;
+// Soft locking in the all-band solve: a band whose residual already meets
+// the tolerance contributes no correction column to the next Rayleigh-Ritz
+// subspace (it stays in X).  Off: every band keeps its column, the
+// reference's 2m-wide subspace.
+bool g_lock = true;
 
 // zheevd with vectors (upper triangle of A referenced; eigenvectors replace
 // A, eigenvalues ascending in w; e is workspace of n values).  Orders up to
@@ -211,14 +219,16 @@ double now() {
 }  // namespace
 
 bool set_solver_tuning(const std::string &key, int value) {
-  if (key != "solver.hetrd" || value < 0 || value > 1) return false;
-  g_hetrd = value != 0;
+  bool *flag = key == "solver.hetrd" ? &g_hetrd : key == "solver.lock" ? &g_lock : nullptr;
+  if (!flag || value < 0 || value > 1) return false;
+  *flag = value != 0;
   return true;
 }
 
 bool get_solver_tuning(const std::string &key, int &value) {
-  if (key != "solver.hetrd") return false;
-  value = g_hetrd ? 1 : 0;
+  const bool *flag = key == "solver.hetrd" ? &g_hetrd : key == "solver.lock" ? &g_lock : nullptr;
+  if (!flag) return false;
+  value = *flag ? 1 : 0;
   return true;
 }
 
@@ -349,7 +359,7 @@ struct AccWork {
   DeviceArray<double2> Z, HZ, SZ, R, tmp, cross, metric, hsub, ssub, gwork;
   DeviceArray<float2> p32, hp32, sp32;
   DeviceArray<double> w, mu, E, res2, kin, dscale, pivots;
-  DeviceArray<int> seeds;
+  DeviceArray<int> seeds, cols;
   DeviceArray<rocblas_int> info;
 };
 
@@ -385,6 +395,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   wk.E.resize(m2);
   wk.res2.resize(m);
   wk.seeds.resize(m);
+  wk.cols.resize(m);
   wk.info.resize(2);
   wk.dscale.resize(m);
   wk.pivots.resize(m + 1);
@@ -401,19 +412,20 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   bool fp64_phase = !ctl.mixed;
   res.applies = 0;
 
-  auto apply = [&](double2 *in, double2 *hout, double2 *sout) {
+  // H and S applied to `cnt` columns
+  auto apply = [&](int cnt, double2 *in, double2 *hout, double2 *sout) {
     if (fp64_phase) {
-      op64.apply(m, in, ld, hout, sout, ld, s);
+      op64.apply(cnt, in, ld, hout, sout, ld, s);
     } else {
-      convert_states(in, ld, wk.p32.get(), ld, n, m, s);
-      op32->apply(m, wk.p32.get(), ld, wk.hp32.get(), wk.sp32.get(), ld, s);
-      convert_states(wk.hp32.get(), ld, hout, ld, n, m, s);
-      convert_states(wk.sp32.get(), ld, sout, ld, n, m, s);
+      convert_states(in, ld, wk.p32.get(), ld, n, cnt, s);
+      op32->apply(cnt, wk.p32.get(), ld, wk.hp32.get(), wk.sp32.get(), ld, s);
+      convert_states(wk.hp32.get(), ld, hout, ld, n, cnt, s);
+      convert_states(wk.sp32.get(), ld, sout, ld, n, cnt, s);
     }
-    res.applies += m;
+    res.applies += cnt;
   };
-  auto copy = [&](double2 *dst, const double2 *src) {
-    HALF_HIP(hipMemcpyAsync(dst, src, sizeof(double2) * size_t(n) * m, hipMemcpyDeviceToDevice, s));
+  auto copy = [&](double2 *dst, const double2 *src, int cnt) {
+    HALF_HIP(hipMemcpyAsync(dst, src, sizeof(double2) * size_t(n) * cnt, hipMemcpyDeviceToDevice, s));
   };
   // generalized k x k problem in (hsub, ssub), vectors in hsub, values in w
   auto small_generalized = [&](int k) {
@@ -428,13 +440,13 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   auto ritz_rotate = [&](int k) {
     zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, k, one, wk.Z.get(), ld, wk.hsub.get(), k,
           zero, wk.tmp.get(), ld);
-    copy(X, wk.tmp.get());
+    copy(X, wk.tmp.get(), m);
     zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, k, one, wk.HZ.get(), ld, wk.hsub.get(), k,
           zero, wk.tmp.get(), ld);
-    copy(HX, wk.tmp.get());
+    copy(HX, wk.tmp.get(), m);
     zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, k, one, wk.SZ.get(), ld, wk.hsub.get(), k,
           zero, wk.tmp.get(), ld);
-    copy(SX, wk.tmp.get());
+    copy(SX, wk.tmp.get(), m);
   };
   auto rayleigh_ritz_x = [&]() {
     gram_hs(m, X, HX, SX);
@@ -451,12 +463,14 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     hipLaunchKernelGGL(init_states_kernel, dim3(blocks_for(int64_t(n) * m)), dim3(kThreads), 0, s, X, ld,
                        d_idx.get(), n, m);
     HALF_HIP(hipGetLastError());
-    apply(X, HX, SX);
+    apply(m, X, HX, SX);
     rayleigh_ritz_x();
     HALF_HIP(hipStreamSynchronize(s));
   }
   std::vector<double> r2(m);
-  std::vector<int> seeds(m);
+  std::vector<int> seeds(m), cols(m);
+  int64_t order_sum = 0;
+  int rr_steps = 0;
   std::vector<double> pivots(m + 1);
   res.metric_fallbacks = 0;
   res.history.clear();
@@ -484,7 +498,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     if (!fp64_phase && max_res <= std::max(ctl.switch_tol, ctl.tol)) {
       // FP64 correction sweeps: exact HX, SX for the current subspace
       fp64_phase = true;
-      apply(X, HX, SX);
+      apply(m, X, HX, SX);
       rayleigh_ritz_x();
       --iter;
       continue;
@@ -493,19 +507,37 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
       converged = true;
       break;
     }
+    // correction columns: every band (the reference), or with soft locking
+    // only the buffer bands and the requested bands above the current
+    // phase's target (tol; switch_tol in the FP32 phase of the mixed form).
+    // The tests above still read every requested band, so a locked band
+    // whose residual grows past the target gets its column back.
+    const double lock_at = fp64_phase ? ctl.tol : std::max(ctl.switch_tol, ctl.tol);
+    int np = 0;
+    for (int j = 0; j < m; ++j) {
+      if (g_lock && j < mreq && std::sqrt(std::max(0.0, r2[j])) <= lock_at) continue;
+      cols[np] = j;
+      seeds[np] = seeds[j];
+      ++np;
+    }
+    if (np == 0) break;  // unreachable: the test above stops first
+    order_sum += m + np;
+    ++rr_steps;
     trace_mark("acc_precondition", s);
     wk.seeds.upload(seeds, s);
-    hipLaunchKernelGGL(precondition_kernel, dim3(blocks_for(int64_t(n) * m)), dim3(kThreads), 0, s, wk.R.get(),
-                       wk.kin.get(), wk.w.get(), wk.seeds.get(), P, n, m, ld);
+    wk.cols.upload(cols, s);
+    hipLaunchKernelGGL(precondition_kernel, dim3(blocks_for(int64_t(n) * np)), dim3(kThreads), 0, s, wk.R.get(),
+                       wk.kin.get(), wk.w.get(), wk.cols.get(), wk.seeds.get(), P, n, np, ld);
     HALF_HIP(hipGetLastError());
-    apply(P, HP, SP);
-    // S-orthogonalise the correction block against X
+    apply(np, P, HP, SP);
+    // S-orthogonalise the correction block against X (cross: m x np)
     trace_mark("acc_orthogonalise_gemm", s);
-    gram<double>(n, m, m, X, ld, SP, ld, wk.cross.get(), m, nullptr, 0, nullptr, 0, wk.gwork, s);
-    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, m, mone, X, ld, wk.cross.get(), m, one, P, ld);
-    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, m, mone, HX, ld, wk.cross.get(), m, one, HP,
+    gram<double>(n, m, np, X, ld, SP, ld, wk.cross.get(), m, nullptr, 0, nullptr, 0, wk.gwork, s);
+    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, np, m, mone, X, ld, wk.cross.get(), m, one, P,
           ld);
-    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, m, mone, SX, ld, wk.cross.get(), m, one, SP,
+    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, np, m, mone, HX, ld, wk.cross.get(), m, one, HP,
+          ld);
+    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, np, m, mone, SX, ld, wk.cross.get(), m, one, SP,
           ld);
     // normalise it in the S metric.  With D = diag(P^H S P)^-1/2 and
     // D P^H S P D = U^H U, P <- P D U^-1 spans the same space as the
@@ -514,60 +546,62 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     // pairs.  It is used when the unit-diagonal factor's squared pivots all
     // exceed 1e-10 (no column within 1e-5 of the span of the others);
     // otherwise the reference's eigen-decomposition form runs.
-    gram<double>(n, m, m, P, ld, SP, ld, wk.metric.get(), m, nullptr, 0, nullptr, 0, wk.gwork, s, true);
+    gram<double>(n, np, np, P, ld, SP, ld, wk.metric.get(), np, nullptr, 0, nullptr, 0, wk.gwork, s, true);
     trace_mark("acc_metric_cholesky", s);
-    hipLaunchKernelGGL(unit_diagonal_kernel, dim3(blocks_for(m)), dim3(kThreads), 0, s, wk.metric.get(),
-                       wk.dscale.get(), m);
-    hipLaunchKernelGGL(scale_both_kernel, dim3(blocks_for(int64_t(m) * m)), dim3(kThreads), 0, s, wk.metric.get(),
-                       wk.dscale.get(), m);
+    hipLaunchKernelGGL(unit_diagonal_kernel, dim3(blocks_for(np)), dim3(kThreads), 0, s, wk.metric.get(),
+                       wk.dscale.get(), np);
+    hipLaunchKernelGGL(scale_both_kernel, dim3(blocks_for(int64_t(np) * np)), dim3(kThreads), 0, s,
+                       wk.metric.get(), wk.dscale.get(), np);
     HALF_HIP(hipGetLastError());
-    HALF_BLAS(rocsolver_zpotrf(blas.h, rocblas_fill_upper, m, rz(wk.metric.get()), m, wk.info.get()));
-    hipLaunchKernelGGL(pivots_kernel, dim3(blocks_for(m)), dim3(kThreads), 0, s, wk.metric.get(), wk.info.get(),
-                       wk.pivots.get(), m);
+    HALF_BLAS(rocsolver_zpotrf(blas.h, rocblas_fill_upper, np, rz(wk.metric.get()), np, wk.info.get()));
+    hipLaunchKernelGGL(pivots_kernel, dim3(blocks_for(np)), dim3(kThreads), 0, s, wk.metric.get(), wk.info.get(),
+                       wk.pivots.get(), np);
     HALF_HIP(hipGetLastError());
-    HALF_HIP(hipMemcpyAsync(pivots.data(), wk.pivots.get(), sizeof(double) * (m + 1), hipMemcpyDeviceToHost, s));
+    HALF_HIP(hipMemcpyAsync(pivots.data(), wk.pivots.get(), sizeof(double) * (np + 1), hipMemcpyDeviceToHost, s));
     HALF_HIP(hipStreamSynchronize(s));
-    bool chol_ok = pivots[m] == 0.0;
-    for (int j = 0; j < m && chol_ok; ++j) chol_ok = std::isfinite(pivots[j]) && pivots[j] > 1e-10;
+    bool chol_ok = pivots[np] == 0.0;
+    for (int j = 0; j < np && chol_ok; ++j) chol_ok = std::isfinite(pivots[j]) && pivots[j] > 1e-10;
     if (chol_ok) {
-      hipLaunchKernelGGL(unscale_columns_kernel, dim3(blocks_for(int64_t(m) * m)), dim3(kThreads), 0, s,
-                         wk.metric.get(), wk.dscale.get(), m);
+      hipLaunchKernelGGL(unscale_columns_kernel, dim3(blocks_for(int64_t(np) * np)), dim3(kThreads), 0, s,
+                         wk.metric.get(), wk.dscale.get(), np);
       HALF_HIP(hipGetLastError());
       trace_mark("acc_normalise_trsm", s);
       const rocblas_double_complex z1{1.0, 0.0};
       for (double2 *blk : {P, HP, SP})
         HALF_BLAS(rocblas_ztrsm(blas.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                                rocblas_diagonal_non_unit, n, m, &z1, rz(wk.metric.get()), m, rz(blk),
+                                rocblas_diagonal_non_unit, n, np, &z1, rz(wk.metric.get()), np, rz(blk),
                                 static_cast<rocblas_int>(ld)));
     } else {
       ++res.metric_fallbacks;
-      gram<double>(n, m, m, P, ld, SP, ld, wk.metric.get(), m, nullptr, 0, nullptr, 0, wk.gwork, s);
+      gram<double>(n, np, np, P, ld, SP, ld, wk.metric.get(), np, nullptr, 0, nullptr, 0, wk.gwork, s);
       trace_mark("acc_metric_zheevd", s);
-      HALF_BLAS(rocsolver_zheevd(blas.h, rocblas_evect_original, rocblas_fill_upper, m, rz(wk.metric.get()), m,
+      HALF_BLAS(rocsolver_zheevd(blas.h, rocblas_evect_original, rocblas_fill_upper, np, rz(wk.metric.get()), np,
                                  wk.mu.get(), wk.E.get(), wk.info.get()));
       check_info(wk.info, s, "ACC correction metric eigensolve");
-      hipLaunchKernelGGL(scale_columns_kernel, dim3(blocks_for(int64_t(m) * m)), dim3(kThreads), 0, s,
-                         wk.metric.get(), wk.mu.get(), m, 1e-14);
+      hipLaunchKernelGGL(scale_columns_kernel, dim3(blocks_for(int64_t(np) * np)), dim3(kThreads), 0, s,
+                         wk.metric.get(), wk.mu.get(), np, 1e-14);
       HALF_HIP(hipGetLastError());
       trace_mark("acc_normalise_gemm", s);
       for (double2 *blk : {P, HP, SP}) {
-        zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, m, one, blk, ld, wk.metric.get(), m,
+        zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, np, np, one, blk, ld, wk.metric.get(), np,
               zero, wk.tmp.get(), ld);
-        copy(blk, wk.tmp.get());
+        copy(blk, wk.tmp.get(), np);
       }
     }
     // Rayleigh-Ritz in span[X P]
+    const int k = m + np;
     trace_mark("acc_projection_gemm", s);
-    gram_hs(m2, wk.Z.get(), wk.HZ.get(), wk.SZ.get());
+    gram_hs(k, wk.Z.get(), wk.HZ.get(), wk.SZ.get());
     trace_mark("acc_subspace_eigensolve", s);
-    small_generalized(m2);
+    small_generalized(k);
     trace_mark("acc_rotate_gemm", s);
-    ritz_rotate(m2);
+    ritz_rotate(k);
     trace_close(s);
   }
   res.iterations = std::min(iter, ctl.maxiter);
   res.fp64_iterations = fp64_iters;
   res.residual = max_res;
+  res.rr_order_mean = rr_steps > 0 ? double(order_sum) / rr_steps : 0.0;
   res.eigenvalues.resize(mreq);
   HALF_HIP(hipMemcpyAsync(res.eigenvalues.data(), wk.w.get(), sizeof(double) * mreq, hipMemcpyDeviceToHost, s));
   if (want_vectors) {

@@ -29,6 +29,7 @@ struct SolveResult {
   double seconds_operator = 0, seconds_solver = 0;
   int64_t applies = 0;      // band applications performed
   int metric_fallbacks = 0; // iterations normalised by the metric's eigen-decomposition
+  double rr_order_mean = 0;  // mean order of the iterations' Rayleigh-Ritz problems (2m without locking)
   std::vector<double> history;  // max residual over the requested bands, per iteration
 };
 

@@ -63,7 +63,7 @@ class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("fp64_iterations", C.c_int32),
                 ("max_residual_eV", C.c_double), ("seconds", C.c_double),
                 ("band_applications", C.c_int64), ("metric_fallbacks", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("rr_order_mean", C.c_int32)]
 
 
 _lib = None

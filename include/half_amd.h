@@ -49,7 +49,9 @@ typedef struct half_amd_solve_stats {
   int32_t metric_fallbacks; /* iterations whose correction block was normalised
                                by the metric's eigen-decomposition instead of
                                its Cholesky factor */
-  int32_t reserved;
+  int32_t rr_order_mean;    /* mean order of the per-iteration Rayleigh-Ritz
+                               problems, rounded (2 x bands without soft
+                               locking, tuning key "solver.lock") */
 } half_amd_solve_stats;
 
 /* All-band solve with explicit controls (the reference ABI fixes tol 1e-4 eV,

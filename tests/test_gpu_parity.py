@@ -385,6 +385,33 @@ def test_hfo2_acc_fp64_and_mixed(hfo2):
     assert np.abs(w0 - w64).max() < 1e-10
 
 
+def test_hfo2_acc_soft_locking(hfo2):
+    """Soft locking (the default: bands already below the tolerance add no
+    correction column) and the reference's full 2m-wide Rayleigh-Ritz
+    subspace converge to the same eigenvalues, FP64 and mixed; locking
+    shrinks the subspace problems."""
+    g, ctx = hfo2
+    ref = np.array(g["eigenvalues"])
+    out = {}
+    for lock in (1, 0):
+        H.set_tuning("solver.lock", lock)
+        try:
+            for mixed in (False, True):
+                w, _, st = ctx.solve_acc(64, mixed=mixed, tol=1e-6, maxiter=300, extra=8)
+                print(f"[gpu] hfo2 acc lock={lock} mixed={mixed}", st, np.abs(w - ref).max(), flush=True)
+                out[lock, mixed] = (w, st)
+        finally:
+            H.set_tuning("solver.lock", 1)
+    for mixed in (False, True):
+        (wl, stl), (wf, stf) = out[1, mixed], out[0, mixed]
+        assert stf["rr_order_mean"] == 144
+        assert stl["rr_order_mean"] < 144
+        assert stl["max_residual_eV"] <= 1e-6 and stf["max_residual_eV"] <= 1e-6
+        assert np.abs(wl - ref).max() < (1e-8 if mixed else 1e-9)
+        assert np.abs(wf - ref).max() < (1e-8 if mixed else 1e-9)
+    assert H.get_tuning("solver.lock") == 1
+
+
 # --------------------------------------------------------------------------
 # potential-dependent D correction (QDEP) on the device
 # --------------------------------------------------------------------------
