@@ -331,6 +331,7 @@ def kmesh_bench(args):
                                  "iterations": st["iterations"], "fp64_iterations": st["fp64_iterations"],
                                  "max_residual_eV": st["max_residual_eV"],
                                  "band_applications": st["band_applications"],
+                                 "rr_order_mean": st["rr_order_mean"],
                                  "band_applications_per_s": st["band_applications"] / st["seconds"]}
         print(f"bench: batched {key} {out[f'batched_{key}']}", file=sys.stderr, flush=True)
     out["max_abs_diff_mixed_vs_fp64_eV"] = float(np.abs(mesh["mixed"][0] - mesh["fp64"][0]).max())

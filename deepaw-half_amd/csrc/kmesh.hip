@@ -91,15 +91,18 @@ __global__ void column_norm2_kernel(const double2 *r, int64_t ld, int N, double 
   if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
-// P = -R/(|T - eps| + 1) or a unit seed vector (reference :42-64)
+// P = -R/(|T - eps| + 1) or a unit seed vector (reference :42-64).  Column c
+// (< np) of k point k's P corrects band cols[k m + c] with seed seeds[k m + c].
 __global__ void precondition_kernel(const double2 *r, int64_t rk, const double *kin, int64_t kk, const double *w,
-                                    int64_t wk, const int *seeds, double2 *p, int64_t zk, int N, int m, int64_t ld) {
+                                    int64_t wk, const int *cols, const int *seeds, double2 *p, int64_t zk, int N,
+                                    int m, int np, int64_t ld) {
   const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (t >= int64_t(N) * m) return;
+  if (t >= int64_t(N) * np) return;
   const int k = blockIdx.y;
-  const int b = static_cast<int>(t / N), g = static_cast<int>(t - int64_t(b) * N);
-  const int64_t o = k * zk + g + b * ld;
-  const int seed = seeds[k * m + b];
+  const int c = static_cast<int>(t / N), g = static_cast<int>(t - int64_t(c) * N);
+  const int b = cols[k * m + c];
+  const int64_t o = k * zk + g + c * ld;
+  const int seed = seeds[k * m + c];
   if (seed >= 0) {
     p[o] = make_double2(g == seed ? 1.0 : 0.0, 0.0);
   } else {
@@ -358,7 +361,7 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
       ssub(size_t(nk) * gk), gvec(size_t(nk) * gk), gwork;
   DeviceArray<double> w(size_t(nk) * m2), E(size_t(nk) * m2), mu(size_t(nk) * m), res2(size_t(nk) * m),
       dscale(size_t(nk) * m), piv(size_t(nk) * (m + 1));
-  DeviceArray<int> dseeds(size_t(nk) * m);
+  DeviceArray<int> dseeds(size_t(nk) * m), dcols(size_t(nk) * m);
   DeviceArray<rocblas_int> info(size_t(3) * nk);
   DeviceArray<float2> p32, hp32, sp32;
   for (auto *a : {&Z, &HZ, &SZ, &Rr})
@@ -373,32 +376,37 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
   double2 *HX = HZ.get(), *HP = HZ.get() + rk;
   double2 *SX = SZ.get(), *SP = SZ.get() + rk;
   const rocblas_double_complex one{1, 0}, zero{0, 0};
-  const dim3 gNm(blocks_for(int64_t(N) * m), nk), gmm(blocks_for(int64_t(m) * m), nk), gm(blocks_for(m), nk);
+  const dim3 gNm(blocks_for(int64_t(N) * m), nk);
   bool fp64_phase = !ctl.mixed;
   res.applies = 0;
   res.metric_fallbacks = 0;
   res.history.clear();
 
-  auto apply = [&](double2 *in, double2 *hout, double2 *sout) {
+  // H and S applied to `cnt` columns of every k point
+  auto apply = [&](int cnt, double2 *in, double2 *hout, double2 *sout) {
     if (fp64_phase) {
-      op64.apply(m, in, ld, zk, hout, sout, ld, zk, s);
+      op64.apply(cnt, in, ld, zk, hout, sout, ld, zk, s);
     } else {
-      hipLaunchKernelGGL((convert_kernel<double2, float2>), gNm, dim3(kThreads), 0, s, in, ld, zk, p32.get(), ld, rk,
-                         N, m);
-      op32->apply(m, p32.get(), ld, rk, hp32.get(), sp32.get(), ld, rk, s);
-      hipLaunchKernelGGL((convert_kernel<float2, double2>), gNm, dim3(kThreads), 0, s, hp32.get(), ld, rk, hout, ld,
-                         zk, N, m);
-      hipLaunchKernelGGL((convert_kernel<float2, double2>), gNm, dim3(kThreads), 0, s, sp32.get(), ld, rk, sout, ld,
-                         zk, N, m);
+      const dim3 gNc(blocks_for(int64_t(N) * cnt), nk);
+      hipLaunchKernelGGL((convert_kernel<double2, float2>), gNc, dim3(kThreads), 0, s, in, ld, zk, p32.get(), ld, rk,
+                         N, cnt);
+      op32->apply(cnt, p32.get(), ld, rk, hp32.get(), sp32.get(), ld, rk, s);
+      hipLaunchKernelGGL((convert_kernel<float2, double2>), gNc, dim3(kThreads), 0, s, hp32.get(), ld, rk, hout, ld,
+                         zk, N, cnt);
+      hipLaunchKernelGGL((convert_kernel<float2, double2>), gNc, dim3(kThreads), 0, s, sp32.get(), ld, rk, sout, ld,
+                         zk, N, cnt);
       HALF_HIP(hipGetLastError());
     }
-    res.applies += int64_t(m) * nk;
+    res.applies += int64_t(cnt) * nk;
   };
-  // the correction block's H.psi: eager once per precision, then captured
-  // and replayed (same launches, same pointers every iteration)
+  // the correction block's H.psi: eager once per precision and width, then
+  // captured and replayed (same launches, same pointers every iteration).
+  // One graph per precision, captured again when the width changes; every
+  // workspace was sized by the full-width apply of the initial subspace.
   const bool use_graph = kc.graph && !trace_enabled();
   hipGraph_t graphs[2] = {nullptr, nullptr};
   hipGraphExec_t execs[2] = {nullptr, nullptr};
+  int graph_np[2] = {0, 0};
   struct GraphGuard {
     hipGraph_t *g;
     hipGraphExec_t *e;
@@ -409,22 +417,29 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
       }
     }
   } guard{graphs, execs};
-  auto apply_p = [&]() {
+  auto apply_p = [&](int np) {
     const int ph = fp64_phase ? 1 : 0;
+    if (execs[ph] && graph_np[ph] != np) {  // its last replay ended at this iteration's residual sync
+      HALF_HIP(hipGraphExecDestroy(execs[ph]));
+      execs[ph] = nullptr;
+      HALF_HIP(hipGraphDestroy(graphs[ph]));
+      graphs[ph] = nullptr;
+    }
     if (!use_graph || execs[ph]) {
       if (execs[ph]) {
         HALF_HIP(hipGraphLaunch(execs[ph], s));
-        res.applies += int64_t(m) * nk;
+        res.applies += int64_t(np) * nk;
       } else {
-        apply(P, HP, SP);
+        apply(np, P, HP, SP);
       }
       return;
     }
-    apply(P, HP, SP);  // this iteration's work; sizes every workspace
+    apply(np, P, HP, SP);  // this iteration's work, eager
     HALF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const int64_t applies0 = res.applies;
+    graph_np[ph] = np;
     try {
-      apply(P, HP, SP);
+      apply(np, P, HP, SP);
     } catch (...) {
       hipGraph_t broken = nullptr;  // leave the stream usable
       if (hipStreamEndCapture(s, &broken) == hipSuccess && broken) (void)hipGraphDestroy(broken);
@@ -468,22 +483,22 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
         throw Error(kInternal, "k-mesh ACC subspace eigensolve failed at k point " + std::to_string(j));
     }
   };
-  // blk(:, :m) <- blk(:, :k) W(:k, :m) for every k point: in place on the
-  // matrix cores when the m output columns fit one tile, else through tmp
-  auto right_multiply = [&](double2 *blk, const double2 *W, int k, int64_t ldw, int64_t sw) {
-    if (m <= gemm_tall_inplace_max) {
-      gemm_tall_batched<double>(nk, N, m, k, 1.0, blk, ld, zk, W, ldw, sw, 0.0, blk, ld, zk, nullptr, 0, 0, nullptr,
+  // blk(:, :c) <- blk(:, :k) W(:k, :c) for every k point: in place on the
+  // matrix cores when the c output columns fit one tile, else through tmp
+  auto right_multiply = [&](double2 *blk, const double2 *W, int k, int64_t ldw, int64_t sw, int c) {
+    if (c <= gemm_tall_inplace_max) {
+      gemm_tall_batched<double>(nk, N, c, k, 1.0, blk, ld, zk, W, ldw, sw, 0.0, blk, ld, zk, nullptr, 0, 0, nullptr,
                                 0, 0, s);
     } else {
-      gemm_tall_batched<double>(nk, N, m, k, 1.0, blk, ld, zk, W, ldw, sw, 0.0, tmp.get(), ld, rk, nullptr, 0, 0,
+      gemm_tall_batched<double>(nk, N, c, k, 1.0, blk, ld, zk, W, ldw, sw, 0.0, tmp.get(), ld, rk, nullptr, 0, 0,
                                 nullptr, 0, 0, s);
-      HALF_HIP(hipMemcpy2DAsync(blk, sizeof(double2) * zk, tmp.get(), sizeof(double2) * rk, sizeof(double2) * rk, nk,
-                                hipMemcpyDeviceToDevice, s));
+      HALF_HIP(hipMemcpy2DAsync(blk, sizeof(double2) * zk, tmp.get(), sizeof(double2) * rk,
+                                sizeof(double2) * int64_t(N) * c, nk, hipMemcpyDeviceToDevice, s));
     }
   };
   // X, HX, SX <- first k columns of (Z, HZ, SZ) times the lowest m Ritz vectors
   auto rotate = [&](int k, int64_t gstride) {
-    for (double2 *blk : {Z.get(), HZ.get(), SZ.get()}) right_multiply(blk, gvec.get(), k, k, gstride);
+    for (double2 *blk : {Z.get(), HZ.get(), SZ.get()}) right_multiply(blk, gvec.get(), k, k, gstride, m);
   };
   auto rayleigh_ritz_x = [&]() {
     gram_batched<double>(nk, N, m, m, X, ld, zk, HX, ld, zk, hsub.get(), m, int64_t(m) * m, SX, ld, zk, ssub.get(), m,
@@ -503,12 +518,15 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
     d_idx.upload(idx, s);
     hipLaunchKernelGGL(init_states_kernel, gNm, dim3(kThreads), 0, s, X, ld, zk, d_idx.get(), N, m);
     HALF_HIP(hipGetLastError());
-    apply(X, HX, SX);
+    apply(m, X, HX, SX);
     rayleigh_ritz_x();
     HALF_HIP(hipStreamSynchronize(s));
   }
   std::vector<double> r2(size_t(nk) * m), pivh(size_t(nk) * (m + 1));
-  std::vector<int> seeds(size_t(nk) * m);
+  std::vector<int> seeds(size_t(nk) * m), cols(size_t(nk) * m), pseeds(size_t(nk) * m);
+  const bool lock = solver_locking();
+  int64_t order_sum = 0;
+  int rr_steps = 0;
   double max_res = HUGE_VAL;
   int iter = 0, fp64_iters = 0;
   for (iter = 1; iter <= ctl.maxiter; ++iter) {
@@ -531,87 +549,126 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
     if (fp64_phase) ++fp64_iters;
     if (!fp64_phase && max_res <= std::max(ctl.switch_tol, ctl.tol)) {
       fp64_phase = true;
-      apply(X, HX, SX);
+      apply(m, X, HX, SX);
       rayleigh_ritz_x();
       --iter;
       continue;
     }
     if (fp64_phase && max_res <= ctl.tol) break;
+    // correction columns, one width np for all k points (the batched algebra
+    // and the replayed graph need one shape): without locking every band;
+    // with it, each k point's buffer bands and requested bands above the
+    // current target first, np = the largest such count over the k points
+    // rounded up to a multiple of 8, the rest of a k point's columns filled
+    // with its other bands in order (extra columns only widen its subspace)
+    const double lock_at = fp64_phase ? ctl.tol : std::max(ctl.switch_tol, ctl.tol);
+    auto active = [&](size_t o, int b) {
+      return !lock || b >= mreq || std::sqrt(std::max(0.0, r2[o + b])) > lock_at;
+    };
+    int np = m;
+    if (lock) {
+      int most = 0;
+      for (int j = 0; j < nk; ++j) {
+        int c = 0;
+        for (int b = 0; b < m; ++b) c += active(size_t(j) * m, b) ? 1 : 0;
+        most = std::max(most, c);
+      }
+      np = std::min(m, std::max(8, (most + 7) / 8 * 8));
+    }
+    for (int j = 0; j < nk; ++j) {
+      int c = 0;
+      const size_t o = size_t(j) * m;
+      for (int pass = 0; pass < 2; ++pass)
+        for (int b = 0; b < m && c < np; ++b) {
+          if (active(o, b) == (pass == 0)) {
+            cols[o + c] = b;
+            pseeds[o + c] = seeds[o + b];
+            ++c;
+          }
+        }
+    }
+    order_sum += m + np;
+    ++rr_steps;
+    const dim3 gNp(blocks_for(int64_t(N) * np), nk), gpp(blocks_for(int64_t(np) * np), nk), gp(blocks_for(np), nk);
+    const int64_t pstride = int64_t(np) * np;  // metric stride at width np
     trace_mark("acc_precondition", s);
-    dseeds.upload(seeds, s);
-    hipLaunchKernelGGL(precondition_kernel, gNm, dim3(kThreads), 0, s, Rr.get(), rk, op64.kin.get(),
-                       int64_t(op64.nmax), w.get(), int64_t(m2), dseeds.get(), P, zk, N, m, ld);
+    dseeds.upload(pseeds, s);
+    dcols.upload(cols, s);
+    hipLaunchKernelGGL(precondition_kernel, gNp, dim3(kThreads), 0, s, Rr.get(), rk, op64.kin.get(),
+                       int64_t(op64.nmax), w.get(), int64_t(m2), dcols.get(), dseeds.get(), P, zk, N, m, np, ld);
     HALF_HIP(hipGetLastError());
-    apply_p();
-    // S-orthogonalise against X
+    apply_p(np);
+    // S-orthogonalise against X (cross: m x np per k point)
     trace_mark("acc_orthogonalise", s);
-    gram_batched<double>(nk, N, m, m, X, ld, zk, SP, ld, zk, cross.get(), m, int64_t(m) * m, nullptr, 0, 0, nullptr, 0,
-                         0, gwork, s);
+    gram_batched<double>(nk, N, m, np, X, ld, zk, SP, ld, zk, cross.get(), m, int64_t(m) * m, nullptr, 0, 0, nullptr,
+                         0, 0, gwork, s);
     for (auto pr : {std::make_pair(X, P), std::make_pair(HX, HP), std::make_pair(SX, SP)})
-      gemm_tall_batched<double>(nk, N, m, m, -1.0, pr.first, ld, zk, cross.get(), m, int64_t(m) * m, 1.0, pr.second,
-                                ld, zk, nullptr, 0, 0, nullptr, 0, 0, s);
+      gemm_tall_batched<double>(nk, N, np, m, -1.0, pr.first, ld, zk, cross.get(), m, int64_t(m) * m, 1.0,
+                                pr.second, ld, zk, nullptr, 0, 0, nullptr, 0, 0, s);
     // normalise in the S metric (unit-diagonal Cholesky; eigen-decomposition
     // form when any k point's factor is ill-conditioned)
     trace_mark("acc_metric_cholesky", s);
-    gram_batched<double>(nk, N, m, m, P, ld, zk, SP, ld, zk, metric.get(), m, int64_t(m) * m, nullptr, 0, 0, nullptr,
-                         0, 0, gwork, s, true);
-    hipLaunchKernelGGL(unit_diagonal_kernel, gm, dim3(kThreads), 0, s, metric.get(), dscale.get(), m);
-    hipLaunchKernelGGL(scale_both_kernel, gmm, dim3(kThreads), 0, s, metric.get(), dscale.get(), m);
+    gram_batched<double>(nk, N, np, np, P, ld, zk, SP, ld, zk, metric.get(), np, pstride, nullptr, 0, 0, nullptr, 0,
+                         0, gwork, s, true);
+    hipLaunchKernelGGL(unit_diagonal_kernel, gp, dim3(kThreads), 0, s, metric.get(), dscale.get(), np);
+    hipLaunchKernelGGL(scale_both_kernel, gpp, dim3(kThreads), 0, s, metric.get(), dscale.get(), np);
     HALF_HIP(hipGetLastError());
-    KM_BLAS(rocsolver_zpotrf_strided_batched(blas.h, rocblas_fill_upper, m, rz(metric.get()), m, int64_t(m) * m,
+    KM_BLAS(rocsolver_zpotrf_strided_batched(blas.h, rocblas_fill_upper, np, rz(metric.get()), np, pstride,
                                              info.get(), nk));
-    hipLaunchKernelGGL(pivots_kernel, gm, dim3(kThreads), 0, s, metric.get(), info.get(), piv.get(), m);
+    hipLaunchKernelGGL(pivots_kernel, gp, dim3(kThreads), 0, s, metric.get(), info.get(), piv.get(), np);
     HALF_HIP(hipGetLastError());
-    HALF_HIP(hipMemcpyAsync(pivh.data(), piv.get(), sizeof(double) * pivh.size(), hipMemcpyDeviceToHost, s));
+    HALF_HIP(hipMemcpyAsync(pivh.data(), piv.get(), sizeof(double) * size_t(nk) * (np + 1), hipMemcpyDeviceToHost,
+                            s));
     HALF_HIP(hipStreamSynchronize(s));
     bool chol_ok = true;
     for (int j = 0; j < nk && chol_ok; ++j) {
-      const double *pj = pivh.data() + size_t(j) * (m + 1);
-      chol_ok = pj[m] == 0.0;
-      for (int b = 0; b < m && chol_ok; ++b) chol_ok = std::isfinite(pj[b]) && pj[b] > 1e-10;
+      const double *pj = pivh.data() + size_t(j) * (np + 1);
+      chol_ok = pj[np] == 0.0;
+      for (int b = 0; b < np && chol_ok; ++b) chol_ok = std::isfinite(pj[b]) && pj[b] > 1e-10;
     }
     if (chol_ok) {
-      hipLaunchKernelGGL(unscale_columns_kernel, gmm, dim3(kThreads), 0, s, metric.get(), dscale.get(), m);
+      hipLaunchKernelGGL(unscale_columns_kernel, gpp, dim3(kThreads), 0, s, metric.get(), dscale.get(), np);
       HALF_HIP(hipGetLastError());
       trace_mark("acc_normalise", s);
       // P <- P U^-1 with U^-1 explicit (a triangular factor whose scaled
       // pivots all exceed 1e-10)
-      KM_BLAS(rocsolver_ztrtri_strided_batched(blas.h, rocblas_fill_upper, rocblas_diagonal_non_unit, m,
-                                               rz(metric.get()), m, int64_t(m) * m, info.get(), nk));
-      hipLaunchKernelGGL(upper_only_kernel, gmm, dim3(kThreads), 0, s, metric.get(), m, int64_t(m) * m);
+      KM_BLAS(rocsolver_ztrtri_strided_batched(blas.h, rocblas_fill_upper, rocblas_diagonal_non_unit, np,
+                                               rz(metric.get()), np, pstride, info.get(), nk));
+      hipLaunchKernelGGL(upper_only_kernel, gpp, dim3(kThreads), 0, s, metric.get(), np, pstride);
       HALF_HIP(hipGetLastError());
-      for (double2 *blk : {P, HP, SP}) right_multiply(blk, metric.get(), m, m, int64_t(m) * m);
+      for (double2 *blk : {P, HP, SP}) right_multiply(blk, metric.get(), np, np, pstride, np);
     } else {
       ++res.metric_fallbacks;
-      gram_batched<double>(nk, N, m, m, P, ld, zk, SP, ld, zk, metric.get(), m, int64_t(m) * m, nullptr, 0, 0,
-                           nullptr, 0, 0, gwork, s, true);
+      gram_batched<double>(nk, N, np, np, P, ld, zk, SP, ld, zk, metric.get(), np, pstride, nullptr, 0, 0, nullptr,
+                           0, 0, gwork, s, true);
       trace_mark("acc_metric_zheevd", s);
-      KM_BLAS(rocsolver_zheevd_strided_batched(blas.h, rocblas_evect_original, rocblas_fill_upper, m,
-                                               rz(metric.get()), m, int64_t(m) * m, mu.get(), m, E.get(), m2,
+      KM_BLAS(rocsolver_zheevd_strided_batched(blas.h, rocblas_evect_original, rocblas_fill_upper, np,
+                                               rz(metric.get()), np, pstride, mu.get(), np, E.get(), m2,
                                                info.get(), nk));
       std::vector<rocblas_int> h(nk);
       HALF_HIP(hipMemcpyAsync(h.data(), info.get(), sizeof(rocblas_int) * nk, hipMemcpyDeviceToHost, s));
       HALF_HIP(hipStreamSynchronize(s));
       for (int j = 0; j < nk; ++j)
         if (h[j] != 0) throw Error(kInternal, "k-mesh ACC metric eigensolve failed");
-      hipLaunchKernelGGL(scale_columns_kernel, gmm, dim3(kThreads), 0, s, metric.get(), mu.get(), m, 1e-14);
+      hipLaunchKernelGGL(scale_columns_kernel, gpp, dim3(kThreads), 0, s, metric.get(), mu.get(), np, 1e-14);
       HALF_HIP(hipGetLastError());
-      for (double2 *blk : {P, HP, SP}) right_multiply(blk, metric.get(), m, m, int64_t(m) * m);
+      for (double2 *blk : {P, HP, SP}) right_multiply(blk, metric.get(), np, np, pstride, np);
     }
     // Rayleigh-Ritz in span[X P] for every k point
+    const int kz = m + np;
     trace_mark("acc_projection_gram", s);
-    gram_batched<double>(nk, N, m2, m2, Z.get(), ld, zk, HZ.get(), ld, zk, hsub.get(), m2, gk, SZ.get(), ld, zk,
-                         ssub.get(), m2, gk, gwork, s, true);
+    gram_batched<double>(nk, N, kz, kz, Z.get(), ld, zk, HZ.get(), ld, zk, hsub.get(), kz, gk, SZ.get(), ld, zk,
+                         ssub.get(), kz, gk, gwork, s, true);
     trace_mark("acc_subspace_eigensolve", s);
-    generalized(m2, gk);
+    generalized(kz, gk);
     trace_mark("acc_rotate_gemm", s);
-    rotate(m2, gk);
+    rotate(kz, gk);
     trace_close(s);
   }
   res.iterations = std::min(iter, ctl.maxiter);
   res.fp64_iterations = fp64_iters;
   res.residual = max_res;
-  res.rr_order_mean = m2;  // every k point keeps its full [X P] subspace
+  res.rr_order_mean = rr_steps > 0 ? double(order_sum) / rr_steps : 0.0;
   std::vector<double> wall(size_t(nk) * m2);
   HALF_HIP(hipMemcpyAsync(wall.data(), w.get(), sizeof(double) * wall.size(), hipMemcpyDeviceToHost, s));
   HALF_HIP(hipStreamSynchronize(s));

@@ -225,6 +225,8 @@ bool set_solver_tuning(const std::string &key, int value) {
   return true;
 }
 
+bool solver_locking() { return g_lock; }
+
 bool get_solver_tuning(const std::string &key, int &value) {
   const bool *flag = key == "solver.hetrd" ? &g_hetrd : key == "solver.lock" ? &g_lock : nullptr;
   if (!flag) return false;

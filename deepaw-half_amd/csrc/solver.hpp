@@ -59,6 +59,9 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
 // rocSOLVER's zheevd throughout.
 bool set_solver_tuning(const std::string &key, int value);
 bool get_solver_tuning(const std::string &key, int &value);
+// Whether the all-band solves drop the correction columns of converged bands
+// (tuning key "solver.lock").
+bool solver_locking();
 
 // Heap-sort order of plane waves by kinetic energy, ties as in the reference
 // (reference: src/half_cuda_iterative_solver.cuf:277-310).  Zero-based.

@@ -71,6 +71,31 @@ def test_kmesh_graph_replay_identical(si2ctx, mixed):
     assert np.array_equal(wg, we)
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_kmesh_soft_locking(si2ctx, mixed):
+    """Soft locking in the mesh solve (one correction width for all k points,
+    padded per k point with its converged bands) and the full 2m subspace
+    reach the same eigenvalues; locking narrows the subspace problems, and
+    graph replay across width changes matches eager launches bit for bit."""
+    nb = 16
+    ref = _dense(si2ctx, nb, SI2_MESH)
+    tol, bar = (1e-6, 1e-8) if mixed else (1e-7, 1e-9)
+    H.set_tuning("solver.lock", 0)
+    try:
+        wf, sf = si2ctx.solve_kmesh(nb, SI2_MESH, mixed=mixed, tol=tol, maxiter=200, extra=4)
+    finally:
+        H.set_tuning("solver.lock", 1)
+    wl, sl = si2ctx.solve_kmesh(nb, SI2_MESH, mixed=mixed, tol=tol, maxiter=200, extra=4)
+    we, se = si2ctx.solve_kmesh(nb, SI2_MESH, mixed=mixed, tol=tol, maxiter=200, extra=4, graph=False)
+    print(f"[gpu] si2 kmesh mixed={mixed} full", sf, "locked", sl, flush=True)
+    assert sf["rr_order_mean"] == 2 * (nb + 4)
+    assert sl["rr_order_mean"] < 2 * (nb + 4)
+    for w, st in ((wf, sf), (wl, sl)):
+        assert st["max_residual_eV"] <= tol
+        assert np.abs(w - ref).max() < bar
+    assert sl["iterations"] == se["iterations"] and np.array_equal(wl, we)
+
+
 def test_kmesh_overlap_matches_in_order(si2ctx):
     """The mesh operator with its projector stage on the second stream gives
     the eigenvalues of the in-order form."""
