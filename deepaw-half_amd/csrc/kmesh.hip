@@ -168,6 +168,17 @@ __global__ void scale_columns_kernel(double2 *u, const double *mu, int m, double
   v = make_double2(v.x * f, v.y * f);
 }
 
+// b_k <- a_k - b_k for m columns of N rows (a at stride za, b at stride zb)
+__global__ void difference_kernel(const double2 *a, int64_t za, double2 *b, int64_t zb, int N, int m, int64_t ld) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(N) * m) return;
+  const int k = blockIdx.y;
+  const int c = static_cast<int>(t / N), g = static_cast<int>(t - int64_t(c) * N);
+  const double2 x = a[k * za + g + c * ld];
+  double2 &y = b[k * zb + g + c * ld];
+  y = make_double2(x.x - y.x, x.y - y.y);
+}
+
 // strided precision conversion of m columns of N rows for every k point
 template <typename VI, typename VO>
 __global__ void convert_kernel(const VI *in, int64_t ldi, int64_t ski, VO *out, int64_t ldo, int64_t sko, int N,
@@ -364,7 +375,7 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
   DeviceArray<int> dseeds(size_t(nk) * m), dcols(size_t(nk) * m);
   DeviceArray<rocblas_int> info(size_t(3) * nk);
   DeviceArray<float2> p32, hp32, sp32;
-  for (auto *a : {&Z, &HZ, &SZ, &Rr})
+  for (auto *a : {&Z, &HZ, &SZ, &Rr, &tmp})
     HALF_HIP(hipMemsetAsync(a->get(), 0, sizeof(double2) * a->size(), s));
   if (ctl.mixed) {
     p32.resize(size_t(nk) * rk);
@@ -378,6 +389,7 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
   const rocblas_double_complex one{1, 0}, zero{0, 0};
   const dim3 gNm(blocks_for(int64_t(N) * m), nk);
   bool fp64_phase = !ctl.mixed;
+  double fp32_floor = 0;
   res.applies = 0;
   res.metric_fallbacks = 0;
   res.history.clear();
@@ -519,7 +531,28 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
     hipLaunchKernelGGL(init_states_kernel, gNm, dim3(kThreads), 0, s, X, ld, zk, d_idx.get(), N, m);
     HALF_HIP(hipGetLastError());
     apply(m, X, HX, SX);
+    if (!fp64_phase) {
+      // FP32 residual floor from the FP32 operator's error on these vectors
+      // (Rr, tmp are free until the loop; their padding rows are zero)
+      op64.apply(m, X, ld, zk, Rr.get(), tmp.get(), ld, rk, s);
+      hipLaunchKernelGGL(difference_kernel, gNm, dim3(kThreads), 0, s, HX, zk, Rr.get(), rk, N, m, ld);
+      hipLaunchKernelGGL(difference_kernel, gNm, dim3(kThreads), 0, s, SX, zk, tmp.get(), rk, N, m, ld);
+      hipLaunchKernelGGL(column_norm2_kernel, dim3(nk * m), dim3(kThreads), 0, s, Rr.get(), ld, N, res2.get());
+      hipLaunchKernelGGL(column_norm2_kernel, dim3(nk * m), dim3(kThreads), 0, s, tmp.get(), ld, N, mu.get());
+      HALF_HIP(hipGetLastError());
+    }
     rayleigh_ritz_x();
+    if (!fp64_phase) {
+      std::vector<double> eh(size_t(nk) * m), es(size_t(nk) * m), ev(size_t(nk) * m2);
+      HALF_HIP(hipMemcpyAsync(eh.data(), res2.get(), sizeof(double) * eh.size(), hipMemcpyDeviceToHost, s));
+      HALF_HIP(hipMemcpyAsync(es.data(), mu.get(), sizeof(double) * es.size(), hipMemcpyDeviceToHost, s));
+      HALF_HIP(hipMemcpyAsync(ev.data(), w.get(), sizeof(double) * ev.size(), hipMemcpyDeviceToHost, s));
+      HALF_HIP(hipStreamSynchronize(s));
+      std::vector<double> e(size_t(nk) * m);
+      for (int j = 0; j < nk; ++j) std::copy(ev.begin() + size_t(j) * m2, ev.begin() + size_t(j) * m2 + m,
+                                             e.begin() + size_t(j) * m);
+      fp32_floor = fp32_floor_estimate(eh, es, e);
+    }
     HALF_HIP(hipStreamSynchronize(s));
   }
   std::vector<double> r2(size_t(nk) * m), pivh(size_t(nk) * (m + 1));
@@ -527,6 +560,7 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
   const bool lock = solver_locking();
   int64_t order_sum = 0;
   int rr_steps = 0;
+  StallWatch fp32_stalled;
   double max_res = HUGE_VAL;
   int iter = 0, fp64_iters = 0;
   for (iter = 1; iter <= ctl.maxiter; ++iter) {
@@ -547,7 +581,10 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
       }
     res.history.push_back(max_res);
     if (fp64_phase) ++fp64_iters;
-    if (!fp64_phase && max_res <= std::max(ctl.switch_tol, ctl.tol)) {
+    const double fp32_target = std::max({ctl.switch_tol, ctl.tol, fp32_floor});
+    const bool fp32_stall = !fp64_phase && ((max_res < kStallArm * fp32_target && fp32_stalled.update(max_res)) ||
+                                            2 * iter > ctl.maxiter);
+    if (!fp64_phase && (fp32_stall || max_res <= fp32_target)) {
       fp64_phase = true;
       apply(m, X, HX, SX);
       rayleigh_ritz_x();
@@ -561,7 +598,7 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
     // current target first, np = the largest such count over the k points
     // rounded up to a multiple of 8, the rest of a k point's columns filled
     // with its other bands in order (extra columns only widen its subspace)
-    const double lock_at = fp64_phase ? ctl.tol : std::max(ctl.switch_tol, ctl.tol);
+    const double lock_at = fp64_phase ? ctl.tol : fp32_target;
     auto active = [&](size_t o, int b) {
       return !lock || b >= mreq || std::sqrt(std::max(0.0, r2[o + b])) > lock_at;
     };

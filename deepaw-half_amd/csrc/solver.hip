@@ -73,6 +73,31 @@ __global__ void residual_kernel(const double2 *hx, const double2 *sx, const doub
   r[o] = make_double2(hx[o].x - w[b] * sx[o].x, hx[o].y - w[b] * sx[o].y);
 }
 
+// p(g, j) = exp(i theta) with a pseudo-random phase per element (integer
+// hash of g and j): dense columns of norm sqrt(n)
+__global__ void phase_block_kernel(double2 *p, int n, int m, int64_t ld) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(n) * m) return;
+  const int j = static_cast<int>(t / n);
+  const int g = static_cast<int>(t - int64_t(j) * n);
+  uint32_t h = static_cast<uint32_t>(g) * 2654435761u ^ (static_cast<uint32_t>(j) + 0x9e3779b9u) * 2246822519u;
+  h ^= h >> 15;
+  h *= 2654435761u;
+  h ^= h >> 13;
+  double sn, cs;
+  sincos(6.283185307179586 * (h * 2.3283064365386963e-10), &sn, &cs);
+  p[g + j * ld] = make_double2(cs, sn);
+}
+
+// b <- a - b for m columns of n rows
+__global__ void difference_kernel(const double2 *a, double2 *b, int n, int m, int64_t ld) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(n) * m) return;
+  const int j = static_cast<int>(t / n);
+  const int64_t o = t - int64_t(j) * n + j * ld;
+  b[o] = make_double2(a[o].x - b[o].x, a[o].y - b[o].y);
+}
+
 // squared column norms, fixed-order per column
 __global__ void column_norm2_kernel(const double2 *r, int64_t ld, int n, double *out) {
   __shared__ double red[kThreads];
@@ -218,7 +243,25 @@ double now() {
 
 }  // namespace
 
+int g_stall = 4;
+
+int solver_stall_window() { return g_stall; }
+
+double fp32_floor_estimate(const std::vector<double> &dh2, const std::vector<double> &ds2,
+                           const std::vector<double> &eps) {
+  double h = 0, sn = 0, e = 0;
+  for (double v : dh2) h = std::max(h, std::sqrt(std::max(0.0, v)));
+  for (double v : ds2) sn = std::max(sn, std::sqrt(std::max(0.0, v)));
+  for (double v : eps) e = std::max(e, std::fabs(v));
+  return h + e * sn;
+}
+
 bool set_solver_tuning(const std::string &key, int value) {
+  if (key == "solver.stall") {
+    if (value < 0 || value > 1000) return false;
+    g_stall = value;
+    return true;
+  }
   bool *flag = key == "solver.hetrd" ? &g_hetrd : key == "solver.lock" ? &g_lock : nullptr;
   if (!flag || value < 0 || value > 1) return false;
   *flag = value != 0;
@@ -228,6 +271,10 @@ bool set_solver_tuning(const std::string &key, int value) {
 bool solver_locking() { return g_lock; }
 
 bool get_solver_tuning(const std::string &key, int &value) {
+  if (key == "solver.stall") {
+    value = g_stall;
+    return true;
+  }
   const bool *flag = key == "solver.hetrd" ? &g_hetrd : key == "solver.lock" ? &g_lock : nullptr;
   if (!flag) return false;
   value = *flag ? 1 : 0;
@@ -412,6 +459,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   double2 *SX = wk.SZ.get(), *SP = wk.SZ.get() + size_t(n) * m;
   const double2 one = make_double2(1, 0), zero = make_double2(0, 0), mone = make_double2(-1, 0);
   bool fp64_phase = !ctl.mixed;
+  double fp32_floor = 0;
   res.applies = 0;
 
   // H and S applied to `cnt` columns
@@ -466,13 +514,44 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
                        d_idx.get(), n, m);
     HALF_HIP(hipGetLastError());
     apply(m, X, HX, SX);
+    if (!fp64_phase) {
+      // FP32 residual floor: the FP32 operator's error on a dense probe
+      // block, |dH p| + max|eps| |dS p| per unit column (P, HP, SP, R and
+      // tmp are free until the loop).  Unit vectors would understate it:
+      // their projector coefficients involve no accumulation.
+      hipLaunchKernelGGL(phase_block_kernel, dim3(blocks_for(int64_t(n) * m)), dim3(kThreads), 0, s, P, n, m, ld);
+      HALF_HIP(hipGetLastError());
+      apply(m, P, HP, SP);
+      op64.apply(m, P, ld, wk.R.get(), wk.tmp.get(), ld, s);
+      res.applies += m;
+      hipLaunchKernelGGL(difference_kernel, dim3(blocks_for(int64_t(n) * m)), dim3(kThreads), 0, s, HP, wk.R.get(),
+                         n, m, ld);
+      hipLaunchKernelGGL(difference_kernel, dim3(blocks_for(int64_t(n) * m)), dim3(kThreads), 0, s, SP,
+                         wk.tmp.get(), n, m, ld);
+      hipLaunchKernelGGL(column_norm2_kernel, dim3(m), dim3(kThreads), 0, s, wk.R.get(), ld, n, wk.res2.get());
+      hipLaunchKernelGGL(column_norm2_kernel, dim3(m), dim3(kThreads), 0, s, wk.tmp.get(), ld, n, wk.mu.get());
+      HALF_HIP(hipGetLastError());
+    }
     rayleigh_ritz_x();
+    if (!fp64_phase) {
+      std::vector<double> eh(m), es(m), ev(m);
+      HALF_HIP(hipMemcpyAsync(eh.data(), wk.res2.get(), sizeof(double) * m, hipMemcpyDeviceToHost, s));
+      HALF_HIP(hipMemcpyAsync(es.data(), wk.mu.get(), sizeof(double) * m, hipMemcpyDeviceToHost, s));
+      HALF_HIP(hipMemcpyAsync(ev.data(), wk.w.get(), sizeof(double) * m, hipMemcpyDeviceToHost, s));
+      HALF_HIP(hipStreamSynchronize(s));
+      for (int j = 0; j < m; ++j) {  // per unit column: |p_j|^2 = n
+        eh[j] /= n;
+        es[j] /= n;
+      }
+      fp32_floor = fp32_floor_estimate(eh, es, ev);
+    }
     HALF_HIP(hipStreamSynchronize(s));
   }
   std::vector<double> r2(m);
   std::vector<int> seeds(m), cols(m);
   int64_t order_sum = 0;
   int rr_steps = 0;
+  StallWatch fp32_stalled;
   std::vector<double> pivots(m + 1);
   res.metric_fallbacks = 0;
   res.history.clear();
@@ -497,7 +576,10 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     }
     res.history.push_back(max_res);
     if (fp64_phase) ++fp64_iters;
-    if (!fp64_phase && max_res <= std::max(ctl.switch_tol, ctl.tol)) {
+    const double fp32_target = std::max({ctl.switch_tol, ctl.tol, fp32_floor});
+    const bool fp32_stall = !fp64_phase && ((max_res < kStallArm * fp32_target && fp32_stalled.update(max_res)) ||
+                                            2 * iter > ctl.maxiter);
+    if (!fp64_phase && (fp32_stall || max_res <= fp32_target)) {
       // FP64 correction sweeps: exact HX, SX for the current subspace
       fp64_phase = true;
       apply(m, X, HX, SX);
@@ -514,7 +596,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     // phase's target (tol; switch_tol in the FP32 phase of the mixed form).
     // The tests above still read every requested band, so a locked band
     // whose residual grows past the target gets its column back.
-    const double lock_at = fp64_phase ? ctl.tol : std::max(ctl.switch_tol, ctl.tol);
+    const double lock_at = fp64_phase ? ctl.tol : fp32_target;
     int np = 0;
     for (int j = 0; j < m; ++j) {
       if (g_lock && j < mreq && std::sqrt(std::max(0.0, r2[j])) <= lock_at) continue;

@@ -3,6 +3,9 @@
 // with FP64 or FP32-mixed H.psi.
 #pragma once
 
+#include <cmath>
+#include <vector>
+
 #include <rocblas/rocblas.h>
 
 #include "paw.hpp"
@@ -31,6 +34,43 @@ struct SolveResult {
   int metric_fallbacks = 0; // iterations normalised by the metric's eigen-decomposition
   double rr_order_mean = 0;  // mean order of the iterations' Rayleigh-Ritz problems (2m without locking)
   std::vector<double> history;  // max residual over the requested bands, per iteration
+};
+
+// FP32 H.psi has a noise floor in the residual (about 1e-4 eV on the
+// 216-atom cell); a switch_tol below it would never be reached.  A mixed
+// solve therefore estimates the floor at the start (the FP32 operator's
+// error on the initial subspace, fp32_floor_estimate) and ends the FP32
+// phase at max(switch_tol, tol, floor), or when the residual stagnates
+// within 10x of that target.  The max residual is far from monotone early
+// in a solve (it can rise a hundredfold when higher bands reorder), so the
+// stagnation rule is only armed near the target.  The measured floor can
+// still sit above the estimate (the residual also carries the FP32 error
+// accumulated in HX and SX through the rotations: on the 216-atom cell the
+// estimate stays below 1e-5 eV while the residual stalls at 2e-4), so the
+// FP32 phase also ends after half of maxiter, leaving the FP64 sweeps the
+// rest.  update() takes the max
+// residual of each armed FP32 iteration and returns true once `window`
+// iterations in a row (tuning key "solver.stall"; 0 turns the rule off)
+// have not brought it below 0.9 times the best value so far.
+constexpr double kStallArm = 10.0;
+int solver_stall_window();
+// Floor estimate from the initial subspace: max_j |dH x_j| + max_j |eps_j|
+// max_j |dS x_j| from squared column norms of the FP32 - FP64 differences.
+double fp32_floor_estimate(const std::vector<double> &dh2, const std::vector<double> &ds2,
+                           const std::vector<double> &eps);
+struct StallWatch {
+  int window = solver_stall_window();
+  double best = HUGE_VAL;
+  int since = 0;
+  bool update(double r) {
+    if (r < 0.9 * best) {
+      best = r;
+      since = 0;
+    } else {
+      ++since;
+    }
+    return window > 0 && since >= window;
+  }
 };
 
 struct AccControls {

@@ -71,6 +71,18 @@ def test_kmesh_graph_replay_identical(si2ctx, mixed):
     assert np.array_equal(wg, we)
 
 
+def test_kmesh_mixed_switch_below_fp32_floor(si2ctx):
+    """The mesh solve's FP32 phase also ends on stagnation when switch_tol is
+    out of FP32's reach."""
+    nb = 16
+    ref = _dense(si2ctx, nb, SI2_MESH)
+    w, st = si2ctx.solve_kmesh(nb, SI2_MESH, mixed=True, tol=1e-6, switch_tol=1e-12, maxiter=200)
+    print("[gpu] si2 kmesh mixed, switch below the FP32 floor", st, flush=True)
+    assert 1 <= st["fp64_iterations"] < st["iterations"]
+    assert st["max_residual_eV"] <= 1e-6
+    assert np.abs(w - ref).max() < 1e-8
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_kmesh_soft_locking(si2ctx, mixed):
     """Soft locking in the mesh solve (one correction width for all k points,

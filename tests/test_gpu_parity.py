@@ -385,6 +385,18 @@ def test_hfo2_acc_fp64_and_mixed(hfo2):
     assert np.abs(w0 - w64).max() < 1e-10
 
 
+def test_hfo2_mixed_switch_below_fp32_floor(hfo2):
+    """A switch_tol below what FP32 H.psi can reach: the FP32 phase ends when
+    its residual stagnates and the FP64 sweeps still converge the solve."""
+    g, ctx = hfo2
+    ref = np.array(g["eigenvalues"])
+    w, _, st = ctx.solve_acc(64, mixed=True, tol=1e-6, switch_tol=1e-12, maxiter=300, extra=8)
+    print("[gpu] hfo2 mixed, switch below the FP32 floor", st, np.abs(w - ref).max(), flush=True)
+    assert 1 <= st["fp64_iterations"] < st["iterations"]
+    assert st["max_residual_eV"] <= 1e-6
+    assert np.abs(w - ref).max() < 1e-8
+
+
 def test_hfo2_acc_soft_locking(hfo2):
     """Soft locking (the default: bands already below the tolerance add no
     correction column) and the reference's full 2m-wide Rayleigh-Ritz
