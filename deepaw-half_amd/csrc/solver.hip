@@ -89,6 +89,14 @@ __global__ void phase_block_kernel(double2 *p, int n, int m, int64_t ld) {
   p[g + j * ld] = make_double2(cs, sn);
 }
 
+// zero the strictly lower triangle of a k x k matrix
+__global__ void upper_only_kernel(double2 *M, int k) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(k) * k) return;
+  const int j = static_cast<int>(t / k), i = static_cast<int>(t - int64_t(j) * k);
+  if (i > j) M[t] = make_double2(0.0, 0.0);
+}
+
 // b <- a - b for m columns of n rows
 __global__ void difference_kernel(const double2 *a, double2 *b, int n, int m, int64_t ld) {
   const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
@@ -234,6 +242,41 @@ void generalized_eigh(Blas &blas, int k, double2 *A, double2 *B, double *w, doub
   if (h[0] != 0)
     throw Error(kInternal, std::string(what) + " failed: overlap not positive definite (info " +
                                std::to_string(h[0]) + ")");
+  if (h[1] != 0) throw Error(kInternal, std::string(what) + " failed (info " + std::to_string(h[1]) + ")");
+}
+
+// The same problem for the Rayleigh-Ritz steps of the all-band solve, where
+// only the lowest `nv` eigenvectors are used (written to A(:, :nv)).  At
+// orders up to kTridiagMaxOrder the triangular factor is inverted once
+// (ztrtri) and applied by matrix products: three triangular solves cost
+// about 50 us each at these orders, the inverse and three products less.
+// Larger orders take generalized_eigh.
+void generalized_eigh_ritz(Blas &blas, int k, int nv, double2 *A, double2 *B, double *w, double *e,
+                           rocblas_int *info, const char *what) {
+  if (!g_hetrd || k > kTridiagMaxOrder) {
+    generalized_eigh(blas, k, A, B, w, e, info, what);
+    return;
+  }
+  const double2 one = make_double2(1, 0), zero = make_double2(0, 0);
+  blas.eig_z.ensure(size_t(k) * k);
+  double2 *T = blas.eig_z.get();
+  HALF_BLAS(rocsolver_zpotrf(blas.h, rocblas_fill_upper, k, rz(B), k, info));
+  HALF_BLAS(rocsolver_ztrtri(blas.h, rocblas_fill_upper, rocblas_diagonal_non_unit, k, rz(B), k, info + 2));
+  hipLaunchKernelGGL(upper_only_kernel, dim3(blocks_for(int64_t(k) * k)), dim3(kThreads), 0, blas.stream, B, k);
+  HALF_HIP(hipGetLastError());
+  // A <- U^-H (A U^-1); A holds the full Hermitian matrix
+  zgemm(blas, rocblas_operation_none, rocblas_operation_none, k, k, k, one, A, k, B, k, zero, T, k);
+  zgemm(blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, k, k, k, one, B, k, T, k, zero, A, k);
+  heevd_vectors(blas, k, A, w, e, info + 1);
+  // A(:, :nv) <- U^-1 V(:, :nv)
+  zgemm(blas, rocblas_operation_none, rocblas_operation_none, k, nv, k, one, B, k, A, k, zero, T, k);
+  HALF_HIP(hipMemcpyAsync(A, T, sizeof(double2) * size_t(k) * nv, hipMemcpyDeviceToDevice, blas.stream));
+  rocblas_int h[3] = {0, 0, 0};
+  HALF_HIP(hipMemcpyAsync(h, info, sizeof(h), hipMemcpyDeviceToHost, blas.stream));
+  HALF_HIP(hipStreamSynchronize(blas.stream));
+  if (h[0] != 0 || h[2] != 0)
+    throw Error(kInternal, std::string(what) + " failed: overlap not positive definite (info " +
+                               std::to_string(h[0] != 0 ? h[0] : h[2]) + ")");
   if (h[1] != 0) throw Error(kInternal, std::string(what) + " failed (info " + std::to_string(h[1]) + ")");
 }
 
@@ -445,7 +488,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   wk.res2.resize(m);
   wk.seeds.resize(m);
   wk.cols.resize(m);
-  wk.info.resize(2);
+  wk.info.resize(3);
   wk.dscale.resize(m);
   wk.pivots.resize(m + 1);
   wk.kin.upload(op64.basis.kinetic, s);
@@ -477,10 +520,11 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   auto copy = [&](double2 *dst, const double2 *src, int cnt) {
     HALF_HIP(hipMemcpyAsync(dst, src, sizeof(double2) * size_t(n) * cnt, hipMemcpyDeviceToDevice, s));
   };
-  // generalized k x k problem in (hsub, ssub), vectors in hsub, values in w
+  // generalized k x k problem in (hsub, ssub): the lowest m eigenvectors in
+  // hsub(:, :m), values in w
   auto small_generalized = [&](int k) {
-    generalized_eigh(blas, k, wk.hsub.get(), wk.ssub.get(), wk.w.get(), wk.E.get(), wk.info.get(),
-                     "ACC small generalized eigensolve");
+    generalized_eigh_ritz(blas, k, m, wk.hsub.get(), wk.ssub.get(), wk.w.get(), wk.E.get(), wk.info.get(),
+                          "ACC small generalized eigensolve");
   };
   // (hsub, ssub) = (A^H HA, A^H SA) for k columns of Z-shaped blocks
   auto gram_hs = [&](int k, const double2 *A, const double2 *HA, const double2 *SA) {
