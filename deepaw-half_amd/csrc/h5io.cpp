@@ -245,6 +245,7 @@ void read_h5_chgcar(const std::string &path, Crystal &crystal, ChargeGrid &charg
     Id set(h.dopen(grp, "system", kDefault), h.dclose);
     Id type(h.dtype(set), h.tclose);
     check(set >= 0 && type >= 0, "HDF5: cannot open system");
+    check(dims_of(h, set).empty(), "HDF5 input: system is not a scalar string");
     if (h.tvarstr(type) > 0) {
       char *v = nullptr;
       check(h.dread(set, type, kDefault, kDefault, kDefault, &v) >= 0, "HDF5: cannot read system");
@@ -268,6 +269,10 @@ void read_h5_chgcar(const std::string &path, Crystal &crystal, ChargeGrid &charg
     Id set(h.dopen(grp, "ion_types", kDefault), h.dclose);
     Id mem(h.tcopy(*h.c_s1), h.tclose);
     check(set >= 0 && mem >= 0 && h.tset_size(mem, 16) >= 0, "HDF5 input lacks ion_types");
+    // the read fills one 16-byte slot per stored entry: the extent must be
+    // exactly the type count the buffer was sized for
+    const auto d = dims_of(h, set);
+    check(d.size() == 1 && d[0] == hsize_t(ntypes), "HDF5 input: ion_types does not hold number_ion_types entries");
     check(h.dread(set, mem, kDefault, kDefault, kDefault, species.data()) >= 0, "HDF5: cannot read ion_types");
   }
   const int direct = get<int>(h, grp, "direct_coordinates", *h.native_int, 1)[0];

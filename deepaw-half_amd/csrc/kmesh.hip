@@ -557,7 +557,7 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
   }
   std::vector<double> r2(size_t(nk) * m), pivh(size_t(nk) * (m + 1));
   std::vector<int> seeds(size_t(nk) * m), cols(size_t(nk) * m), pseeds(size_t(nk) * m);
-  const bool lock = solver_locking();
+  const bool lock = ctl.locking();
   int64_t order_sum = 0;
   int rr_steps = 0;
   StallWatch fp32_stalled;

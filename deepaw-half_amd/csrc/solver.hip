@@ -313,6 +313,8 @@ bool set_solver_tuning(const std::string &key, int value) {
 
 bool solver_locking() { return g_lock; }
 
+bool AccControls::locking() const { return lock < 0 ? g_lock : lock != 0; }
+
 bool get_solver_tuning(const std::string &key, int &value) {
   if (key == "solver.stall") {
     value = g_stall;
@@ -503,6 +505,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   const double2 one = make_double2(1, 0), zero = make_double2(0, 0), mone = make_double2(-1, 0);
   bool fp64_phase = !ctl.mixed;
   double fp32_floor = 0;
+  const bool locking = ctl.locking();
   res.applies = 0;
 
   // H and S applied to `cnt` columns
@@ -643,7 +646,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     const double lock_at = fp64_phase ? ctl.tol : fp32_target;
     int np = 0;
     for (int j = 0; j < m; ++j) {
-      if (g_lock && j < mreq && std::sqrt(std::max(0.0, r2[j])) <= lock_at) continue;
+      if (locking && j < mreq && std::sqrt(std::max(0.0, r2[j])) <= lock_at) continue;
       cols[np] = j;
       seeds[np] = seeds[j];
       ++np;

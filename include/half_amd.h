@@ -57,11 +57,18 @@ typedef struct half_amd_solve_stats {
 /* All-band solve with explicit controls (the reference ABI fixes tol 1e-4 eV,
  * 40 iterations, 16-band blocks: src/half_cuda_iterative_solver.cuf:113-119).
  * block: bands per FFT launch, 0 = all bands within a 2 GiB workspace.
- * mixed != 0 runs FP32 H.psi iterations until max residual <= switch_tol,
- * then FP64 H.psi sweeps to tol; Rayleigh-Ritz, Cholesky-type metric
- * normalisation and the subspace eigensolves are FP64 throughout.  extra
- * buffer bands are carried internally and not reported.  eigenvectors
- * (host, ld >= npw) may be null. */
+ * mixed != 0 runs FP32 H.psi iterations, then FP64 H.psi sweeps to tol.  The
+ * FP32 phase ends at the first of: max residual <= max(switch_tol (0: 2e-3),
+ * tol, the FP32 operator's error measured on a probe block at the start);
+ * no 10% gain in `solver.stall` iterations (default 4) once within 10x of
+ * that target; half of maxiter.  Rayleigh-Ritz, Cholesky-type metric
+ * normalisation and the subspace eigensolves are FP64 throughout.  Soft
+ * locking (key "solver.lock", default 1): requested bands whose residual
+ * meets the current phase's target add no correction column to the next
+ * Rayleigh-Ritz step; the stopping rule still reads every requested band.
+ * extra buffer bands are carried internally and not reported.  eigenvectors
+ * (host, ld >= npw) may be null.  ACC through half_solve_kpoint never locks
+ * (the reference's iteration). */
 int half_amd_solve_acc(half_handle handle, const double kpoint[3], int nbands, int mixed, double tol,
                        double switch_tol, int maxiter, int block, int extra, double *eigenvalues_eV,
                        HALF_COMPLEX *eigenvectors, int64_t ld, half_amd_solve_stats *stats,
@@ -77,9 +84,11 @@ int half_amd_solve_history(half_handle handle, int capacity, double *residuals_e
 /* All-band solves at nk k points (fractional, nk x 3) of one context in
  * lockstep: the H.psi of every k point is one launch per stage, the
  * Rayleigh-Ritz algebra strided-batched over k, and the correction block's
- * H.psi replayed from a captured hipGraph when graph != 0.  Controls as in
- * half_amd_solve_acc; the solve stops when every k point's requested bands
- * meet tol.  eigenvalues: nk x nbands, k-major.  stats: max_residual_eV is
+ * H.psi replayed from a captured hipGraph when graph != 0.  Controls,
+ * FP32-phase exit rules and soft locking as in half_amd_solve_acc (with
+ * locking, one correction width for all k points: the largest active count
+ * rounded up to a multiple of 8); the solve stops when every k point's
+ * requested bands meet tol.  eigenvalues: nk x nbands, k-major.  stats: max_residual_eV is
  * the largest over k, band_applications counts all k points.  The
  * reference has no batched entry; it is the loop of half_solve_kpoint over
  * k points (src/half_library.F90:120-150) at ACC (solver 4). */
@@ -131,6 +140,39 @@ int half_amd_set_solver(half_handle handle, int solver, char *error, int error_c
  * src/half_cuda_potential.cuf:338-348): e_H, e_xc, e_xc,pot in eV. */
 int half_amd_energy_terms(half_handle handle, double *e_hartree, double *e_xc,
                           double *e_xc_potential, char *error, int error_capacity);
+/* Harris energy of solved bands, as the reference CLI assembles it
+ * (app/half_cli.F90:896-920): occupations of the nk x nbands eigenvalues
+ * (k-major; k weights summing to one) for the context's valence electron
+ * count sum ZVAL x ions (src/half_energy.F90:41-120: sigma = 0 fills states
+ * in ascending order with degenerate groups sharing electrons; sigma > 0
+ * Gaussian (ismear 0) or Fermi-Dirac (ismear -1) smearing), the Ewald energy
+ * of the ions (src/half_energy.F90:122-162), local_g0 = sum psp_local(0) x
+ * count x nelect / volume and sum EATOM x count.  harris_energy = band -
+ * e_H - e_xc,pot + e_xc + ewald + local_g0 + atomic_reference; free_energy
+ * adds the entropy term.  The PAW atomic double-counting term the CLI adds
+ * for use_uspp contexts is not included.  occupations (may be null): nk x
+ * nbands, k-major.  Invalid weights, sigma, ismear or too few bands:
+ * HALF_ERROR_INVALID_ARGUMENT with the reference's message, as is an
+ * occupied highest band under smearing (the CLI stops there). */
+typedef struct half_amd_harris_terms {
+  double band_energy;
+  double fermi;
+  double entropy;
+  double e_hartree;
+  double e_xc;
+  double e_xc_potential;
+  double ewald;
+  double local_g0;
+  double atomic_reference;
+  double harris_energy;
+  double free_energy;
+  double nelect;
+} half_amd_harris_terms;
+
+int half_amd_harris_energy(half_handle handle, int nk, const double *kweights, int nbands,
+                           const double *eigenvalues_eV, double sigma, int ismear, double *occupations,
+                           half_amd_harris_terms *terms, char *error, int error_capacity);
+
 /* Veff on the C-order box (n1 slowest), eV; capacity in doubles. */
 int half_amd_get_veff(half_handle handle, int64_t capacity, double *veff, char *error,
                       int error_capacity);

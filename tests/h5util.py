@@ -122,7 +122,8 @@ def _put(lib, parent, name, kind, data, strlen=None, count=None):
 
 
 def write_vaspout(path, lattice, species, counts, positions, grid, values, potcar_text=None,
-                  group="structure/positions", direct=True, spin_axis=True, system="synthetic vaspout"):
+                  group="structure/positions", direct=True, spin_axis=True, system="synthetic vaspout",
+                  extra_types=0, system_count=None):
     """A vaspout.h5-like file: structure (fractional or Cartesian positions),
     charge (values x fastest, with or without the leading spin axis) and,
     if given, the POTCAR text at /input/potcar/content."""
@@ -136,10 +137,13 @@ def write_vaspout(path, lattice, species, counts, positions, grid, values, potca
         for p in parts:
             g = lib.H5Gcreate2(g, p.encode(), 0, 0, 0)
             opened.append(g)
-        _put(lib, g, "system", "s", system.encode(), len(system))
+        if system_count is None:
+            _put(lib, g, "system", "s", system.encode(), len(system))
+        else:  # malformed: an array of strings
+            _put(lib, g, "system", "s", system.encode() * system_count, len(system), count=system_count)
         _put(lib, g, "lattice_vectors", "f8", np.asarray(lattice, float).reshape(3, 3))
-        names = b"".join(s.encode().ljust(16) for s in species)
-        _put(lib, g, "ion_types", "s", names, 16, count=len(species))
+        names = b"".join(s.encode().ljust(16) for s in list(species) + ["X"] * extra_types)
+        _put(lib, g, "ion_types", "s", names, 16, count=len(species) + extra_types)
         _put(lib, g, "number_ion_types", "i4", np.asarray(counts))
         pos = np.asarray(positions, float).reshape(-1, 3)
         if not direct:

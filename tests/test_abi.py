@@ -71,9 +71,22 @@ def test_create_argument_validation():
     assert st == H.IO and "charge" in msg
     st, h, msg = _create(SI2[0], "/nonexistent")
     assert st == H.IO and "potential" in msg
-    # this library carries no CPU numerics and no QDEP
+    # this library carries no CPU numerics
     assert _create(*SI2, backend=H.BACKEND_CPU)[0] == H.UNAVAILABLE
-    assert _create(*SI2, uspp=1)[0] == H.UNAVAILABLE
+
+
+def test_qdep_without_partial_waves():
+    """use_uspp_dij = 1 needs the datasets' PAW radial grid and partial
+    waves (src/half_uspp.F90:142-170 integrates over them); si2.POTCAR has
+    none.  With a device the create fails with INVALID_ARGUMENT naming what
+    is missing; without one the missing device is reported first."""
+    from half_amd import hip
+    st, h, msg = _create(*SI2, uspp=1)
+    assert h == 0
+    if hip.device_count() == 0:
+        assert st == H.UNAVAILABLE and "device" in msg
+    else:
+        assert st == H.INVALID_ARGUMENT and "partial waves" in msg
 
 
 def test_escn_entry_points():

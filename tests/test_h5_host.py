@@ -40,3 +40,28 @@ def test_h5util_round_trip(tmp_path):
     assert h5util.read(path, "input/potcar/content", "s", len(pot.encode()))[0].decode() == pot
     with open(path, "rb") as fh:
         assert fh.read(8) == b"\x89HDF\r\n\x1a\n"
+
+
+@pytest.mark.skipif(not h5util.available(), reason="HDF5 1.10 C library not available")
+@pytest.mark.parametrize("bad", ["extra_types", "system_array"])
+def test_malformed_structure_rejected(tmp_path, bad):
+    """The library's vaspout.h5 reader checks every extent before reading:
+    an ion_types list longer than number_ion_types, or a system name stored
+    as an array, is an I/O error with a message (the file is read before the
+    device is touched, so this runs without a GPU)."""
+    import ctypes as C
+    import half_amd as H
+    crystal, grid, vals = O.read_chgcar(SI2[0])
+    pot = open(SI2[1]).read()
+    path = tmp_path / "bad.h5"
+    kw = {"extra_types": 3} if bad == "extra_types" else {"system_count": 4}
+    h5util.write_vaspout(path, crystal.lattice, crystal.species, crystal.counts, crystal.positions, grid, vals, pot,
+                         **kw)
+    lib = H.load()
+    h = C.c_int64(0)
+    e = C.create_string_buffer(256)
+    st = lib.half_create_from_files_backend(str(path).encode(), str(path).encode(), 400.0, H.XC_PBE, 0,
+                                            H.BACKEND_AUTO, H.SOLVER_EVD, C.byref(h), e, 256)
+    assert st == H.IO and h.value == 0
+    msg = e.value.decode()
+    assert ("ion_types" in msg) if bad == "extra_types" else ("system" in msg), msg
