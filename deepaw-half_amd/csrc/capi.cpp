@@ -22,6 +22,7 @@
 #include "../../include/half_amd.h"
 #include "common.hpp"
 #include "forces.hpp"
+#include "energy.hpp"
 #include "fset.hpp"
 #include "h5io.hpp"
 #include "kmesh.hpp"
@@ -294,7 +295,10 @@ void run_solve(Context &c, const double *k, int nbands, bool want_vectors, Devic
   KOperator<double> &op = c.op64(k);
   if (nbands < 1 || nbands > op.n) throw Error(kInvalidArgument, "invalid requested band count");
   if (c.solver == kSolverAcc) {
-    AccControls ctl;  // reference defaults: tol 1e-4 eV, 40 iterations, 16-band blocks
+    // reference controls: tol 1e-4 eV, 40 iterations, every band's correction
+    // column in each Rayleigh-Ritz step (src/half_cuda_iterative_solver.cuf:113-119)
+    AccControls ctl;
+    ctl.lock = 0;
     solve_acc(c.blas, op, nullptr, nbands, ctl, want_vectors, vec, res);
   } else {
     solve_dense(c.blas, op, c.solver, nbands, want_vectors, vec, res);
@@ -920,6 +924,35 @@ int half_amd_energy_terms(half_handle handle, double *eh, double *exc, double *e
   *exc = s.e_xc;
   *exv = s.e_xc_potential;
   return int(kSuccess);
+}
+
+int half_amd_harris_energy(half_handle handle, int nk, const double *kweights, int nbands, const double *eigenvalues,
+                           double sigma, int ismear, double *occupations, half_amd_harris_terms *terms, char *error,
+                           int capacity) {
+  clear_error(error, capacity);
+  const int slot = context_slot(handle);
+  if (!slot) return bad_handle(error, capacity);
+  return guarded(error, capacity, [&]() {
+    if (nk < 1 || nbands < 1 || !kweights || !eigenvalues || !terms)
+      return export_error(kInvalidArgument, "invalid band arguments", error, capacity);
+    Context &c = *g_contexts[slot];
+    std::vector<double> occ;
+    const HarrisTerms t = harris_energy(c.sys, nk, nbands, eigenvalues, kweights, sigma, ismear, occ, c.stream);
+    terms->band_energy = t.band_energy;
+    terms->fermi = t.fermi;
+    terms->entropy = t.entropy;
+    terms->e_hartree = t.e_hartree;
+    terms->e_xc = t.e_xc;
+    terms->e_xc_potential = t.e_xc_potential;
+    terms->ewald = t.ewald;
+    terms->local_g0 = t.local_g0;
+    terms->atomic_reference = t.atomic_reference;
+    terms->harris_energy = t.harris;
+    terms->free_energy = t.free_energy;
+    terms->nelect = t.nelect;
+    if (occupations) std::copy(occ.begin(), occ.end(), occupations);
+    return int(kSuccess);
+  });
 }
 
 int half_amd_set_projector_positions(half_handle handle, int nions, const double *frac, char *error,

@@ -195,39 +195,110 @@ double smallest_singular(const double m[3][3]) {
   return std::sqrt(std::max(0.0, std::min({a[0][0], a[1][1], a[2][2]})));
 }
 
-void ewald_forces_device(const System &sys, std::vector<double> &forces, hipStream_t s) {
-  const Crystal &cr = sys.crystal;
-  const int nions = sys.natoms;
+// Ewald energy, real-space part of atom blockIdx.x: sum over (j, lattice
+// image) within rcut of q_j erfc(alpha d) / d (reference:
+// src/half_energy.F90:143-148; the host multiplies by q_i, sums over i in
+// atom order and scales by felect / 2)
+__global__ __launch_bounds__(kThreads) void ewald_real_energy_kernel(EwaldArgs a, double *out) {
+  const int i = blockIdx.x;
+  const int side = 2 * a.rb + 1;
+  const int64_t nimg = int64_t(side) * side * side;
+  const int64_t total = nimg * a.nions;
+  double e[3] = {0, 0, 0};
+  for (int64_t t = threadIdx.x; t < total; t += blockDim.x) {
+    const int j = static_cast<int>(t / nimg);
+    const int64_t im = t - int64_t(j) * nimg;
+    const int n1 = static_cast<int>(im / (side * side)) - a.rb;
+    const int n2 = static_cast<int>((im / side) % side) - a.rb;
+    const int n3 = static_cast<int>(im % side) - a.rb;
+    double d[3];
+    for (int c = 0; c < 3; ++c)
+      d[c] = a.pos[3 * i + c] - a.pos[3 * j + c] + n1 * a.lat[c] + n2 * a.lat[3 + c] + n3 * a.lat[6 + c];
+    const double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (dist > 1e-13 && dist <= a.rcut) e[0] += a.charge[j] * erfc(a.alpha * dist) / dist;
+  }
+  double r[3];
+  block_reduce3(e, r);
+  if (threadIdx.x == 0) out[i] = r[0];
+}
+
+// Ewald energy, reciprocal part: sum over G within gcut of
+// exp(-G^2 / 4 alpha^2) |S(G)|^2 / G^2 with S the charge structure factor
+// (reference: src/half_energy.F90:149-157), one workgroup, G vectors strided
+// over its threads, fixed-order tree sum
+__global__ __launch_bounds__(kThreads) void ewald_recip_energy_kernel(EwaldArgs a, double *out) {
+  const int side = 2 * a.gb + 1;
+  const int64_t ng = int64_t(side) * side * side;
+  double e[3] = {0, 0, 0};
+  for (int64_t t = threadIdx.x; t < ng; t += blockDim.x) {
+    const int n1 = static_cast<int>(t / (side * side)) - a.gb;
+    const int n2 = static_cast<int>((t / side) % side) - a.gb;
+    const int n3 = static_cast<int>(t % side) - a.gb;
+    double g[3];
+    for (int c = 0; c < 3; ++c) g[c] = n1 * a.rec[c] + n2 * a.rec[3 + c] + n3 * a.rec[6 + c];
+    const double gn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    if (!(gn > 1e-13 && gn <= a.gcut)) continue;
+    double sr = 0, si = 0;
+    for (int j = 0; j < a.nions; ++j) {
+      const double ph = g[0] * a.pos[3 * j] + g[1] * a.pos[3 * j + 1] + g[2] * a.pos[3 * j + 2];
+      double sn, cs;
+      sincos(ph, &sn, &cs);
+      sr += a.charge[j] * cs;
+      si -= a.charge[j] * sn;
+    }
+    e[0] += exp(-gn * gn / (4 * a.alpha * a.alpha)) * (sr * sr + si * si) / (gn * gn);
+  }
+  double r[3];
+  block_reduce3(e, r);
+  if (threadIdx.x == 0) out[0] = r[0];
+}
+
+// Ewald parameters of the reference (src/half_energy.F90:131-141: eta =
+// sqrt(pi) / V^(1/3), tolerance 1e-11) and the Cartesian positions and
+// charges on the device
+struct EwaldSetup {
   EwaldArgs a{};
-  a.nions = nions;
-  a.volume = cr.volume;
-  a.alpha = std::sqrt(kPi) / std::cbrt(cr.volume);
-  const double scale = std::sqrt(-std::log(1e-11));
-  a.rcut = scale / a.alpha;
-  a.gcut = 2 * a.alpha * scale;
-  double lat[3][3], rec[3][3];
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) {
-      lat[i][j] = cr.lattice[i][j];
-      rec[i][j] = cr.reciprocal[i][j];
-      a.lat[3 * i + j] = lat[i][j];
-      a.rec[3 * i + j] = rec[i][j];
-    }
-  a.rb = static_cast<int>(std::ceil(a.rcut / smallest_singular(lat))) + 1;
-  a.gb = static_cast<int>(std::ceil(a.gcut / smallest_singular(rec))) + 1;
-  std::vector<double> charge(nions);
-  for (int at = 0; at < nions; ++at) charge[at] = sys.sets[sys.atom_species[at]].zval;
-  DeviceArray<double> d_pos, d_charge, d_real(size_t(3) * nions), d_recip(size_t(3) * nions);
-  std::vector<double> pos(size_t(3) * nions);
-  for (int at = 0; at < nions; ++at)
-    for (int j = 0; j < 3; ++j) {
-      const double *f = &cr.positions[3 * at];
-      pos[3 * at + j] = f[0] * lat[0][j] + f[1] * lat[1][j] + f[2] * lat[2][j];
-    }
-  d_pos.upload(pos, s);
-  d_charge.upload(charge, s);
-  a.pos = d_pos.get();
-  a.charge = d_charge.get();
+  DeviceArray<double> pos, charge;
+  std::vector<double> q;
+  EwaldSetup(const System &sys, hipStream_t s) {
+    const Crystal &cr = sys.crystal;
+    const int nions = sys.natoms;
+    a.nions = nions;
+    a.volume = cr.volume;
+    a.alpha = std::sqrt(kPi) / std::cbrt(cr.volume);
+    const double scale = std::sqrt(-std::log(1e-11));
+    a.rcut = scale / a.alpha;
+    a.gcut = 2 * a.alpha * scale;
+    double lat[3][3], rec[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        lat[i][j] = cr.lattice[i][j];
+        rec[i][j] = cr.reciprocal[i][j];
+        a.lat[3 * i + j] = lat[i][j];
+        a.rec[3 * i + j] = rec[i][j];
+      }
+    a.rb = static_cast<int>(std::ceil(a.rcut / smallest_singular(lat))) + 1;
+    a.gb = static_cast<int>(std::ceil(a.gcut / smallest_singular(rec))) + 1;
+    q.resize(nions);
+    for (int at = 0; at < nions; ++at) q[at] = sys.sets[sys.atom_species[at]].zval;
+    std::vector<double> p(size_t(3) * nions);
+    for (int at = 0; at < nions; ++at)
+      for (int j = 0; j < 3; ++j) {
+        const double *f = &cr.positions[3 * at];
+        p[3 * at + j] = f[0] * lat[0][j] + f[1] * lat[1][j] + f[2] * lat[2][j];
+      }
+    pos.upload(p, s);
+    charge.upload(q, s);
+    a.pos = pos.get();
+    a.charge = charge.get();
+  }
+};
+
+void ewald_forces_device(const System &sys, std::vector<double> &forces, hipStream_t s) {
+  const int nions = sys.natoms;
+  EwaldSetup setup(sys, s);
+  const EwaldArgs &a = setup.a;
+  DeviceArray<double> d_real(size_t(3) * nions), d_recip(size_t(3) * nions);
   hipLaunchKernelGGL(ewald_real_kernel, dim3(nions), dim3(kThreads), 0, s, a, d_real.get());
   hipLaunchKernelGGL(ewald_recip_kernel, dim3(nions), dim3(kThreads), 0, s, a, d_recip.get());
   HALF_HIP(hipGetLastError());
@@ -240,6 +311,32 @@ void ewald_forces_device(const System &sys, std::vector<double> &forces, hipStre
 }
 
 }  // namespace
+
+double ewald_energy_device(const System &sys, double parts[4], hipStream_t s) {
+  const int nions = sys.natoms;
+  EwaldSetup setup(sys, s);
+  const EwaldArgs &a = setup.a;
+  DeviceArray<double> d_real(nions), d_recip(1);
+  hipLaunchKernelGGL(ewald_real_energy_kernel, dim3(nions), dim3(kThreads), 0, s, a, d_real.get());
+  hipLaunchKernelGGL(ewald_recip_energy_kernel, dim3(1), dim3(kThreads), 0, s, a, d_recip.get());
+  HALF_HIP(hipGetLastError());
+  std::vector<double> er(nions);
+  double eg = 0;
+  HALF_HIP(hipMemcpyAsync(er.data(), d_real.get(), sizeof(double) * nions, hipMemcpyDeviceToHost, s));
+  HALF_HIP(hipMemcpyAsync(&eg, d_recip.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+  HALF_HIP(hipStreamSynchronize(s));
+  double re = 0, q2 = 0, qsum = 0;
+  for (int i = 0; i < nions; ++i) {
+    re += setup.q[i] * er[i];
+    q2 += setup.q[i] * setup.q[i];
+    qsum += setup.q[i];
+  }
+  parts[0] = 0.5 * kFelect * re;
+  parts[1] = 2 * kPi * kFelect / a.volume * eg;
+  parts[2] = -kFelect * a.alpha / std::sqrt(kPi) * q2;
+  parts[3] = -kFelect * kPi * qsum * qsum / (2 * a.alpha * a.alpha * a.volume);
+  return parts[0] + parts[1] + parts[2] + parts[3];
+}
 
 void ForceSetState::reset(const System &sys, hipStream_t s) {
   const int64_t nr = sys.charge.size();

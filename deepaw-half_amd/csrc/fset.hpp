@@ -36,4 +36,11 @@ void force_set_add_k(System &sys, KOperator<double> &op, int nb, const double2 *
 // forces; total = their sum (reference: app/half_cli.F90:921-962).
 void force_set_finish(System &sys, ForceSetState &state, bool want_density, ForceSetResult &out, hipStream_t s);
 
+// Ewald energy of the context's ions with the valence charges as point
+// charges (reference: src/half_energy.F90:122-162, eta = sqrt(pi) / V^(1/3),
+// tolerance 1e-11): real-space pairs per atom and the reciprocal sum on the
+// device, the atom sum and the self and background terms on the host.
+// parts: real, reciprocal, self, background.  Returns their sum (eV).
+double ewald_energy_device(const System &sys, double parts[4], hipStream_t s);
+
 }  // namespace half

@@ -40,7 +40,7 @@ EXT_SYMBOLS = (
     "half_amd_get_veff", "half_amd_nonlocal_forces", "half_amd_device_info",
     "half_amd_trace", "half_amd_trace_read", "half_amd_set_tuning", "half_amd_get_tuning",
     "half_amd_set_projector_positions", "half_amd_solve_history", "half_amd_solve_kmesh",
-    "half_amd_force_set", "half_amd_write_vaspwave",
+    "half_amd_force_set", "half_amd_write_vaspwave", "half_amd_harris_energy",
 )
 
 
@@ -64,6 +64,12 @@ class SolveStats(C.Structure):
                 ("max_residual_eV", C.c_double), ("seconds", C.c_double),
                 ("band_applications", C.c_int64), ("metric_fallbacks", C.c_int32),
                 ("rr_order_mean", C.c_int32)]
+
+
+class HarrisTerms(C.Structure):
+    _fields_ = [(name, C.c_double) for name in (
+        "band_energy", "fermi", "entropy", "e_hartree", "e_xc", "e_xc_potential", "ewald", "local_g0",
+        "atomic_reference", "harris_energy", "free_energy", "nelect")]
 
 
 _lib = None
@@ -119,6 +125,7 @@ def load(path: str | None = None):
                                pd, pd, cp, i],
         "half_amd_write_vaspwave": [i64, cp, i, pd, i, pd, pd, d, C.POINTER(C.c_void_p), C.POINTER(C.c_int64),
                                     cp, i],
+        "half_amd_harris_energy": [i64, i, pd, i, pd, d, i, pd, C.POINTER(HarrisTerms), cp, i],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -258,6 +265,20 @@ class Context:
         e = _err()
         _check(self.lib.half_amd_energy_terms(self.handle, C.byref(a), C.byref(b), C.byref(c), e, 512), e)
         return a.value, b.value, c.value
+
+    def harris_energy(self, eigenvalues, kweights=None, sigma=0.0, ismear=0):
+        """Harris energy terms of solved bands (eigenvalues nk x nbands, or one
+        k point's nbands) and their occupations (reference:
+        app/half_cli.F90:896-920).  Returns (terms dict, occupations)."""
+        eig = np.ascontiguousarray(np.atleast_2d(np.asarray(eigenvalues, float)))
+        nk, nb = eig.shape
+        w = np.ascontiguousarray(np.full(nk, 1.0 / nk) if kweights is None else np.asarray(kweights, float))
+        occ = np.zeros((nk, nb))
+        t = HarrisTerms()
+        e = _err()
+        _check(self.lib.half_amd_harris_energy(self.handle, nk, _pd(w), nb, _pd(eig), float(sigma), int(ismear),
+                                               _pd(occ), C.byref(t), e, 512), e)
+        return {name: getattr(t, name) for name, _ in HarrisTerms._fields_}, occ
 
     def veff(self):
         info = self.system_info()
