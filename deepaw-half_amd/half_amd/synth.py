@@ -201,9 +201,10 @@ def _fmt_block(values, per_line=5, fmt="%.12E"):
     return "\n".join(out)
 
 
-def write_chgcar(path: str, w: Workload, header_fmt: str = "%.12f") -> None:
-    """header_fmt formats the lattice and position numbers of the header."""
-    rho = smooth_density(w)
+def write_chgcar(path: str, w: Workload, header_fmt: str = "%.12f", rho=None) -> None:
+    """header_fmt formats the lattice and position numbers of the header;
+    rho (C-order box of CHGCAR values) defaults to smooth_density(w)."""
+    rho = smooth_density(w) if rho is None else rho
     lines = [f"synthetic {w.name}", "1.0"]
     for v in w.lattice:
         lines.append("  " + "  ".join(header_fmt % x for x in v))
@@ -405,6 +406,52 @@ def make_inputs(name: str, outdir: str, partial_waves: bool = False, partial_cor
         write_potcar(tmp, w, partial_waves, partial_core)
         os.replace(tmp, pot)
     return chg, pot, w
+
+
+# --------------------------------------------------------------------------
+# band folding: a perfect supercell and its primitive cell
+# --------------------------------------------------------------------------
+
+def write_folded_inputs(outdir: str, xc_seed: int = 20260006):
+    """Inputs of a band-folding pair at the 216-atom workload's sizes: the
+    3x3x3 supercell of the 8-atom Si cell without positional noise (216
+    atoms, L = 16.29 A, 96^3 box) and the 8-atom cell (a = 5.43 A, 32^3 box),
+    the supercell's density the 3x3x3 tiling of the primitive cell's
+    (Gaussian sum with the seeded +-1% noise), one POTCAR for both.  At
+    Gamma the supercell's spectrum is the union of the primitive cell's
+    spectra at the 27 k points (i, j, l)/3, i, j, l in {0, 1, 2}: supercell
+    G = 3 G' + (i, j, l) in the primitive reciprocal basis, and both boxes
+    enumerate the same vectors.  Returns (supercell CHGCAR, primitive
+    CHGCAR, POTCAR, k points, supercell workload)."""
+    a = 5.43
+    base = _diamond8()
+    prim = Workload("si8f", a * np.eye(3), ["Si"], [8], base, (32, 32, 32), 245.5, np.zeros((1, 3)), 40, xc_seed)
+    frac = []
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                frac.extend((base + np.array([i, j, k])) / 3.0)
+    sup = Workload("si216f", 3 * a * np.eye(3), ["Si"], [216], np.array(frac), (96, 96, 96), 245.5,
+                   np.zeros((1, 3)), 512, xc_seed)
+    os.makedirs(outdir, exist_ok=True)
+    rho_p = smooth_density(prim)
+    # CHGCAR values are rho x cell volume: the supercell holds 27 cells
+    rho_s = np.tile(rho_p, (3, 3, 3)) * 27.0
+    paths = []
+    for w, rho in ((sup, rho_s), (prim, rho_p)):
+        path = os.path.join(outdir, f"{w.name}.CHGCAR")
+        if not os.path.exists(path):
+            tmp = path + f".tmp{os.getpid()}"
+            write_chgcar(tmp, w, rho=rho)
+            os.replace(tmp, path)
+        paths.append(path)
+    pot = os.path.join(outdir, "si216f.POTCAR")
+    if not os.path.exists(pot):
+        tmp = pot + f".tmp{os.getpid()}"
+        write_potcar(tmp, sup)
+        os.replace(tmp, pot)
+    ks = np.array([[i, j, k] for i in range(3) for j in range(3) for k in range(3)], float) / 3.0
+    return paths[0], paths[1], pot, ks, sup
 
 
 # --------------------------------------------------------------------------
