@@ -860,21 +860,33 @@ def assemble_dense(veff, basis, paw):
     return h, s
 
 
-def apply_hs(veff, basis, paw, psi):
+def apply_hs(veff, basis, paw, psi, workers=None):
     """Matrix-free H.psi, S.psi (src/half_cuda_assembly.cuf:468-519):
     scatter -> inverse FFT -> * Veff/N_r -> forward FFT -> gather + T psi,
-    then + B D B^H psi and S psi = psi + B Q B^H psi."""
+    then + B D B^H psi and S psi = psi + B Q B^H psi.  workers: FFT threads
+    (scipy.fft over the whole band block); None transforms band by band with
+    numpy."""
     grid = basis.grid
     nr = veff.size
     psi = np.asarray(psi, complex)
     if psi.ndim == 1:
         psi = psi[:, None]
     hpsi = np.empty_like(psi)
-    for j in range(psi.shape[1]):
-        box = np.zeros(nr, complex)
-        box[basis.fft_index] = psi[:, j]
-        r = fft3_backward(box, grid) * (veff / nr)
-        hpsi[:, j] = fft3_forward(r, grid)[basis.fft_index] + basis.kinetic * psi[:, j]
+    if workers:
+        import scipy.fft as sf
+        nb = psi.shape[1]
+        box = np.zeros((nb, nr), complex)
+        box[:, basis.fft_index] = psi.T
+        # ifftn carries the 1/N_r of Veff/N_r applied after an unnormalised backward transform
+        r = sf.ifftn(box.reshape(nb, *grid), axes=(1, 2, 3), workers=workers) * veff.reshape(grid)
+        f = sf.fftn(r, axes=(1, 2, 3), workers=workers).reshape(nb, nr)
+        hpsi[:] = f[:, basis.fft_index].T + basis.kinetic[:, None] * psi
+    else:
+        for j in range(psi.shape[1]):
+            box = np.zeros(nr, complex)
+            box[basis.fft_index] = psi[:, j]
+            r = fft3_backward(box, grid) * (veff / nr)
+            hpsi[:, j] = fft3_forward(r, grid)[basis.fft_index] + basis.kinetic * psi[:, j]
     spsi = psi.copy()
     for sp in paw:
         nat, nlm = sp["natoms"], sp["nlm"]

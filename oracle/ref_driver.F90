@@ -10,6 +10,7 @@
 !   half_ref_driver solve CHGCAR POTCAR ENCUT lda|pbe NBANDS KX KY KZ VEFF_OUT [uspp]
 !   half_ref_driver kat
 !   half_ref_driver forces CHGCAR POTCAR ENCUT lda|pbe NBANDS KX KY KZ [uspp]
+!   half_ref_driver time CHGCAR POTCAR ENCUT lda|pbe NBANDS
 !
 ! With "uspp" the potential-dependent D correction of src/half_uspp.F90
 ! (build_uspp_dij_cpu, QDEP) is applied after the projectors are built, as
@@ -27,6 +28,10 @@
 ! augmentation occupancy and density (src/half_uspp.F90:26-111), Ewald,
 ! local, non-local (projector + dD/dR parts), NLCC and Harris-correction
 ! forces.
+! "time" runs the reference CPU context's Gamma pipeline once -- Veff
+! (build_veff_*), basis, projectors (build_paw_operators) and the dense
+! solve (solve_dense_gamma: assembly, overlap spectrum, zhegv) -- and prints
+! the wall time of each stage ("time_<stage> seconds", system_clock).
 program half_ref_driver
   use half_kinds, only: dp, i32, i64
   use half_types, only: crystal_t, charge_grid_t, plane_wave_basis_t, potcar_t
@@ -53,6 +58,8 @@ program half_ref_driver
     call run_kat()
   case ('forces')
     call run_forces()
+  case ('time')
+    call run_time()
   case default
     write(0,'(A)') 'usage: half_ref_driver solve|kat ...'
     stop 2
@@ -287,6 +294,48 @@ contains
         f_harris(i, :)
     end do
   end subroutine run_forces
+
+  subroutine run_time()
+    character(len=1024) :: chg_path, pot_path, arg
+    type(crystal_t) :: crystal
+    type(charge_grid_t) :: charge
+    type(potcar_t), allocatable :: potcars(:)
+    type(plane_wave_basis_t) :: basis
+    type(paw_species_t), allocatable :: paw(:)
+    real(dp), allocatable :: veff(:), values(:)
+    real(dp) :: encut, eh, exc, exv, smin, smax, kpt(3)
+    integer :: nbands
+    integer(i64) :: c0, c1, c2, c3, c4, rate
+    logical :: pbe
+    call get_command_argument(2, chg_path); call get_command_argument(3, pot_path)
+    call get_command_argument(4, arg); read(arg, *) encut
+    call get_command_argument(5, arg); pbe = trim(arg) == 'pbe'
+    call get_command_argument(6, arg); read(arg, *) nbands
+    kpt = 0
+    call read_chgcar(trim(chg_path), crystal, charge)
+    call read_potcar(trim(pot_path), potcars)
+    call validate_potcar_structure(potcars, crystal)
+    call system_clock(c0, rate)
+    if (pbe) then
+      call build_veff_pbe(charge, potcars, crystal, veff, eh, exc, exv)
+    else
+      call build_veff_lda(charge, potcars, crystal, veff, eh, exc, exv)
+    end if
+    call system_clock(c1)
+    call build_plane_wave_basis(crystal, charge%shape, encut, kpt, basis)
+    call system_clock(c2)
+    call build_paw_operators(potcars, crystal, basis, paw)
+    call system_clock(c3)
+    call solve_dense_gamma(veff, basis, paw, values, smin, smax)
+    call system_clock(c4)
+    write(*,'(A,I0)') 'npw ', basis%npw
+    write(*,'(A,ES25.17)') 'time_veff ', real(c1 - c0, dp)/real(rate, dp)
+    write(*,'(A,ES25.17)') 'time_basis ', real(c2 - c1, dp)/real(rate, dp)
+    write(*,'(A,ES25.17)') 'time_projectors ', real(c3 - c2, dp)/real(rate, dp)
+    write(*,'(A,ES25.17)') 'time_dense_solve ', real(c4 - c3, dp)/real(rate, dp)
+    write(*,'(A,ES25.17)') 'eig_first ', values(1)
+    write(*,'(A,ES25.17)') 'eig_last ', values(nbands)
+  end subroutine run_time
 
   subroutine run_kat()
     ! Known-answer inputs of app/half_energy_check.F90:19-72.

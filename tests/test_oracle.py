@@ -568,3 +568,15 @@ def test_local_force_is_energy_derivative():
         pm = ctx.crystal.positions[0] - h * inv[ax]
         d = (energy(pp) - energy(pm)) / (2 * h)
         assert abs(d + f[0, ax]) < 1e-6 * max(1.0, abs(f[0, ax])), ax
+
+
+def test_apply_threaded_fft_matches():
+    """The threaded-FFT form of the oracle's H.psi (the CPU baseline's) is
+    the band-by-band form to rounding."""
+    ctx = O.Context(*SI2, 400.0, "pbe")
+    basis, paw = ctx.prepare((0.1, 0.2, 0.3))
+    rng = np.random.default_rng(3)
+    psi = rng.standard_normal((basis.kinetic.size, 5)) + 1j * rng.standard_normal((basis.kinetic.size, 5))
+    h0, s0 = O.apply_hs(ctx.veff, basis, paw, psi)
+    h1, s1 = O.apply_hs(ctx.veff, basis, paw, psi, workers=4)
+    assert np.abs(h1 - h0).max() < 1e-12 * np.abs(h0).max() and np.abs(s1 - s0).max() == 0.0
