@@ -42,8 +42,10 @@ struct GramArgs {
   int upper;             // grid enumerates only tiles with ta <= tb (tiles_a per side)
 };
 
+// At least two waves per SIMD: with one (the widest FP64 forms otherwise
+// need 258 registers) every barrier and slab store leaves the matrix core idle.
 template <typename R, int WT, int NB>
-__global__ __launch_bounds__(256) void gram_kernel(GramArgs<R> g) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gram_kernel(GramArgs<R> g) {
   using V = cplx_t<R>;
   using A4 = typename Acc<R>::type;
   constexpr int TW = 32 * WT;        // workgroup tile width
@@ -196,7 +198,7 @@ struct TallArgs {
 };
 
 template <typename R, int WT, int NB>
-__global__ __launch_bounds__(256) void tall_kernel(TallArgs<R> g) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void tall_kernel(TallArgs<R> g) {
   using V = cplx_t<R>;
   using A4 = typename Acc<R>::type;
   constexpr int TW = 32 * WT;
