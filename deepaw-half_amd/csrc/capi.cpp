@@ -26,6 +26,7 @@
 #include "fset.hpp"
 #include "h5io.hpp"
 #include "kmesh.hpp"
+#include "noalloc.hpp"
 #include "paw.hpp"
 #include "qdep.hpp"
 #include "solver.hpp"
@@ -683,6 +684,31 @@ int half_amd_operator_apply(half_amd_operator op, int stage, int nstates, const 
         k->apply_local(nstates, psi, ldpsi, hp, sp, ldout, s);
       else
         k->apply_projectors(nstates, psi, ldpsi, hp, sp, ldout, s);
+    };
+    if (e.precision == 64)
+      run(e.op64.get(), static_cast<const double2 *>(d_psi), static_cast<double2 *>(d_hpsi), static_cast<double2 *>(d_spsi));
+    else
+      run(e.op32.get(), static_cast<const float2 *>(d_psi), static_cast<float2 *>(d_hpsi), static_cast<float2 *>(d_spsi));
+    return int(kSuccess);
+  });
+}
+
+int half_amd_operator_apply_noalloc(half_amd_operator op, int nstates, const void *d_psi, int64_t ldpsi,
+                                    void *d_hpsi, void *d_spsi, int64_t ldout, void *stream, char *error,
+                                    int capacity) {
+  clear_error(error, capacity);
+  auto it = g_ops.find(op);
+  if (it == g_ops.end() || !context_slot(it->second.owner))
+    return export_error(kInvalidHandle, "invalid HALF operator handle", error, capacity);
+  if (nstates < 1 || !d_psi || !d_hpsi || !d_spsi)
+    return export_error(kInvalidArgument, "invalid state block arguments", error, capacity);
+  return guarded(error, capacity, [&]() {
+    OperatorEntry &e = it->second;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : e.ctx->stream;
+    void *work = noalloc_scratch();
+    auto run = [&](auto *k, auto *psi, auto *hp, auto *sp) {
+      if (ldpsi < k->n || ldout < k->n) throw Error(kCapacity, "state leading dimension is too small");
+      k->apply_external(nstates, psi, ldpsi, hp, sp, ldout, s, work, kNoAllocScratchBytes);
     };
     if (e.precision == 64)
       run(e.op64.get(), static_cast<const double2 *>(d_psi), static_cast<double2 *>(d_hpsi), static_cast<double2 *>(d_spsi));

@@ -314,11 +314,44 @@ void launch_tall(dim3 grid, const TallArgs<R> &a, hipStream_t s) {
 
 }  // namespace
 
+namespace {
+
+// Tile width, grid and row chunking of a gram launch.
+struct GramPlan {
+  int wt, tiles_a, tiles, nchunks, chunk;
+  size_t work_elems;  // partial tiles (all outputs and batch entries); 0 without row chunks
+};
+
+GramPlan plan_gram(int batch, int n, int ka, int kb, int nout, bool hermitian) {
+  GramPlan p{};
+  // 64-wide tiles; for the subspace Grams (Hermitian, matrix-core bound)
+  // 32-wide ones where they cover the result with less padding (80 columns:
+  // 96 against 128) below a few hundred columns, where the padding of the
+  // wider tiles stops mattering and their halved operand re-reads win
+  const int kmax = std::max(ka, kb);
+  const bool pad_less_32 = (kmax + 31) / 32 * 32 < (kmax + 63) / 64 * 64;
+  p.wt = (kmax <= 32 || (hermitian && pad_less_32 && kmax < 256)) ? 1 : 2;
+  const int tw = 32 * p.wt;
+  p.tiles_a = (ka + tw - 1) / tw;
+  p.tiles = hermitian ? p.tiles_a * (p.tiles_a + 1) / 2 : p.tiles_a * ((kb + tw - 1) / tw);
+  // about a thousand workgroups over the batch, each chunk at least 64 rows
+  int nchunks = std::max(1, std::min({1024 / std::max(p.tiles * batch, 1), (n + 63) / 64, 64}));
+  int chunk = (n + nchunks - 1) / nchunks;
+  chunk = (chunk + KS - 1) / KS * KS;
+  p.nchunks = (n + chunk - 1) / chunk;
+  p.chunk = chunk;
+  p.work_elems = p.nchunks > 1 ? size_t(ka) * kb * p.nchunks * batch * nout : 0;
+  return p;
+}
+
+// gram_batched with its partial tiles in `work` (at least plan.work_elems
+// elements) or, when `grow` is given, in that array grown to fit
 template <typename R>
-void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, int64_t sA, const cplx_t<R> *B,
-                  int64_t ldb, int64_t sB, cplx_t<R> *C, int64_t ldc, int64_t sC, const cplx_t<R> *B2, int64_t ldb2,
-                  int64_t sB2, cplx_t<R> *C2, int64_t ldc2, int64_t sC2, DeviceArray<cplx_t<R>> &work, hipStream_t s,
-                  bool hermitian) {
+void gram_batched_work(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, int64_t sA,
+                       const cplx_t<R> *B, int64_t ldb, int64_t sB, cplx_t<R> *C, int64_t ldc, int64_t sC,
+                       const cplx_t<R> *B2, int64_t ldb2, int64_t sB2, cplx_t<R> *C2, int64_t ldc2, int64_t sC2,
+                       DeviceArray<cplx_t<R>> *grow, cplx_t<R> *work, size_t work_elems, hipStream_t s,
+                       bool hermitian) {
   using V = cplx_t<R>;
   if (ka <= 0 || kb <= 0 || batch <= 0) return;
   if (batch > 65535) throw Error(kInvalidArgument, "gram: batch too large");
@@ -333,21 +366,16 @@ void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t 
   }
   if (lda < n || ldb < n || ldc < ka || (B2 && (ldb2 < n || ldc2 < ka)))
     throw Error(kInvalidArgument, "gram: leading dimension too small");
-  // 64-wide tiles; for the subspace Grams (Hermitian, matrix-core bound)
-  // 32-wide ones where they cover the result with less padding (80 columns:
-  // 96 against 128) below a few hundred columns, where the padding of the
-  // wider tiles stops mattering and their halved operand re-reads win
-  const int kmax = std::max(ka, kb);
-  const bool pad_less_32 = (kmax + 31) / 32 * 32 < (kmax + 63) / 64 * 64;
-  const int wt = (kmax <= 32 || (hermitian && pad_less_32 && kmax < 256)) ? 1 : 2;
-  const int tw = 32 * wt;
-  const int tiles_a = (ka + tw - 1) / tw;
-  const int tiles = hermitian ? tiles_a * (tiles_a + 1) / 2 : tiles_a * ((kb + tw - 1) / tw);
-  // about a thousand workgroups over the batch, each chunk at least 64 rows
-  int nchunks = std::max(1, std::min({1024 / std::max(tiles * batch, 1), (n + 63) / 64, 64}));
-  int chunk = (n + nchunks - 1) / nchunks;
-  chunk = (chunk + KS - 1) / KS * KS;
-  nchunks = (n + chunk - 1) / chunk;
+  const GramPlan plan = plan_gram(batch, n, ka, kb, nb, hermitian);
+  const int nchunks = plan.nchunks;
+  if (plan.work_elems > 0) {
+    if (grow) {
+      grow->ensure(plan.work_elems);
+      work = grow->get();
+    } else if (!work || work_elems < plan.work_elems) {
+      throw Error(kInternal, "gram: workspace too small");
+    }
+  }
   GramArgs<R> a{};
   a.A = A;
   a.lda = lda;
@@ -361,8 +389,8 @@ void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t 
   a.n = n;
   a.ka = ka;
   a.kb = kb;
-  a.chunk = chunk;
-  a.tiles_a = tiles_a;
+  a.chunk = plan.chunk;
+  a.tiles_a = plan.tiles_a;
   a.upper = hermitian ? 1 : 0;
   const int64_t tile_elems = int64_t(ka) * kb;
   if (nchunks == 1) {
@@ -374,15 +402,14 @@ void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t 
     a.bOut[1] = sC2;
     a.chunk_stride = 0;
   } else {
-    work.ensure(size_t(tile_elems) * nchunks * batch * nb);
-    a.out[0] = work.get();
-    a.out[1] = work.get() + tile_elems * nchunks * batch;
+    a.out[0] = work;
+    a.out[1] = work + tile_elems * nchunks * batch;
     a.ldo[0] = a.ldo[1] = ka;
     a.bOut[0] = a.bOut[1] = tile_elems * nchunks;
     a.chunk_stride = tile_elems;
   }
-  const dim3 grid(tiles, nchunks, batch);
-  if (wt == 1)
+  const dim3 grid(plan.tiles, nchunks, batch);
+  if (plan.wt == 1)
     nb == 1 ? launch_gram<R, 1, 1>(grid, a, s) : launch_gram<R, 1, 2>(grid, a, s);
   else
     nb == 1 ? launch_gram<R, 2, 1>(grid, a, s) : launch_gram<R, 2, 2>(grid, a, s);
@@ -402,6 +429,29 @@ void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t 
     if (B2) hipLaunchKernelGGL((mirror_lower_kernel<V>), mg, dim3(256), 0, s, C2, ldc2, ka, sC2);
     HALF_HIP(hipGetLastError());
   }
+}
+
+}  // namespace
+
+template <typename R>
+void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, int64_t sA, const cplx_t<R> *B,
+                  int64_t ldb, int64_t sB, cplx_t<R> *C, int64_t ldc, int64_t sC, const cplx_t<R> *B2, int64_t ldb2,
+                  int64_t sB2, cplx_t<R> *C2, int64_t ldc2, int64_t sC2, DeviceArray<cplx_t<R>> &work, hipStream_t s,
+                  bool hermitian) {
+  gram_batched_work<R>(batch, n, ka, kb, A, lda, sA, B, ldb, sB, C, ldc, sC, B2, ldb2, sB2, C2, ldc2, sC2, &work,
+                       nullptr, 0, s, hermitian);
+}
+
+size_t gram_work_elems(int n, int ka, int kb) {
+  if (n <= 0 || ka <= 0 || kb <= 0) return 0;
+  return plan_gram(1, n, ka, kb, 1, false).work_elems;
+}
+
+template <typename R>
+void gram_into(int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, const cplx_t<R> *B, int64_t ldb, cplx_t<R> *C,
+               int64_t ldc, cplx_t<R> *work, size_t work_elems, hipStream_t s) {
+  gram_batched_work<R>(1, n, ka, kb, A, lda, 0, B, ldb, 0, C, ldc, 0, nullptr, 0, 0, nullptr, 0, 0, nullptr, work,
+                       work_elems, s, false);
 }
 
 template <typename R>
@@ -474,6 +524,8 @@ void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, co
   template void gram<R>(int, int, int, const cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *, \
                         int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *, int64_t, DeviceArray<cplx_t<R>> &, \
                         hipStream_t, bool);                                                                  \
+  template void gram_into<R>(int, int, int, const cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *,  \
+                             int64_t, cplx_t<R> *, size_t, hipStream_t);                                     \
   template void gram_batched<R>(int, int, int, int, const cplx_t<R> *, int64_t, int64_t, const cplx_t<R> *,    \
                                 int64_t, int64_t, cplx_t<R> *, int64_t, int64_t, const cplx_t<R> *, int64_t,   \
                                 int64_t, cplx_t<R> *, int64_t, int64_t, DeviceArray<cplx_t<R>> &, hipStream_t, \

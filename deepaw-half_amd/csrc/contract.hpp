@@ -28,6 +28,16 @@ void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t 
                   int64_t sB2, cplx_t<R> *C2, int64_t ldc2, int64_t sC2, DeviceArray<cplx_t<R>> &work, hipStream_t s,
                   bool hermitian = false);
 
+// Elements of partial-tile workspace gram needs for C = A^H B (n rows, ka x
+// kb result, one output, not Hermitian); 0 when the rows are not split.
+size_t gram_work_elems(int n, int ka, int kb);
+
+// That gram with a caller-provided workspace of work_elems elements (no
+// device allocation); kInternal if it is smaller than gram_work_elems.
+template <typename R>
+void gram_into(int n, int ka, int kb, const cplx_t<R> *A, int64_t lda, const cplx_t<R> *B, int64_t ldb, cplx_t<R> *C,
+               int64_t ldc, cplx_t<R> *work, size_t work_elems, hipStream_t s);
+
 // C = beta C + alpha A W for A (n x k) and W (k x m); C is n x m.  When W2 is
 // given, also C2 = beta C2 + alpha A W2.  beta = 0 does not read C.  C must
 // not overlap W, and either does not overlap A or is A itself (same pointer,

@@ -600,6 +600,88 @@ void KOperator<R>::apply(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> 
   apply_local_added(nb, psi, ldpsi, hpsi, spsi, ldout, st, nl_h.get(), nl_s.get(), aux.join);
 }
 
+namespace {
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+
+}  // namespace
+
+template <typename R>
+size_t KOperator<R>::external_bytes(int bb) const {
+  using V = cplx_t<R>;
+  size_t b = align_up(size_t(sys->fft.nr) * bb * sizeof(V));
+  if (np > 0) {
+    b += 3 * align_up(size_t(np) * bb * sizeof(V));
+    // partials of any batch up to bb bands (the last batch may be narrower,
+    // and narrower batches can split the rows into more chunks)
+    size_t worst = 0;
+    for (int c = 1; c <= bb; ++c) worst = std::max(worst, gram_work_elems(n, np, c));
+    b += align_up(worst * sizeof(V));
+  }
+  return b;
+}
+
+template <typename R>
+int KOperator<R>::external_batch(int nb, size_t capacity) const {
+  // the largest band batch whose workspace fits, scanning down from the
+  // count the boxes alone allow (the partial-tile term is not monotone in
+  // the batch: wider batches split the rows into fewer chunks)
+  const size_t box = size_t(sys->fft.nr) * sizeof(cplx_t<R>);
+  int bb = static_cast<int>(std::min<size_t>(std::max(1, nb), std::max<size_t>(1, capacity / box)));
+  while (bb > 1 && external_bytes(bb) > capacity) --bb;
+  return external_bytes(bb) <= capacity ? bb : 0;
+}
+
+template <typename R>
+void KOperator<R>::apply_external(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
+                                  int64_t ldout, hipStream_t st, void *work, size_t capacity) {
+  using V = cplx_t<R>;
+  if (nb <= 0) return;
+  const int bb = external_batch(nb, capacity);
+  if (bb < 1)
+    throw Error(kCapacity, "one band of this operator needs " + std::to_string(external_bytes(1)) +
+                               " bytes of workspace; " + std::to_string(capacity) + " are available");
+  // workspace layout for a batch of bb bands: boxes, c, D c, Q c, gram partials
+  char *base = static_cast<char *>(work);
+  V *wboxes = reinterpret_cast<V *>(base);
+  size_t off = align_up(size_t(sys->fft.nr) * bb * sizeof(V));
+  V *c = nullptr, *dc = nullptr, *qc = nullptr, *part = nullptr;
+  size_t part_elems = 0;
+  if (np > 0) {
+    const size_t cb = align_up(size_t(np) * bb * sizeof(V));
+    c = reinterpret_cast<V *>(base + off);
+    dc = reinterpret_cast<V *>(base + off + cb);
+    qc = reinterpret_cast<V *>(base + off + 2 * cb);
+    off += 3 * cb;
+    part = reinterpret_cast<V *>(base + off);
+    part_elems = (capacity - off) / sizeof(V);
+  }
+  const R *veff = nullptr;
+  const cplx_t<R> *vx = nullptr;
+  if constexpr (std::is_same<R, double>::value) {
+    veff = sys->veff.get();
+    vx = sys->veff_x.get();
+  } else {
+    veff = sys->veff32.get();
+    vx = sys->veff_x32.get();
+  }
+  for (int b0 = 0; b0 < nb; b0 += bb) {
+    const int cnt = std::min(bb, nb - b0);
+    const V *p = psi + int64_t(b0) * ldpsi;
+    V *h = hpsi + int64_t(b0) * ldout, *sp = spsi + int64_t(b0) * ldout;
+    local_apply<R>(sys->fft, lmap, veff, vx, kinetic.get(), cnt, p, ldpsi, h, sp, ldout, wboxes, st);
+    if (np == 0) continue;
+    trace_mark("proj_coeff_mfma", st);
+    gram_into<R>(n, np, cnt, B.get(), ldb, p, ldpsi, c, np, part, part_elems, st);
+    trace_mark("proj_mix", st);
+    mix_projector_coefficients<R>(*sys, cnt, c, dc, qc, st);
+    trace_mark("proj_expand_mfma", st);
+    gemm_tall<R>(n, cnt, np, R(1), B.get(), ldb, dc, np, R(1), h, ldout, qc, np, sp, ldout, st);
+  }
+  trace_close(st);
+}
+
 template struct KOperator<double>;
 template struct KOperator<float>;
 

@@ -143,6 +143,14 @@ struct KOperator {
   void project(int nb, const cplx_t<R> *psi, int64_t ldpsi, hipStream_t st);
   // BD and BQ, once
   void mixed_blocks(hipStream_t st);
+  // H psi and S psi in order on st with every temporary (FFT boxes,
+  // projector coefficients, contraction partials) in the caller's `work`
+  // of `capacity` bytes, bands in batches that fit it: no device
+  // allocation.  kCapacity if one band does not fit.
+  void apply_external(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
+                      int64_t ldout, hipStream_t st, void *work, size_t capacity);
+  size_t external_bytes(int bands) const;              // workspace of one batch of `bands`
+  int external_batch(int nb, size_t capacity) const;   // largest batch within capacity (0: none)
   // projector stage on the skinny kernels for nb bands
   bool skinny(int nb) const;
 };

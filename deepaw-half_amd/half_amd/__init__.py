@@ -41,6 +41,7 @@ EXT_SYMBOLS = (
     "half_amd_trace", "half_amd_trace_read", "half_amd_set_tuning", "half_amd_get_tuning",
     "half_amd_set_projector_positions", "half_amd_solve_history", "half_amd_solve_kmesh",
     "half_amd_force_set", "half_amd_write_vaspwave", "half_amd_harris_energy",
+    "half_amd_operator_apply_noalloc",
 )
 
 
@@ -108,6 +109,7 @@ def load(path: str | None = None):
         "half_amd_operator_create": [i64, pd, i, pi64, pi64, pi32, cp, i],
         "half_amd_operator_apply": [i64, i, i, vp, i64, vp, vp, i64, i, vp, cp, i],
         "half_amd_operator_destroy": [i64, cp, i],
+        "half_amd_operator_apply_noalloc": [i64, i, vp, i64, vp, vp, i64, vp, cp, i],
         "half_amd_solve_acc": [i64, pd, i, i, d, d, i, i, i, pd, vp, i64, C.POINTER(SolveStats), cp, i],
         "half_amd_set_solver": [i64, i, cp, i],
         "half_amd_energy_terms": [i64, pd, pd, pd, cp, i],
@@ -496,6 +498,15 @@ class Operator:
         _check(self.lib.half_amd_operator_apply(self.handle, int(stage), int(nstates), C.c_void_p(d_psi), ld,
                                                 C.c_void_p(d_hpsi), C.c_void_p(d_spsi), ld, int(box_batch),
                                                 C.c_void_p(stream) if stream else None, e, 512), e)
+
+    def apply_device_noalloc(self, nstates, d_psi, d_hpsi, d_spsi, ld=None, stream=None):
+        """The full operator on device pointers through the entry point that
+        allocates no device memory (half_amd_operator_apply_noalloc)."""
+        ld = ld or self.npw
+        e = _err()
+        _check(self.lib.half_amd_operator_apply_noalloc(self.handle, int(nstates), C.c_void_p(d_psi), ld,
+                                                        C.c_void_p(d_hpsi), C.c_void_p(d_spsi), ld,
+                                                        C.c_void_p(stream) if stream else None, e, 512), e)
 
     def close(self):
         if self.handle:

@@ -69,6 +69,13 @@ def synchronize():
     _ok(lib().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
+def mem_info():
+    """(free, total) device memory in bytes (hipMemGetInfo)."""
+    free, total = C.c_size_t(0), C.c_size_t(0)
+    _ok(lib().hipMemGetInfo(C.byref(free), C.byref(total)), "hipMemGetInfo")
+    return free.value, total.value
+
+
 def memset(buf: "Buffer", value: int):
     _ok(lib().hipMemset(buf.ptr, int(value), buf.nbytes), "hipMemset")
 

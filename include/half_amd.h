@@ -39,6 +39,23 @@ int half_amd_operator_apply(half_amd_operator op, int stage, int nstates, const 
                             void *stream, char *error, int error_capacity);
 int half_amd_operator_destroy(half_amd_operator op, char *error, int error_capacity);
 
+/* H.psi and S.psi of half_amd_operator_apply (stage 0) with no device
+ * allocation.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Every temporary (FFT boxes,
+ * projector coefficients, contraction partials) lives in a 128 MiB array
+ * the library's code object holds on each device until the process ends;
+ * the bands are applied in batches that fit it, the two stages in order on
+ * the stream.  HALF_ERROR_CAPACITY if one band of this operator needs more
+ * than the array (an FFT box of 8 M points in FP64, for example).  All calls
+ * on one device share that array: calls that overlap in time on one device
+ * (other streams, other threads) race and must be serialised by the caller.
+ * Results equal half_amd_operator_apply's to rounding (the projector stage
+ * always takes the LDS-tiled contractions here). */
+int half_amd_operator_apply_noalloc(half_amd_operator op, int nstates, const void *d_psi, int64_t ldpsi,
+                                    void *d_hpsi, void *d_spsi, int64_t ldout, void *stream, char *error,
+                                    int error_capacity);
+
 /* Solver statistics of the last solve. */
 typedef struct half_amd_solve_stats {
   int32_t iterations;
