@@ -188,11 +188,17 @@ void check_info(const DeviceArray<rocblas_int> &info, hipStream_t s, const char 
 }
 
 bool g_hetrd = true;
-// Soft locking in the all-band solve: a band whose residual already meets
-// the tolerance contributes no correction column to the next Rayleigh-Ritz
-// subspace (it stays in X).  Off: every band keeps its column, the
+// Locking in the all-band solve (key "solver.lock").  1 (soft): a band
+// whose residual already meets the current target contributes no correction
+// column to the next Rayleigh-Ritz subspace (it stays in X).  2 (soft plus
+// a hard-locked prefix): in addition the lowest bands that all meet the
+// target are frozen -- their vectors, H and S images and Ritz values no
+// longer enter the Rayleigh-Ritz problem, which covers only the remaining
+// bands and the correction columns (the corrections stay S-orthogonal to
+// every band, frozen ones included); a final Rayleigh-Ritz step over all
+// bands orders and refines the result.  0: every band keeps its column, the
 // reference's 2m-wide subspace.
-bool g_lock = true;
+int g_lock = 2;
 
 // zheevd with vectors (upper triangle of A referenced; eigenvectors replace
 // A, eigenvalues ascending in w; e is workspace of n values).  Orders up to
@@ -305,24 +311,31 @@ bool set_solver_tuning(const std::string &key, int value) {
     g_stall = value;
     return true;
   }
-  bool *flag = key == "solver.hetrd" ? &g_hetrd : key == "solver.lock" ? &g_lock : nullptr;
-  if (!flag || value < 0 || value > 1) return false;
-  *flag = value != 0;
+  if (key == "solver.lock") {
+    if (value < 0 || value > 2) return false;
+    g_lock = value;
+    return true;
+  }
+  if (key != "solver.hetrd" || value < 0 || value > 1) return false;
+  g_hetrd = value != 0;
   return true;
 }
 
-bool solver_locking() { return g_lock; }
+bool solver_locking() { return g_lock != 0; }
 
-bool AccControls::locking() const { return lock < 0 ? g_lock : lock != 0; }
+int AccControls::lock_mode() const { return lock < 0 ? g_lock : lock; }
+bool AccControls::locking() const { return lock_mode() != 0; }
 
 bool get_solver_tuning(const std::string &key, int &value) {
   if (key == "solver.stall") {
     value = g_stall;
-    return true;
+  } else if (key == "solver.lock") {
+    value = g_lock;
+  } else if (key == "solver.hetrd") {
+    value = g_hetrd ? 1 : 0;
+  } else {
+    return false;
   }
-  const bool *flag = key == "solver.hetrd" ? &g_hetrd : key == "solver.lock" ? &g_lock : nullptr;
-  if (!flag) return false;
-  value = *flag ? 1 : 0;
   return true;
 }
 
@@ -506,6 +519,9 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   bool fp64_phase = !ctl.mixed;
   double fp32_floor = 0;
   const bool locking = ctl.locking();
+  const bool hard = ctl.lock_mode() == 2;
+  int L = 0;              // hard-locked prefix: bands 0..L-1 frozen
+  bool hard_used = false;
   res.applies = 0;
 
   // H and S applied to `cnt` columns
@@ -525,25 +541,29 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   };
   // generalized k x k problem in (hsub, ssub): the lowest m eigenvectors in
   // hsub(:, :m), values in w
-  auto small_generalized = [&](int k) {
-    generalized_eigh_ritz(blas, k, m, wk.hsub.get(), wk.ssub.get(), wk.w.get(), wk.E.get(), wk.info.get(),
-                          "ACC small generalized eigensolve");
+  // the lowest nv (default m) eigenvectors in hsub(:, :nv), values from w + w0
+  auto small_generalized = [&](int k, int nv = -1, int w0 = 0) {
+    generalized_eigh_ritz(blas, k, nv < 0 ? m : nv, wk.hsub.get(), wk.ssub.get(), wk.w.get() + w0, wk.E.get(),
+                          wk.info.get(), "ACC small generalized eigensolve");
   };
   // (hsub, ssub) = (A^H HA, A^H SA) for k columns of Z-shaped blocks
   auto gram_hs = [&](int k, const double2 *A, const double2 *HA, const double2 *SA) {
     gram<double>(n, k, k, A, ld, HA, ld, wk.hsub.get(), k, SA, ld, wk.ssub.get(), k, wk.gwork, s, true);
   };
-  // rotate the first k columns of (Z, HZ, SZ) by hsub(:, :m) into X, HX, SX
-  auto ritz_rotate = [&](int k) {
-    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, k, one, wk.Z.get(), ld, wk.hsub.get(), k,
-          zero, wk.tmp.get(), ld);
-    copy(X, wk.tmp.get(), m);
-    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, k, one, wk.HZ.get(), ld, wk.hsub.get(), k,
-          zero, wk.tmp.get(), ld);
-    copy(HX, wk.tmp.get(), m);
-    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, k, one, wk.SZ.get(), ld, wk.hsub.get(), k,
-          zero, wk.tmp.get(), ld);
-    copy(SX, wk.tmp.get(), m);
+  // rotate k columns of (Z, HZ, SZ) from column c0 by hsub(:, :m - c0)
+  // into columns c0.. of X, HX, SX
+  auto ritz_rotate = [&](int k, int c0 = 0) {
+    const int nv = m - c0;
+    const size_t off = size_t(n) * c0;
+    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, nv, k, one, wk.Z.get() + off, ld, wk.hsub.get(),
+          k, zero, wk.tmp.get(), ld);
+    copy(X + off, wk.tmp.get(), nv);
+    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, nv, k, one, wk.HZ.get() + off, ld, wk.hsub.get(),
+          k, zero, wk.tmp.get(), ld);
+    copy(HX + off, wk.tmp.get(), nv);
+    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, nv, k, one, wk.SZ.get() + off, ld, wk.hsub.get(),
+          k, zero, wk.tmp.get(), ld);
+    copy(SX + off, wk.tmp.get(), nv);
   };
   auto rayleigh_ritz_x = [&]() {
     gram_hs(m, X, HX, SX);
@@ -627,8 +647,10 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     const bool fp32_stall = !fp64_phase && ((max_res < kStallArm * fp32_target && fp32_stalled.update(max_res)) ||
                                             2 * iter > ctl.maxiter);
     if (!fp64_phase && (fp32_stall || max_res <= fp32_target)) {
-      // FP64 correction sweeps: exact HX, SX for the current subspace
+      // FP64 correction sweeps: exact HX, SX for the current subspace; the
+      // bands frozen at FP32 accuracy take part again
       fp64_phase = true;
+      L = 0;
       apply(m, X, HX, SX);
       rayleigh_ritz_x();
       --iter;
@@ -644,15 +666,19 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     // The tests above still read every requested band, so a locked band
     // whose residual grows past the target gets its column back.
     const double lock_at = fp64_phase ? ctl.tol : fp32_target;
+    if (hard) {  // extend the frozen prefix over requested bands that meet the target
+      while (L < mreq && std::sqrt(std::max(0.0, r2[L])) <= lock_at) ++L;
+      if (L > 0) hard_used = true;
+    }
     int np = 0;
-    for (int j = 0; j < m; ++j) {
+    for (int j = L; j < m; ++j) {
       if (locking && j < mreq && std::sqrt(std::max(0.0, r2[j])) <= lock_at) continue;
       cols[np] = j;
       seeds[np] = seeds[j];
       ++np;
     }
     if (np == 0) break;  // unreachable: the test above stops first
-    order_sum += m + np;
+    order_sum += m - L + np;
     ++rr_steps;
     trace_mark("acc_precondition", s);
     wk.seeds.upload(seeds, s);
@@ -719,14 +745,30 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
         copy(blk, wk.tmp.get(), np);
       }
     }
-    // Rayleigh-Ritz in span[X P]
-    const int k = m + np;
+    // Rayleigh-Ritz in span[X P] (without the frozen prefix: columns L..)
+    const int k = m - L + np;
+    const size_t zoff = size_t(n) * L;
     trace_mark("acc_projection_gemm", s);
-    gram_hs(k, wk.Z.get(), wk.HZ.get(), wk.SZ.get());
+    gram_hs(k, wk.Z.get() + zoff, wk.HZ.get() + zoff, wk.SZ.get() + zoff);
     trace_mark("acc_subspace_eigensolve", s);
-    small_generalized(k);
+    small_generalized(k, m - L, L);
     trace_mark("acc_rotate_gemm", s);
-    ritz_rotate(k);
+    ritz_rotate(k, L);
+    trace_close(s);
+  }
+  if (hard_used) {
+    // one Rayleigh-Ritz step over every band: ascending Ritz values across
+    // the frozen prefix and the rest, and the current residual reported
+    trace_mark("acc_final_rayleigh_ritz", s);
+    rayleigh_ritz_x();
+    hipLaunchKernelGGL(residual_kernel, dim3(blocks_for(int64_t(n) * m)), dim3(kThreads), 0, s, HX, SX,
+                       wk.w.get(), wk.R.get(), n, m, ld);
+    hipLaunchKernelGGL(column_norm2_kernel, dim3(m), dim3(kThreads), 0, s, wk.R.get(), ld, n, wk.res2.get());
+    HALF_HIP(hipGetLastError());
+    HALF_HIP(hipMemcpyAsync(r2.data(), wk.res2.get(), sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    HALF_HIP(hipStreamSynchronize(s));
+    max_res = 0;
+    for (int j = 0; j < mreq; ++j) max_res = std::max(max_res, std::sqrt(std::max(0.0, r2[j])));
     trace_close(s);
   }
   res.iterations = std::min(iter, ctl.maxiter);

@@ -80,8 +80,10 @@ struct AccControls {
   bool mixed = false;     // FP32 H.psi iterations before FP64 correction sweeps
   double switch_tol = 2e-3;  // mixed form: residual at which FP64 sweeps start
   int extra = 0;          // buffer bands carried internally, not reported
-  int lock = -1;          // soft locking: 1 on, 0 off (the reference's 2m-wide
-                          // subspace), -1 the process-wide key "solver.lock"
+  int lock = -1;          // 0 off (the reference's 2m-wide subspace), 1 soft
+                          // locking, 2 soft locking plus a frozen prefix of
+                          // converged bands, -1 the process-wide key "solver.lock"
+  int lock_mode() const;
   bool locking() const;
 };
 

@@ -129,17 +129,20 @@ def test_no_device_is_reported_not_aborted():
 
 
 def test_solver_tuning_keys_host_only():
-    """solver.lock (soft locking, 0/1) and solver.stall (FP32-phase
+    """solver.lock (0 off, 1 soft locking, 2 soft locking plus a frozen
+    prefix of converged bands, the default) and solver.stall (FP32-phase
     stagnation window, 0..1000) are host settings: set, read back, restored;
     out-of-range values are rejected with INVALID_ARGUMENT."""
     saved = {k: H.get_tuning(k) for k in ("solver.lock", "solver.stall", "solver.hetrd")}
-    assert saved["solver.lock"] == 1 and saved["solver.stall"] == 4
+    assert saved["solver.lock"] == 2 and saved["solver.stall"] == 4
     try:
         H.set_tuning("solver.lock", 0)
         assert H.get_tuning("solver.lock") == 0
         H.set_tuning("solver.stall", 7)
         assert H.get_tuning("solver.stall") == 7
-        for key, bad in (("solver.lock", 2), ("solver.stall", -1), ("solver.stall", 1001), ("solver.nope", 0)):
+        H.set_tuning("solver.lock", 1)
+        assert H.get_tuning("solver.lock") == 1
+        for key, bad in (("solver.lock", 3), ("solver.lock", -1), ("solver.stall", -1), ("solver.stall", 1001), ("solver.nope", 0)):
             with pytest.raises(Exception, match="unknown tuning key or value out of range"):
                 H.set_tuning(key, bad)
         assert H.get_tuning("solver.stall") == 7

@@ -92,11 +92,12 @@ def test_kmesh_soft_locking(si2ctx, mixed):
     nb = 16
     ref = _dense(si2ctx, nb, SI2_MESH)
     tol, bar = (1e-6, 1e-8) if mixed else (1e-7, 1e-9)
+    saved = H.get_tuning("solver.lock")
     H.set_tuning("solver.lock", 0)
     try:
         wf, sf = si2ctx.solve_kmesh(nb, SI2_MESH, mixed=mixed, tol=tol, maxiter=200, extra=4)
     finally:
-        H.set_tuning("solver.lock", 1)
+        H.set_tuning("solver.lock", saved)
     wl, sl = si2ctx.solve_kmesh(nb, SI2_MESH, mixed=mixed, tol=tol, maxiter=200, extra=4)
     we, se = si2ctx.solve_kmesh(nb, SI2_MESH, mixed=mixed, tol=tol, maxiter=200, extra=4, graph=False)
     print(f"[gpu] si2 kmesh mixed={mixed} full", sf, "locked", sl, flush=True)

@@ -133,7 +133,8 @@ def test_acc_abi_defaults(si2):
     # whatever the process-wide locking key says: the same Ritz pairs under
     # either key value, and the same as the extension's solve with locking
     # off at the reference's controls
-    for lock in (1, 0):
+    saved = H.get_tuning("solver.lock")
+    for lock in (2, 1, 0):
         H.set_tuning("solver.lock", lock)
         try:
             ctx.set_solver(H.SOLVER_ACC)
@@ -141,12 +142,12 @@ def test_acc_abi_defaults(si2):
             ctx.set_solver(H.SOLVER_EVD)
             assert np.abs(wa - w).max() == 0.0 and np.abs(va - v).max() == 0.0
         finally:
-            H.set_tuning("solver.lock", 1)
+            H.set_tuning("solver.lock", saved)
     H.set_tuning("solver.lock", 0)
     try:
         we, ve, st = ctx.solve_acc(g["nbands"], g["kpoint"], tol=1e-4, maxiter=40, vectors=True)
     finally:
-        H.set_tuning("solver.lock", 1)
+        H.set_tuning("solver.lock", saved)
     assert st["rr_order_mean"] == 2 * g["nbands"]
     assert np.abs(we - w).max() < 1e-12 and np.abs(ve - v).max() < 1e-10
 
@@ -432,14 +433,16 @@ def test_hfo2_mixed_switch_below_fp32_floor(hfo2):
 
 
 def test_hfo2_acc_soft_locking(hfo2):
-    """Soft locking (the default: bands already below the tolerance add no
-    correction column) and the reference's full 2m-wide Rayleigh-Ritz
-    subspace converge to the same eigenvalues, FP64 and mixed; locking
-    shrinks the subspace problems."""
+    """Locking (soft: bands already below the tolerance add no correction
+    column; soft plus a frozen prefix of converged bands, the default) and
+    the reference's full 2m-wide Rayleigh-Ritz subspace converge to the same
+    eigenvalues, FP64 and mixed; locking shrinks the subspace problems, the
+    frozen prefix more."""
     g, ctx = hfo2
     ref = np.array(g["eigenvalues"][:64])
+    saved = H.get_tuning("solver.lock")
     out = {}
-    for lock in (1, 0):
+    for lock in (2, 1, 0):
         H.set_tuning("solver.lock", lock)
         try:
             for mixed in (False, True):
@@ -447,15 +450,17 @@ def test_hfo2_acc_soft_locking(hfo2):
                 print(f"[gpu] hfo2 acc lock={lock} mixed={mixed}", st, np.abs(w - ref).max(), flush=True)
                 out[lock, mixed] = (w, st)
         finally:
-            H.set_tuning("solver.lock", 1)
+            H.set_tuning("solver.lock", saved)
     for mixed in (False, True):
-        (wl, stl), (wf, stf) = out[1, mixed], out[0, mixed]
+        stf = out[0, mixed][1]
         assert stf["rr_order_mean"] == 144
-        assert stl["rr_order_mean"] < 144
-        assert stl["max_residual_eV"] <= 1e-6 and stf["max_residual_eV"] <= 1e-6
-        assert np.abs(wl - ref).max() < (1e-8 if mixed else 1e-9)
-        assert np.abs(wf - ref).max() < (1e-8 if mixed else 1e-9)
-    assert H.get_tuning("solver.lock") == 1
+        assert out[2, mixed][1]["rr_order_mean"] < out[1, mixed][1]["rr_order_mean"] < 144
+        for lock in (2, 1, 0):
+            w, st = out[lock, mixed]
+            assert st["max_residual_eV"] <= 1e-6
+            assert np.all(np.diff(w) >= 0)
+            assert np.abs(w - ref).max() < (1e-8 if mixed else 1e-9), (lock, mixed)
+    assert H.get_tuning("solver.lock") == saved
 
 
 # --------------------------------------------------------------------------
