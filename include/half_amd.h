@@ -67,7 +67,7 @@ typedef struct half_amd_solve_stats {
                                by the metric's eigen-decomposition instead of
                                its Cholesky factor */
   int32_t rr_order_mean;    /* mean order of the per-iteration Rayleigh-Ritz
-                               problems, rounded (2 x bands without soft
+                               problems, rounded (2 x bands without
                                locking, tuning key "solver.lock") */
 } half_amd_solve_stats;
 
@@ -79,10 +79,14 @@ typedef struct half_amd_solve_stats {
  * tol, the FP32 operator's error measured on a probe block at the start);
  * no 10% gain in `solver.stall` iterations (default 4) once within 10x of
  * that target; half of maxiter.  Rayleigh-Ritz, Cholesky-type metric
- * normalisation and the subspace eigensolves are FP64 throughout.  Soft
- * locking (key "solver.lock", default 1): requested bands whose residual
- * meets the current phase's target add no correction column to the next
- * Rayleigh-Ritz step; the stopping rule still reads every requested band.
+ * normalisation and the subspace eigensolves are FP64 throughout.  Locking
+ * (key "solver.lock"): 1 soft -- requested bands whose residual meets the
+ * current phase's target add no correction column to the next Rayleigh-Ritz
+ * step; 2 (default) soft plus a frozen prefix -- the lowest requested bands
+ * that all meet it also leave the Rayleigh-Ritz problem (the corrections
+ * stay S-orthogonal to them; the FP32 phase's frozen bands rejoin at the
+ * switch to FP64), and one Rayleigh-Ritz step over all bands ends the solve;
+ * 0 off.  The stopping rule still reads every requested band.
  * extra buffer bands are carried internally and not reported.  eigenvectors
  * (host, ld >= npw) may be null.  ACC through half_solve_kpoint never locks
  * (the reference's iteration). */
@@ -101,10 +105,11 @@ int half_amd_solve_history(half_handle handle, int capacity, double *residuals_e
 /* All-band solves at nk k points (fractional, nk x 3) of one context in
  * lockstep: the H.psi of every k point is one launch per stage, the
  * Rayleigh-Ritz algebra strided-batched over k, and the correction block's
- * H.psi replayed from a captured hipGraph when graph != 0.  Controls,
- * FP32-phase exit rules and soft locking as in half_amd_solve_acc (with
- * locking, one correction width for all k points: the largest active count
- * rounded up to a multiple of 8); the solve stops when every k point's
+ * H.psi replayed from a captured hipGraph when graph != 0.  Controls and
+ * FP32-phase exit rules as in half_amd_solve_acc; "solver.lock" 1 and 2
+ * both give soft locking here, with one correction width for all k points
+ * (the largest active count rounded up to a multiple of 8); the solve stops
+ * when every k point's
  * requested bands meet tol.  eigenvalues: nk x nbands, k-major.  stats: max_residual_eV is
  * the largest over k, band_applications counts all k points.  The
  * reference has no batched entry; it is the loop of half_solve_kpoint over
