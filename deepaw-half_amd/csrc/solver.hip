@@ -62,6 +62,15 @@ __global__ void init_states_kernel(double2 *x, int64_t ld, const int *indices, i
   x[g + b * ld] = make_double2(g == indices[b] ? 1.0 : 0.0, 0.0);
 }
 
+// out column i = in column perm[i] (i < cnt), for reordering blocks of states
+__global__ void gather_columns_kernel(const double2 *in, double2 *out, const int *perm, int n, int cnt, int64_t ld) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(n) * cnt) return;
+  const int b = static_cast<int>(t / n);
+  const int g = static_cast<int>(t - int64_t(b) * n);
+  out[g + b * ld] = in[g + perm[b] * ld];
+}
+
 // R = HX - SX diag(w) (reference: src/half_cuda_iterative_solver.cuf:29-40)
 __global__ void residual_kernel(const double2 *hx, const double2 *sx, const double *w, double2 *r, int n, int m,
                                 int64_t ld) {
@@ -191,13 +200,13 @@ bool g_hetrd = true;
 // Locking in the all-band solve (key "solver.lock").  1 (soft): a band
 // whose residual already meets the current target contributes no correction
 // column to the next Rayleigh-Ritz subspace (it stays in X).  2 (soft plus
-// a hard-locked prefix): in addition the lowest bands that all meet the
-// target are frozen -- their vectors, H and S images and Ritz values no
-// longer enter the Rayleigh-Ritz problem, which covers only the remaining
-// bands and the correction columns (the corrections stay S-orthogonal to
-// every band, frozen ones included); a final Rayleigh-Ritz step over all
-// bands orders and refines the result.  0: every band keeps its column, the
-// reference's 2m-wide subspace.
+// freezing): in addition every requested band that meets the target is
+// frozen -- moved to the front of X, its vector, H and S images and Ritz
+// value no longer enter the Rayleigh-Ritz problem, which covers only the
+// remaining bands and the correction columns (the corrections stay
+// S-orthogonal to every band, frozen ones included); at the end the pairs
+// are sorted by Ritz value.  0: every band keeps its column, the reference's
+// 2m-wide subspace.
 int g_lock = 2;
 
 // zheevd with vectors (upper triangle of A referenced; eigenvectors replace
@@ -520,7 +529,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   double fp32_floor = 0;
   const bool locking = ctl.locking();
   const bool hard = ctl.lock_mode() == 2;
-  int L = 0;              // hard-locked prefix: bands 0..L-1 frozen
+  int L = 0;              // frozen bands: columns 0..L-1 of X, HX, SX
   bool hard_used = false;
   res.applies = 0;
 
@@ -614,8 +623,9 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     }
     HALF_HIP(hipStreamSynchronize(s));
   }
-  std::vector<double> r2(m);
-  std::vector<int> seeds(m), cols(m);
+  std::vector<double> r2(m), wh(m);
+  std::vector<int> seeds(m), cols(m), perm(m);
+  DeviceArray<int> d_perm(m);
   int64_t order_sum = 0;
   int rr_steps = 0;
   StallWatch fp32_stalled;
@@ -634,12 +644,12 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     HALF_HIP(hipGetLastError());
     trace_mark("acc_norms_to_host", s);
     HALF_HIP(hipMemcpyAsync(r2.data(), wk.res2.get(), sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    if (hard) HALF_HIP(hipMemcpyAsync(wh.data(), wk.w.get(), sizeof(double) * m, hipMemcpyDeviceToHost, s));
     HALF_HIP(hipStreamSynchronize(s));
     max_res = 0;
     for (int j = 0; j < m; ++j) {
       const double r = std::sqrt(std::max(0.0, r2[j]));
       if (j < mreq) max_res = std::max(max_res, r);
-      seeds[j] = r < std::max(ctl.tol * 1e-2, 1e-12) ? (m + j) % n : -1;
     }
     res.history.push_back(max_res);
     if (fp64_phase) ++fp64_iters;
@@ -666,10 +676,40 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     // The tests above still read every requested band, so a locked band
     // whose residual grows past the target gets its column back.
     const double lock_at = fp64_phase ? ctl.tol : fp32_target;
-    if (hard) {  // extend the frozen prefix over requested bands that meet the target
-      while (L < mreq && std::sqrt(std::max(0.0, r2[L])) <= lock_at) ++L;
+    if (hard) {
+      // freeze the requested bands that meet the target: they move, in
+      // order, to just after the frozen ones (columns 0..L-1); the other
+      // bands keep their order behind them, so columns below mreq still
+      // hold requested bands and the buffer bands stay last
+      int nconv = 0, moved = 0;
+      for (int j = L; j < mreq; ++j)
+        if (std::sqrt(std::max(0.0, r2[j])) <= lock_at) perm[L + nconv++] = j;
+      for (int j = L, c = L + nconv; j < m; ++j)
+        if (!(j < mreq && std::sqrt(std::max(0.0, r2[j])) <= lock_at)) perm[c++] = j;
+      for (int i = L; i < m; ++i) moved += perm[i] != i;
+      if (moved) {
+        trace_mark("acc_freeze_reorder", s);
+        const int cnt = m - L;
+        d_perm.upload(perm.data() + L, size_t(cnt), s);
+        for (double2 *blk : {X, HX, SX, wk.R.get()}) {
+          hipLaunchKernelGGL(gather_columns_kernel, dim3(blocks_for(int64_t(n) * cnt)), dim3(kThreads), 0, s,
+                             blk, wk.tmp.get(), d_perm.get(), n, cnt, ld);
+          HALF_HIP(hipGetLastError());
+          copy(blk + size_t(n) * L, wk.tmp.get(), cnt);
+        }
+        std::vector<double> w2(wh), r22(r2);
+        for (int i = L; i < m; ++i) {
+          wh[i] = w2[perm[i]];
+          r2[i] = r22[perm[i]];
+        }
+        HALF_HIP(hipMemcpyAsync(wk.w.get() + L, wh.data() + L, sizeof(double) * cnt, hipMemcpyHostToDevice, s));
+        HALF_HIP(hipStreamSynchronize(s));  // wh is reused by the next iteration's download
+      }
+      L += nconv;
       if (L > 0) hard_used = true;
     }
+    for (int j = 0; j < m; ++j)
+      seeds[j] = std::sqrt(std::max(0.0, r2[j])) < std::max(ctl.tol * 1e-2, 1e-12) ? (m + j) % n : -1;
     int np = 0;
     for (int j = L; j < m; ++j) {
       if (locking && j < mreq && std::sqrt(std::max(0.0, r2[j])) <= lock_at) continue;
@@ -745,7 +785,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
         copy(blk, wk.tmp.get(), np);
       }
     }
-    // Rayleigh-Ritz in span[X P] (without the frozen prefix: columns L..)
+    // Rayleigh-Ritz in span[X P] without the frozen bands (columns L..)
     const int k = m - L + np;
     const size_t zoff = size_t(n) * L;
     trace_mark("acc_projection_gemm", s);
@@ -756,19 +796,37 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     ritz_rotate(k, L);
     trace_close(s);
   }
-  if (hard_used) {
-    // one Rayleigh-Ritz step over every band: ascending Ritz values across
-    // the frozen prefix and the rest, and the current residual reported
-    trace_mark("acc_final_rayleigh_ritz", s);
-    rayleigh_ritz_x();
-    hipLaunchKernelGGL(residual_kernel, dim3(blocks_for(int64_t(n) * m)), dim3(kThreads), 0, s, HX, SX,
-                       wk.w.get(), wk.R.get(), n, m, ld);
-    hipLaunchKernelGGL(column_norm2_kernel, dim3(m), dim3(kThreads), 0, s, wk.R.get(), ld, n, wk.res2.get());
-    HALF_HIP(hipGetLastError());
-    HALF_HIP(hipMemcpyAsync(r2.data(), wk.res2.get(), sizeof(double) * m, hipMemcpyDeviceToHost, s));
+  if (hard_used && L > 0) {
+    // frozen bands keep the Ritz values they had when frozen, so the bands
+    // are out of order: sort the pairs by value (a permutation, so every
+    // vector keeps its residual)
+    trace_mark("acc_final_sort", s);
+    HALF_HIP(hipMemcpyAsync(wh.data(), wk.w.get(), sizeof(double) * m, hipMemcpyDeviceToHost, s));
     HALF_HIP(hipStreamSynchronize(s));
-    max_res = 0;
-    for (int j = 0; j < mreq; ++j) max_res = std::max(max_res, std::sqrt(std::max(0.0, r2[j])));
+    std::vector<int> order(m);
+    for (int j = 0; j < m; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return wh[a] < wh[b]; });
+    bool sorted = true;
+    for (int j = 0; j < m; ++j) sorted = sorted && order[j] == j;
+    if (!sorted) {
+      d_perm.upload(order, s);
+      for (double2 *blk : {X, HX, SX}) {
+        hipLaunchKernelGGL(gather_columns_kernel, dim3(blocks_for(int64_t(n) * m)), dim3(kThreads), 0, s, blk,
+                           wk.tmp.get(), d_perm.get(), n, m, ld);
+        HALF_HIP(hipGetLastError());
+        copy(blk, wk.tmp.get(), m);
+      }
+      std::vector<double> w2(m), r22(m);
+      for (int j = 0; j < m; ++j) {
+        w2[j] = wh[order[j]];
+        r22[j] = r2[order[j]];
+      }
+      HALF_HIP(hipMemcpyAsync(wk.w.get(), w2.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+      HALF_HIP(hipStreamSynchronize(s));
+      // the residual reported is that of the bands now first
+      max_res = 0;
+      for (int j = 0; j < mreq; ++j) max_res = std::max(max_res, std::sqrt(std::max(0.0, r22[j])));
+    }
     trace_close(s);
   }
   res.iterations = std::min(iter, ctl.maxiter);

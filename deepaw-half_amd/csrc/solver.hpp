@@ -81,7 +81,7 @@ struct AccControls {
   double switch_tol = 2e-3;  // mixed form: residual at which FP64 sweeps start
   int extra = 0;          // buffer bands carried internally, not reported
   int lock = -1;          // 0 off (the reference's 2m-wide subspace), 1 soft
-                          // locking, 2 soft locking plus a frozen prefix of
+                          // locking, 2 soft locking plus freezing of
                           // converged bands, -1 the process-wide key "solver.lock"
   int lock_mode() const;
   bool locking() const;
