@@ -59,6 +59,8 @@ def main():
                     help="residual tolerance (eV) of the untimed FP64 solve the eigenvalue errors are measured "
                          "against, where no golden eigenvalues cover the bands")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="length of the CPU baseline sample")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="set a kernel tunable (half_amd_set_tuning) before measuring; repeatable")
     ap.add_argument("--kmesh", type=int, default=0, metavar="NK",
                     help="only time the batched all-band solve of the workload's first NK mesh k points "
                          "(-1: all) against per-k solves, and print that line")
@@ -76,6 +78,10 @@ def main():
     import numpy as np
     import half_amd as H
     from half_amd import hip, synth
+
+    for kv in args.tune:
+        key, value = kv.split("=")
+        H.set_tuning(key, int(value))
 
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20
@@ -192,6 +198,14 @@ def main():
             solve["max_abs_diff_mixed_vs_fp64_eV"] = float(np.abs(evals["mixed"] - evals["fp64"]).max())
         solve["controls"] = {"tol_eV": args.tol, "extra_bands": extra, "maxiter": 400}
 
+    if not args.no_solve and n <= 6000:
+        # dense FP64 solve (assembly, overlap spectrum, generalized eigensolve),
+        # the reference CPU path's algorithm, for comparison with its timing
+        ctx.solve(nb, k, vectors=False)
+        t = time.perf_counter()
+        wd, _, _, _ = ctx.solve(nb, k, vectors=False)
+        solve["dense_evd"] = {"seconds": time.perf_counter() - t, "bands": nb}
+        print(f"bench: dense solve {solve['dense_evd']}", file=sys.stderr, flush=True)
     gv, _ = ctx.basis(k)
     nlines, nplanes = sphere_counts(gv, info["grid"])
     roofline = kernel_roofline(kernels[64], nr, info["grid"], n, nproj, nb, args.steps, nlines, nplanes, True,
@@ -236,6 +250,8 @@ def main():
         "kernel_rooflines": per_kernel,
         "cpu_baseline": cpu,
     }
+    if args.tune:
+        out["tuning"] = dict(kv.split("=") for kv in args.tune)
     print(json.dumps(out))
 
 

@@ -752,7 +752,9 @@ YxyKernel yxy_len(int n2, bool vx_lds) {
 template <typename R>
 YxyKernel yxy_for(int planes, int n1, int n2, int G, int ld) {
   int pmax = planes <= 16 ? 16 : planes <= 24 ? 24 : planes <= 32 ? 32 : 48;
-  if (std::is_same<R, double>::value && pmax > 32) return {};
+  // above 32 planes the separate passes are faster in either precision
+  // (96^3 box, 41 planes, FP32: 2.34 ms fused against 1.92 ms per 303 bands)
+  if (pmax > 32) return {};
   if (planes > 48 || pmax > n1 + 1) return {};  // window arithmetic assumes PMAX <= n1 + 1
   const bool vx_lds = g_tune.vxlds[std::is_same<R, double>::value ? 0 : 1] && G >= 2 && ld == n2 + 1;
   switch (pmax) {
