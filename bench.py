@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--extra", type=int, default=-1, help="buffer bands of the all-band solves (-1: bands / 8, "
                                                           "at least 8)")
     ap.add_argument("--tol", type=float, default=1e-5, help="residual tolerance (eV) of the all-band solves")
+    ap.add_argument("--switch-tol", type=float, default=0.0,
+                    help="residual (eV) at which the mixed solve's FP32 phase hands over to FP64 sweeps "
+                         "(0: the library's default)")
     ap.add_argument("--ref-tol", type=float, default=1e-7,
                     help="residual tolerance (eV) of the untimed FP64 solve the eigenvalue errors are measured "
                          "against, where no golden eigenvalues cover the bands")
@@ -175,12 +178,13 @@ def main():
             print(f"bench: reference solve {solve['reference']}", file=sys.stderr, flush=True)
         for mixed in (False, True):
             t = time.perf_counter()
-            wv, _, st = ctx.solve_acc(nb, k, mixed=mixed, tol=args.tol, maxiter=400, extra=extra)
+            wv, _, st = ctx.solve_acc(nb, k, mixed=mixed, tol=args.tol, switch_tol=args.switch_tol, maxiter=400,
+                                      extra=extra)
             el = time.perf_counter() - t
             key = "mixed" if mixed else "fp64"
             # the same solve again on the launch-boundary timeline: where the time goes
             H.trace(True)
-            ctx.solve_acc(nb, k, mixed=mixed, tol=args.tol, maxiter=400, extra=extra)
+            ctx.solve_acc(nb, k, mixed=mixed, tol=args.tol, switch_tol=args.switch_tol, maxiter=400, extra=extra)
             phases = H.trace_read()
             H.trace(False)
             err = np.abs(wv - ref)
@@ -196,7 +200,9 @@ def main():
         if "fp64" in solve and "mixed" in solve:
             solve["speedup_mixed_vs_fp64"] = solve["fp64"]["seconds"] / solve["mixed"]["seconds"]
             solve["max_abs_diff_mixed_vs_fp64_eV"] = float(np.abs(evals["mixed"] - evals["fp64"]).max())
-        solve["controls"] = {"tol_eV": args.tol, "extra_bands": extra, "maxiter": 400}
+        solve["controls"] = {"tol_eV": args.tol, "extra_bands": extra, "maxiter": 400,
+                             "switch_tol_eV": args.switch_tol or "library default",
+                             "lock": H.get_tuning("solver.lock"), "precond": H.get_tuning("solver.precond")}
 
     if not args.no_solve and n <= 6000:
         # dense FP64 solve (assembly, overlap spectrum, generalized eigensolve),

@@ -300,6 +300,7 @@ void run_solve(Context &c, const double *k, int nbands, bool want_vectors, Devic
     // column in each Rayleigh-Ritz step (src/half_cuda_iterative_solver.cuf:113-119)
     AccControls ctl;
     ctl.lock = 0;
+    ctl.precond = 0;
     solve_acc(c.blas, op, nullptr, nbands, ctl, want_vectors, vec, res);
   } else {
     solve_dense(c.blas, op, c.solver, nbands, want_vectors, vec, res);

@@ -150,6 +150,79 @@ __global__ void precondition_kernel(const double2 *r, const double *kin, const d
   }
 }
 
+// Kinetic energy expectation <x|T|x> / <x|x> of band cols[c] of X (one
+// workgroup per correction column, fixed-order tree sums)
+__global__ void band_kinetic_kernel(const double2 *x, const double *kin, const int *cols, int n, int64_t ld,
+                                    double *ekin) {
+  __shared__ double red[2][kThreads];
+  const double2 *c = x + cols[blockIdx.x] * ld;
+  double a = 0, b = 0;
+  for (int g = threadIdx.x; g < n; g += blockDim.x) {
+    const double v = c[g].x * c[g].x + c[g].y * c[g].y;
+    a += kin[g] * v;
+    b += v;
+  }
+  red[0][threadIdx.x] = a;
+  red[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ekin[blockIdx.x] = red[1][0] > 0 ? red[0][0] / red[1][0] : 1.0;
+}
+
+// Teter-Payne-Allan preconditioner: P = -K(x) R with x = T_G / T_b (T_b the
+// band's kinetic energy expectation) and
+// K(x) = (27 + 18x + 12x^2 + 8x^3) / (27 + 18x + 12x^2 + 8x^3 + 16x^4)
+// (Teter, Payne and Allan, Phys. Rev. B 40, 12255 (1989)); seeds as in
+// precondition_kernel
+__global__ void precondition_tpa_kernel(const double2 *r, const double *kin, const double *ekin, const int *cols,
+                                        const int *seeds, double2 *p, int n, int np, int64_t ld) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(n) * np) return;
+  const int c = static_cast<int>(t / n);
+  const int g = static_cast<int>(t - int64_t(c) * n);
+  const int b = cols[c];
+  const int64_t o = g + c * ld, ro = g + b * ld;
+  if (seeds[c] >= 0) {
+    p[o] = make_double2(g == seeds[c] ? 1.0 : 0.0, 0.0);
+  } else {
+    const double x = kin[g] / fmax(ekin[c], 1e-12);
+    const double num = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x));
+    const double k = num / (num + 16.0 * x * x * x * x);
+    p[o] = make_double2(-k * r[ro].x, -k * r[ro].y);
+  }
+}
+
+// w[j] = Re(x_j^H (H x)_j) / Re(x_j^H (S x)_j) for column j = blockIdx.x + c0
+// (fixed-order tree sums)
+__global__ void rayleigh_quotient_kernel(const double2 *x, const double2 *hx, const double2 *sx, int n, int64_t ld,
+                                         int c0, double *w) {
+  __shared__ double red[2][kThreads];
+  const int j = c0 + blockIdx.x;
+  const double2 *a = x + j * ld, *h = hx + j * ld, *q = sx + j * ld;
+  double num = 0, den = 0;
+  for (int g = threadIdx.x; g < n; g += blockDim.x) {
+    num += a[g].x * h[g].x + a[g].y * h[g].y;
+    den += a[g].x * q[g].x + a[g].y * q[g].y;
+  }
+  red[0][threadIdx.x] = num;
+  red[1][threadIdx.x] = den;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && red[1][0] > 0) w[j] = red[0][0] / red[1][0];
+}
+
 // M <- D M D with D = diag(M)^-1/2 (zero where the diagonal is not positive);
 // d keeps D's diagonal
 __global__ void unit_diagonal_kernel(double2 *M, double *d, int m) {
@@ -208,6 +281,16 @@ bool g_hetrd = true;
 // are sorted by Ritz value.  0: every band keeps its column, the reference's
 // 2m-wide subspace.
 int g_lock = 2;
+// Preconditioner of the correction columns (key "solver.precond"): 0 the
+// reference's -R / (|T - eps| + 1 eV), 1 Teter-Payne-Allan.
+int g_precond = 0;
+// Mixed form: every `g_refresh` FP32 iterations (key "solver.refresh", 0 =
+// never) H X and S X of the bands still in the Rayleigh-Ritz problem are
+// recomputed with the FP32 operator and their Ritz values replaced by the
+// Rayleigh quotients: the rotations otherwise accumulate one FP32 operator
+// error per iteration into H X (about sqrt(iterations) times the error of
+// one application), which sets the floor the FP32 phase stalls at.
+int g_refresh = 0;
 
 // zheevd with vectors (upper triangle of A referenced; eigenvectors replace
 // A, eigenvalues ascending in w; e is workspace of n values).  Orders up to
@@ -325,6 +408,16 @@ bool set_solver_tuning(const std::string &key, int value) {
     g_lock = value;
     return true;
   }
+  if (key == "solver.precond") {
+    if (value < 0 || value > 1) return false;
+    g_precond = value;
+    return true;
+  }
+  if (key == "solver.refresh") {
+    if (value < 0 || value > 1000) return false;
+    g_refresh = value;
+    return true;
+  }
   if (key != "solver.hetrd" || value < 0 || value > 1) return false;
   g_hetrd = value != 0;
   return true;
@@ -333,6 +426,7 @@ bool set_solver_tuning(const std::string &key, int value) {
 bool solver_locking() { return g_lock != 0; }
 
 int AccControls::lock_mode() const { return lock < 0 ? g_lock : lock; }
+int AccControls::precond_mode() const { return precond < 0 ? g_precond : precond; }
 bool AccControls::locking() const { return lock_mode() != 0; }
 
 bool get_solver_tuning(const std::string &key, int &value) {
@@ -340,6 +434,10 @@ bool get_solver_tuning(const std::string &key, int &value) {
     value = g_stall;
   } else if (key == "solver.lock") {
     value = g_lock;
+  } else if (key == "solver.precond") {
+    value = g_precond;
+  } else if (key == "solver.refresh") {
+    value = g_refresh;
   } else if (key == "solver.hetrd") {
     value = g_hetrd ? 1 : 0;
   } else {
@@ -474,7 +572,7 @@ struct AccWork {
   int n, m, m2;
   DeviceArray<double2> Z, HZ, SZ, R, tmp, cross, metric, hsub, ssub, gwork;
   DeviceArray<float2> p32, hp32, sp32;
-  DeviceArray<double> w, mu, E, res2, kin, dscale, pivots;
+  DeviceArray<double> w, mu, E, res2, kin, dscale, pivots, ekin;
   DeviceArray<int> seeds, cols;
   DeviceArray<rocblas_int> info;
 };
@@ -515,6 +613,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   wk.info.resize(3);
   wk.dscale.resize(m);
   wk.pivots.resize(m + 1);
+  wk.ekin.resize(m);
   wk.kin.upload(op64.basis.kinetic, s);
   if (ctl.mixed) {
     wk.p32.resize(size_t(n) * m);
@@ -529,6 +628,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   double fp32_floor = 0;
   const bool locking = ctl.locking();
   const bool hard = ctl.lock_mode() == 2;
+  const bool tpa = ctl.precond_mode() == 1;
   int L = 0;              // frozen bands: columns 0..L-1 of X, HX, SX
   bool hard_used = false;
   res.applies = 0;
@@ -636,7 +736,17 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   int iter = 0;
   bool converged = false;
   int fp64_iters = 0;
+  int fp32_iters = 0;
   for (iter = 1; iter <= ctl.maxiter; ++iter) {
+    if (!fp64_phase && g_refresh > 0 && fp32_iters > 0 && fp32_iters % g_refresh == 0 && L < m) {
+      trace_mark("acc_fp32_refresh", s);
+      const size_t off = size_t(n) * L;
+      apply(m - L, X + off, HX + off, SX + off);
+      hipLaunchKernelGGL(rayleigh_quotient_kernel, dim3(m - L), dim3(kThreads), 0, s, X, HX, SX, n, ld, L,
+                         wk.w.get());
+      HALF_HIP(hipGetLastError());
+    }
+    if (!fp64_phase) ++fp32_iters;
     trace_mark("acc_residual", s);
     hipLaunchKernelGGL(residual_kernel, dim3(blocks_for(int64_t(n) * m)), dim3(kThreads), 0, s, HX, SX,
                        wk.w.get(), wk.R.get(), n, m, ld);
@@ -723,8 +833,15 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     trace_mark("acc_precondition", s);
     wk.seeds.upload(seeds, s);
     wk.cols.upload(cols, s);
-    hipLaunchKernelGGL(precondition_kernel, dim3(blocks_for(int64_t(n) * np)), dim3(kThreads), 0, s, wk.R.get(),
-                       wk.kin.get(), wk.w.get(), wk.cols.get(), wk.seeds.get(), P, n, np, ld);
+    if (tpa) {
+      hipLaunchKernelGGL(band_kinetic_kernel, dim3(np), dim3(kThreads), 0, s, X, wk.kin.get(), wk.cols.get(), n, ld,
+                         wk.ekin.get());
+      hipLaunchKernelGGL(precondition_tpa_kernel, dim3(blocks_for(int64_t(n) * np)), dim3(kThreads), 0, s,
+                         wk.R.get(), wk.kin.get(), wk.ekin.get(), wk.cols.get(), wk.seeds.get(), P, n, np, ld);
+    } else {
+      hipLaunchKernelGGL(precondition_kernel, dim3(blocks_for(int64_t(n) * np)), dim3(kThreads), 0, s, wk.R.get(),
+                         wk.kin.get(), wk.w.get(), wk.cols.get(), wk.seeds.get(), P, n, np, ld);
+    }
     HALF_HIP(hipGetLastError());
     apply(np, P, HP, SP);
     // S-orthogonalise the correction block against X (cross: m x np)

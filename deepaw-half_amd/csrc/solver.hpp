@@ -83,7 +83,10 @@ struct AccControls {
   int lock = -1;          // 0 off (the reference's 2m-wide subspace), 1 soft
                           // locking, 2 soft locking plus freezing of
                           // converged bands, -1 the process-wide key "solver.lock"
+  int precond = -1;       // 0 the reference's preconditioner, 1 Teter-Payne-Allan,
+                          // -1 the process-wide key "solver.precond"
   int lock_mode() const;
+  int precond_mode() const;
   bool locking() const;
 };
 
