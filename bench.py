@@ -62,6 +62,9 @@ def main():
                     help="residual tolerance (eV) of the untimed FP64 solve the eigenvalue errors are measured "
                          "against, where no golden eigenvalues cover the bands")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="length of the CPU baseline sample")
+    ap.add_argument("--precond", type=int, default=1,
+                    help="preconditioner of the all-band solves (solver.precond: 0 the reference's, 1 "
+                         "Teter-Payne-Allan)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="set a kernel tunable (half_amd_set_tuning) before measuring; repeatable")
     ap.add_argument("--kmesh", type=int, default=0, metavar="NK",
@@ -82,6 +85,7 @@ def main():
     import half_amd as H
     from half_amd import hip, synth
 
+    H.set_tuning("solver.precond", args.precond)
     for kv in args.tune:
         key, value = kv.split("=")
         H.set_tuning(key, int(value))
@@ -322,7 +326,8 @@ def kernel_roofline(kern, nr, grid, n, nproj, nb, steps, nlines, nplanes, fp64=T
     if name in per_step_flops:
         peak = MFMA_FP64_TFLOPS if fp64 else MFMA_FP32_TFLOPS
         ach = per_step_flops[name] * steps / t / 1e12
-        out.update(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak, traffic=None)
+        out.update(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
+                   traffic=measured_traffic(workload, nb, fp64, name) if workload else None)
         return out
     b = kernel_bytes(name, nr, grid, n, nlines, nplanes, nb, sc, sr)
     ach = b * steps / t / 1e9 if b else None

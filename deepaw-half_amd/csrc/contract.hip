@@ -16,6 +16,8 @@
 // current one is multiplied.  gram splits the rows into chunks whose partial
 // tiles are summed in a fixed order by a second kernel.
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "contract.hpp"
 #include "cplx.hpp"
@@ -322,7 +324,55 @@ struct GramPlan {
   size_t work_elems;  // partial tiles (all outputs and batch entries); 0 without row chunks
 };
 
-GramPlan plan_gram(int batch, int n, int ka, int kb, int nout, bool hermitian) {
+// Workgroups of gram_kernel<R, WT, NB> the device holds at once (occupancy
+// per CU x CUs of the current device), cached per variant and device.
+template <typename R, int WT, int NB>
+int gram_capacity() {
+  static std::mutex mu;
+  static std::map<int, int> cache;
+  int dev = 0;
+  HALF_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(dev);
+  if (it != cache.end()) return it->second;
+  int per_cu = 0, cus = 0;
+  HALF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void *>(&gram_kernel<R, WT, NB>), 256, 0));
+  HALF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int cap = std::max(1, per_cu) * std::max(1, cus);
+  cache[dev] = cap;
+  return cap;
+}
+
+template <typename R>
+int gram_capacity_of(int wt, int nout) {
+  if (wt == 1) return nout == 1 ? gram_capacity<R, 1, 1>() : gram_capacity<R, 1, 2>();
+  return nout == 1 ? gram_capacity<R, 2, 1>() : gram_capacity<R, 2, 2>();
+}
+
+// Row chunks for `units` output tiles (over the batch) on a device holding
+// `cap` workgroups at once: the fewest chunks that fill the device in rounds
+// at least 90% full (a grid of 1.1 rounds leaves the second round nearly
+// empty), at most 64 and at least 64 rows each; failing that, the count with
+// the fullest rounds.
+int choose_chunks(int units, int n, int cap) {
+  const int max_chunks = std::max(1, std::min(64, (n + 63) / 64));
+  int best = 1;
+  double best_eff = -1;
+  for (int c = 1; c <= max_chunks; ++c) {
+    const int64_t total = int64_t(units) * c;
+    const int64_t rounds = (total + cap - 1) / cap;
+    const double eff = double(total) / double(rounds * cap);
+    if (total >= cap && eff >= 0.9) return c;
+    if (eff > best_eff + 1e-9) {
+      best_eff = eff;
+      best = c;
+    }
+  }
+  return best;
+}
+
+GramPlan plan_gram(int batch, int n, int ka, int kb, int nout, bool hermitian, int cap_wt1, int cap_wt2) {
   GramPlan p{};
   // 64-wide tiles; for the subspace Grams (Hermitian, matrix-core bound)
   // 32-wide ones where they cover the result with less padding (80 columns:
@@ -334,8 +384,7 @@ GramPlan plan_gram(int batch, int n, int ka, int kb, int nout, bool hermitian) {
   const int tw = 32 * p.wt;
   p.tiles_a = (ka + tw - 1) / tw;
   p.tiles = hermitian ? p.tiles_a * (p.tiles_a + 1) / 2 : p.tiles_a * ((kb + tw - 1) / tw);
-  // about a thousand workgroups over the batch, each chunk at least 64 rows
-  int nchunks = std::max(1, std::min({1024 / std::max(p.tiles * batch, 1), (n + 63) / 64, 64}));
+  int nchunks = choose_chunks(p.tiles * batch, n, p.wt == 1 ? cap_wt1 : cap_wt2);
   int chunk = (n + nchunks - 1) / nchunks;
   chunk = (chunk + KS - 1) / KS * KS;
   p.nchunks = (n + chunk - 1) / chunk;
@@ -366,7 +415,8 @@ void gram_batched_work(int batch, int n, int ka, int kb, const cplx_t<R> *A, int
   }
   if (lda < n || ldb < n || ldc < ka || (B2 && (ldb2 < n || ldc2 < ka)))
     throw Error(kInvalidArgument, "gram: leading dimension too small");
-  const GramPlan plan = plan_gram(batch, n, ka, kb, nb, hermitian);
+  const GramPlan plan = plan_gram(batch, n, ka, kb, nb, hermitian, gram_capacity_of<R>(1, nb),
+                                  gram_capacity_of<R>(2, nb));
   const int nchunks = plan.nchunks;
   if (plan.work_elems > 0) {
     if (grow) {
@@ -442,9 +492,10 @@ void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t 
                        nullptr, 0, s, hermitian);
 }
 
+template <typename R>
 size_t gram_work_elems(int n, int ka, int kb) {
   if (n <= 0 || ka <= 0 || kb <= 0) return 0;
-  return plan_gram(1, n, ka, kb, 1, false).work_elems;
+  return plan_gram(1, n, ka, kb, 1, false, gram_capacity_of<R>(1, 1), gram_capacity_of<R>(2, 1)).work_elems;
 }
 
 template <typename R>
@@ -524,6 +575,7 @@ void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, co
   template void gram<R>(int, int, int, const cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *, \
                         int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *, int64_t, DeviceArray<cplx_t<R>> &, \
                         hipStream_t, bool);                                                                  \
+  template size_t gram_work_elems<R>(int, int, int);                                                        \
   template void gram_into<R>(int, int, int, const cplx_t<R> *, int64_t, const cplx_t<R> *, int64_t, cplx_t<R> *,  \
                              int64_t, cplx_t<R> *, size_t, hipStream_t);                                     \
   template void gram_batched<R>(int, int, int, int, const cplx_t<R> *, int64_t, int64_t, const cplx_t<R> *,    \

@@ -30,6 +30,7 @@ void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t 
 
 // Elements of partial-tile workspace gram needs for C = A^H B (n rows, ka x
 // kb result, one output, not Hermitian); 0 when the rows are not split.
+template <typename R>
 size_t gram_work_elems(int n, int ka, int kb);
 
 // That gram with a caller-provided workspace of work_elems elements (no

@@ -616,7 +616,7 @@ size_t KOperator<R>::external_bytes(int bb) const {
     // partials of any batch up to bb bands (the last batch may be narrower,
     // and narrower batches can split the rows into more chunks)
     size_t worst = 0;
-    for (int c = 1; c <= bb; ++c) worst = std::max(worst, gram_work_elems(n, np, c));
+    for (int c = 1; c <= bb; ++c) worst = std::max(worst, gram_work_elems<R>(n, np, c));
     b += align_up(worst * sizeof(V));
   }
   return b;

@@ -282,7 +282,11 @@ bool g_hetrd = true;
 // 2m-wide subspace.
 int g_lock = 2;
 // Preconditioner of the correction columns (key "solver.precond"): 0 the
-// reference's -R / (|T - eps| + 1 eV), 1 Teter-Payne-Allan.
+// reference's -R / (|T - eps| + 1 eV) (default; the drop-in half_solve_kpoint
+// path always uses it), 1 Teter-Payne-Allan: 63 instead of 96 iterations on
+// the 216-atom cell (512 bands), but slower where a requested band has no
+// buffer above it (2-atom cell, 16 bands: 148 instead of 90 iterations to
+// 1e-7 eV in the oracle's restatement).
 int g_precond = 0;
 // Mixed form: every `g_refresh` FP32 iterations (key "solver.refresh", 0 =
 // never) H X and S X of the bands still in the Rayleigh-Ritz problem are

@@ -129,25 +129,28 @@ def test_acc_abi_defaults(si2):
     # returned vectors are S-orthonormal Ritz vectors
     _, s = ctx.assemble_hs(g["kpoint"])
     assert np.abs(v.conj().T @ s @ v - np.eye(g["nbands"])).max() < 1e-10
-    # the drop-in path runs the reference's iteration (full 2m-wide subspace)
-    # whatever the process-wide locking key says: the same Ritz pairs under
-    # either key value, and the same as the extension's solve with locking
-    # off at the reference's controls
+    # the drop-in path runs the reference's iteration (full 2m-wide subspace,
+    # the reference's preconditioner) whatever the process-wide locking key
+    # says: the same Ritz pairs under any key value, and the same as the
+    # extension's solve with locking off at the reference's controls
     saved = H.get_tuning("solver.lock")
-    for lock in (2, 1, 0):
-        H.set_tuning("solver.lock", lock)
-        try:
+    saved_pc = H.get_tuning("solver.precond")
+    try:
+        # and whatever the preconditioner key says
+        for lock, pc in ((2, 1), (1, 0), (0, 1)):
+            H.set_tuning("solver.lock", lock)
+            H.set_tuning("solver.precond", pc)
             ctx.set_solver(H.SOLVER_ACC)
             wa, va, _, _ = ctx.solve(g["nbands"], g["kpoint"], vectors=True)
             ctx.set_solver(H.SOLVER_EVD)
             assert np.abs(wa - w).max() == 0.0 and np.abs(va - v).max() == 0.0
-        finally:
-            H.set_tuning("solver.lock", saved)
-    H.set_tuning("solver.lock", 0)
-    try:
+        H.set_tuning("solver.lock", 0)
+        H.set_tuning("solver.precond", 0)
         we, ve, st = ctx.solve_acc(g["nbands"], g["kpoint"], tol=1e-4, maxiter=40, vectors=True)
     finally:
+        ctx.set_solver(H.SOLVER_EVD)
         H.set_tuning("solver.lock", saved)
+        H.set_tuning("solver.precond", saved_pc)
     assert st["rr_order_mean"] == 2 * g["nbands"]
     assert np.abs(we - w).max() < 1e-12 and np.abs(ve - v).max() < 1e-10
 
@@ -418,6 +421,23 @@ def test_hfo2_mixed_96_bands(hfo2):
     print("[gpu] hfo2 96-band per-band |d eps| (eV):", " ".join(f"{e:.1e}" for e in err), flush=True)
     assert st["fp64_iterations"] >= 1 and st["max_residual_eV"] <= 1e-6
     assert err.max() < 1e-8
+
+
+def test_hfo2_acc_tpa_preconditioner(hfo2):
+    """The optional Teter-Payne-Allan preconditioner (solver.precond 1)
+    converges to the reference's eigenvalues, FP64 and mixed."""
+    g, ctx = hfo2
+    ref = np.array(g["eigenvalues"][:64])
+    saved = H.get_tuning("solver.precond")
+    H.set_tuning("solver.precond", 1)
+    try:
+        for mixed in (False, True):
+            w, _, st = ctx.solve_acc(64, mixed=mixed, tol=1e-6, maxiter=300, extra=8)
+            print(f"[gpu] hfo2 acc tpa mixed={mixed}", st, np.abs(w - ref).max(), flush=True)
+            assert st["max_residual_eV"] <= 1e-6
+            assert np.abs(w - ref).max() < (1e-8 if mixed else 1e-9)
+    finally:
+        H.set_tuning("solver.precond", saved)
 
 
 def test_hfo2_mixed_switch_below_fp32_floor(hfo2):
