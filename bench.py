@@ -288,13 +288,14 @@ def kernel_bytes(name, nr, grid, n, nlines, nplanes, nb, sc, sr):
         "local_y_bwd": line + plane,
         "local_x_veff": 2 * plane + nr * sr,
         "local_x_conv": 2 * plane,
+        "local_x_mfma": 2 * plane,
         "local_y_fwd": plane + line,
         "local_z_fwd": line + n * (3 * sc + sr + 8),
         # fused y-x-y pass: each active line's values read and written once
         "local_yxy": 2 * line,
     }
     # the convolution forms read the i1-axis DFT of Veff (nr complex) once per launch
-    once = {"local_x_conv": nr * sc, "local_yxy": nr * sc}
+    once = {"local_x_conv": nr * sc, "local_x_mfma": nr * sc, "local_yxy": nr * sc}
     return None if name not in table else nb * table[name] + once.get(name, 0)
 
 

@@ -21,6 +21,7 @@
 
 #include "../../include/half_amd.h"
 #include "common.hpp"
+#include "contract.hpp"
 #include "forces.hpp"
 #include "energy.hpp"
 #include "fset.hpp"
@@ -1054,7 +1055,8 @@ int half_amd_nonlocal_forces(half_handle handle, const double kpoint[3], int nba
 
 int half_amd_set_tuning(const char *key, int value, char *error, int capacity) {
   clear_error(error, capacity);
-  if (!key || !(set_local_tuning(key, value) || set_operator_tuning(key, value) || set_solver_tuning(key, value)))
+  if (!key || !(set_local_tuning(key, value) || set_operator_tuning(key, value) || set_solver_tuning(key, value) ||
+                set_contract_tuning(key, value)))
     return export_error(kInvalidArgument, "unknown tuning key or value out of range", error, capacity);
   return int(kSuccess);
 }
@@ -1062,7 +1064,8 @@ int half_amd_set_tuning(const char *key, int value, char *error, int capacity) {
 int half_amd_get_tuning(const char *key, int *value, char *error, int capacity) {
   clear_error(error, capacity);
   if (!key || !value ||
-      !(get_local_tuning(key, *value) || get_operator_tuning(key, *value) || get_solver_tuning(key, *value)))
+      !(get_local_tuning(key, *value) || get_operator_tuning(key, *value) || get_solver_tuning(key, *value) ||
+        get_contract_tuning(key, *value)))
     return export_error(kInvalidArgument, "unknown tuning key", error, capacity);
   return int(kSuccess);
 }

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <string>
 
 #include "contract.hpp"
 #include "cplx.hpp"
@@ -318,6 +319,19 @@ void launch_tall(dim3 grid, const TallArgs<R> &a, hipStream_t s) {
 
 namespace {
 
+// Tuning (half_amd_set_tuning keys contract.<fp64|fp32>.*; index 0 FP64,
+// 1 FP32): tile width of gram and gemm_tall (0 = the shape rule below, 1 =
+// 32-wide, 2 = 64-wide) and the fill a gram row-chunk count must reach in
+// every round of workgroups (percent).
+int g_gram_wt[2] = {0, 0};
+int g_gram_fill[2] = {90, 90};
+int g_tall_wt[2] = {0, 0};
+
+template <typename R>
+constexpr int prec_index() {
+  return sizeof(R) == 8 ? 0 : 1;
+}
+
 // Tile width, grid and row chunking of a gram launch.
 struct GramPlan {
   int wt, tiles_a, tiles, nchunks, chunk;
@@ -352,10 +366,10 @@ int gram_capacity_of(int wt, int nout) {
 
 // Row chunks for `units` output tiles (over the batch) on a device holding
 // `cap` workgroups at once: the fewest chunks that fill the device in rounds
-// at least 90% full (a grid of 1.1 rounds leaves the second round nearly
-// empty), at most 64 and at least 64 rows each; failing that, the count with
-// the fullest rounds.
-int choose_chunks(int units, int n, int cap) {
+// at least `fill` percent full (default 90: a grid of 1.1 rounds leaves the
+// second round nearly empty), at most 64 and at least 64 rows each; failing
+// that, the count with the fullest rounds.
+int choose_chunks(int units, int n, int cap, int fill) {
   const int max_chunks = std::max(1, std::min(64, (n + 63) / 64));
   int best = 1;
   double best_eff = -1;
@@ -363,7 +377,7 @@ int choose_chunks(int units, int n, int cap) {
     const int64_t total = int64_t(units) * c;
     const int64_t rounds = (total + cap - 1) / cap;
     const double eff = double(total) / double(rounds * cap);
-    if (total >= cap && eff >= 0.9) return c;
+    if (total >= cap && eff >= 0.01 * fill) return c;
     if (eff > best_eff + 1e-9) {
       best_eff = eff;
       best = c;
@@ -372,7 +386,8 @@ int choose_chunks(int units, int n, int cap) {
   return best;
 }
 
-GramPlan plan_gram(int batch, int n, int ka, int kb, int nout, bool hermitian, int cap_wt1, int cap_wt2) {
+GramPlan plan_gram(int batch, int n, int ka, int kb, int nout, bool hermitian, int cap_wt1, int cap_wt2, int wt_force,
+                   int fill) {
   GramPlan p{};
   // 64-wide tiles; for the subspace Grams (Hermitian, matrix-core bound)
   // 32-wide ones where they cover the result with less padding (80 columns:
@@ -380,11 +395,11 @@ GramPlan plan_gram(int batch, int n, int ka, int kb, int nout, bool hermitian, i
   // wider tiles stops mattering and their halved operand re-reads win
   const int kmax = std::max(ka, kb);
   const bool pad_less_32 = (kmax + 31) / 32 * 32 < (kmax + 63) / 64 * 64;
-  p.wt = (kmax <= 32 || (hermitian && pad_less_32 && kmax < 256)) ? 1 : 2;
+  p.wt = wt_force ? wt_force : (kmax <= 32 || (hermitian && pad_less_32 && kmax < 256)) ? 1 : 2;
   const int tw = 32 * p.wt;
   p.tiles_a = (ka + tw - 1) / tw;
   p.tiles = hermitian ? p.tiles_a * (p.tiles_a + 1) / 2 : p.tiles_a * ((kb + tw - 1) / tw);
-  int nchunks = choose_chunks(p.tiles * batch, n, p.wt == 1 ? cap_wt1 : cap_wt2);
+  int nchunks = choose_chunks(p.tiles * batch, n, p.wt == 1 ? cap_wt1 : cap_wt2, fill);
   int chunk = (n + nchunks - 1) / nchunks;
   chunk = (chunk + KS - 1) / KS * KS;
   p.nchunks = (n + chunk - 1) / chunk;
@@ -416,7 +431,8 @@ void gram_batched_work(int batch, int n, int ka, int kb, const cplx_t<R> *A, int
   if (lda < n || ldb < n || ldc < ka || (B2 && (ldb2 < n || ldc2 < ka)))
     throw Error(kInvalidArgument, "gram: leading dimension too small");
   const GramPlan plan = plan_gram(batch, n, ka, kb, nb, hermitian, gram_capacity_of<R>(1, nb),
-                                  gram_capacity_of<R>(2, nb));
+                                  gram_capacity_of<R>(2, nb), g_gram_wt[prec_index<R>()],
+                                  g_gram_fill[prec_index<R>()]);
   const int nchunks = plan.nchunks;
   if (plan.work_elems > 0) {
     if (grow) {
@@ -495,7 +511,9 @@ void gram_batched(int batch, int n, int ka, int kb, const cplx_t<R> *A, int64_t 
 template <typename R>
 size_t gram_work_elems(int n, int ka, int kb) {
   if (n <= 0 || ka <= 0 || kb <= 0) return 0;
-  return plan_gram(1, n, ka, kb, 1, false, gram_capacity_of<R>(1, 1), gram_capacity_of<R>(2, 1)).work_elems;
+  return plan_gram(1, n, ka, kb, 1, false, gram_capacity_of<R>(1, 1), gram_capacity_of<R>(2, 1),
+                   g_gram_wt[prec_index<R>()], g_gram_fill[prec_index<R>()])
+      .work_elems;
 }
 
 template <typename R>
@@ -532,7 +550,8 @@ void gemm_tall_batched(int batch, int n, int m, int k, R alpha, const cplx_t<R> 
   // 64-wide tiles when they still give a few hundred workgroups (always in
   // place: one column tile, so a workgroup owns whole rows)
   const int64_t t64 = int64_t((n + 63) / 64) * ((m + 63) / 64) * batch;
-  const int wt = (inplace || t64 >= 512) ? 2 : 1;
+  const int forced = g_tall_wt[prec_index<R>()];
+  const int wt = inplace ? 2 : forced ? forced : t64 >= 512 ? 2 : 1;
   const int tw = 32 * wt;
   TallArgs<R> a{};
   a.A = A;
@@ -569,6 +588,42 @@ void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, co
                R beta, cplx_t<R> *C, int64_t ldc, const cplx_t<R> *W2, int64_t ldw2, cplx_t<R> *C2, int64_t ldc2,
                hipStream_t s) {
   gemm_tall_batched<R>(1, n, m, k, alpha, A, lda, 0, W, ldw, 0, beta, C, ldc, 0, W2, ldw2, 0, C2, ldc2, 0, s);
+}
+
+bool set_contract_tuning(const std::string &key, int value) {
+  for (int p = 0; p < 2; ++p) {
+    const std::string pre = p == 0 ? "contract.fp64." : "contract.fp32.";
+    if (key == pre + "gram.wt" || key == pre + "tall.wt") {
+      if (value < 0 || value > 2) return false;
+      (key == pre + "gram.wt" ? g_gram_wt : g_tall_wt)[p] = value;
+      return true;
+    }
+    if (key == pre + "gram.fill") {
+      if (value < 10 || value > 100) return false;
+      g_gram_fill[p] = value;
+      return true;
+    }
+  }
+  return false;
+}
+
+bool get_contract_tuning(const std::string &key, int &value) {
+  for (int p = 0; p < 2; ++p) {
+    const std::string pre = p == 0 ? "contract.fp64." : "contract.fp32.";
+    if (key == pre + "gram.wt") {
+      value = g_gram_wt[p];
+      return true;
+    }
+    if (key == pre + "tall.wt") {
+      value = g_tall_wt[p];
+      return true;
+    }
+    if (key == pre + "gram.fill") {
+      value = g_gram_fill[p];
+      return true;
+    }
+  }
+  return false;
 }
 
 #define HALF_CONTRACT(R)                                                                                     \

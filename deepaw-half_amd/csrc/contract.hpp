@@ -52,6 +52,13 @@ void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, co
 
 constexpr int gemm_tall_inplace_max = 64;
 
+// Tuning keys contract.<fp64|fp32>.gram.wt / tall.wt (0 = shape rule, 1 =
+// 32-wide tiles, 2 = 64-wide) and contract.<fp64|fp32>.gram.fill (10-100:
+// percent of the device's workgroup slots every round of a row-chunked gram
+// must fill).  False for an unknown key or a value out of range.
+bool set_contract_tuning(const std::string &key, int value);
+bool get_contract_tuning(const std::string &key, int &value);
+
 // Batched form of gemm_tall (entry b's operands at base + b stride).
 template <typename R>
 void gemm_tall_batched(int batch, int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, int64_t sA,

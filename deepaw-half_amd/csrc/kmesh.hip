@@ -64,6 +64,25 @@ __global__ void init_states_kernel(double2 *x, int64_t ld, int64_t zk, const int
   x[k * zk + g + b * ld] = make_double2(g == idx[k * m + b] ? 1.0 : 0.0, 0.0);
 }
 
+// p_k(g, j) = exp(i theta) for g < npw_k, zero on the padding rows: dense
+// probe columns of norm sqrt(npw_k), a pseudo-random phase per element
+// (integer hash of g and j, the same at every k point)
+__global__ void phase_block_kernel(double2 *p, int64_t zk, const int *npw, int N, int m, int64_t ld) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= int64_t(N) * m) return;
+  const int k = blockIdx.y;
+  const int j = static_cast<int>(t / N), g = static_cast<int>(t - int64_t(j) * N);
+  double2 v = make_double2(0.0, 0.0);
+  if (g < npw[k]) {
+    uint32_t h = static_cast<uint32_t>(g) * 2654435761u ^ (static_cast<uint32_t>(j) + 0x9e3779b9u) * 2246822519u;
+    h ^= h >> 15;
+    h *= 2654435761u;
+    h ^= h >> 13;
+    sincos(6.283185307179586 * (h * 2.3283064365386963e-10), &v.y, &v.x);
+  }
+  p[k * zk + g + j * ld] = v;
+}
+
 // R_k = HX_k - SX_k diag(w_k) (reference :29-40)
 __global__ void residual_kernel(const double2 *hx, const double2 *sx, int64_t zk, const double *w, int64_t wk,
                                 double2 *r, int64_t rk, int N, int m, int64_t ld) {
@@ -530,13 +549,22 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
     d_idx.upload(idx, s);
     hipLaunchKernelGGL(init_states_kernel, gNm, dim3(kThreads), 0, s, X, ld, zk, d_idx.get(), N, m);
     HALF_HIP(hipGetLastError());
+    DeviceArray<int> d_npw;
     apply(m, X, HX, SX);
     if (!fp64_phase) {
-      // FP32 residual floor from the FP32 operator's error on these vectors
-      // (Rr, tmp are free until the loop; their padding rows are zero)
-      op64.apply(m, X, ld, zk, Rr.get(), tmp.get(), ld, rk, s);
-      hipLaunchKernelGGL(difference_kernel, gNm, dim3(kThreads), 0, s, HX, zk, Rr.get(), rk, N, m, ld);
-      hipLaunchKernelGGL(difference_kernel, gNm, dim3(kThreads), 0, s, SX, zk, tmp.get(), rk, N, m, ld);
+      // FP32 residual floor: the FP32 operator's error on a dense probe
+      // block per k point, |dH p| + max|eps| |dS p| per unit column (P, HP,
+      // SP, Rr and tmp are free until the loop; padding rows stay zero).
+      // Unit vectors would understate it: their projector coefficients
+      // involve no accumulation.
+      d_npw.upload(op64.npw, s);
+      hipLaunchKernelGGL(phase_block_kernel, gNm, dim3(kThreads), 0, s, P, zk, d_npw.get(), N, m, ld);
+      HALF_HIP(hipGetLastError());
+      apply(m, P, HP, SP);
+      op64.apply(m, P, ld, zk, Rr.get(), tmp.get(), ld, rk, s);
+      res.applies += int64_t(m) * nk;
+      hipLaunchKernelGGL(difference_kernel, gNm, dim3(kThreads), 0, s, HP, zk, Rr.get(), rk, N, m, ld);
+      hipLaunchKernelGGL(difference_kernel, gNm, dim3(kThreads), 0, s, SP, zk, tmp.get(), rk, N, m, ld);
       hipLaunchKernelGGL(column_norm2_kernel, dim3(nk * m), dim3(kThreads), 0, s, Rr.get(), ld, N, res2.get());
       hipLaunchKernelGGL(column_norm2_kernel, dim3(nk * m), dim3(kThreads), 0, s, tmp.get(), ld, N, mu.get());
       HALF_HIP(hipGetLastError());
@@ -549,8 +577,13 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
       HALF_HIP(hipMemcpyAsync(ev.data(), w.get(), sizeof(double) * ev.size(), hipMemcpyDeviceToHost, s));
       HALF_HIP(hipStreamSynchronize(s));
       std::vector<double> e(size_t(nk) * m);
-      for (int j = 0; j < nk; ++j) std::copy(ev.begin() + size_t(j) * m2, ev.begin() + size_t(j) * m2 + m,
-                                             e.begin() + size_t(j) * m);
+      for (int j = 0; j < nk; ++j) {
+        std::copy(ev.begin() + size_t(j) * m2, ev.begin() + size_t(j) * m2 + m, e.begin() + size_t(j) * m);
+        for (int b = 0; b < m; ++b) {  // per unit column: |p|^2 = npw_k
+          eh[size_t(j) * m + b] /= op64.npw[j];
+          es[size_t(j) * m + b] /= op64.npw[j];
+        }
+      }
       fp32_floor = fp32_floor_estimate(eh, es, e);
     }
     HALF_HIP(hipStreamSynchronize(s));

@@ -10,14 +10,15 @@
 //           from psi (no box clearing), backward transform along i3
 //   y_bwd   active i1 planes: backward transform along i2, loading only the
 //           active lines of the plane (the rest are zeros, never stored)
-//   x_conv  every (i2,i3) column: restricted to the active i1 planes the
+//   x_mfma  every (i2,i3) column: restricted to the active i1 planes the
 //           backward transform along i1, the multiply by Veff/N_r and the
 //           forward transform are one circular convolution,
 //             out[p'] = sum_p Vx[(p' - p) mod n1][c] in[p],
 //           with Vx[d][c] = sum_x Veff[x][c]/N_r e^{-2 pi i d x/n1} computed
-//           once per potential; each active plane is read and written once
-//           (x_veff, the transform-based form, serves boxes whose active
-//           plane count exceeds the register budget)
+//           once per potential, run as a Toeplitz product on the matrix
+//           cores (x_conv: the same on the vector units); each active plane
+//           is read and written once (x_veff, the transform-based form,
+//           serves boxes whose active plane count exceeds both)
 //   y_fwd   active planes: forward transform along i2, store active lines
 //   z_fwd   active lines: forward transform along i3, gather the plane-wave
 //           coefficients and add T psi (S psi = psi is written here too)
@@ -36,6 +37,7 @@
 #include <algorithm>
 
 #include "local.hpp"
+#include "mfma.hpp"
 #include "stockham.hpp"
 #include "trace.hpp"
 
@@ -50,7 +52,11 @@ struct Tuning {
   // defaults from the autotuner on the 48^3 HfO2-like box, 64 bands
   int tile[2][3] = {{1536, 1024, 768}, {3072, 2048, 1024}};  // [fp64|fp32][x|y|z]
   int xgroup[2] = {8, 8};
-  int xconv[2] = {1, 1};  // x pass as a per-column convolution when it fits
+  // x pass where every k point's active planes are one run: 0 transforms,
+  // 1 per-column convolution on the vector units, 2 the same as a Toeplitz
+  // product on the matrix cores (96^3 box, 41 planes, 512 bands: 1.76 ms
+  // against 3.22 ms FP64, 0.92 against 1.49 ms FP32; transforms 10.1 ms)
+  int xconv[2] = {2, 2};
   int fused[2] = {1, 1};  // y, x, y passes in one kernel where the convolution applies
   int ygroup[2] = {4, 4}; // bands per workgroup of the fused kernel (fewer where LDS is short)
   int vxlds[2] = {1, 1};  // fused kernel with the slice's Vx rows in LDS where it applies
@@ -303,6 +309,136 @@ __global__ __launch_bounds__(256) void x_conv_kernel(cplx_t<R> *box, const cplx_
     int i1 = s0 + op;
     i1 -= i1 >= t.n1 ? t.n1 : 0;
     col[static_cast<long long>(i1) * C] = acc;
+  }
+}
+
+// The same convolution on the matrix cores.  For one (i2, i3) column c the x
+// pass over the P active planes and a group of bands is the product
+//   Out[p'][b] = sum_p T_c[p'][p] In[p][b],  T_c[p'][p] = Vx[(p' - p) mod n1][c],
+// a P x P Toeplitz matrix times a P x 16 block of 16 bands.  A workgroup
+// takes kXmCols consecutive columns and one 16-band tile; each wave owns two
+// columns and accumulates their complex 16x16 output tiles (ceil(P / 16) per
+// column) over ceil(P / 4) steps of 4 planes.  Per step the workgroup stages
+// the In block [4 planes][kXmCols columns][16 bands] in LDS (loads along the
+// columns: each plane row of the tile is one contiguous run), double-
+// buffered so the next step's loads are in flight during this step's MFMAs;
+// the B operand comes from there, the A operand (T_c) from the column's
+// 2P - 1 Vx values, also in LDS.  The results go back through LDS, one
+// 16-plane output tile at a time, and are stored along the columns over the
+// input planes (every step has been read before the first store).  Column
+// groups are numbered XCD-major: workgroup x runs on XCD x mod 8 (round-robin
+// dispatch), so groups that share 128-byte lines (FP32: two per line) meet in
+// one L2 (gridDim.x is a multiple of 8; groups past the end are empty).
+constexpr int kXmCols = 8;
+
+template <typename R, int PMAX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void x_mfma_kernel(
+    cplx_t<R> *box, const cplx_t<R> *vx, const LocalTables *tabs, KStride ks, int nb, int groups_per_xcd) {
+  using V = cplx_t<R>;
+  using A4 = typename Acc<R>::type;
+  constexpr int MT = (PMAX + 15) / 16;  // 16-plane output tiles
+  constexpr int SB = 17;                // band row length of a staged step (16 + 1 pad)
+  constexpr int OC = kXmCols + 1;       // column row length of an output tile (8 + 1 pad)
+  constexpr int STAGE = 4 * kXmCols * SB;
+  constexpr int UNION = (2 * STAGE > 16 * 16 * OC) ? 2 * STAGE : 16 * 16 * OC;
+  __shared__ V tw[kXmCols][2 * PMAX - 1];
+  __shared__ V buf[UNION];  // two staged steps, then one output tile
+  const int kp = blockIdx.z;
+  const LocalTables t = tabs[kp];
+  const int C = t.n2 * t.n3;
+  const int P = t.nplanes;
+  if (P > PMAX) return;  // the same for the whole workgroup; the host never launches it
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c0 = ((blockIdx.x & 7) * groups_per_xcd + (blockIdx.x >> 3)) * kXmCols;
+  const int b0 = blockIdx.y * 16;
+  box += kp * ks.box;
+  const int s0 = t.plane_s0 < 0 ? t.plane_s0 + t.n1 : t.plane_s0;
+  // tw[w][d + P - 1] = Vx[d mod n1][c0 + w] for d in [-(P - 1), P - 1]
+  for (int e = tid; e < kXmCols * (2 * P - 1); e += blockDim.x) {
+    const int w = e % kXmCols, k = e / kXmCols;
+    const int c = c0 + w;
+    int d = (k - (P - 1)) % t.n1;
+    d += d < 0 ? t.n1 : 0;
+    tw[w][k] = c < C ? vx[static_cast<long long>(d) * C + c] : mk<V>(R(0), R(0));
+  }
+  // staging: element e = tid + 256 j of a step is column e & 7, band
+  // (e >> 3) & 15, plane (e >> 7) of the step
+  auto fetch = [&](int k4, V (&r)[2]) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int e = tid + 256 * j;
+      const int cc = e & 7, band = (e >> 3) & 15, q = 4 * k4 + (e >> 7);
+      int i1 = s0 + q;
+      i1 -= i1 >= t.n1 ? t.n1 : 0;
+      const bool ok = q < P && b0 + band < nb && c0 + cc < C;
+      r[j] = ok ? box[static_cast<long long>(b0 + band) * t.nr + static_cast<long long>(i1) * C + c0 + cc]
+                : mk<V>(R(0), R(0));
+    }
+  };
+  auto stash = [&](int st, const V (&r)[2]) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int e = tid + 256 * j;
+      buf[st * STAGE + ((e >> 7) * kXmCols + (e & 7)) * SB + ((e >> 3) & 15)] = r[j];
+    }
+  };
+  const int nk4 = (P + 3) / 4;
+  const int row = lane & 15, kq = lane >> 4;
+  A4 acc[2][MT][kCmacParts<R>];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int u = 0; u < kCmacParts<R>; ++u) acc[j][mt][u] = A4{0, 0, 0, 0};
+  V r[2];
+  fetch(0, r);
+  stash(0, r);
+  for (int k4 = 0; k4 < nk4; ++k4) {
+    __syncthreads();  // step k4 staged; every reader of the other buffer is done
+    if (k4 + 1 < nk4) fetch(k4 + 1, r);
+    const V *stg = buf + (k4 & 1) * STAGE;
+    const int q = 4 * k4 + kq;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int cc = 2 * wave + j;
+      const V bv = stg[(kq * kXmCols + cc) * SB + row];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        if (16 * mt >= P) continue;  // uniform
+        const int op = 16 * mt + row;
+        const bool live = op < P && q < P;
+        const V a = tw[cc][live ? op - q + P - 1 : 0];
+        half::cmac<R>(live ? a.x : R(0), live ? a.y : R(0), bv.x, bv.y, acc[j][mt]);
+      }
+    }
+    if (k4 + 1 < nk4) stash((k4 + 1) & 1, r);
+  }
+  // output tiles through LDS: buf[(op' * 16 + band) * OC + column]
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    if (16 * mt >= P) continue;  // uniform
+    __syncthreads();  // the staged steps (or the previous tile) are no longer read
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        R re, im;
+        cmac_result<R>(acc[j][mt], rr, re, im);
+        buf[(mfma_row<R>(lane, rr) * 16 + row) * OC + 2 * wave + j] = mk<V>(re, im);
+      }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+      const int e = tid + 256 * it;
+      const int cc = e & 7, band = (e >> 3) & 15, opl = e >> 7;
+      const int op = 16 * mt + opl;
+      int i1 = s0 + op;
+      i1 -= i1 >= t.n1 ? t.n1 : 0;
+      if (op < P && b0 + band < nb && c0 + cc < C)
+        box[static_cast<long long>(b0 + band) * t.nr + static_cast<long long>(i1) * C + c0 + cc] =
+            buf[(opl * 16 + band) * OC + cc];
+    }
   }
 }
 
@@ -696,6 +832,14 @@ const void *x_conv_for(int planes, int &pmax) {
   return nullptr;
 }
 
+template <typename R>
+const void *x_mfma_for(int planes) {
+  if (planes <= 16) return reinterpret_cast<const void *>(&x_mfma_kernel<R, 16>);
+  if (planes <= 32) return reinterpret_cast<const void *>(&x_mfma_kernel<R, 32>);
+  if (planes <= 48) return reinterpret_cast<const void *>(&x_mfma_kernel<R, 48>);
+  return nullptr;
+}
+
 // Transform lengths with compile-time kernels (the boxes of the workloads
 // in use); any other length runs the runtime-plan kernels.
 #define HALF_CT_LENGTHS(X) X(24) X(40) X(48) X(96)
@@ -906,13 +1050,24 @@ void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff,
     launch<R>("local_y_bwd", y_for<R, true>(n2), ay, s, args, gy);
   }
   // x pass: per-column convolution where every k point's active planes are
-  // one run and the widest fits the register budget, else transform-based
+  // one run and the widest fits the register budget (on the matrix cores
+  // with xconv 2), else transform-based
   int pmax = 0;
   int pw = kb.max_planes;
   const size_t xconv_lds = size_t(2 * pw - 1) * kXcCols * sizeof(V);
-  const bool xconv_ok = vx && g_tune.xconv[prec] && kb.contiguous && pw <= n1 && xconv_lds <= 65536;
-  const void *xconv = xconv_ok ? x_conv_for<R>(pw, pmax) : nullptr;
-  if (xconv) {
+  const bool conv_ok = vx && kb.contiguous && pw <= n1;
+  const void *xmfma = conv_ok && g_tune.xconv[prec] == 2 ? x_mfma_for<R>(pw) : nullptr;
+  const bool xconv_ok = conv_ok && g_tune.xconv[prec] && xconv_lds <= 65536;
+  const void *xconv = !xmfma && xconv_ok ? x_conv_for<R>(pw, pmax) : nullptr;
+  if (xmfma) {
+    const int groups = (C + kXmCols - 1) / kXmCols;
+    int per_xcd = (groups + 7) / 8;
+    const unsigned gx = static_cast<unsigned>(8 * per_xcd);
+    const unsigned gyb = static_cast<unsigned>((nb + 15) / 16);
+    void *args[] = {&boxes, &vx, &tabs, &ks, &nbi, &per_xcd};
+    trace_mark("local_x_mfma", s);
+    HALF_HIP(hipLaunchKernel(xmfma, dim3(gx, gyb, nk), dim3(256), args, 0, s));
+  } else if (xconv) {
     void *args[] = {&boxes, &vx, &tabs, &ks, &nbi, &pw};
     trace_mark("local_x_conv", s);
     HALF_HIP(hipLaunchKernel(xconv, dim3((C + kXcCols - 1) / kXcCols, (nb + kXcBands - 1) / kXcBands, nk),
@@ -972,7 +1127,7 @@ bool set_local_tuning(const std::string &key, int value) {
       return true;
     }
     if (key == pre + ".xconv") {
-      if (value < 0 || value > 1) return false;
+      if (value < 0 || value > 2) return false;
       g_tune.xconv[p] = value;
       return true;
     }

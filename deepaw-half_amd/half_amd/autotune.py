@@ -1,19 +1,27 @@
-"""Autotuner for the local-stage tile shapes and forms of the H.psi operator.
+"""Autotuner for the tile shapes and forms of the H.psi operator.
 
 For one workload it times the FP64 and FP32 operators on a resident band
 block while sweeping, per precision:
   local.<p>.tile.{x,y,z}   FFT points per workgroup tile of each pass
   local.<p>.xgroup         bands sharing one Veff tile in the x pass
+  local.<p>.xconv          x pass as transforms (0), a per-column convolution on
+                           the vector units (1) or on the matrix cores (2)
   local.<p>.fused          y, x, y passes fused per (band, i3) slice or not
   local.<p>.ygroup         bands per workgroup of the fused kernel
   local.<p>.vxlds          fused kernel with the Vx rows staged in LDS or read from L2
   operator.overlap         projector stage on a second stream or in order
+  operator.projector       skinny projector kernels (B D, B Q formed at build)
+                           or the LDS-tiled contractions
+  contract.<p>.gram.wt     tile width of the coefficient contraction B^H psi
+  contract.<p>.gram.fill   round fill its row-chunk count must reach
+  contract.<p>.tall.wt     tile width of the expansion B (D c), B (Q c)
 The tiles are swept with the separate passes (fused off, so every pass runs):
 each candidate is applied to all three passes at once and every pass keeps
 the value that minimised its own traced time (the passes are independent
-kernels); xgroup is swept afterwards with the chosen tiles.  The forms are
-then chosen by the whole operator's event-timed step (no launch timeline,
-so the overlap is in effect).
+kernels); xgroup and xconv are swept afterwards with the chosen tiles, the
+contraction keys by the traced time of their own kernel (generic projector
+path).  The forms are then chosen by the whole operator's event-timed step
+(no launch timeline, so the overlap is in effect).
 
     python -m half_amd.autotune --workload hfo2 --bands 64 --out tuning.json
 
@@ -30,7 +38,7 @@ import numpy as np
 
 from . import BACKEND_CUDA, SOLVER_EVD, Context, get_tuning, hip, set_tuning, synth, trace, trace_read
 
-PASSES = {"x": ("local_x_veff", "local_x_conv"), "y": ("local_y_bwd", "local_y_fwd"), "z": ("local_z_bwd", "local_z_fwd")}
+PASSES = {"x": ("local_x_veff", "local_x_conv", "local_x_mfma"), "y": ("local_y_bwd", "local_y_fwd"), "z": ("local_z_bwd", "local_z_fwd")}
 
 
 def _pass_ms(kern, axis, steps):
@@ -108,28 +116,66 @@ def tune(workload="hfo2", bands=0, steps=10, verbose=True):
                 gbest = (g, ms)
         set_tuning(f"local.{p}.xgroup", gbest[0])
         result["best"][f"local.{p}.xgroup"] = gbest[0]
+        xbest = (1, float("inf"))
+        for xc in (0, 1, 2):
+            set_tuning(f"local.{p}.xconv", xc)
+            ms = _pass_ms(run(), "x", steps)
+            result["timings"][f"{p}.xconv={xc}"] = ms
+            if ms < xbest[1]:
+                xbest = (xc, ms)
+        set_tuning(f"local.{p}.xconv", xbest[0])
+        result["best"][f"local.{p}.xconv"] = xbest[0]
+        # contraction tiles on the generic projector path
+        set_tuning("operator.projector", 0)
+        cbest = ((0, 90), float("inf"))
+        for wt in (0, 1, 2):
+            for fill in (70, 90, 100):
+                set_tuning(f"contract.{p}.gram.wt", wt)
+                set_tuning(f"contract.{p}.gram.fill", fill)
+                kern = run()
+                ms = kern["proj_coeff_mfma"][0] / steps if "proj_coeff_mfma" in kern else float("inf")
+                result["timings"][f"{p}.gram.wt={wt},fill={fill}"] = ms
+                if ms < cbest[1]:
+                    cbest = ((wt, fill), ms)
+        set_tuning(f"contract.{p}.gram.wt", cbest[0][0])
+        set_tuning(f"contract.{p}.gram.fill", cbest[0][1])
+        result["best"][f"contract.{p}.gram.wt"], result["best"][f"contract.{p}.gram.fill"] = cbest[0]
+        tbest = (0, float("inf"))
+        for wt in (0, 1, 2):
+            set_tuning(f"contract.{p}.tall.wt", wt)
+            kern = run()
+            ms = kern["proj_expand_mfma"][0] / steps if "proj_expand_mfma" in kern else float("inf")
+            result["timings"][f"{p}.tall.wt={wt}"] = ms
+            if ms < tbest[1]:
+                tbest = (wt, ms)
+        set_tuning(f"contract.{p}.tall.wt", tbest[0])
+        result["best"][f"contract.{p}.tall.wt"] = tbest[0]
         # the Vx rows go to LDS only with band groups of two or more
-        forms = ([(0, 1, 0, ov) for ov in (0, 1)] + [(1, 1, 0, ov) for ov in (0, 1)]
-                 + [(1, g, v, ov) for g in (2, 4) for v in (0, 1) for ov in (0, 1)])
+        forms = ([(0, 1, 0, ov, pj) for ov in (0, 1) for pj in (0, 1)]
+                 + [(1, 1, 0, ov, pj) for ov in (0, 1) for pj in (0, 1)]
+                 + [(1, g, v, ov, pj) for g in (2, 4) for v in (0, 1) for ov in (0, 1) for pj in (0, 1)])
         fbest = (None, float("inf"))
-        for fused, g, v, ov in forms:
+        for fused, g, v, ov, pj in forms:
             set_tuning(f"local.{p}.fused", fused)
             set_tuning(f"local.{p}.ygroup", g)
             set_tuning(f"local.{p}.vxlds", v)
             set_tuning("operator.overlap", ov)
+            set_tuning("operator.projector", pj)
             ms = step_ms()
-            result["timings"][f"{p}.fused={fused},ygroup={g},vxlds={v},overlap={ov}"] = ms
+            result["timings"][f"{p}.fused={fused},ygroup={g},vxlds={v},overlap={ov},projector={pj}"] = ms
             if ms < fbest[1]:
-                fbest = ((fused, g, v, ov), ms)
-        (fused, g, v, ov) = fbest[0]
+                fbest = ((fused, g, v, ov, pj), ms)
+        (fused, g, v, ov, pj) = fbest[0]
         set_tuning(f"local.{p}.fused", fused)
         set_tuning(f"local.{p}.ygroup", g)
         set_tuning(f"local.{p}.vxlds", v)
         set_tuning("operator.overlap", ov)
+        set_tuning("operator.projector", pj)
         result["best"][f"local.{p}.fused"] = fused
         result["best"][f"local.{p}.ygroup"] = g
         result["best"][f"local.{p}.vxlds"] = v
         result["best"][f"operator.overlap.{p}"] = ov
+        result["best"][f"operator.projector.{p}"] = pj
         result["timings"][f"{p}.tuned_step_ms"] = fbest[1]
         if verbose:
             print(f"autotune {p}: {result['best']} step {result['timings'][f'{p}.tuned_step_ms']:.4f} ms",

@@ -155,3 +155,25 @@ def test_solver_tuning_keys_host_only():
     finally:
         for k, v in saved.items():
             H.set_tuning(k, v)
+
+
+def test_contract_tuning_keys_host_only():
+    """contract.<fp64|fp32>.gram.wt / tall.wt (0 = shape rule, 1, 2) and
+    contract.<fp64|fp32>.gram.fill (10..100 percent) are host settings: set,
+    read back, restored; out-of-range values are rejected."""
+    keys = [f"contract.{p}.{k}" for p in ("fp64", "fp32") for k in ("gram.wt", "tall.wt", "gram.fill")]
+    saved = {k: H.get_tuning(k) for k in keys}
+    assert all(saved[k] == 0 for k in keys if k.endswith(".wt"))
+    assert all(saved[k] == 90 for k in keys if k.endswith(".fill"))
+    try:
+        for k in keys:
+            good = 75 if k.endswith(".fill") else 1
+            H.set_tuning(k, good)
+            assert H.get_tuning(k) == good
+        for key, bad in (("contract.fp64.gram.wt", 3), ("contract.fp32.tall.wt", -1),
+                         ("contract.fp64.gram.fill", 9), ("contract.fp32.gram.fill", 101), ("contract.fp16.gram.wt", 1)):
+            with pytest.raises(Exception, match="unknown tuning key or value out of range"):
+                H.set_tuning(key, bad)
+    finally:
+        for k, v in saved.items():
+            H.set_tuning(k, v)

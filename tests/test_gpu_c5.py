@@ -11,7 +11,8 @@ Checks:
 - Veff and the energy terms against the oracle (PBE pointwise <= 1e-7 eV);
 - H.psi / S.psi of a 520-band block, FP64 and FP32, against the oracle's
   matrix-free operator on columns spread over the box batches (FP64
-  <= 1e-11 relative, FP32 <= 2e-6);
+  <= 1e-11 relative, FP32 <= 2e-6), with the x pass (41 planes) as the
+  vector-unit convolution and as the Toeplitz product on the matrix cores;
 - the FP64 all-band solve (tol 1e-6): Ritz pairs satisfy the oracle's
   operator (Rayleigh quotients <= 1e-9 eV, residuals <= 2e-6 eV,
   S-orthonormal); the mixed solve per band <= 1e-8 eV from it;
@@ -78,20 +79,29 @@ def test_c5_operator_520_bands(c5):
     cols = [0, 1, 137, 260, 333, 450, 518, 519]
     rh, rs = O.apply_hs(ref.veff, basis, paw, psi[cols].T)
     stream = hip.Stream()
-    for prec, dt, tol in ((64, np.complex128, 1e-11), (32, np.complex64, 2e-6)):
-        op = ctx.operator((0, 0, 0), prec)
-        x = hip.Buffer.from_numpy(psi.astype(dt))
-        hp, sp = hip.Buffer(x.shape, dt), hip.Buffer(x.shape, dt)
-        op.apply_device(nb, x.ptr, hp.ptr, sp.ptr, stream=stream.ptr)
-        stream.synchronize()
-        h = hp.numpy()[cols].T.astype(np.complex128)
-        s = sp.numpy()[cols].T.astype(np.complex128)
-        eh = np.abs(h - rh).max() / np.abs(rh).max()
-        es = np.abs(s - rs).max() / np.abs(rs).max()
-        print(f"[gpu] si216 fp{prec} 520-band apply: rel |dH psi| {eh:.2e} |dS psi| {es:.2e}", flush=True)
-        assert eh < tol and es < tol
-        op.close()
-        x.free(), hp.free(), sp.free()
+    saved = {p: H.get_tuning(f"local.fp{p}.xconv") for p in (64, 32)}
+    try:
+        for prec, dt, tol in ((64, np.complex128, 1e-11), (32, np.complex64, 2e-6)):
+            op = ctx.operator((0, 0, 0), prec)
+            x = hip.Buffer.from_numpy(psi.astype(dt))
+            hp, sp = hip.Buffer(x.shape, dt), hip.Buffer(x.shape, dt)
+            # x pass on the vector units and as the Toeplitz product on the matrix cores
+            for xconv in (1, 2):
+                H.set_tuning(f"local.fp{prec}.xconv", xconv)
+                op.apply_device(nb, x.ptr, hp.ptr, sp.ptr, stream=stream.ptr)
+                stream.synchronize()
+                h = hp.numpy()[cols].T.astype(np.complex128)
+                s = sp.numpy()[cols].T.astype(np.complex128)
+                eh = np.abs(h - rh).max() / np.abs(rh).max()
+                es = np.abs(s - rs).max() / np.abs(rs).max()
+                print(f"[gpu] si216 fp{prec} xconv {xconv} 520-band apply: rel |dH psi| {eh:.2e} |dS psi| {es:.2e}",
+                      flush=True)
+                assert eh < tol and es < tol
+            op.close()
+            x.free(), hp.free(), sp.free()
+    finally:
+        for p, v in saved.items():
+            H.set_tuning(f"local.fp{p}.xconv", v)
 
 
 def test_c5_acc_fp64_and_mixed(c5):

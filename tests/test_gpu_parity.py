@@ -286,7 +286,8 @@ def test_local_x_pass_forms(si2):
     band groups of 1, 2 and 4 over 5 bands, so the last group is partial;
     groups of two or more with the Vx rows staged in LDS, and 2 also with
     them read from L2), separate passes with the x pass as a per-column
-    convolution, and transform / multiply / transform."""
+    convolution on the vector units or as a Toeplitz product on the matrix
+    cores, and transform / multiply / transform."""
     from half_amd import hip
     name, g, ctx = si2
     n = g["npw"]
@@ -294,7 +295,7 @@ def test_local_x_pass_forms(si2):
     psi = psi_block(n, nb)
     ref = np.load(f"tests/golden/{name}.apply.npy")
     forms = {"fused1": (1, 1, 1, 1), "fused2": (1, 1, 2, 1), "fused4": (1, 1, 4, 1), "fused2_l2": (1, 1, 2, 0),
-             "conv": (0, 1, 1, 1), "fft": (0, 0, 1, 1)}
+             "conv": (0, 1, 1, 1), "mfma": (0, 2, 1, 1), "fft": (0, 0, 1, 1)}
     saved = {f"local.fp{p}.{k}": H.get_tuning(f"local.fp{p}.{k}") for p in (64, 32)
              for k in ("fused", "xconv", "ygroup", "vxlds")}
     stream = hip.Stream()
