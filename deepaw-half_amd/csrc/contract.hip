@@ -47,7 +47,7 @@ struct GramArgs {
 
 // At least two waves per SIMD: with one (the widest FP64 forms otherwise
 // need 258 registers) every barrier and slab store leaves the matrix core idle.
-template <typename R, int WT, int NB>
+template <typename R, int WT, int NB, int PARTS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gram_kernel(GramArgs<R> g) {
   using V = cplx_t<R>;
   using A4 = typename Acc<R>::type;
@@ -89,11 +89,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
       for (int j = 0; j < NB; ++j) rb[j][i] = (rv && cb) ? pb[j][r] : mk<V>(0, 0);
     }
   };
-  A4 acc[NB][WT][WT][kCmacParts<R>];
+  A4 acc[NB][WT][WT][PARTS];
   for (int j = 0; j < NB; ++j)
     for (int t = 0; t < WT; ++t)
       for (int u = 0; u < WT; ++u)
-        for (int q = 0; q < kCmacParts<R>; ++q) acc[j][t][u][q] = A4{0, 0, 0, 0};
+        for (int q = 0; q < PARTS; ++q) acc[j][t][u][q] = A4{0, 0, 0, 0};
   if (r_begin < r_end) load(r_begin);
   for (int r0 = r_begin; r0 < r_end; r0 += KS) {
     __syncthreads();
@@ -198,9 +198,10 @@ struct TallArgs {
   int64_t bA, bW[2], bC[2];  // batch strides (blockIdx.z)
   int n, m, k;
   R alpha, beta;
+  int split;  // 1: blockIdx.z selects the output (W[z], C[z]) of one problem, not a batch entry
 };
 
-template <typename R, int WT, int NB>
+template <typename R, int WT, int NB, int PARTS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void tall_kernel(TallArgs<R> g) {
   using V = cplx_t<R>;
   using A4 = typename Acc<R>::type;
@@ -218,7 +219,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
   const int akc = tid / TPA, ar0 = (tid % TPA) * L;  // A: inner column akc, rows ar0..
   const int wcol = tid / TPW, wk0 = (tid % TPW) * L;  // W: column wcol, inner rows wk0..
   const bool wcv = c0 + wcol < g.m;
-  const int64_t bz = blockIdx.z;
+  const int64_t bz = g.split ? 0 : blockIdx.z;
+  const int zo = g.split ? static_cast<int>(blockIdx.z) : 0;  // first output of this workgroup
   V ra[L], rw[NB][L];
   auto load = [&](int k0) {
     const int kc = k0 + akc;
@@ -229,18 +231,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
       ra[i] = (kv && r < g.n) ? pa[r] : mk<V>(0, 0);
     }
     for (int j = 0; j < NB; ++j) {
-      const V *pw = g.W[j] + bz * g.bW[j] + int64_t(min(c0 + wcol, g.m - 1)) * g.ldw[j];
+      const V *pw = g.W[j + zo] + bz * g.bW[j + zo] + int64_t(min(c0 + wcol, g.m - 1)) * g.ldw[j + zo];
       for (int i = 0; i < L; ++i) {
         const int kr = k0 + wk0 + i;
         rw[j][i] = (wcv && kr < g.k) ? pw[kr] : mk<V>(0, 0);
       }
     }
   };
-  A4 acc[NB][WT][WT][kCmacParts<R>];
+  A4 acc[NB][WT][WT][PARTS];
   for (int j = 0; j < NB; ++j)
     for (int t = 0; t < WT; ++t)
       for (int u = 0; u < WT; ++u)
-        for (int q = 0; q < kCmacParts<R>; ++q) acc[j][t][u][q] = A4{0, 0, 0, 0};
+        for (int q = 0; q < PARTS; ++q) acc[j][t][u][q] = A4{0, 0, 0, 0};
   load(0);
   for (int k0 = 0; k0 < g.k; k0 += KS) {
     __syncthreads();
@@ -273,6 +275,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
   // updates is loaded first (independent loads in flight together) and
   // only then are the results stored
   for (int j = 0; j < NB; ++j) {
+    V *const Cj = g.C[j + zo] + bz * g.bC[j + zo];
+    const int64_t ldcj = g.ldc[j + zo];
     V old[WT][WT][4];
     if (g.beta != R(0)) {
       for (int u = 0; u < WT; ++u)
@@ -280,8 +284,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
           for (int i = 0; i < 4; ++i) {
             const int col = c0 + wc + 16 * u + row;
             const int r = g0 + wr + 16 * t + mfma_row<R>(lane, i);
-            old[u][t][i] = (col < g.m && r < g.n) ? g.C[j][bz * g.bC[j] + r + int64_t(col) * g.ldc[j]]
-                                                   : mk<V>(R(0), R(0));
+            old[u][t][i] = (col < g.m && r < g.n) ? Cj[r + int64_t(col) * ldcj] : mk<V>(R(0), R(0));
           }
     }
     for (int u = 0; u < WT; ++u) {
@@ -299,20 +302,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
             vr += g.beta * old[u][t][i].x;
             vi += g.beta * old[u][t][i].y;
           }
-          g.C[j][bz * g.bC[j] + r + int64_t(col) * g.ldc[j]] = mk<V>(vr, vi);
+          Cj[r + int64_t(col) * ldcj] = mk<V>(vr, vi);
         }
     }
   }
 }
 
-template <typename R, int WT, int NB>
+template <typename R, int WT, int NB, int PARTS>
 void launch_gram(dim3 grid, const GramArgs<R> &a, hipStream_t s) {
-  hipLaunchKernelGGL((gram_kernel<R, WT, NB>), grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL((gram_kernel<R, WT, NB, PARTS>), grid, dim3(256), 0, s, a);
 }
 
-template <typename R, int WT, int NB>
+template <typename R, int WT, int NB, int PARTS>
 void launch_tall(dim3 grid, const TallArgs<R> &a, hipStream_t s) {
-  hipLaunchKernelGGL((tall_kernel<R, WT, NB>), grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL((tall_kernel<R, WT, NB, PARTS>), grid, dim3(256), 0, s, a);
 }
 
 }  // namespace
@@ -326,6 +329,14 @@ namespace {
 int g_gram_wt[2] = {0, 0};
 int g_gram_fill[2] = {90, 90};
 int g_tall_wt[2] = {0, 0};
+// FP64 complex products with three MFMAs (Gauss) instead of four where a
+// workgroup has one output; the expansion with both outputs then runs as one
+// launch with one output per workgroup (grid.z), which keeps a 64-wide
+// tile's three accumulators within the register budget of three waves per
+// SIMD.  216-atom cell, 512 bands: B^H psi 4.70 -> 3.96 ms, B (D c), B (Q c)
+// 9.52 -> 8.05 ms.  FP32 always uses three.
+int g_gram_gauss = 1;
+int g_tall_gauss = 1;
 
 template <typename R>
 constexpr int prec_index() {
@@ -340,7 +351,7 @@ struct GramPlan {
 
 // Workgroups of gram_kernel<R, WT, NB> the device holds at once (occupancy
 // per CU x CUs of the current device), cached per variant and device.
-template <typename R, int WT, int NB>
+template <typename R, int WT, int NB, int PARTS>
 int gram_capacity() {
   static std::mutex mu;
   static std::map<int, int> cache;
@@ -351,17 +362,28 @@ int gram_capacity() {
   if (it != cache.end()) return it->second;
   int per_cu = 0, cus = 0;
   HALF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void *>(&gram_kernel<R, WT, NB>), 256, 0));
+      &per_cu, reinterpret_cast<const void *>(&gram_kernel<R, WT, NB, PARTS>), 256, 0));
   HALF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const int cap = std::max(1, per_cu) * std::max(1, cus);
   cache[dev] = cap;
   return cap;
 }
 
+// (the two-output 64-wide FP64 tile would spill with three accumulators)
+template <typename R>
+int gram_parts(int nout) {
+  return (sizeof(R) == 8 && !(g_gram_gauss && nout == 1)) ? 2 : 3;
+}
+
 template <typename R>
 int gram_capacity_of(int wt, int nout) {
-  if (wt == 1) return nout == 1 ? gram_capacity<R, 1, 1>() : gram_capacity<R, 1, 2>();
-  return nout == 1 ? gram_capacity<R, 2, 1>() : gram_capacity<R, 2, 2>();
+  constexpr int P4 = kCmacParts<R>;  // the default product form of R
+  if (gram_parts<R>(nout) == P4) {
+    if (wt == 1) return nout == 1 ? gram_capacity<R, 1, 1, P4>() : gram_capacity<R, 1, 2, P4>();
+    return nout == 1 ? gram_capacity<R, 2, 1, P4>() : gram_capacity<R, 2, 2, P4>();
+  }
+  if (wt == 1) return nout == 1 ? gram_capacity<R, 1, 1, 3>() : gram_capacity<R, 1, 2, 3>();
+  return nout == 1 ? gram_capacity<R, 2, 1, 3>() : gram_capacity<R, 2, 2, 3>();
 }
 
 // Row chunks for `units` output tiles (over the batch) on a device holding
@@ -475,10 +497,18 @@ void gram_batched_work(int batch, int n, int ka, int kb, const cplx_t<R> *A, int
     a.chunk_stride = tile_elems;
   }
   const dim3 grid(plan.tiles, nchunks, batch);
-  if (plan.wt == 1)
-    nb == 1 ? launch_gram<R, 1, 1>(grid, a, s) : launch_gram<R, 1, 2>(grid, a, s);
-  else
-    nb == 1 ? launch_gram<R, 2, 1>(grid, a, s) : launch_gram<R, 2, 2>(grid, a, s);
+  constexpr int P4 = kCmacParts<R>;
+  if (gram_parts<R>(nb) == P4) {
+    if (plan.wt == 1)
+      nb == 1 ? launch_gram<R, 1, 1, P4>(grid, a, s) : launch_gram<R, 1, 2, P4>(grid, a, s);
+    else
+      nb == 1 ? launch_gram<R, 2, 1, P4>(grid, a, s) : launch_gram<R, 2, 2, P4>(grid, a, s);
+  } else {
+    if (plan.wt == 1)
+      nb == 1 ? launch_gram<R, 1, 1, 3>(grid, a, s) : launch_gram<R, 1, 2, 3>(grid, a, s);
+    else
+      nb == 1 ? launch_gram<R, 2, 1, 3>(grid, a, s) : launch_gram<R, 2, 2, 3>(grid, a, s);
+  }
   HALF_HIP(hipGetLastError());
   if (nchunks > 1) {
     const dim3 rg(static_cast<unsigned>((tile_elems + 63) / 64), batch);
@@ -574,12 +604,19 @@ void gemm_tall_batched(int batch, int n, int m, int k, R alpha, const cplx_t<R> 
   a.k = k;
   a.alpha = alpha;
   a.beta = beta;
-  const dim3 grid((n + tw - 1) / tw, (m + tw - 1) / tw, batch);
+  constexpr int P4 = kCmacParts<R>;
+  // FP64 Gauss form: both outputs of one problem as grid.z = 2, one each
+  const bool split = sizeof(R) == 8 && g_tall_gauss && nb == 2 && batch == 1 && !inplace;
+  a.split = split ? 1 : 0;
+  const dim3 grid((n + tw - 1) / tw, (m + tw - 1) / tw, split ? 2 : batch);
   if (grid.y > 65535) throw Error(kInvalidArgument, "gemm_tall: too many columns");
-  if (wt == 1)
-    nb == 1 ? launch_tall<R, 1, 1>(grid, a, s) : launch_tall<R, 1, 2>(grid, a, s);
-  else
-    nb == 1 ? launch_tall<R, 2, 1>(grid, a, s) : launch_tall<R, 2, 2>(grid, a, s);
+  if (sizeof(R) == 8 && g_tall_gauss && !inplace && (nb == 1 || split)) {
+    wt == 1 ? launch_tall<R, 1, 1, 3>(grid, a, s) : launch_tall<R, 2, 1, 3>(grid, a, s);
+  } else if (wt == 1) {
+    nb == 1 ? launch_tall<R, 1, 1, P4>(grid, a, s) : launch_tall<R, 1, 2, P4>(grid, a, s);
+  } else {
+    nb == 1 ? launch_tall<R, 2, 1, P4>(grid, a, s) : launch_tall<R, 2, 2, P4>(grid, a, s);
+  }
   HALF_HIP(hipGetLastError());
 }
 
@@ -604,6 +641,11 @@ bool set_contract_tuning(const std::string &key, int value) {
       return true;
     }
   }
+  if (key == "contract.fp64.gram.gauss" || key == "contract.fp64.tall.gauss") {
+    if (value < 0 || value > 1) return false;
+    (key == "contract.fp64.gram.gauss" ? g_gram_gauss : g_tall_gauss) = value;
+    return true;
+  }
   return false;
 }
 
@@ -622,6 +664,10 @@ bool get_contract_tuning(const std::string &key, int &value) {
       value = g_gram_fill[p];
       return true;
     }
+  }
+  if (key == "contract.fp64.gram.gauss" || key == "contract.fp64.tall.gauss") {
+    value = key == "contract.fp64.gram.gauss" ? g_gram_gauss : g_tall_gauss;
+    return true;
   }
   return false;
 }

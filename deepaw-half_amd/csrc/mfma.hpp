@@ -48,18 +48,20 @@ __device__ inline void cmfma(R ar, R ai, R br, R bi, typename Acc<R>::type &cr, 
   ci = Acc<R>::mfma(ai, br, ci);
 }
 
-// Complex multiply-accumulate into the accumulators of one 16x16 tile.  FP32
-// uses three real MFMAs (Gauss): p0 += ar br, p1 += ai bi,
+// Complex multiply-accumulate into the accumulators of one 16x16 tile:
+// PARTS = 3 uses three real MFMAs (Gauss): p0 += ar br, p1 += ai bi,
 // p2 += (ar + ai)(br + bi), with the product's real part p0 - p1 and its
-// imaginary part p2 - p0 - p1 (measured faster for the FP32 contractions);
-// FP64 keeps the four-MFMA form, whose smaller accumulator set holds more
-// waves per SIMD in the 64-wide tiles (the three-MFMA form was slower there).
+// imaginary part p2 - p0 - p1; PARTS = 2 the four-MFMA form with the real
+// and imaginary parts accumulated directly.  The default: three for FP32
+// (measured faster for the FP32 contractions), four for FP64, whose smaller
+// accumulator set holds more waves per SIMD in the widest tiles.
 template <typename R>
 constexpr int kCmacParts = sizeof(R) == 4 ? 3 : 2;
 
-template <typename R>
-__device__ inline void cmac(R ar, R ai, R br, R bi, typename Acc<R>::type (&p)[kCmacParts<R>]) {
-  if constexpr (kCmacParts<R> == 3) {
+template <typename R, int PARTS>
+__device__ inline void cmac(R ar, R ai, R br, R bi, typename Acc<R>::type (&p)[PARTS]) {
+  static_assert(PARTS == 2 || PARTS == 3, "two or three accumulators per tile");
+  if constexpr (PARTS == 3) {
     p[0] = Acc<R>::mfma(ar, br, p[0]);
     p[1] = Acc<R>::mfma(ai, bi, p[1]);
     p[2] = Acc<R>::mfma(ar + ai, br + bi, p[2]);
@@ -69,9 +71,9 @@ __device__ inline void cmac(R ar, R ai, R br, R bi, typename Acc<R>::type (&p)[k
 }
 
 // element i of the tile accumulated by cmac
-template <typename R>
-__device__ inline void cmac_result(const typename Acc<R>::type (&p)[kCmacParts<R>], int i, R &re, R &im) {
-  if constexpr (kCmacParts<R> == 3) {
+template <typename R, int PARTS>
+__device__ inline void cmac_result(const typename Acc<R>::type (&p)[PARTS], int i, R &re, R &im) {
+  if constexpr (PARTS == 3) {
     re = p[0][i] - p[1][i];
     im = p[2][i] - p[0][i] - p[1][i];
   } else {

@@ -12,7 +12,8 @@ Checks:
 - H.psi / S.psi of a 520-band block, FP64 and FP32, against the oracle's
   matrix-free operator on columns spread over the box batches (FP64
   <= 1e-11 relative, FP32 <= 2e-6), with the x pass (41 planes) as the
-  vector-unit convolution and as the Toeplitz product on the matrix cores;
+  vector-unit convolution and as the Toeplitz product on the matrix cores,
+  and (FP64) with the projector contractions' three-MFMA complex products;
 - the FP64 all-band solve (tol 1e-6): Ritz pairs satisfy the oracle's
   operator (Rayleigh quotients <= 1e-9 eV, residuals <= 2e-6 eV,
   S-orthonormal); the mixed solve per band <= 1e-8 eV from it;
@@ -79,29 +80,34 @@ def test_c5_operator_520_bands(c5):
     cols = [0, 1, 137, 260, 333, 450, 518, 519]
     rh, rs = O.apply_hs(ref.veff, basis, paw, psi[cols].T)
     stream = hip.Stream()
-    saved = {p: H.get_tuning(f"local.fp{p}.xconv") for p in (64, 32)}
+    keys = ["local.fp64.xconv", "local.fp32.xconv", "contract.fp64.gram.gauss", "contract.fp64.tall.gauss"]
+    saved = {k: H.get_tuning(k) for k in keys}
     try:
         for prec, dt, tol in ((64, np.complex128, 1e-11), (32, np.complex64, 2e-6)):
             op = ctx.operator((0, 0, 0), prec)
             x = hip.Buffer.from_numpy(psi.astype(dt))
             hp, sp = hip.Buffer(x.shape, dt), hip.Buffer(x.shape, dt)
-            # x pass on the vector units and as the Toeplitz product on the matrix cores
-            for xconv in (1, 2):
+            # x pass on the vector units and as the Toeplitz product on the matrix
+            # cores; FP64 projector contractions with four and three MFMAs per
+            # complex product
+            for xconv, gauss in ((1, 0), (2, 0), (2, 1)) if prec == 64 else ((1, 0), (2, 0)):
                 H.set_tuning(f"local.fp{prec}.xconv", xconv)
+                H.set_tuning("contract.fp64.gram.gauss", gauss)
+                H.set_tuning("contract.fp64.tall.gauss", gauss)
                 op.apply_device(nb, x.ptr, hp.ptr, sp.ptr, stream=stream.ptr)
                 stream.synchronize()
                 h = hp.numpy()[cols].T.astype(np.complex128)
                 s = sp.numpy()[cols].T.astype(np.complex128)
                 eh = np.abs(h - rh).max() / np.abs(rh).max()
                 es = np.abs(s - rs).max() / np.abs(rs).max()
-                print(f"[gpu] si216 fp{prec} xconv {xconv} 520-band apply: rel |dH psi| {eh:.2e} |dS psi| {es:.2e}",
-                      flush=True)
+                print(f"[gpu] si216 fp{prec} xconv {xconv} gauss {gauss} 520-band apply: rel |dH psi| {eh:.2e} "
+                      f"|dS psi| {es:.2e}", flush=True)
                 assert eh < tol and es < tol
             op.close()
             x.free(), hp.free(), sp.free()
     finally:
-        for p, v in saved.items():
-            H.set_tuning(f"local.fp{p}.xconv", v)
+        for k, v in saved.items():
+            H.set_tuning(k, v)
 
 
 def test_c5_acc_fp64_and_mixed(c5):
