@@ -269,6 +269,13 @@ void check_info(const DeviceArray<rocblas_int> &info, hipStream_t s, const char 
   if (h != 0) throw Error(kInternal, std::string(what) + " failed (info " + std::to_string(h) + ")");
 }
 
+// Correction blocks of at least this many elements (n x np) are normalised
+// by the explicit inverse of the Cholesky factor and one batched product
+// (216-atom cell: n x np ~ 6e6); smaller ones by three rocBLAS triangular
+// solves, which were faster there (HfO2-like cell, 3,407 x 72: 3.0 against
+// 3.3 ms per solve with the product in place).
+constexpr int64_t kNormaliseByInverse = int64_t(1) << 20;
+
 bool g_hetrd = true;
 // Locking in the all-band solve (key "solver.lock").  1 (soft): a band
 // whose residual already meets the current target contributes no correction
@@ -296,21 +303,31 @@ int g_precond = 0;
 // one application), which sets the floor the FP32 phase stalls at.
 int g_refresh = 0;
 
-// zheevd with vectors (upper triangle of A referenced; eigenvectors replace
+// zheevd with vectors (the full Hermitian matrix in A; eigenvectors replace
 // A, eigenvalues ascending in w; e is workspace of n values).  Orders up to
-// kTridiagMaxOrder reduce in one workgroup (hetrd_upper), then rocSOLVER
-// solves the tridiagonal problem by divide and conquer (zstedc) and applies
-// the reflectors (zunmtr): the reduction is the part of rocSOLVER's zheevd
-// that is a chain of launches per column at these orders.
+// kTridiagMaxOrder reduce in one workgroup (hetrd_upper), orders up to
+// kTridiagCoopMaxOrder in one cooperative launch of several (hetrd_upper_coop),
+// then rocSOLVER solves the tridiagonal problem by divide and conquer
+// (zstedc) and applies the reflectors (zunmtr): the reduction is the part of
+// rocSOLVER's zheevd that is a chain of launches per column at these orders.
 void heevd_vectors(Blas &blas, int k, double2 *A, double *w, double *e, rocblas_int *info) {
-  if (!g_hetrd || k > kTridiagMaxOrder) {
+  if (!g_hetrd || k > kTridiagCoopMaxOrder) {
     HALF_BLAS(rocsolver_zheevd(blas.h, rocblas_evect_original, rocblas_fill_upper, k, rz(A), k, w, e, info));
     return;
   }
   blas.eig_tau.ensure(size_t(k));
   blas.eig_z.ensure(size_t(k) * k);
   double2 *Z = blas.eig_z.get(), *tau = blas.eig_tau.get();
-  hetrd_upper(k, 1, A, k, 0, w, 0, e, 0, tau, 0, blas.stream);
+  if (k <= kTridiagMaxOrder) {
+    hetrd_upper(k, 1, A, k, 0, w, 0, e, 0, tau, 0, blas.stream);
+  } else {
+    const size_t need = coop_work_elems(k, kTridiagCoopMaxGroups);
+    blas.eig_coop.ensure(need);
+    if (!hetrd_upper_coop(k, A, k, w, e, tau, blas.eig_coop.get(), need, blas.stream)) {
+      HALF_BLAS(rocsolver_zheevd(blas.h, rocblas_evect_original, rocblas_fill_upper, k, rz(A), k, w, e, info));
+      return;
+    }
+  }
   HALF_BLAS(rocsolver_zstedc(blas.h, rocblas_evect_tridiagonal, k, w, e, rz(Z), k, info));
   HALF_BLAS(rocsolver_zunmtr(blas.h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, k, k, rz(A), k,
                              rz(tau), rz(Z), k));
@@ -574,7 +591,11 @@ namespace {
 
 struct AccWork {
   int n, m, m2;
-  DeviceArray<double2> Z, HZ, SZ, R, tmp, cross, metric, hsub, ssub, gwork;
+  // ZZ holds Z, HZ, SZ (n x 2m each) at a stride of n 2m, so the three
+  // blocks' updates are one batched launch; tmp3 the three n x m results of
+  // an out-of-place update at a stride of n m
+  DeviceArray<double2> ZZ, tmp3, R, tmp, cross, metric, hsub, ssub, gwork;
+  double2 *Z = nullptr, *HZ = nullptr, *SZ = nullptr;
   DeviceArray<float2> p32, hp32, sp32;
   DeviceArray<double> w, mu, E, res2, kin, dscale, pivots, ekin;
   DeviceArray<int> seeds, cols;
@@ -599,9 +620,12 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   const int m2 = 2 * m;
   const int64_t ld = n;
   AccWork wk;
-  wk.Z.resize(size_t(n) * m2);
-  wk.HZ.resize(size_t(n) * m2);
-  wk.SZ.resize(size_t(n) * m2);
+  const int64_t zs = int64_t(n) * m2;  // block stride in ZZ
+  wk.ZZ.resize(3 * size_t(zs));
+  wk.Z = wk.ZZ.get();
+  wk.HZ = wk.Z + zs;
+  wk.SZ = wk.HZ + zs;
+  wk.tmp3.resize(3 * size_t(n) * m);
   wk.R.resize(size_t(n) * m);
   wk.tmp.resize(size_t(n) * m);
   wk.cross.resize(size_t(m) * m);
@@ -624,10 +648,10 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     wk.hp32.resize(size_t(n) * m);
     wk.sp32.resize(size_t(n) * m);
   }
-  double2 *X = wk.Z.get(), *P = wk.Z.get() + size_t(n) * m;
-  double2 *HX = wk.HZ.get(), *HP = wk.HZ.get() + size_t(n) * m;
-  double2 *SX = wk.SZ.get(), *SP = wk.SZ.get() + size_t(n) * m;
-  const double2 one = make_double2(1, 0), zero = make_double2(0, 0), mone = make_double2(-1, 0);
+  double2 *X = wk.Z, *P = wk.Z + size_t(n) * m;
+  double2 *HX = wk.HZ, *HP = wk.HZ + size_t(n) * m;
+  double2 *SX = wk.SZ, *SP = wk.SZ + size_t(n) * m;
+  const double2 one = make_double2(1, 0), zero = make_double2(0, 0);
   bool fp64_phase = !ctl.mixed;
   double fp32_floor = 0;
   const bool locking = ctl.locking();
@@ -652,6 +676,13 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   auto copy = [&](double2 *dst, const double2 *src, int cnt) {
     HALF_HIP(hipMemcpyAsync(dst, src, sizeof(double2) * size_t(n) * cnt, hipMemcpyDeviceToDevice, s));
   };
+  // the three results in tmp3 (cnt columns each) back to column c0.. of
+  // Z, HZ, SZ: one strided copy
+  auto copy_back3 = [&](int c0, int cnt) {
+    HALF_HIP(hipMemcpy2DAsync(wk.Z + size_t(n) * c0, sizeof(double2) * size_t(zs), wk.tmp3.get(),
+                              sizeof(double2) * size_t(n) * m, sizeof(double2) * size_t(n) * cnt, 3,
+                              hipMemcpyDeviceToDevice, s));
+  };
   // generalized k x k problem in (hsub, ssub): the lowest m eigenvectors in
   // hsub(:, :m), values in w
   // the lowest nv (default m) eigenvectors in hsub(:, :nv), values from w + w0
@@ -664,19 +695,14 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     gram<double>(n, k, k, A, ld, HA, ld, wk.hsub.get(), k, SA, ld, wk.ssub.get(), k, wk.gwork, s, true);
   };
   // rotate k columns of (Z, HZ, SZ) from column c0 by hsub(:, :m - c0)
-  // into columns c0.. of X, HX, SX
+  // into columns c0.. of X, HX, SX (one batched product for the three
+  // blocks into tmp3, then one strided copy back)
   auto ritz_rotate = [&](int k, int c0 = 0) {
     const int nv = m - c0;
     const size_t off = size_t(n) * c0;
-    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, nv, k, one, wk.Z.get() + off, ld, wk.hsub.get(),
-          k, zero, wk.tmp.get(), ld);
-    copy(X + off, wk.tmp.get(), nv);
-    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, nv, k, one, wk.HZ.get() + off, ld, wk.hsub.get(),
-          k, zero, wk.tmp.get(), ld);
-    copy(HX + off, wk.tmp.get(), nv);
-    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, nv, k, one, wk.SZ.get() + off, ld, wk.hsub.get(),
-          k, zero, wk.tmp.get(), ld);
-    copy(SX + off, wk.tmp.get(), nv);
+    gemm_tall_batched<double>(3, n, nv, k, 1.0, wk.Z + off, ld, zs, wk.hsub.get(), k, 0, 0.0, wk.tmp3.get(), ld,
+                              int64_t(n) * m, nullptr, 0, 0, nullptr, 0, 0, s);
+    copy_back3(c0, nv);
   };
   auto rayleigh_ritz_x = [&]() {
     gram_hs(m, X, HX, SX);
@@ -851,12 +877,9 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     // S-orthogonalise the correction block against X (cross: m x np)
     trace_mark("acc_orthogonalise_gemm", s);
     gram<double>(n, m, np, X, ld, SP, ld, wk.cross.get(), m, nullptr, 0, nullptr, 0, wk.gwork, s);
-    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, np, m, mone, X, ld, wk.cross.get(), m, one, P,
-          ld);
-    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, np, m, mone, HX, ld, wk.cross.get(), m, one, HP,
-          ld);
-    zgemm(blas, rocblas_operation_none, rocblas_operation_none, n, np, m, mone, SX, ld, wk.cross.get(), m, one, SP,
-          ld);
+    // (P, HP, SP) -= (X, HX, SX) cross: one batched launch over the three blocks
+    gemm_tall_batched<double>(3, n, np, m, -1.0, X, ld, zs, wk.cross.get(), m, 0, 1.0, P, ld, zs, nullptr, 0, 0,
+                              nullptr, 0, 0, s);
     // normalise it in the S metric.  With D = diag(P^H S P)^-1/2 and
     // D P^H S P D = U^H U, P <- P D U^-1 spans the same space as the
     // reference's P V mu^-1/2 (eigenpairs (mu, V) of P^H S P, mu floored at
@@ -884,11 +907,25 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
                          wk.metric.get(), wk.dscale.get(), np);
       HALF_HIP(hipGetLastError());
       trace_mark("acc_normalise_trsm", s);
-      const rocblas_double_complex z1{1.0, 0.0};
-      for (double2 *blk : {P, HP, SP})
-        HALF_BLAS(rocblas_ztrsm(blas.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                                rocblas_diagonal_non_unit, n, np, &z1, rz(wk.metric.get()), np, rz(blk),
-                                static_cast<rocblas_int>(ld)));
+      if (int64_t(n) * np >= kNormaliseByInverse) {
+        // U^-1 formed once (ztrtri, its strictly lower part cleared) and
+        // applied to the three blocks by one batched product
+        HALF_BLAS(rocsolver_ztrtri(blas.h, rocblas_fill_upper, rocblas_diagonal_non_unit, np,
+                                   rz(wk.metric.get()), np, wk.info.get() + 1));
+        // (info not read: every pivot was checked above, so U is invertible)
+        hipLaunchKernelGGL(upper_only_kernel, dim3(blocks_for(int64_t(np) * np)), dim3(kThreads), 0, s,
+                           wk.metric.get(), np);
+        HALF_HIP(hipGetLastError());
+        gemm_tall_batched<double>(3, n, np, np, 1.0, P, ld, zs, wk.metric.get(), np, 0, 0.0, wk.tmp3.get(), ld,
+                                  int64_t(n) * m, nullptr, 0, 0, nullptr, 0, 0, s);
+        copy_back3(m, np);
+      } else {
+        const rocblas_double_complex z1{1.0, 0.0};
+        for (double2 *blk : {P, HP, SP})
+          HALF_BLAS(rocblas_ztrsm(blas.h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                  rocblas_diagonal_non_unit, n, np, &z1, rz(wk.metric.get()), np, rz(blk),
+                                  static_cast<rocblas_int>(ld)));
+      }
     } else {
       ++res.metric_fallbacks;
       gram<double>(n, np, np, P, ld, SP, ld, wk.metric.get(), np, nullptr, 0, nullptr, 0, wk.gwork, s);
@@ -910,7 +947,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     const int k = m - L + np;
     const size_t zoff = size_t(n) * L;
     trace_mark("acc_projection_gemm", s);
-    gram_hs(k, wk.Z.get() + zoff, wk.HZ.get() + zoff, wk.SZ.get() + zoff);
+    gram_hs(k, wk.Z + zoff, wk.HZ + zoff, wk.SZ + zoff);
     trace_mark("acc_subspace_eigensolve", s);
     small_generalized(k, m - L, L);
     trace_mark("acc_rotate_gemm", s);

@@ -17,7 +17,7 @@ enum SolverKind { kSolverEvd = 1, kSolverEvj = 2, kSolverEvx = 3, kSolverAcc = 4
 struct Blas {
   rocblas_handle h = nullptr;
   hipStream_t stream = nullptr;
-  DeviceArray<double2> eig_z, eig_tau;  // workspace of the small eigensolver
+  DeviceArray<double2> eig_z, eig_tau, eig_coop;  // workspace of the small eigensolver
   void init(hipStream_t s);
   void release();
   ~Blas();

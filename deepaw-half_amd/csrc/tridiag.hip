@@ -19,6 +19,12 @@
 // its row of w.  The pass streams the block through one CU (about 1 ms at
 // order 144, against 2 ms for rocSOLVER's reduction); orders above 256 are
 // left to rocSOLVER.
+#include <hip/hip_cooperative_groups.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 #include "cplx.hpp"
 #include "tridiag.hpp"
 
@@ -45,6 +51,7 @@ __device__ inline double wave_total(const double *red) {
   for (int k = 0; k < kTrdWaves; ++k) s += red[k];
   return s;
 }
+__device__ inline double wave_total4(const double *red) { return (red[0] + red[1]) + (red[2] + red[3]); }
 
 template <int NMAX>
 __global__ __launch_bounds__(kTrdThreads) void hetrd_upper_kernel(int n, double2 *A, int64_t lda, int64_t sA,
@@ -215,7 +222,189 @@ __global__ __launch_bounds__(kTrdThreads) void hetrd_upper_kernel(int n, double2
   }
 }
 
+// Orders above the one-workgroup kernel: the same reduction by G
+// workgroups of one cooperative launch, the matrix held in their LDS.
+// Workgroup g owns columns g, g + G, g + 2G, ... (all n rows each, so the
+// shrinking active block stays spread over every workgroup), and by
+// Hermitian symmetry forms (A v)_j = sum_r conj(A(r, j)) v(r) for its own
+// columns without any exchange.  One grid barrier per column step:
+//   A  every workgroup reads column c = i + 1 (published by its owner in the
+//      previous step), applies the pending rank-2 update and forms the
+//      reflector itself (the same operations on the same data everywhere,
+//      so every copy of v is identical);
+//   B  each workgroup applies the pending update to its active columns,
+//      forms their entries of A v and its share of v^H A v, and the owner of
+//      column i publishes that column for the next step;
+//   -- grid barrier --
+//   C  every workgroup reads A v and the G shares (summed in workgroup order)
+//      and forms w; workgroup 0 stores the reflector, e(i) and tau(i).
+// Published columns, A v and the shares alternate between two buffers by
+// step parity: a workgroup can only reach step i - 1's writes after every
+// workgroup has passed step i's barrier, i.e. finished reading step i + 1's.
+// The grid barrier (cooperative groups) costs a few microseconds and grows
+// with the workgroup count, so a column step takes about 10 us at order 340
+// (16 workgroups) against about 18 us for rocSOLVER's chain of launches;
+// dense solves of order 339 / 531 / 645 took 14.7 / 24.7 / 31.4 ms against
+// 17.3 / 27.3 / 32.3 ms with rocSOLVER's reduction (one MI355X).
+constexpr int kCoopThreads = 256;
+
+__global__ __launch_bounds__(kCoopThreads) void hetrd_upper_coop_kernel(int n, double2 *A, int64_t lda, double *D,
+                                                                        double *E, double2 *Tau, double2 *colbuf,
+                                                                        double2 *pbuf, double *share, int ncol) {
+  namespace cg = cooperative_groups;
+  cg::grid_group grid = cg::this_grid();
+  const int G = static_cast<int>(gridDim.x), g = static_cast<int>(blockIdx.x);
+  extern __shared__ double2 coop_lds[];
+  double2 *cols = coop_lds;                   // [ncol][n]: column g + lc G at lc n
+  double2 *vc = cols + static_cast<size_t>(ncol) * n;  // this step's v
+  double2 *vp = vc + n, *wp = vp + n;         // the pending update's v, w
+  __shared__ double red[kCoopThreads / 64];
+  __shared__ double2 s_alpha, s_tau;
+  __shared__ double s_beta;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  for (int lc = 0; lc < ncol && g + lc * G < n; ++lc)
+    for (int r = t; r < n; r += kCoopThreads) cols[static_cast<size_t>(lc) * n + r] = A[r + int64_t(g + lc * G) * lda];
+  bool prev = false;
+  for (int i = n - 2; i >= 0; --i) {
+    const int c = i + 1, par = i & 1;
+    // A: column c with the pending update, then the reflector (zlarfg)
+    const double2 *src = (i == n - 2) ? A + int64_t(c) * lda : colbuf + (par ^ 1) * n;
+    const double2 cwc = prev ? wp[c] : mk<double2>(0, 0), cvc = prev ? vp[c] : mk<double2>(0, 0);
+    double nrm = 0;
+    for (int r = t; r <= c; r += kCoopThreads) {
+      double2 x = src[r];
+      if (prev) x = csub(x, cadd(cmulc(vp[r], cwc), cmulc(wp[r], cvc)));
+      vc[r] = x;
+      if (r < i) nrm += x.x * x.x + x.y * x.y;
+      if (r == i) s_alpha = x;
+      if (r == c && g == 0) D[c] = x.x;
+    }
+    nrm = wave_sum(nrm);
+    if (lane == 0) red[wid] = nrm;
+    __syncthreads();
+    {
+      double xn2 = 0;
+#pragma unroll
+      for (int k = 0; k < kCoopThreads / 64; ++k) xn2 += red[k];
+      const double2 alpha = s_alpha;
+      const bool refl = !(xn2 == 0.0 && alpha.y == 0.0);
+      double beta = alpha.x;
+      double2 taui = mk<double2>(0, 0), scal = mk<double2>(1, 0);
+      if (refl) {
+        beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
+        taui = mk<double2>((beta - alpha.x) / beta, -alpha.y / beta);
+        const double zr = alpha.x - beta, zi = alpha.y, q = 1.0 / (zr * zr + zi * zi);
+        scal = mk<double2>(zr * q, -zi * q);  // 1 / (alpha - beta)
+      }
+      for (int r = t; r <= i; r += kCoopThreads) vc[r] = r < i ? cmul(vc[r], scal) : mk<double2>(1, 0);
+      if (t == 0) {
+        s_tau = taui;
+        s_beta = beta;
+      }
+    }
+    __syncthreads();
+    // B: own active columns j <= i (ascending in lc), one wave per column
+    double sv = 0;  // this wave's part of v^H A v
+    for (int lc = wid; lc < ncol; lc += kCoopThreads / 64) {
+      const int j = g + lc * G;
+      if (j > i) break;  // uniform over the wave
+      double2 *col = cols + static_cast<size_t>(lc) * n;
+      const double2 cw = prev ? wp[j] : mk<double2>(0, 0), cv = prev ? vp[j] : mk<double2>(0, 0);
+      double2 acc = mk<double2>(0, 0);
+      for (int r = lane; r <= i; r += 64) {
+        double2 a = col[r];
+        if (prev) {
+          a = csub(a, cadd(cmulc(vp[r], cw), cmulc(wp[r], cv)));
+          col[r] = a;
+        }
+        acc = cadd(acc, cmulc(vc[r], a));  // conj(A(r, j)) v(r)
+        if (j == i) colbuf[par * n + r] = a;
+      }
+      acc.x = wave_sum(acc.x);
+      acc.y = wave_sum(acc.y);
+      if (lane == 0) pbuf[par * n + j] = acc;
+      const double2 vj = vc[j];
+      sv += vj.x * acc.x + vj.y * acc.y;
+    }
+    if (lane == 0) red[wid] = sv;
+    __syncthreads();
+    if (t == 0) share[par * G + g] = wave_total4(red);
+    grid.sync();
+    // C: w = tau A v - |tau|^2 / 2 (v^H A v) v; v, w become the pending update
+    double vav = 0;
+    for (int h = 0; h < G; ++h) vav += share[par * G + h];
+    const double2 taui = s_tau;
+    const double a2 = -0.5 * (taui.x * taui.x + taui.y * taui.y) * vav;
+    for (int r = t; r <= i; r += kCoopThreads) {
+      const double2 v = vc[r];
+      wp[r] = cadd(cmul(taui, pbuf[par * n + r]), cscale(v, a2));
+      vp[r] = v;
+      if (g == 0) A[r + int64_t(c) * lda] = r < i ? v : mk<double2>(s_beta, 0);
+    }
+    if (t == 0 && g == 0) {
+      E[i] = s_beta;
+      Tau[i] = taui;
+    }
+    prev = !(taui.x == 0.0 && taui.y == 0.0);
+    __syncthreads();
+  }
+  if (g == 0 && t == 0) {  // column 0 is workgroup 0's first column
+    double2 a = cols[0];
+    if (prev) a = csub(a, cadd(cmulc(vp[0], wp[0]), cmulc(wp[0], vp[0])));
+    D[0] = a.x;
+  }
+}
+
 }  // namespace
+
+bool hetrd_upper_coop(int n, double2 *A, int64_t lda, double *d, double *e, double2 *tau, double2 *work,
+                      size_t work_elems, hipStream_t s) {
+  if (n <= 1 || lda < n) throw Error(kInvalidArgument, "hetrd_upper_coop: order or leading dimension");
+  static std::mutex mu;
+  static std::map<int, std::pair<int, int>> dev_info;  // device -> (CUs, LDS bytes per workgroup)
+  int dev = 0;
+  HALF_HIP(hipGetDevice(&dev));
+  std::pair<int, int> info;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = dev_info.find(dev);
+    if (it == dev_info.end()) {
+      int cus = 0, lds = 0;
+      HALF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      HALF_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+      it = dev_info.emplace(dev, std::make_pair(cus, lds)).first;
+    }
+    info = it->second;
+  }
+  const int cus = info.first;
+  const size_t lds_max = static_cast<size_t>(info.second) - 1024;  // the kernel's static shared data
+  const size_t vec_bytes = 3 * sizeof(double2) * size_t(n);
+  if (vec_bytes >= lds_max) return false;
+  // fewest workgroups whose columns fit (fewer workgroups: a cheaper grid barrier)
+  const int ncol = static_cast<int>(std::min<size_t>(n, (lds_max - vec_bytes) / (sizeof(double2) * size_t(n))));
+  if (ncol < 1) return false;
+  const int G = std::max(kCoopMinGroups, (n + ncol - 1) / ncol);
+  if (G > cus) return false;
+  const int ncol_used = (n + G - 1) / G;
+  const size_t lds = vec_bytes + sizeof(double2) * size_t(ncol_used) * n;
+  if (work_elems < coop_work_elems(n, G)) throw Error(kInternal, "hetrd_upper_coop: workspace too small");
+  const void *fn = reinterpret_cast<const void *>(&hetrd_upper_coop_kernel);
+  int per_cu = 0;
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)) != hipSuccess) {
+    (void)hipGetLastError();  // this runtime cannot raise the limit: leave the order to the library
+    return false;
+  }
+  HALF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kCoopThreads, lds));
+  if (per_cu < 1 || G > per_cu * cus) return false;  // every workgroup must be resident at once
+  double2 *colbuf = work, *pbuf = work + 2 * size_t(n);
+  double *share = reinterpret_cast<double *>(work + 4 * size_t(n));
+  int nn = n, nc = ncol_used;
+  void *args[] = {&nn, &A, &lda, &d, &e, &tau, &colbuf, &pbuf, &share, &nc};
+  HALF_HIP(hipLaunchCooperativeKernel(fn, dim3(G), dim3(kCoopThreads), args, static_cast<unsigned>(lds), s));
+  return true;
+}
+
+size_t coop_work_elems(int n, int groups) { return 4 * size_t(n) + size_t(groups); }
 
 void hetrd_upper(int n, int batch, double2 *A, int64_t lda, int64_t sA, double *d, int64_t sd, double *e,
                  int64_t se, double2 *tau, int64_t stau, hipStream_t s) {
