@@ -577,12 +577,18 @@ void KOperator<R>::projector_terms(int nb, const cplx_t<R> *psi, int64_t ldpsi, 
 // the projector stage, whose contractions fill the chip poorly at tens of
 // bands, runs on the second stream while the local stage runs here; the
 // local stage's last pass waits for it and adds its result as it writes
-// H psi and S psi.  Otherwise the two stages run in order on `st`.
+// H psi and S psi.  Otherwise the two stages run in order on `st`, as they
+// also do for blocks of more than kOverlapMaxElems coefficients: there
+// every kernel fills the chip and the projector terms' round trip through
+// nl_h / nl_s costs more than the overlap gains (216-atom cell, 512 bands:
+// FP64 17.9 ms overlapped against 17.4 ms in order; HfO2-like cell, 64
+// bands: 0.080 against 0.088 ms).
+constexpr int64_t kOverlapMaxElems = int64_t(1) << 22;
 template <typename R>
 void KOperator<R>::apply(int nb, const cplx_t<R> *psi, int64_t ldpsi, cplx_t<R> *hpsi, cplx_t<R> *spsi,
                          int64_t ldout, hipStream_t st) {
   if (nb <= 0) return;
-  if (np == 0 || !operator_overlap() || trace_enabled()) {
+  if (np == 0 || !operator_overlap() || trace_enabled() || int64_t(n) * nb > kOverlapMaxElems) {
     apply_local(nb, psi, ldpsi, hpsi, spsi, ldout, st);
     apply_projectors(nb, psi, ldpsi, hpsi, spsi, ldout, st);
     trace_close(st);

@@ -15,6 +15,7 @@ block while sweeping, per precision:
   contract.<p>.gram.wt     tile width of the coefficient contraction B^H psi
   contract.<p>.gram.fill   round fill its row-chunk count must reach
   contract.<p>.tall.wt     tile width of the expansion B (D c), B (Q c)
+  contract.fp64.*.gauss    FP64 complex products with three MFMAs or four
 The tiles are swept with the separate passes (fused off, so every pass runs):
 each candidate is applied to all three passes at once and every pass keeps
 the value that minimised its own traced time (the passes are independent
@@ -127,29 +128,45 @@ def tune(workload="hfo2", bands=0, steps=10, verbose=True):
         result["best"][f"local.{p}.xconv"] = xbest[0]
         # contraction tiles on the generic projector path
         set_tuning("operator.projector", 0)
-        cbest = ((0, 90), float("inf"))
-        for wt in (0, 1, 2):
-            for fill in (70, 90, 100):
-                set_tuning(f"contract.{p}.gram.wt", wt)
-                set_tuning(f"contract.{p}.gram.fill", fill)
+        # FP64 also chooses between four- and three-MFMA complex products
+        gauss = (0, 1) if prec == 64 else (None,)
+        cbest = ((None, 0, 90), float("inf"))
+        for ga in gauss:
+            for wt in (0, 1, 2):
+                for fill in (70, 90, 100):
+                    if ga is not None:
+                        set_tuning("contract.fp64.gram.gauss", ga)
+                    set_tuning(f"contract.{p}.gram.wt", wt)
+                    set_tuning(f"contract.{p}.gram.fill", fill)
+                    kern = run()
+                    ms = kern["proj_coeff_mfma"][0] / steps if "proj_coeff_mfma" in kern else float("inf")
+                    result["timings"][f"{p}.gram.gauss={ga},wt={wt},fill={fill}"] = ms
+                    if ms < cbest[1]:
+                        cbest = ((ga, wt, fill), ms)
+        ga, wt, fill = cbest[0]
+        if ga is not None:
+            set_tuning("contract.fp64.gram.gauss", ga)
+            result["best"]["contract.fp64.gram.gauss"] = ga
+        set_tuning(f"contract.{p}.gram.wt", wt)
+        set_tuning(f"contract.{p}.gram.fill", fill)
+        result["best"][f"contract.{p}.gram.wt"], result["best"][f"contract.{p}.gram.fill"] = wt, fill
+        tbest = ((None, 0), float("inf"))
+        for ga in gauss:
+            for wt in (0, 1, 2):
+                if ga is not None:
+                    set_tuning("contract.fp64.tall.gauss", ga)
+                set_tuning(f"contract.{p}.tall.wt", wt)
                 kern = run()
-                ms = kern["proj_coeff_mfma"][0] / steps if "proj_coeff_mfma" in kern else float("inf")
-                result["timings"][f"{p}.gram.wt={wt},fill={fill}"] = ms
-                if ms < cbest[1]:
-                    cbest = ((wt, fill), ms)
-        set_tuning(f"contract.{p}.gram.wt", cbest[0][0])
-        set_tuning(f"contract.{p}.gram.fill", cbest[0][1])
-        result["best"][f"contract.{p}.gram.wt"], result["best"][f"contract.{p}.gram.fill"] = cbest[0]
-        tbest = (0, float("inf"))
-        for wt in (0, 1, 2):
-            set_tuning(f"contract.{p}.tall.wt", wt)
-            kern = run()
-            ms = kern["proj_expand_mfma"][0] / steps if "proj_expand_mfma" in kern else float("inf")
-            result["timings"][f"{p}.tall.wt={wt}"] = ms
-            if ms < tbest[1]:
-                tbest = (wt, ms)
-        set_tuning(f"contract.{p}.tall.wt", tbest[0])
-        result["best"][f"contract.{p}.tall.wt"] = tbest[0]
+                ms = kern["proj_expand_mfma"][0] / steps if "proj_expand_mfma" in kern else float("inf")
+                result["timings"][f"{p}.tall.gauss={ga},wt={wt}"] = ms
+                if ms < tbest[1]:
+                    tbest = ((ga, wt), ms)
+        ga, wt = tbest[0]
+        if ga is not None:
+            set_tuning("contract.fp64.tall.gauss", ga)
+            result["best"]["contract.fp64.tall.gauss"] = ga
+        set_tuning(f"contract.{p}.tall.wt", wt)
+        result["best"][f"contract.{p}.tall.wt"] = wt
         # the Vx rows go to LDS only with band groups of two or more
         forms = ([(0, 1, 0, ov, pj) for ov in (0, 1) for pj in (0, 1)]
                  + [(1, 1, 0, ov, pj) for ov in (0, 1) for pj in (0, 1)]

@@ -18,7 +18,9 @@ PER_PREC = ["local.{p}.tile.x", "local.{p}.tile.y", "local.{p}.tile.z", "local.{
 
 
 def test_autotune_si2_keys_and_result():
-    keys = [k.format(p=p) for p in ("fp64", "fp32") for k in PER_PREC] + ["operator.overlap", "operator.projector"]
+    keys = [k.format(p=p) for p in ("fp64", "fp32") for k in PER_PREC] + ["operator.overlap", "operator.projector",
+                                                                         "contract.fp64.gram.gauss",
+                                                                         "contract.fp64.tall.gauss"]
     saved = {k: H.get_tuning(k) for k in keys}
     try:
         print("[gpu] autotune si2", flush=True)
@@ -28,7 +30,9 @@ def test_autotune_si2_keys_and_result():
             for k in PER_PREC:
                 assert k.format(p=p) in best
             assert f"{p}.tuned_step_ms" in res["timings"] and res["timings"][f"{p}.tuned_step_ms"] > 0
-            assert f"{p}.xconv=0" in res["timings"] and f"{p}.tall.wt=2" in res["timings"]
+            assert f"{p}.xconv=2" in res["timings"]
+        assert "fp64.tall.gauss=1,wt=2" in res["timings"] and "fp32.tall.gauss=None,wt=2" in res["timings"]
+        assert best["contract.fp64.gram.gauss"] in (0, 1) and best["contract.fp64.tall.gauss"] in (0, 1)
         # the FP32 choices are in effect now; the shared operator keys hold the FP32 sweep's
         for k in PER_PREC:
             assert H.get_tuning(k.format(p="fp32")) == best[k.format(p="fp32")]
