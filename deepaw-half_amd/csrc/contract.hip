@@ -47,9 +47,12 @@ struct GramArgs {
 
 // At least two waves per SIMD: with one (the widest FP64 forms otherwise
 // need 258 registers) every barrier and slab store leaves the matrix core idle.
-template <typename R, int WT, int NB, int PARTS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gram_kernel(GramArgs<R> g) {
-  using V = cplx_t<R>;
+// S: the operands' storage type in memory, R: the type of the LDS slabs and
+// the matrix-core products (S = double, R = float: FP32 products of FP64
+// blocks, the FP32 phase of the mixed solve).
+template <typename S, typename R, int WT, int NB, int PARTS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gram_kernel(GramArgs<S> g) {
+  using VS = cplx_t<S>;
   using A4 = typename Acc<R>::type;
   constexpr int TW = 32 * WT;        // workgroup tile width
   constexpr int LW = TW + PAD;       // LDS row length
@@ -77,16 +80,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
   const int lc = tid / TPC, lr = (tid % TPC) * L;
   const bool ca = a0 + lc < g.ka, cb = b0 + lc < g.kb;
   const int64_t bz = blockIdx.z;
-  const V *pa = g.A + bz * g.bA + int64_t(min(a0 + lc, g.ka - 1)) * g.lda;
-  const V *pb[NB];
+  const VS *pa = g.A + bz * g.bA + int64_t(min(a0 + lc, g.ka - 1)) * g.lda;
+  const VS *pb[NB];
   for (int j = 0; j < NB; ++j) pb[j] = g.B[j] + bz * g.bB[j] + int64_t(min(b0 + lc, g.kb - 1)) * g.ldb[j];
-  V ra[L], rb[NB][L];
+  VS ra[L], rb[NB][L];
   auto load = [&](int r0) {
     for (int i = 0; i < L; ++i) {
       const int r = r0 + lr + i;
       const bool rv = r < r_end;
-      ra[i] = (rv && ca) ? pa[r] : mk<V>(0, 0);
-      for (int j = 0; j < NB; ++j) rb[j][i] = (rv && cb) ? pb[j][r] : mk<V>(0, 0);
+      ra[i] = (rv && ca) ? pa[r] : mk<VS>(0, 0);
+      for (int j = 0; j < NB; ++j) rb[j][i] = (rv && cb) ? pb[j][r] : mk<VS>(0, 0);
     }
   };
   A4 acc[NB][WT][WT][PARTS];
@@ -98,11 +101,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
   for (int r0 = r_begin; r0 < r_end; r0 += KS) {
     __syncthreads();
     for (int i = 0; i < L; ++i) {
-      sa[0][lr + i][lc] = ra[i].x;
-      sa[1][lr + i][lc] = -ra[i].y;  // conj(A)
+      sa[0][lr + i][lc] = static_cast<R>(ra[i].x);
+      sa[1][lr + i][lc] = -static_cast<R>(ra[i].y);  // conj(A)
       for (int j = 0; j < NB; ++j) {
-        sb[j][0][lr + i][lc] = rb[j][i].x;
-        sb[j][1][lr + i][lc] = rb[j][i].y;
+        sb[j][0][lr + i][lc] = static_cast<R>(rb[j][i].x);
+        sb[j][1][lr + i][lc] = static_cast<R>(rb[j][i].y);
       }
     }
     __syncthreads();
@@ -133,7 +136,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
           if (ra2 < g.ka) {
             R re, im;
             cmac_result<R>(acc[j][t][u], i, re, im);
-            g.out[j][bz * g.bOut[j] + off + ra2 + int64_t(cb2) * g.ldo[j]] = mk<V>(re, im);
+            g.out[j][bz * g.bOut[j] + off + ra2 + int64_t(cb2) * g.ldo[j]] =
+                mk<VS>(static_cast<S>(re), static_cast<S>(im));
           }
         }
       }
@@ -201,9 +205,11 @@ struct TallArgs {
   int split;  // 1: blockIdx.z selects the output (W[z], C[z]) of one problem, not a batch entry
 };
 
-template <typename R, int WT, int NB, int PARTS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void tall_kernel(TallArgs<R> g) {
-  using V = cplx_t<R>;
+// S: storage type, R: product type (as for gram_kernel); the epilogue
+// (alpha, beta C) runs in S.
+template <typename S, typename R, int WT, int NB, int PARTS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void tall_kernel(TallArgs<S> g) {
+  using VS = cplx_t<S>;
   using A4 = typename Acc<R>::type;
   constexpr int TW = 32 * WT;
   constexpr int LW = TW + PAD;
@@ -221,20 +227,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
   const bool wcv = c0 + wcol < g.m;
   const int64_t bz = g.split ? 0 : blockIdx.z;
   const int zo = g.split ? static_cast<int>(blockIdx.z) : 0;  // first output of this workgroup
-  V ra[L], rw[NB][L];
+  VS ra[L], rw[NB][L];
   auto load = [&](int k0) {
     const int kc = k0 + akc;
     const bool kv = kc < g.k;
-    const V *pa = g.A + bz * g.bA + int64_t(min(kc, g.k - 1)) * g.lda;
+    const VS *pa = g.A + bz * g.bA + int64_t(min(kc, g.k - 1)) * g.lda;
     for (int i = 0; i < L; ++i) {
       const int r = g0 + ar0 + i;
-      ra[i] = (kv && r < g.n) ? pa[r] : mk<V>(0, 0);
+      ra[i] = (kv && r < g.n) ? pa[r] : mk<VS>(0, 0);
     }
     for (int j = 0; j < NB; ++j) {
-      const V *pw = g.W[j + zo] + bz * g.bW[j + zo] + int64_t(min(c0 + wcol, g.m - 1)) * g.ldw[j + zo];
+      const VS *pw = g.W[j + zo] + bz * g.bW[j + zo] + int64_t(min(c0 + wcol, g.m - 1)) * g.ldw[j + zo];
       for (int i = 0; i < L; ++i) {
         const int kr = k0 + wk0 + i;
-        rw[j][i] = (wcv && kr < g.k) ? pw[kr] : mk<V>(0, 0);
+        rw[j][i] = (wcv && kr < g.k) ? pw[kr] : mk<VS>(0, 0);
       }
     }
   };
@@ -247,11 +253,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
   for (int k0 = 0; k0 < g.k; k0 += KS) {
     __syncthreads();
     for (int i = 0; i < L; ++i) {
-      sa[0][akc][ar0 + i] = ra[i].x;
-      sa[1][akc][ar0 + i] = ra[i].y;
+      sa[0][akc][ar0 + i] = static_cast<R>(ra[i].x);
+      sa[1][akc][ar0 + i] = static_cast<R>(ra[i].y);
       for (int j = 0; j < NB; ++j) {
-        sw[j][0][wk0 + i][wcol] = rw[j][i].x;
-        sw[j][1][wk0 + i][wcol] = rw[j][i].y;
+        sw[j][0][wk0 + i][wcol] = static_cast<R>(rw[j][i].x);
+        sw[j][1][wk0 + i][wcol] = static_cast<R>(rw[j][i].y);
       }
     }
     __syncthreads();
@@ -275,16 +281,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
   // updates is loaded first (independent loads in flight together) and
   // only then are the results stored
   for (int j = 0; j < NB; ++j) {
-    V *const Cj = g.C[j + zo] + bz * g.bC[j + zo];
+    VS *const Cj = g.C[j + zo] + bz * g.bC[j + zo];
     const int64_t ldcj = g.ldc[j + zo];
-    V old[WT][WT][4];
-    if (g.beta != R(0)) {
+    VS old[WT][WT][4];
+    if (g.beta != S(0)) {
       for (int u = 0; u < WT; ++u)
         for (int t = 0; t < WT; ++t)
           for (int i = 0; i < 4; ++i) {
             const int col = c0 + wc + 16 * u + row;
             const int r = g0 + wr + 16 * t + mfma_row<R>(lane, i);
-            old[u][t][i] = (col < g.m && r < g.n) ? Cj[r + int64_t(col) * ldcj] : mk<V>(R(0), R(0));
+            old[u][t][i] = (col < g.m && r < g.n) ? Cj[r + int64_t(col) * ldcj] : mk<VS>(S(0), S(0));
           }
     }
     for (int u = 0; u < WT; ++u) {
@@ -294,28 +300,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         for (int i = 0; i < 4; ++i) {
           const int r = g0 + wr + 16 * t + mfma_row<R>(lane, i);
           if (r >= g.n) continue;
-          R vr, vi;
-          cmac_result<R>(acc[j][t][u], i, vr, vi);
-          vr *= g.alpha;
-          vi *= g.alpha;
-          if (g.beta != R(0)) {
+          R pr, pi;
+          cmac_result<R>(acc[j][t][u], i, pr, pi);
+          S vr = static_cast<S>(pr) * g.alpha, vi = static_cast<S>(pi) * g.alpha;
+          if (g.beta != S(0)) {
             vr += g.beta * old[u][t][i].x;
             vi += g.beta * old[u][t][i].y;
           }
-          Cj[r + int64_t(col) * ldcj] = mk<V>(vr, vi);
+          Cj[r + int64_t(col) * ldcj] = mk<VS>(vr, vi);
         }
     }
   }
 }
 
-template <typename R, int WT, int NB, int PARTS>
-void launch_gram(dim3 grid, const GramArgs<R> &a, hipStream_t s) {
-  hipLaunchKernelGGL((gram_kernel<R, WT, NB, PARTS>), grid, dim3(256), 0, s, a);
+template <typename S, typename R, int WT, int NB, int PARTS>
+void launch_gram(dim3 grid, const GramArgs<S> &a, hipStream_t s) {
+  hipLaunchKernelGGL((gram_kernel<S, R, WT, NB, PARTS>), grid, dim3(256), 0, s, a);
 }
 
-template <typename R, int WT, int NB, int PARTS>
-void launch_tall(dim3 grid, const TallArgs<R> &a, hipStream_t s) {
-  hipLaunchKernelGGL((tall_kernel<R, WT, NB, PARTS>), grid, dim3(256), 0, s, a);
+template <typename S, typename R, int WT, int NB, int PARTS>
+void launch_tall(dim3 grid, const TallArgs<S> &a, hipStream_t s) {
+  hipLaunchKernelGGL((tall_kernel<S, R, WT, NB, PARTS>), grid, dim3(256), 0, s, a);
 }
 
 }  // namespace
@@ -349,9 +354,24 @@ struct GramPlan {
   size_t work_elems;  // partial tiles (all outputs and batch entries); 0 without row chunks
 };
 
-// Workgroups of gram_kernel<R, WT, NB> the device holds at once (occupancy
-// per CU x CUs of the current device), cached per variant and device.
-template <typename R, int WT, int NB, int PARTS>
+// FP32 products for FP64 blocks, set for the calling thread by
+// ContractFp32Scope (the FP32 phase of a mixed solve).
+thread_local bool t_fp32_products = false;
+
+// Product form of a launch on blocks of type S: 0 S products with four
+// MFMAs per complex product (FP64 only), 1 S products with three, 2 FP32
+// products (three MFMAs) of FP64 blocks.
+template <typename S>
+int product_form(bool gauss_ok) {
+  if (sizeof(S) == 4) return 1;
+  if (t_fp32_products) return 2;
+  return gauss_ok ? 1 : 0;
+}
+
+// Workgroups of gram_kernel<S, R, WT, NB, PARTS> the device holds at once
+// (occupancy per CU x CUs of the current device), cached per variant and
+// device.
+template <typename S, typename R, int WT, int NB, int PARTS>
 int gram_capacity() {
   static std::mutex mu;
   static std::map<int, int> cache;
@@ -362,28 +382,43 @@ int gram_capacity() {
   if (it != cache.end()) return it->second;
   int per_cu = 0, cus = 0;
   HALF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void *>(&gram_kernel<R, WT, NB, PARTS>), 256, 0));
+      &per_cu, reinterpret_cast<const void *>(&gram_kernel<S, R, WT, NB, PARTS>), 256, 0));
   HALF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const int cap = std::max(1, per_cu) * std::max(1, cus);
   cache[dev] = cap;
   return cap;
 }
 
-// (the two-output 64-wide FP64 tile would spill with three accumulators)
-template <typename R>
-int gram_parts(int nout) {
-  return (sizeof(R) == 8 && !(g_gram_gauss && nout == 1)) ? 2 : 3;
+// (the two-output 64-wide FP64 tile would spill with three FP64 accumulators)
+template <typename S>
+int gram_form(int nout) {
+  return product_form<S>(g_gram_gauss && nout == 1);
 }
 
-template <typename R>
+template <typename S, typename R, int PARTS>
+int gram_capacity_wt(int wt, int nout) {
+  if (wt == 1) return nout == 1 ? gram_capacity<S, R, 1, 1, PARTS>() : gram_capacity<S, R, 1, 2, PARTS>();
+  return nout == 1 ? gram_capacity<S, R, 2, 1, PARTS>() : gram_capacity<S, R, 2, 2, PARTS>();
+}
+
+template <typename S>
 int gram_capacity_of(int wt, int nout) {
-  constexpr int P4 = kCmacParts<R>;  // the default product form of R
-  if (gram_parts<R>(nout) == P4) {
-    if (wt == 1) return nout == 1 ? gram_capacity<R, 1, 1, P4>() : gram_capacity<R, 1, 2, P4>();
-    return nout == 1 ? gram_capacity<R, 2, 1, P4>() : gram_capacity<R, 2, 2, P4>();
+  const int form = gram_form<S>(nout);
+  if constexpr (std::is_same<S, double>::value) {
+    if (form == 0) return gram_capacity_wt<double, double, 2>(wt, nout);
+    if (form == 2) return gram_capacity_wt<double, float, 3>(wt, nout);
+    return gram_capacity_wt<double, double, 3>(wt, nout);
+  } else {
+    return gram_capacity_wt<float, float, 3>(wt, nout);
   }
-  if (wt == 1) return nout == 1 ? gram_capacity<R, 1, 1, 3>() : gram_capacity<R, 1, 2, 3>();
-  return nout == 1 ? gram_capacity<R, 2, 1, 3>() : gram_capacity<R, 2, 2, 3>();
+}
+
+template <typename S, typename R, int PARTS>
+void launch_gram_wt(int wt, int nout, dim3 grid, const GramArgs<S> &a, hipStream_t s) {
+  if (wt == 1)
+    nout == 1 ? launch_gram<S, R, 1, 1, PARTS>(grid, a, s) : launch_gram<S, R, 1, 2, PARTS>(grid, a, s);
+  else
+    nout == 1 ? launch_gram<S, R, 2, 1, PARTS>(grid, a, s) : launch_gram<S, R, 2, 2, PARTS>(grid, a, s);
 }
 
 // Row chunks for `units` output tiles (over the batch) on a device holding
@@ -497,17 +532,16 @@ void gram_batched_work(int batch, int n, int ka, int kb, const cplx_t<R> *A, int
     a.chunk_stride = tile_elems;
   }
   const dim3 grid(plan.tiles, nchunks, batch);
-  constexpr int P4 = kCmacParts<R>;
-  if (gram_parts<R>(nb) == P4) {
-    if (plan.wt == 1)
-      nb == 1 ? launch_gram<R, 1, 1, P4>(grid, a, s) : launch_gram<R, 1, 2, P4>(grid, a, s);
+  if constexpr (std::is_same<R, double>::value) {
+    const int form = gram_form<R>(nb);
+    if (form == 0)
+      launch_gram_wt<double, double, 2>(plan.wt, nb, grid, a, s);
+    else if (form == 2)
+      launch_gram_wt<double, float, 3>(plan.wt, nb, grid, a, s);
     else
-      nb == 1 ? launch_gram<R, 2, 1, P4>(grid, a, s) : launch_gram<R, 2, 2, P4>(grid, a, s);
+      launch_gram_wt<double, double, 3>(plan.wt, nb, grid, a, s);
   } else {
-    if (plan.wt == 1)
-      nb == 1 ? launch_gram<R, 1, 1, 3>(grid, a, s) : launch_gram<R, 1, 2, 3>(grid, a, s);
-    else
-      nb == 1 ? launch_gram<R, 2, 1, 3>(grid, a, s) : launch_gram<R, 2, 2, 3>(grid, a, s);
+    launch_gram_wt<float, float, 3>(plan.wt, nb, grid, a, s);
   }
   HALF_HIP(hipGetLastError());
   if (nchunks > 1) {
@@ -604,18 +638,35 @@ void gemm_tall_batched(int batch, int n, int m, int k, R alpha, const cplx_t<R> 
   a.k = k;
   a.alpha = alpha;
   a.beta = beta;
-  constexpr int P4 = kCmacParts<R>;
   // FP64 Gauss form: both outputs of one problem as grid.z = 2, one each
-  const bool split = sizeof(R) == 8 && g_tall_gauss && nb == 2 && batch == 1 && !inplace;
+  const bool fp32_products = sizeof(R) == 8 && t_fp32_products;
+  const bool split = sizeof(R) == 8 && !fp32_products && g_tall_gauss && nb == 2 && batch == 1 && !inplace;
   a.split = split ? 1 : 0;
   const dim3 grid((n + tw - 1) / tw, (m + tw - 1) / tw, split ? 2 : batch);
   if (grid.y > 65535) throw Error(kInvalidArgument, "gemm_tall: too many columns");
-  if (sizeof(R) == 8 && g_tall_gauss && !inplace && (nb == 1 || split)) {
-    wt == 1 ? launch_tall<R, 1, 1, 3>(grid, a, s) : launch_tall<R, 2, 1, 3>(grid, a, s);
-  } else if (wt == 1) {
-    nb == 1 ? launch_tall<R, 1, 1, P4>(grid, a, s) : launch_tall<R, 1, 2, P4>(grid, a, s);
+  const int nbl = split ? 1 : nb;  // outputs per workgroup
+  if constexpr (std::is_same<R, double>::value) {
+    const int form = product_form<double>(g_tall_gauss && !inplace && nbl == 1);
+    auto go = [&](auto s_tag, auto r_tag, auto parts) {
+      using SS = decltype(s_tag);
+      using RR = decltype(r_tag);
+      constexpr int PP = decltype(parts)::value;
+      if (wt == 1)
+        nbl == 1 ? launch_tall<SS, RR, 1, 1, PP>(grid, a, s) : launch_tall<SS, RR, 1, 2, PP>(grid, a, s);
+      else
+        nbl == 1 ? launch_tall<SS, RR, 2, 1, PP>(grid, a, s) : launch_tall<SS, RR, 2, 2, PP>(grid, a, s);
+    };
+    if (form == 0)
+      go(double(), double(), std::integral_constant<int, 2>());
+    else if (form == 2)
+      go(double(), float(), std::integral_constant<int, 3>());
+    else
+      go(double(), double(), std::integral_constant<int, 3>());
   } else {
-    nb == 1 ? launch_tall<R, 2, 1, P4>(grid, a, s) : launch_tall<R, 2, 2, P4>(grid, a, s);
+    if (wt == 1)
+      nbl == 1 ? launch_tall<float, float, 1, 1, 3>(grid, a, s) : launch_tall<float, float, 1, 2, 3>(grid, a, s);
+    else
+      nbl == 1 ? launch_tall<float, float, 2, 1, 3>(grid, a, s) : launch_tall<float, float, 2, 2, 3>(grid, a, s);
   }
   HALF_HIP(hipGetLastError());
 }
@@ -626,6 +677,9 @@ void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, co
                hipStream_t s) {
   gemm_tall_batched<R>(1, n, m, k, alpha, A, lda, 0, W, ldw, 0, beta, C, ldc, 0, W2, ldw2, 0, C2, ldc2, 0, s);
 }
+
+ContractFp32Scope::ContractFp32Scope(bool on) : prev_(t_fp32_products) { t_fp32_products = on; }
+ContractFp32Scope::~ContractFp32Scope() { t_fp32_products = prev_; }
 
 bool set_contract_tuning(const std::string &key, int value) {
   for (int p = 0; p < 2; ++p) {

@@ -52,6 +52,22 @@ void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, co
 
 constexpr int gemm_tall_inplace_max = 64;
 
+// While alive, gram / gemm_tall on FP64 blocks in the calling thread form
+// their products in FP32 on the matrix cores (operands rounded to FP32 as
+// they are staged, results and the beta C update in FP64): the FP32 phase
+// of a mixed solve, whose iterates carry FP32 operator errors anyway.
+// Scopes nest; the previous setting is restored on destruction.
+class ContractFp32Scope {
+ public:
+  explicit ContractFp32Scope(bool on);
+  ~ContractFp32Scope();
+  ContractFp32Scope(const ContractFp32Scope &) = delete;
+  ContractFp32Scope &operator=(const ContractFp32Scope &) = delete;
+
+ private:
+  bool prev_;
+};
+
 // Tuning keys contract.<fp64|fp32>.gram.wt / tall.wt (0 = shape rule, 1 =
 // 32-wide tiles, 2 = 64-wide) and contract.<fp64|fp32>.gram.fill (10-100:
 // percent of the device's workgroup slots every round of a row-chunked gram

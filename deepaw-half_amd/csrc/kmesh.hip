@@ -668,6 +668,8 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
                        int64_t(op64.nmax), w.get(), int64_t(m2), dcols.get(), dseeds.get(), P, zk, N, m, np, ld);
     HALF_HIP(hipGetLastError());
     apply_p(np);
+    // FP32 phase: the dense algebra below with FP32 products (as solve_acc)
+    ContractFp32Scope fp32_products(!fp64_phase && solver_fp32_algebra());
     // S-orthogonalise against X (cross: m x np per k point)
     trace_mark("acc_orthogonalise", s);
     gram_batched<double>(nk, N, m, np, X, ld, zk, SP, ld, zk, cross.get(), m, int64_t(m) * m, nullptr, 0, 0, nullptr,

@@ -302,6 +302,9 @@ int g_precond = 0;
 // error per iteration into H X (about sqrt(iterations) times the error of
 // one application), which sets the floor the FP32 phase stalls at.
 int g_refresh = 0;
+// Mixed form: the FP32 phase's dense algebra with FP32 products (1, default)
+// or FP64 (0), key "solver.fp32algebra".
+int g_fp32_algebra = 1;
 
 // zheevd with vectors (the full Hermitian matrix in A; eigenvectors replace
 // A, eigenvalues ascending in w; e is workspace of n values).  Orders up to
@@ -439,12 +442,18 @@ bool set_solver_tuning(const std::string &key, int value) {
     g_refresh = value;
     return true;
   }
+  if (key == "solver.fp32algebra") {
+    if (value < 0 || value > 1) return false;
+    g_fp32_algebra = value;
+    return true;
+  }
   if (key != "solver.hetrd" || value < 0 || value > 1) return false;
   g_hetrd = value != 0;
   return true;
 }
 
 bool solver_locking() { return g_lock != 0; }
+bool solver_fp32_algebra() { return g_fp32_algebra != 0; }
 
 int AccControls::lock_mode() const { return lock < 0 ? g_lock : lock; }
 int AccControls::precond_mode() const { return precond < 0 ? g_precond : precond; }
@@ -459,6 +468,8 @@ bool get_solver_tuning(const std::string &key, int &value) {
     value = g_precond;
   } else if (key == "solver.refresh") {
     value = g_refresh;
+  } else if (key == "solver.fp32algebra") {
+    value = g_fp32_algebra;
   } else if (key == "solver.hetrd") {
     value = g_hetrd ? 1 : 0;
   } else {
@@ -874,6 +885,11 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
     }
     HALF_HIP(hipGetLastError());
     apply(np, P, HP, SP);
+    // the rest of an FP32-phase iteration forms its Grams, updates and
+    // rotations with FP32 products on the matrix cores (FP64 blocks; the
+    // subspace eigensolves stay FP64); the switch to FP64 recomputes H X,
+    // S X and the Rayleigh-Ritz step in FP64 (key solver.fp32algebra)
+    ContractFp32Scope fp32_products(!fp64_phase && g_fp32_algebra != 0);
     // S-orthogonalise the correction block against X (cross: m x np)
     trace_mark("acc_orthogonalise_gemm", s);
     gram<double>(n, m, np, X, ld, SP, ld, wk.cross.get(), m, nullptr, 0, nullptr, 0, wk.gwork, s);

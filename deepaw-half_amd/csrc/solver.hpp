@@ -106,6 +106,9 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
 // (tridiag.hip) before rocSOLVER's zstedc and zunmtr (default), or run
 // rocSOLVER's zheevd throughout.
 bool set_solver_tuning(const std::string &key, int value);
+// key solver.fp32algebra: FP32 products in the dense algebra of a mixed
+// solve's FP32 phase
+bool solver_fp32_algebra();
 bool get_solver_tuning(const std::string &key, int &value);
 // Whether the all-band solves drop the correction columns of converged bands
 // (tuning key "solver.lock").
