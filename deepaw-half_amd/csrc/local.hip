@@ -54,9 +54,11 @@ struct Tuning {
   int xgroup[2] = {8, 8};
   // x pass where every k point's active planes are one run: 0 transforms,
   // 1 per-column convolution on the vector units, 2 the same as a Toeplitz
-  // product on the matrix cores (96^3 box, 41 planes, 512 bands: 1.76 ms
-  // against 3.22 ms FP64, 0.92 against 1.49 ms FP32; transforms 10.1 ms)
-  int xconv[2] = {2, 2};
+  // product on the matrix cores (four waves per workgroup), 3 the same with
+  // eight waves and three-MFMA products (96^3 box, 41 planes, 512 bands:
+  // FP64 1.64 ms (3) / 1.75 ms (2) against 3.22 ms (1) and 8.9 ms (0); FP32
+  // 0.92 ms (2) / 0.94 ms (3) against 1.49 ms (1))
+  int xconv[2] = {3, 2};
   int fused[2] = {1, 1};  // y, x, y passes in one kernel where the convolution applies
   int ygroup[2] = {4, 4}; // bands per workgroup of the fused kernel (fewer where LDS is short)
   int vxlds[2] = {1, 1};  // fused kernel with the slice's Vx rows in LDS where it applies
@@ -331,11 +333,20 @@ __global__ __launch_bounds__(256) void x_conv_kernel(cplx_t<R> *box, const cplx_
 // one L2 (gridDim.x is a multiple of 8; groups past the end are empty).
 constexpr int kXmCols = 8;
 
-template <typename R, int PMAX>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void x_mfma_kernel(
+// WAVES = 4: two columns per wave, the default complex-product form of R,
+// at least three waves per SIMD (the 48-plane form needs up to 168
+// registers; left to itself the compiler took more and the kernel ran 10%
+// slower); WAVES = 8: one column per wave, three-MFMA (Gauss) products, whose
+// extra accumulators fit because each wave holds half the tiles.
+template <typename R, int PMAX, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES == 4 ? 3 : 2))) void x_mfma_kernel(
     cplx_t<R> *box, const cplx_t<R> *vx, const LocalTables *tabs, KStride ks, int nb, int groups_per_xcd) {
   using V = cplx_t<R>;
   using A4 = typename Acc<R>::type;
+  constexpr int NT = 64 * WAVES;        // threads
+  constexpr int CPW = kXmCols / WAVES;  // columns per wave
+  constexpr int PARTS = WAVES == 8 ? 3 : kCmacParts<R>;
+  constexpr int JS = 4 * kXmCols * 16 / NT;  // staged elements per thread and step
   constexpr int MT = (PMAX + 15) / 16;  // 16-plane output tiles
   constexpr int SB = 17;                // band row length of a staged step (16 + 1 pad)
   constexpr int OC = kXmCols + 1;       // column row length of an output tile (8 + 1 pad)
@@ -361,12 +372,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void x
     d += d < 0 ? t.n1 : 0;
     tw[w][k] = c < C ? vx[static_cast<long long>(d) * C + c] : mk<V>(R(0), R(0));
   }
-  // staging: element e = tid + 256 j of a step is column e & 7, band
+  // staging: element e = tid + NT j of a step is column e & 7, band
   // (e >> 3) & 15, plane (e >> 7) of the step
-  auto fetch = [&](int k4, V (&r)[2]) {
+  auto fetch = [&](int k4, V (&r)[JS]) {
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int e = tid + 256 * j;
+    for (int j = 0; j < JS; ++j) {
+      const int e = tid + NT * j;
       const int cc = e & 7, band = (e >> 3) & 15, q = 4 * k4 + (e >> 7);
       int i1 = s0 + q;
       i1 -= i1 >= t.n1 ? t.n1 : 0;
@@ -375,23 +386,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void x
                 : mk<V>(R(0), R(0));
     }
   };
-  auto stash = [&](int st, const V (&r)[2]) {
+  auto stash = [&](int st, const V (&r)[JS]) {
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int e = tid + 256 * j;
+    for (int j = 0; j < JS; ++j) {
+      const int e = tid + NT * j;
       buf[st * STAGE + ((e >> 7) * kXmCols + (e & 7)) * SB + ((e >> 3) & 15)] = r[j];
     }
   };
   const int nk4 = (P + 3) / 4;
   const int row = lane & 15, kq = lane >> 4;
-  A4 acc[2][MT][kCmacParts<R>];
+  A4 acc[CPW][MT][PARTS];
 #pragma unroll
-  for (int j = 0; j < 2; ++j)
+  for (int j = 0; j < CPW; ++j)
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-      for (int u = 0; u < kCmacParts<R>; ++u) acc[j][mt][u] = A4{0, 0, 0, 0};
-  V r[2];
+      for (int u = 0; u < PARTS; ++u) acc[j][mt][u] = A4{0, 0, 0, 0};
+  V r[JS];
   fetch(0, r);
   stash(0, r);
   for (int k4 = 0; k4 < nk4; ++k4) {
@@ -400,8 +411,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void x
     const V *stg = buf + (k4 & 1) * STAGE;
     const int q = 4 * k4 + kq;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int cc = 2 * wave + j;
+    for (int j = 0; j < CPW; ++j) {
+      const int cc = CPW * wave + j;
       const V bv = stg[(kq * kXmCols + cc) * SB + row];
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
@@ -420,17 +431,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void x
     if (16 * mt >= P) continue;  // uniform
     __syncthreads();  // the staged steps (or the previous tile) are no longer read
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < CPW; ++j)
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
         R re, im;
         cmac_result<R>(acc[j][mt], rr, re, im);
-        buf[(mfma_row<R>(lane, rr) * 16 + row) * OC + 2 * wave + j] = mk<V>(re, im);
+        buf[(mfma_row<R>(lane, rr) * 16 + row) * OC + CPW * wave + j] = mk<V>(re, im);
       }
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < 8; ++it) {
-      const int e = tid + 256 * it;
+    for (int it = 0; it < 16 * 16 * kXmCols / NT; ++it) {
+      const int e = tid + NT * it;
       const int cc = e & 7, band = (e >> 3) & 15, opl = e >> 7;
       const int op = 16 * mt + opl;
       int i1 = s0 + op;
@@ -832,12 +843,19 @@ const void *x_conv_for(int planes, int &pmax) {
   return nullptr;
 }
 
-template <typename R>
-const void *x_mfma_for(int planes) {
-  if (planes <= 16) return reinterpret_cast<const void *>(&x_mfma_kernel<R, 16>);
-  if (planes <= 32) return reinterpret_cast<const void *>(&x_mfma_kernel<R, 32>);
-  if (planes <= 48) return reinterpret_cast<const void *>(&x_mfma_kernel<R, 48>);
+template <typename R, int WAVES>
+const void *x_mfma_for_w(int planes) {
+  if (planes <= 16) return reinterpret_cast<const void *>(&x_mfma_kernel<R, 16, WAVES>);
+  if (planes <= 32) return reinterpret_cast<const void *>(&x_mfma_kernel<R, 32, WAVES>);
+  if (planes <= 48) return reinterpret_cast<const void *>(&x_mfma_kernel<R, 48, WAVES>);
   return nullptr;
+}
+
+// xconv 2: four waves per workgroup; 3: eight, with three-MFMA products
+template <typename R>
+const void *x_mfma_for(int planes, int form, int &threads) {
+  threads = form == 3 ? 512 : 256;
+  return form == 3 ? x_mfma_for_w<R, 8>(planes) : x_mfma_for_w<R, 4>(planes);
 }
 
 // Transform lengths with compile-time kernels (the boxes of the workloads
@@ -1056,7 +1074,9 @@ void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff,
   int pw = kb.max_planes;
   const size_t xconv_lds = size_t(2 * pw - 1) * kXcCols * sizeof(V);
   const bool conv_ok = vx && kb.contiguous && pw <= n1;
-  const void *xmfma = conv_ok && g_tune.xconv[prec] == 2 ? x_mfma_for<R>(pw) : nullptr;
+  int xm_threads = 256;
+  const void *xmfma = conv_ok && g_tune.xconv[prec] >= 2 ? x_mfma_for<R>(pw, g_tune.xconv[prec], xm_threads)
+                                                          : nullptr;
   const bool xconv_ok = conv_ok && g_tune.xconv[prec] && xconv_lds <= 65536;
   const void *xconv = !xmfma && xconv_ok ? x_conv_for<R>(pw, pmax) : nullptr;
   if (xmfma) {
@@ -1066,7 +1086,7 @@ void local_apply_batch(const FftPlan &plan, const LocalBatch &kb, const R *veff,
     const unsigned gyb = static_cast<unsigned>((nb + 15) / 16);
     void *args[] = {&boxes, &vx, &tabs, &ks, &nbi, &per_xcd};
     trace_mark("local_x_mfma", s);
-    HALF_HIP(hipLaunchKernel(xmfma, dim3(gx, gyb, nk), dim3(256), args, 0, s));
+    HALF_HIP(hipLaunchKernel(xmfma, dim3(gx, gyb, nk), dim3(static_cast<unsigned>(xm_threads)), args, 0, s));
   } else if (xconv) {
     void *args[] = {&boxes, &vx, &tabs, &ks, &nbi, &pw};
     trace_mark("local_x_conv", s);
@@ -1127,7 +1147,7 @@ bool set_local_tuning(const std::string &key, int value) {
       return true;
     }
     if (key == pre + ".xconv") {
-      if (value < 0 || value > 2) return false;
+      if (value < 0 || value > 3) return false;
       g_tune.xconv[p] = value;
       return true;
     }

@@ -5,7 +5,8 @@ block while sweeping, per precision:
   local.<p>.tile.{x,y,z}   FFT points per workgroup tile of each pass
   local.<p>.xgroup         bands sharing one Veff tile in the x pass
   local.<p>.xconv          x pass as transforms (0), a per-column convolution on
-                           the vector units (1) or on the matrix cores (2)
+                           the vector units (1) or on the matrix cores (2: four
+                           waves per workgroup, 3: eight, three-MFMA products)
   local.<p>.fused          y, x, y passes fused per (band, i3) slice or not
   local.<p>.ygroup         bands per workgroup of the fused kernel
   local.<p>.vxlds          fused kernel with the Vx rows staged in LDS or read from L2
@@ -118,7 +119,7 @@ def tune(workload="hfo2", bands=0, steps=10, verbose=True):
         set_tuning(f"local.{p}.xgroup", gbest[0])
         result["best"][f"local.{p}.xgroup"] = gbest[0]
         xbest = (1, float("inf"))
-        for xc in (0, 1, 2):
+        for xc in (0, 1, 2, 3):
             set_tuning(f"local.{p}.xconv", xc)
             ms = _pass_ms(run(), "x", steps)
             result["timings"][f"{p}.xconv={xc}"] = ms

@@ -90,7 +90,7 @@ def test_c5_operator_520_bands(c5):
             # x pass on the vector units and as the Toeplitz product on the matrix
             # cores; FP64 projector contractions with four and three MFMAs per
             # complex product
-            for xconv, gauss in ((1, 0), (2, 0), (2, 1)) if prec == 64 else ((1, 0), (2, 0)):
+            for xconv, gauss in ((1, 0), (2, 0), (2, 1), (3, 1)) if prec == 64 else ((1, 0), (2, 0), (3, 0)):
                 H.set_tuning(f"local.fp{prec}.xconv", xconv)
                 H.set_tuning("contract.fp64.gram.gauss", gauss)
                 H.set_tuning("contract.fp64.tall.gauss", gauss)

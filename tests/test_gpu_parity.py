@@ -327,7 +327,7 @@ def test_local_x_pass_forms(si2):
     psi = psi_block(n, nb)
     ref = np.load(f"tests/golden/{name}.apply.npy")
     forms = {"fused1": (1, 1, 1, 1), "fused2": (1, 1, 2, 1), "fused4": (1, 1, 4, 1), "fused2_l2": (1, 1, 2, 0),
-             "conv": (0, 1, 1, 1), "mfma": (0, 2, 1, 1), "fft": (0, 0, 1, 1)}
+             "conv": (0, 1, 1, 1), "mfma": (0, 2, 1, 1), "mfma8": (0, 3, 1, 1), "fft": (0, 0, 1, 1)}
     saved = {f"local.fp{p}.{k}": H.get_tuning(f"local.fp{p}.{k}") for p in (64, 32)
              for k in ("fused", "xconv", "ygroup", "vxlds")}
     stream = hip.Stream()
