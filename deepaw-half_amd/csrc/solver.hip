@@ -277,6 +277,7 @@ void check_info(const DeviceArray<rocblas_int> &info, hipStream_t s, const char 
 constexpr int64_t kNormaliseByInverse = int64_t(1) << 20;
 
 bool g_hetrd = true;
+int g_hetrd_mode = 1;  // solver.hetrd value: 0 rocSOLVER, 1 own reductions, 2 cooperative one with columns in A
 // Locking in the all-band solve (key "solver.lock").  1 (soft): a band
 // whose residual already meets the current target contributes no correction
 // column to the next Rayleigh-Ritz subspace (it stays in X).  2 (soft plus
@@ -326,7 +327,7 @@ void heevd_vectors(Blas &blas, int k, double2 *A, double *w, double *e, rocblas_
   } else {
     const size_t need = coop_work_elems(k, kTridiagCoopMaxGroups);
     blas.eig_coop.ensure(need);
-    if (!hetrd_upper_coop(k, A, k, w, e, tau, blas.eig_coop.get(), need, blas.stream)) {
+    if (!hetrd_upper_coop(k, A, k, w, e, tau, blas.eig_coop.get(), need, blas.stream, g_hetrd_mode == 2)) {
       HALF_BLAS(rocsolver_zheevd(blas.h, rocblas_evect_original, rocblas_fill_upper, k, rz(A), k, w, e, info));
       return;
     }
@@ -447,8 +448,9 @@ bool set_solver_tuning(const std::string &key, int value) {
     g_fp32_algebra = value;
     return true;
   }
-  if (key != "solver.hetrd" || value < 0 || value > 1) return false;
+  if (key != "solver.hetrd" || value < 0 || value > 2) return false;
   g_hetrd = value != 0;
+  g_hetrd_mode = value;
   return true;
 }
 
@@ -471,7 +473,7 @@ bool get_solver_tuning(const std::string &key, int &value) {
   } else if (key == "solver.fp32algebra") {
     value = g_fp32_algebra;
   } else if (key == "solver.hetrd") {
-    value = g_hetrd ? 1 : 0;
+    value = g_hetrd_mode;
   } else {
     return false;
   }

@@ -246,8 +246,13 @@ __global__ __launch_bounds__(kTrdThreads) void hetrd_upper_kernel(int n, double2
 // (16 workgroups) against about 18 us for rocSOLVER's chain of launches;
 // dense solves of order 339 / 531 / 645 took 14.7 / 24.7 / 31.4 ms against
 // 17.3 / 27.3 / 32.3 ms with rocSOLVER's reduction (one MI355X).
+// LDS_COLS false: the columns stay in A (global memory, L2 / Infinity Cache
+// resident at these orders) and each workgroup updates its own in place,
+// for orders whose columns would need more workgroups than a cheap barrier
+// allows.
 constexpr int kCoopThreads = 256;
 
+template <bool LDS_COLS>
 __global__ __launch_bounds__(kCoopThreads) void hetrd_upper_coop_kernel(int n, double2 *A, int64_t lda, double *D,
                                                                         double *E, double2 *Tau, double2 *colbuf,
                                                                         double2 *pbuf, double *share, int ncol) {
@@ -255,15 +260,17 @@ __global__ __launch_bounds__(kCoopThreads) void hetrd_upper_coop_kernel(int n, d
   cg::grid_group grid = cg::this_grid();
   const int G = static_cast<int>(gridDim.x), g = static_cast<int>(blockIdx.x);
   extern __shared__ double2 coop_lds[];
-  double2 *cols = coop_lds;                   // [ncol][n]: column g + lc G at lc n
-  double2 *vc = cols + static_cast<size_t>(ncol) * n;  // this step's v
+  double2 *cols = coop_lds;                   // LDS_COLS: [ncol][n], column g + lc G at lc n
+  double2 *vc = cols + (LDS_COLS ? static_cast<size_t>(ncol) * n : 0);  // this step's v
   double2 *vp = vc + n, *wp = vp + n;         // the pending update's v, w
   __shared__ double red[kCoopThreads / 64];
   __shared__ double2 s_alpha, s_tau;
   __shared__ double s_beta;
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-  for (int lc = 0; lc < ncol && g + lc * G < n; ++lc)
-    for (int r = t; r < n; r += kCoopThreads) cols[static_cast<size_t>(lc) * n + r] = A[r + int64_t(g + lc * G) * lda];
+  if (LDS_COLS)
+    for (int lc = 0; lc < ncol && g + lc * G < n; ++lc)
+      for (int r = t; r < n; r += kCoopThreads)
+        cols[static_cast<size_t>(lc) * n + r] = A[r + int64_t(g + lc * G) * lda];
   bool prev = false;
   for (int i = n - 2; i >= 0; --i) {
     const int c = i + 1, par = i & 1;
@@ -308,7 +315,7 @@ __global__ __launch_bounds__(kCoopThreads) void hetrd_upper_coop_kernel(int n, d
     for (int lc = wid; lc < ncol; lc += kCoopThreads / 64) {
       const int j = g + lc * G;
       if (j > i) break;  // uniform over the wave
-      double2 *col = cols + static_cast<size_t>(lc) * n;
+      double2 *col = LDS_COLS ? cols + static_cast<size_t>(lc) * n : A + int64_t(j) * lda;
       const double2 cw = prev ? wp[j] : mk<double2>(0, 0), cv = prev ? vp[j] : mk<double2>(0, 0);
       double2 acc = mk<double2>(0, 0);
       for (int r = lane; r <= i; r += 64) {
@@ -349,7 +356,7 @@ __global__ __launch_bounds__(kCoopThreads) void hetrd_upper_coop_kernel(int n, d
     __syncthreads();
   }
   if (g == 0 && t == 0) {  // column 0 is workgroup 0's first column
-    double2 a = cols[0];
+    double2 a = LDS_COLS ? cols[0] : A[0];
     if (prev) a = csub(a, cadd(cmulc(vp[0], wp[0]), cmulc(wp[0], vp[0])));
     D[0] = a.x;
   }
@@ -358,7 +365,7 @@ __global__ __launch_bounds__(kCoopThreads) void hetrd_upper_coop_kernel(int n, d
 }  // namespace
 
 bool hetrd_upper_coop(int n, double2 *A, int64_t lda, double *d, double *e, double2 *tau, double2 *work,
-                      size_t work_elems, hipStream_t s) {
+                      size_t work_elems, hipStream_t s, bool force_global) {
   if (n <= 1 || lda < n) throw Error(kInvalidArgument, "hetrd_upper_coop: order or leading dimension");
   static std::mutex mu;
   static std::map<int, std::pair<int, int>> dev_info;  // device -> (CUs, LDS bytes per workgroup)
@@ -380,15 +387,18 @@ bool hetrd_upper_coop(int n, double2 *A, int64_t lda, double *d, double *e, doub
   const size_t lds_max = static_cast<size_t>(info.second) - 1024;  // the kernel's static shared data
   const size_t vec_bytes = 3 * sizeof(double2) * size_t(n);
   if (vec_bytes >= lds_max) return false;
-  // fewest workgroups whose columns fit (fewer workgroups: a cheaper grid barrier)
-  const int ncol = static_cast<int>(std::min<size_t>(n, (lds_max - vec_bytes) / (sizeof(double2) * size_t(n))));
-  if (ncol < 1) return false;
-  const int G = std::max(kCoopMinGroups, (n + ncol - 1) / ncol);
-  if (G > cus) return false;
+  // fewest workgroups whose columns fit in LDS (fewer workgroups: a cheaper
+  // grid barrier); above kCoopLdsMaxGroups of them the columns stay in A
+  const int ncol_fit = static_cast<int>(std::min<size_t>(n, (lds_max - vec_bytes) / (sizeof(double2) * size_t(n))));
+  const int G_lds = ncol_fit >= 1 ? std::max(kCoopMinGroups, (n + ncol_fit - 1) / ncol_fit) : cus + 1;
+  const bool lds_cols = !force_global && G_lds <= kCoopLdsMaxGroups;
+  const int G = lds_cols ? G_lds : std::min(kCoopLdsMaxGroups, cus);
+  if (G > cus || G < 1) return false;
   const int ncol_used = (n + G - 1) / G;
-  const size_t lds = vec_bytes + sizeof(double2) * size_t(ncol_used) * n;
+  const size_t lds = vec_bytes + (lds_cols ? sizeof(double2) * size_t(ncol_used) * n : 0);
   if (work_elems < coop_work_elems(n, G)) throw Error(kInternal, "hetrd_upper_coop: workspace too small");
-  const void *fn = reinterpret_cast<const void *>(&hetrd_upper_coop_kernel);
+  const void *fn = lds_cols ? reinterpret_cast<const void *>(&hetrd_upper_coop_kernel<true>)
+                            : reinterpret_cast<const void *>(&hetrd_upper_coop_kernel<false>);
   int per_cu = 0;
   if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)) != hipSuccess) {
     (void)hipGetLastError();  // this runtime cannot raise the limit: leave the order to the library

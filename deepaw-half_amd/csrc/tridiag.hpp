@@ -22,17 +22,22 @@ void hetrd_upper(int n, int batch, double2 *A, int64_t lda, int64_t sA, double *
                  int64_t se, double2 *tau, int64_t stau, hipStream_t s);
 
 // The same reduction (uplo = 'U', one matrix, the full Hermitian matrix in
-// A on entry) by a cooperative launch of several workgroups holding the
-// matrix in their LDS, for orders above kTridiagMaxOrder.  Returns false,
-// launching nothing, when the order does not fit the device's LDS or the
-// workgroups cannot all be resident; `work` needs coop_work_elems(n,
-// kTridiagCoopMaxGroups) elements.
+// A on entry) by a cooperative launch of several workgroups, for orders above
+// kTridiagMaxOrder: the matrix's columns in their LDS where at most
+// kCoopLdsMaxGroups workgroups hold it, else (or with force_global) in A,
+// each workgroup updating its own columns in place.  Returns false, launching
+// nothing, when the workgroups cannot all be resident; `work` needs
+// coop_work_elems(n, kTridiagCoopMaxGroups) elements.
 bool hetrd_upper_coop(int n, double2 *A, int64_t lda, double *d, double *e, double2 *tau, double2 *work,
-                      size_t work_elems, hipStream_t s);
+                      size_t work_elems, hipStream_t s, bool force_global = false);
 size_t coop_work_elems(int n, int groups);
 constexpr int kCoopMinGroups = 8;
-// Largest order handed to hetrd_upper_coop (the columns of larger orders
-// need more workgroups than a cheap grid barrier allows).
+constexpr int kCoopLdsMaxGroups = 32;
+// Largest order handed to hetrd_upper_coop: above it rocSOLVER's reduction
+// is faster (dense solves of order 869 / 1067: 47.5 / 64.7 ms with the
+// columns in A and 32 workgroups, 44.9 / 56.2 ms with rocSOLVER's; their
+// columns would need more workgroups than a cheap grid barrier allows to
+// stay in LDS).
 constexpr int kTridiagCoopMaxOrder = 704;
 constexpr int kTridiagCoopMaxGroups = 1024;
 

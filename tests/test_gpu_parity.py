@@ -118,14 +118,15 @@ def test_small_eigensolver_matches_library(si2):
 
 def test_coop_tridiagonal_matches_library(si2):
     """Dense generalized solves of orders 257..704 (the si2 cell at higher
-    relative cutoffs), whose tridiagonal reduction runs as one cooperative
-    launch of several workgroups holding the matrix in LDS, against
-    rocSOLVER's zheevd throughout: the same eigenvalues, and eigenvectors
-    that satisfy H v = e S v and are S-orthonormal."""
+    relative cutoffs: 339 and 645), whose tridiagonal reduction runs as one
+    cooperative launch of several workgroups (the columns in their LDS at 339,
+    in global memory at 645; at 339 also forced into global memory),
+    against rocSOLVER's zheevd throughout: the same eigenvalues, and
+    eigenvectors that satisfy H v = e S v and are S-orthonormal."""
     name, g, _ = si2
     if name != "si2_pbe":
         pytest.skip("one dataset suffices")
-    for frac in (0.62, 0.95):
+    for frac, modes in ((0.62, (1, 2, 0)), (0.95, (1, 0))):
         ctx = H.Context(*SI2, frac * g["encut"], g["xc"], H.BACKEND_CUDA, H.SOLVER_EVD)
         try:
             h, s = ctx.assemble_hs(g["kpoint"])
@@ -133,7 +134,7 @@ def test_coop_tridiagonal_matches_library(si2):
             assert 256 < n <= 704
             nb = 24
             out = {}
-            for on in (1, 0):
+            for on in modes:
                 H.set_tuning("solver.hetrd", on)
                 try:
                     out[on] = ctx.solve(nb, g["kpoint"], vectors=True)
@@ -141,11 +142,12 @@ def test_coop_tridiagonal_matches_library(si2):
                     H.set_tuning("solver.hetrd", 1)
         finally:
             ctx.close()
-        w1, v1 = out[1][0], out[1][1]
-        print(f"[gpu] order {n}: |dw| {np.abs(w1 - out[0][0]).max():.2e}", flush=True)
-        assert np.abs(w1 - out[0][0]).max() < 1e-10
-        assert np.abs(h @ v1 - (s @ v1) * w1[None, :]).max() < 1e-9
-        assert np.abs(v1.conj().T @ s @ v1 - np.eye(nb)).max() < 1e-11
+        for mode in modes[:-1]:
+            w1, v1 = out[mode][0], out[mode][1]
+            print(f"[gpu] order {n} mode {mode}: |dw| {np.abs(w1 - out[0][0]).max():.2e}", flush=True)
+            assert np.abs(w1 - out[0][0]).max() < 1e-10
+            assert np.abs(h @ v1 - (s @ v1) * w1[None, :]).max() < 1e-9
+            assert np.abs(v1.conj().T @ s @ v1 - np.eye(nb)).max() < 1e-11
 
 
 def test_acc_abi_defaults(si2):
