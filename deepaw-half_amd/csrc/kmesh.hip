@@ -14,13 +14,20 @@
 //   - the Rayleigh-Ritz algebra of all k points is batched: Grams, the
 //     S-orthogonalisation update, the metric normalisation and the Ritz
 //     rotations on the batched MFMA contractions (the tall right-multiplies
-//     in place), Cholesky factors and their explicit inverses with
-//     rocsolver_zpotrf/ztrtri_strided_batched, the small products with
-//     rocblas_zgemm_strided_batched, eigensolves with
-//     rocsolver_zheevd_strided_batched;
-//   - the correction block's H.psi (the same launches every iteration) is
-//     captured once per precision into a hipGraph and replayed.
-// One host round trip per iteration carries the nk x m residual norms.
+//     in place); Cholesky factors with their explicit inverses, the
+//     reductions to tridiagonal form and the tridiagonal eigensolves are
+//     one-workgroup-per-k kernels (smalleig.hip, tridiag.hip) where every
+//     subspace problem has order <= kSmallMaxOrder, else
+//     rocsolver_zpotrf/ztrtri/zheevd_strided_batched with
+//     rocblas_zgemm_strided_batched;
+//   - with the one-workgroup kernels an iteration needs no host decision
+//     between its launches (the metric's eigen-decomposition form is chosen
+//     per k point on the device), so the whole iteration -- correction
+//     columns, their H.psi, orthogonalisation, normalisation, Rayleigh-Ritz,
+//     rotations and the next residual norms -- is captured once per
+//     precision and correction width into a hipGraph and replayed.
+// One host round trip per iteration carries the nk x m residual norms (and
+// the per-k status flags of the device-side factorisations).
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
@@ -31,7 +38,9 @@
 #include "cplx.hpp"
 #include "kmesh.hpp"
 #include "projector.hpp"
+#include "smalleig.hpp"
 #include "trace.hpp"
+#include "tridiag.hpp"
 
 namespace half {
 
@@ -386,16 +395,25 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
   const double t0 = now();
   const int64_t ld = N, zk = int64_t(N) * m2, rk = int64_t(N) * m;
   const int64_t gk = int64_t(m2) * m2;  // Gram stride per k point
+  // every subspace problem (orders m .. 2m, metrics up to m) fits the
+  // one-workgroup kernels: no host decision inside an iteration
+  const bool small = m2 <= kSmallMaxOrder && solver_smalleig() != 0;
+  const bool force_metric_eig = solver_smalleig() == 3;
   DeviceArray<double2> Z(size_t(nk) * zk), HZ(size_t(nk) * zk), SZ(size_t(nk) * zk), Rr(size_t(nk) * rk),
       tmp(size_t(nk) * rk), cross(size_t(nk) * m * m), metric(size_t(nk) * m * m), hsub(size_t(nk) * gk),
-      ssub(size_t(nk) * gk), gvec(size_t(nk) * gk), gwork;
+      ssub(size_t(nk) * gk), gvec(size_t(nk) * gk), tau(size_t(nk) * m2), gwork;
   DeviceArray<double> w(size_t(nk) * m2), E(size_t(nk) * m2), mu(size_t(nk) * m), res2(size_t(nk) * m),
       dscale(size_t(nk) * m), piv(size_t(nk) * (m + 1));
   DeviceArray<int> dseeds(size_t(nk) * m), dcols(size_t(nk) * m);
+  // device-side status per k point: [0, nk) Rayleigh-Ritz Cholesky, [nk, 2nk)
+  // its QL iterations, [2nk, 3nk) metric in the eigen-decomposition form,
+  // [3nk, 4nk) that form's QL iterations (written only where it ran)
+  DeviceArray<int> flags(size_t(4) * nk);
   DeviceArray<rocblas_int> info(size_t(3) * nk);
   DeviceArray<float2> p32, hp32, sp32;
   for (auto *a : {&Z, &HZ, &SZ, &Rr, &tmp})
     HALF_HIP(hipMemsetAsync(a->get(), 0, sizeof(double2) * a->size(), s));
+  HALF_HIP(hipMemsetAsync(flags.get(), 0, sizeof(int) * flags.size(), s));
   if (ctl.mixed) {
     p32.resize(size_t(nk) * rk);
     hp32.resize(size_t(nk) * rk);
@@ -430,61 +448,24 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
     }
     res.applies += int64_t(cnt) * nk;
   };
-  // the correction block's H.psi: eager once per precision and width, then
-  // captured and replayed (same launches, same pointers every iteration).
-  // One graph per precision, captured again when the width changes; every
-  // workspace was sized by the full-width apply of the initial subspace.
-  const bool use_graph = kc.graph && !trace_enabled();
-  hipGraph_t graphs[2] = {nullptr, nullptr};
-  hipGraphExec_t execs[2] = {nullptr, nullptr};
-  int graph_np[2] = {0, 0};
-  struct GraphGuard {
-    hipGraph_t *g;
-    hipGraphExec_t *e;
-    ~GraphGuard() {
-      for (int i = 0; i < 2; ++i) {
-        if (e[i]) (void)hipGraphExecDestroy(e[i]);
-        if (g[i]) (void)hipGraphDestroy(g[i]);
-      }
-    }
-  } guard{graphs, execs};
-  auto apply_p = [&](int np) {
-    const int ph = fp64_phase ? 1 : 0;
-    if (execs[ph] && graph_np[ph] != np) {  // its last replay ended at this iteration's residual sync
-      HALF_HIP(hipGraphExecDestroy(execs[ph]));
-      execs[ph] = nullptr;
-      HALF_HIP(hipGraphDestroy(graphs[ph]));
-      graphs[ph] = nullptr;
-    }
-    if (!use_graph || execs[ph]) {
-      if (execs[ph]) {
-        HALF_HIP(hipGraphLaunch(execs[ph], s));
-        res.applies += int64_t(np) * nk;
-      } else {
-        apply(np, P, HP, SP);
-      }
-      return;
-    }
-    apply(np, P, HP, SP);  // this iteration's work, eager
-    HALF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const int64_t applies0 = res.applies;
-    graph_np[ph] = np;
-    try {
-      apply(np, P, HP, SP);
-    } catch (...) {
-      hipGraph_t broken = nullptr;  // leave the stream usable
-      if (hipStreamEndCapture(s, &broken) == hipSuccess && broken) (void)hipGraphDestroy(broken);
-      throw;
-    }
-    res.applies = applies0;
-    HALF_HIP(hipStreamEndCapture(s, &graphs[ph]));
-    HALF_HIP(hipGraphInstantiate(&execs[ph], graphs[ph], nullptr, nullptr, 0));
+  // k x k generalized problems (A_j = hsub_j, B_j = ssub_j at stride gstride):
+  // B = U^H U, C = U^-H A U^-1, C v = e v, eigenvectors U^-1 V of the lowest m
+  // into gvec (k x m), all k values ascending into w (stride m2)
+  auto generalized_small = [&](int k, int64_t gstride) {
+    ContractFp32Scope fp64_products(false);  // the subspace problem stays FP64 in either phase
+    potrf_inverse_batched(k, nk, ssub.get(), k, gstride, ssub.get(), k, gstride, flags.get(), false, nullptr, 0, s);
+    gemm_tall_batched<double>(nk, k, k, k, 1.0, hsub.get(), k, gstride, ssub.get(), k, gstride, 0.0, gvec.get(), k,
+                              gstride, nullptr, 0, 0, nullptr, 0, 0, s);
+    gram_batched<double>(nk, k, k, k, ssub.get(), k, gstride, gvec.get(), k, gstride, hsub.get(), k, gstride, nullptr,
+                         0, 0, nullptr, 0, 0, gwork, s, true);
+    hetrd_upper(k, nk, hsub.get(), k, gstride, w.get(), m2, E.get(), m2, tau.get(), m2, s);
+    tridiag_eig_batched(k, nk, w.get(), m2, E.get(), m2, hsub.get(), k, gstride, tau.get(), m2, m, ssub.get(), k,
+                        gstride, 0.0, gvec.get(), k, gstride, w.get(), m2, flags.get() + nk, nullptr, s);
   };
-  // k x k generalized problems (A_j, B_j) at stride gstride: B = U^H U,
-  // U^-1 formed explicitly (batched triangular solves are several times
-  // slower than trtri plus products at these sizes), A <- U^-H A U^-1,
-  // eigenpairs, eigenvectors U^-1 V into gvec; values to w (stride m2)
-  auto generalized = [&](int k, int64_t gstride) {
+  // the same with rocSOLVER (orders above kSmallMaxOrder): U^-1 formed
+  // explicitly (batched triangular solves are several times slower than
+  // trtri plus products at these sizes); statuses checked on the host
+  auto generalized_library = [&](int k, int64_t gstride) {
     KM_BLAS(rocsolver_zpotrf_strided_batched(blas.h, rocblas_fill_upper, k, rz(ssub.get()), k, gstride, info.get(),
                                              nk));
     KM_BLAS(rocsolver_ztrtri_strided_batched(blas.h, rocblas_fill_upper, rocblas_diagonal_non_unit, k,
@@ -514,6 +495,32 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
         throw Error(kInternal, "k-mesh ACC subspace eigensolve failed at k point " + std::to_string(j));
     }
   };
+  auto generalized = [&](int k, int64_t gstride) {
+    if (small)
+      generalized_small(k, gstride);
+    else
+      generalized_library(k, gstride);
+  };
+  // the device-side statuses of the small kernels since the last check
+  // (read with the residual norms); failures end the solve
+  std::vector<int> hflags(size_t(4) * nk);
+  bool metric_ran = false;  // an iteration (with its metric factorisation) since the last check
+  auto check_flags = [&]() {
+    bool fallback = false;
+    for (int j = 0; j < nk; ++j) {
+      if (hflags[j] != 0)
+        throw Error(kInternal, "k-mesh ACC subspace solve failed at k point " + std::to_string(j) +
+                                   ": overlap not positive definite");
+      if (hflags[nk + j] != 0)
+        throw Error(kInternal, "k-mesh ACC subspace eigensolve failed at k point " + std::to_string(j));
+      if (metric_ran && hflags[2 * nk + j] != 0) {
+        fallback = true;
+        if (hflags[3 * nk + j] != 0) throw Error(kInternal, "k-mesh ACC metric eigensolve failed");
+      }
+    }
+    if (fallback && metric_ran) ++res.metric_fallbacks;
+    metric_ran = false;
+  };
   // blk(:, :c) <- blk(:, :k) W(:k, :c) for every k point: in place on the
   // matrix cores when the c output columns fit one tile, else through tmp
   auto right_multiply = [&](double2 *blk, const double2 *W, int k, int64_t ldw, int64_t sw, int c) {
@@ -536,6 +543,14 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
                          int64_t(m) * m, gwork, s, true);
     generalized(m, int64_t(m) * m);
     rotate(m, int64_t(m) * m);
+  };
+  // R = HX - SX diag(w) and its squared column norms
+  auto residual_norms = [&]() {
+    trace_mark("acc_residual", s);
+    hipLaunchKernelGGL(residual_kernel, gNm, dim3(kThreads), 0, s, HX, SX, zk, w.get(), int64_t(m2), Rr.get(), rk, N,
+                       m, ld);
+    hipLaunchKernelGGL(column_norm2_kernel, dim3(nk * m), dim3(kThreads), 0, s, Rr.get(), ld, N, res2.get());
+    HALF_HIP(hipGetLastError());
   };
 
   // initial subspace: unit vectors at each k point's m lowest kinetic energies
@@ -586,9 +601,176 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
       }
       fp32_floor = fp32_floor_estimate(eh, es, e);
     }
+    residual_norms();
     HALF_HIP(hipStreamSynchronize(s));
   }
-  std::vector<double> r2(size_t(nk) * m), pivh(size_t(nk) * (m + 1));
+
+  // one iteration after the host's decisions (correction columns cols and
+  // seeds already on the device): preconditioned corrections, their H.psi,
+  // S-orthogonalisation against X, normalisation in the S metric,
+  // Rayleigh-Ritz in span[X P] for every k point, the rotations, and the
+  // residual norms the next decision reads
+  auto iteration = [&](int np) {
+    const dim3 gNp(blocks_for(int64_t(N) * np), nk), gpp(blocks_for(int64_t(np) * np), nk), gp(blocks_for(np), nk);
+    const int64_t pstride = int64_t(np) * np;  // metric stride at width np
+    trace_mark("acc_precondition", s);
+    hipLaunchKernelGGL(precondition_kernel, gNp, dim3(kThreads), 0, s, Rr.get(), rk, op64.kin.get(),
+                       int64_t(op64.nmax), w.get(), int64_t(m2), dcols.get(), dseeds.get(), P, zk, N, m, np, ld);
+    HALF_HIP(hipGetLastError());
+    apply(np, P, HP, SP);
+    // FP32 phase: the dense algebra below with FP32 products (as solve_acc)
+    ContractFp32Scope fp32_products(!fp64_phase && solver_fp32_algebra());
+    // S-orthogonalise against X (cross: m x np per k point)
+    trace_mark("acc_orthogonalise", s);
+    gram_batched<double>(nk, N, m, np, X, ld, zk, SP, ld, zk, cross.get(), m, int64_t(m) * m, nullptr, 0, 0, nullptr,
+                         0, 0, gwork, s);
+    for (auto pr : {std::make_pair(X, P), std::make_pair(HX, HP), std::make_pair(SX, SP)})
+      gemm_tall_batched<double>(nk, N, np, m, -1.0, pr.first, ld, zk, cross.get(), m, int64_t(m) * m, 1.0,
+                                pr.second, ld, zk, nullptr, 0, 0, nullptr, 0, 0, s);
+    // normalise in the S metric: W = U^-1 of the unit-diagonal Cholesky
+    // factor, or the reference's eigen-decomposition form V mu^-1/2 where a
+    // k point's factor is ill-conditioned
+    trace_mark("acc_metric_cholesky", s);
+    gram_batched<double>(nk, N, np, np, P, ld, zk, SP, ld, zk, metric.get(), np, pstride, nullptr, 0, 0, nullptr, 0,
+                         0, gwork, s, true);
+    if (small) {
+      // the copy for the eigen-decomposition form goes to `cross` (free now)
+      potrf_inverse_batched(np, nk, metric.get(), np, pstride, metric.get(), np, pstride, flags.get() + 2 * nk, true,
+                            cross.get(), pstride, s, force_metric_eig);
+      trace_mark("acc_metric_zheevd", s);
+      hetrd_upper(np, nk, cross.get(), np, pstride, mu.get(), m, E.get(), m2, tau.get(), m2, s, flags.get() + 2 * nk);
+      tridiag_eig_batched(np, nk, mu.get(), m, E.get(), m2, cross.get(), np, pstride, tau.get(), m2, np, nullptr, 0, 0,
+                          1e-14, metric.get(), np, pstride, nullptr, 0, flags.get() + 3 * nk, flags.get() + 2 * nk, s);
+      trace_mark("acc_normalise", s);
+      for (double2 *blk : {P, HP, SP}) right_multiply(blk, metric.get(), np, np, pstride, np);
+    } else {
+      hipLaunchKernelGGL(unit_diagonal_kernel, gp, dim3(kThreads), 0, s, metric.get(), dscale.get(), np);
+      hipLaunchKernelGGL(scale_both_kernel, gpp, dim3(kThreads), 0, s, metric.get(), dscale.get(), np);
+      HALF_HIP(hipGetLastError());
+      KM_BLAS(rocsolver_zpotrf_strided_batched(blas.h, rocblas_fill_upper, np, rz(metric.get()), np, pstride,
+                                               info.get(), nk));
+      hipLaunchKernelGGL(pivots_kernel, gp, dim3(kThreads), 0, s, metric.get(), info.get(), piv.get(), np);
+      HALF_HIP(hipGetLastError());
+      std::vector<double> pivh(size_t(nk) * (np + 1));
+      HALF_HIP(hipMemcpyAsync(pivh.data(), piv.get(), sizeof(double) * pivh.size(), hipMemcpyDeviceToHost, s));
+      HALF_HIP(hipStreamSynchronize(s));
+      bool chol_ok = true;
+      for (int j = 0; j < nk && chol_ok; ++j) {
+        const double *pj = pivh.data() + size_t(j) * (np + 1);
+        chol_ok = pj[np] == 0.0;
+        for (int b = 0; b < np && chol_ok; ++b) chol_ok = std::isfinite(pj[b]) && pj[b] > 1e-10;
+      }
+      if (chol_ok) {
+        hipLaunchKernelGGL(unscale_columns_kernel, gpp, dim3(kThreads), 0, s, metric.get(), dscale.get(), np);
+        HALF_HIP(hipGetLastError());
+        trace_mark("acc_normalise", s);
+        // P <- P U^-1 with U^-1 explicit (a triangular factor whose scaled
+        // pivots all exceed 1e-10)
+        KM_BLAS(rocsolver_ztrtri_strided_batched(blas.h, rocblas_fill_upper, rocblas_diagonal_non_unit, np,
+                                                 rz(metric.get()), np, pstride, info.get(), nk));
+        hipLaunchKernelGGL(upper_only_kernel, gpp, dim3(kThreads), 0, s, metric.get(), np, pstride);
+        HALF_HIP(hipGetLastError());
+        for (double2 *blk : {P, HP, SP}) right_multiply(blk, metric.get(), np, np, pstride, np);
+      } else {
+        ++res.metric_fallbacks;
+        gram_batched<double>(nk, N, np, np, P, ld, zk, SP, ld, zk, metric.get(), np, pstride, nullptr, 0, 0, nullptr,
+                             0, 0, gwork, s, true);
+        trace_mark("acc_metric_zheevd", s);
+        KM_BLAS(rocsolver_zheevd_strided_batched(blas.h, rocblas_evect_original, rocblas_fill_upper, np,
+                                                 rz(metric.get()), np, pstride, mu.get(), np, E.get(), m2,
+                                                 info.get(), nk));
+        std::vector<rocblas_int> h(nk);
+        HALF_HIP(hipMemcpyAsync(h.data(), info.get(), sizeof(rocblas_int) * nk, hipMemcpyDeviceToHost, s));
+        HALF_HIP(hipStreamSynchronize(s));
+        for (int j = 0; j < nk; ++j)
+          if (h[j] != 0) throw Error(kInternal, "k-mesh ACC metric eigensolve failed");
+        hipLaunchKernelGGL(scale_columns_kernel, gpp, dim3(kThreads), 0, s, metric.get(), mu.get(), np, 1e-14);
+        HALF_HIP(hipGetLastError());
+        for (double2 *blk : {P, HP, SP}) right_multiply(blk, metric.get(), np, np, pstride, np);
+      }
+    }
+    // Rayleigh-Ritz in span[X P] for every k point
+    const int kz = m + np;
+    trace_mark("acc_projection_gram", s);
+    gram_batched<double>(nk, N, kz, kz, Z.get(), ld, zk, HZ.get(), ld, zk, hsub.get(), kz, gk, SZ.get(), ld, zk,
+                         ssub.get(), kz, gk, gwork, s, true);
+    trace_mark("acc_subspace_eigensolve", s);
+    generalized(kz, gk);
+    trace_mark("acc_rotate_gemm", s);
+    rotate(kz, gk);
+    residual_norms();
+    trace_close(s);
+  };
+
+  // Captured iterations, one graph per (precision phase, correction width):
+  // a width's first iteration runs eagerly (every workspace then has its
+  // size for that shape), is recorded right after and replayed from the next
+  // iteration of that shape on.  A graph also records the contraction
+  // workspace it was captured with and is captured again if that moved.
+  const bool use_graph = kc.graph && small && !trace_enabled();
+  struct Captured {
+    int phase, np;
+    const void *work;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<Captured> graphs;
+  struct GraphGuard {
+    std::vector<Captured> &g;
+    ~GraphGuard() {
+      for (auto &c : g) {
+        if (c.exec) (void)hipGraphExecDestroy(c.exec);
+        if (c.graph) (void)hipGraphDestroy(c.graph);
+      }
+    }
+  } guard{graphs};
+  auto run_iteration = [&](int np) {
+    metric_ran = true;
+    if (!use_graph) {
+      iteration(np);
+      return;
+    }
+    const int phase = fp64_phase ? 1 : 0;
+    for (auto it = graphs.begin(); it != graphs.end(); ++it) {
+      if (it->phase != phase || it->np != np) continue;
+      if (it->work == static_cast<const void *>(gwork.get())) {
+        HALF_HIP(hipGraphLaunch(it->exec, s));
+        res.applies += int64_t(np) * nk;
+        return;
+      }
+      HALF_HIP(hipGraphExecDestroy(it->exec));  // its last replay has ended at a residual sync
+      HALF_HIP(hipGraphDestroy(it->graph));
+      graphs.erase(it);
+      break;
+    }
+    iteration(np);  // this iteration's work, eager
+    HALF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int64_t applies0 = res.applies;
+    const int fallbacks0 = res.metric_fallbacks;
+    Captured c{phase, np, gwork.get(), nullptr, nullptr};
+    try {
+      iteration(np);
+    } catch (...) {
+      hipGraph_t broken = nullptr;  // leave the stream usable
+      if (hipStreamEndCapture(s, &broken) == hipSuccess && broken) (void)hipGraphDestroy(broken);
+      throw;
+    }
+    res.applies = applies0;
+    res.metric_fallbacks = fallbacks0;
+    HALF_HIP(hipStreamEndCapture(s, &c.graph));
+    if (c.work != static_cast<const void *>(gwork.get())) {
+      (void)hipGraphDestroy(c.graph);  // a workspace moved while recording: replay nothing of it
+      throw Error(kInternal, "k-mesh ACC: contraction workspace resized during graph capture");
+    }
+    const hipError_t ie = hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(c.graph);
+      HALF_HIP(ie);
+    }
+    graphs.push_back(c);
+  };
+
+  std::vector<double> r2(size_t(nk) * m);
   std::vector<int> seeds(size_t(nk) * m), cols(size_t(nk) * m), pseeds(size_t(nk) * m);
   const bool lock = ctl.locking();
   int64_t order_sum = 0;
@@ -597,14 +779,12 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
   double max_res = HUGE_VAL;
   int iter = 0, fp64_iters = 0;
   for (iter = 1; iter <= ctl.maxiter; ++iter) {
-    trace_mark("acc_residual", s);
-    hipLaunchKernelGGL(residual_kernel, gNm, dim3(kThreads), 0, s, HX, SX, zk, w.get(), int64_t(m2), Rr.get(), rk, N,
-                       m, ld);
-    hipLaunchKernelGGL(column_norm2_kernel, dim3(nk * m), dim3(kThreads), 0, s, Rr.get(), ld, N, res2.get());
-    HALF_HIP(hipGetLastError());
     trace_mark("acc_norms_to_host", s);
     HALF_HIP(hipMemcpyAsync(r2.data(), res2.get(), sizeof(double) * r2.size(), hipMemcpyDeviceToHost, s));
+    if (small)
+      HALF_HIP(hipMemcpyAsync(hflags.data(), flags.get(), sizeof(int) * hflags.size(), hipMemcpyDeviceToHost, s));
     HALF_HIP(hipStreamSynchronize(s));
+    if (small) check_flags();
     max_res = 0;
     for (int j = 0; j < nk; ++j)
       for (int b = 0; b < m; ++b) {
@@ -621,6 +801,7 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
       fp64_phase = true;
       apply(m, X, HX, SX);
       rayleigh_ritz_x();
+      residual_norms();
       --iter;
       continue;
     }
@@ -659,83 +840,9 @@ void solve_acc_kmesh(Blas &blas, KMeshOperator<double> &op64, KMeshOperator<floa
     }
     order_sum += m + np;
     ++rr_steps;
-    const dim3 gNp(blocks_for(int64_t(N) * np), nk), gpp(blocks_for(int64_t(np) * np), nk), gp(blocks_for(np), nk);
-    const int64_t pstride = int64_t(np) * np;  // metric stride at width np
-    trace_mark("acc_precondition", s);
     dseeds.upload(pseeds, s);
     dcols.upload(cols, s);
-    hipLaunchKernelGGL(precondition_kernel, gNp, dim3(kThreads), 0, s, Rr.get(), rk, op64.kin.get(),
-                       int64_t(op64.nmax), w.get(), int64_t(m2), dcols.get(), dseeds.get(), P, zk, N, m, np, ld);
-    HALF_HIP(hipGetLastError());
-    apply_p(np);
-    // FP32 phase: the dense algebra below with FP32 products (as solve_acc)
-    ContractFp32Scope fp32_products(!fp64_phase && solver_fp32_algebra());
-    // S-orthogonalise against X (cross: m x np per k point)
-    trace_mark("acc_orthogonalise", s);
-    gram_batched<double>(nk, N, m, np, X, ld, zk, SP, ld, zk, cross.get(), m, int64_t(m) * m, nullptr, 0, 0, nullptr,
-                         0, 0, gwork, s);
-    for (auto pr : {std::make_pair(X, P), std::make_pair(HX, HP), std::make_pair(SX, SP)})
-      gemm_tall_batched<double>(nk, N, np, m, -1.0, pr.first, ld, zk, cross.get(), m, int64_t(m) * m, 1.0,
-                                pr.second, ld, zk, nullptr, 0, 0, nullptr, 0, 0, s);
-    // normalise in the S metric (unit-diagonal Cholesky; eigen-decomposition
-    // form when any k point's factor is ill-conditioned)
-    trace_mark("acc_metric_cholesky", s);
-    gram_batched<double>(nk, N, np, np, P, ld, zk, SP, ld, zk, metric.get(), np, pstride, nullptr, 0, 0, nullptr, 0,
-                         0, gwork, s, true);
-    hipLaunchKernelGGL(unit_diagonal_kernel, gp, dim3(kThreads), 0, s, metric.get(), dscale.get(), np);
-    hipLaunchKernelGGL(scale_both_kernel, gpp, dim3(kThreads), 0, s, metric.get(), dscale.get(), np);
-    HALF_HIP(hipGetLastError());
-    KM_BLAS(rocsolver_zpotrf_strided_batched(blas.h, rocblas_fill_upper, np, rz(metric.get()), np, pstride,
-                                             info.get(), nk));
-    hipLaunchKernelGGL(pivots_kernel, gp, dim3(kThreads), 0, s, metric.get(), info.get(), piv.get(), np);
-    HALF_HIP(hipGetLastError());
-    HALF_HIP(hipMemcpyAsync(pivh.data(), piv.get(), sizeof(double) * size_t(nk) * (np + 1), hipMemcpyDeviceToHost,
-                            s));
-    HALF_HIP(hipStreamSynchronize(s));
-    bool chol_ok = true;
-    for (int j = 0; j < nk && chol_ok; ++j) {
-      const double *pj = pivh.data() + size_t(j) * (np + 1);
-      chol_ok = pj[np] == 0.0;
-      for (int b = 0; b < np && chol_ok; ++b) chol_ok = std::isfinite(pj[b]) && pj[b] > 1e-10;
-    }
-    if (chol_ok) {
-      hipLaunchKernelGGL(unscale_columns_kernel, gpp, dim3(kThreads), 0, s, metric.get(), dscale.get(), np);
-      HALF_HIP(hipGetLastError());
-      trace_mark("acc_normalise", s);
-      // P <- P U^-1 with U^-1 explicit (a triangular factor whose scaled
-      // pivots all exceed 1e-10)
-      KM_BLAS(rocsolver_ztrtri_strided_batched(blas.h, rocblas_fill_upper, rocblas_diagonal_non_unit, np,
-                                               rz(metric.get()), np, pstride, info.get(), nk));
-      hipLaunchKernelGGL(upper_only_kernel, gpp, dim3(kThreads), 0, s, metric.get(), np, pstride);
-      HALF_HIP(hipGetLastError());
-      for (double2 *blk : {P, HP, SP}) right_multiply(blk, metric.get(), np, np, pstride, np);
-    } else {
-      ++res.metric_fallbacks;
-      gram_batched<double>(nk, N, np, np, P, ld, zk, SP, ld, zk, metric.get(), np, pstride, nullptr, 0, 0, nullptr,
-                           0, 0, gwork, s, true);
-      trace_mark("acc_metric_zheevd", s);
-      KM_BLAS(rocsolver_zheevd_strided_batched(blas.h, rocblas_evect_original, rocblas_fill_upper, np,
-                                               rz(metric.get()), np, pstride, mu.get(), np, E.get(), m2,
-                                               info.get(), nk));
-      std::vector<rocblas_int> h(nk);
-      HALF_HIP(hipMemcpyAsync(h.data(), info.get(), sizeof(rocblas_int) * nk, hipMemcpyDeviceToHost, s));
-      HALF_HIP(hipStreamSynchronize(s));
-      for (int j = 0; j < nk; ++j)
-        if (h[j] != 0) throw Error(kInternal, "k-mesh ACC metric eigensolve failed");
-      hipLaunchKernelGGL(scale_columns_kernel, gpp, dim3(kThreads), 0, s, metric.get(), mu.get(), np, 1e-14);
-      HALF_HIP(hipGetLastError());
-      for (double2 *blk : {P, HP, SP}) right_multiply(blk, metric.get(), np, np, pstride, np);
-    }
-    // Rayleigh-Ritz in span[X P] for every k point
-    const int kz = m + np;
-    trace_mark("acc_projection_gram", s);
-    gram_batched<double>(nk, N, kz, kz, Z.get(), ld, zk, HZ.get(), ld, zk, hsub.get(), kz, gk, SZ.get(), ld, zk,
-                         ssub.get(), kz, gk, gwork, s, true);
-    trace_mark("acc_subspace_eigensolve", s);
-    generalized(kz, gk);
-    trace_mark("acc_rotate_gemm", s);
-    rotate(kz, gk);
-    trace_close(s);
+    run_iteration(np);
   }
   res.iterations = std::min(iter, ctl.maxiter);
   res.fp64_iterations = fp64_iters;

@@ -22,6 +22,7 @@
 
 #include "contract.hpp"
 #include "cplx.hpp"
+#include "smalleig.hpp"
 #include "solver.hpp"
 #include "trace.hpp"
 #include "tridiag.hpp"
@@ -306,6 +307,12 @@ int g_refresh = 0;
 // Mixed form: the FP32 phase's dense algebra with FP32 products (1, default)
 // or FP64 (0), key "solver.fp32algebra".
 int g_fp32_algebra = 1;
+// Subspace problems of order <= kSmallMaxOrder on the one-workgroup kernels
+// of smalleig.hip (key "solver.smalleig"): 0 rocSOLVER everywhere; 1 (default)
+// the k-point mesh solve (which then captures whole iterations); 2 also the
+// single-k Rayleigh-Ritz steps and dense generalized solves; 3 as 1 with the
+// mesh's correction-block metric always in the eigen-decomposition form.
+int g_smalleig = 1;
 
 // zheevd with vectors (the full Hermitian matrix in A; eigenvectors replace
 // A, eigenvalues ascending in w; e is workspace of n values).  Orders up to
@@ -338,6 +345,35 @@ void heevd_vectors(Blas &blas, int k, double2 *A, double *w, double *e, rocblas_
   HALF_HIP(hipMemcpyAsync(A, Z, sizeof(double2) * size_t(k) * k, hipMemcpyDeviceToDevice, blas.stream));
 }
 
+// The generalized problem below on the one-workgroup kernels (order <=
+// kSmallMaxOrder): U^-1 from potrf_inverse_batched, C = U^-H (A U^-1) by two
+// products, hetrd_upper, then the tridiagonal eigensolve with the
+// back-transformation by the reflectors and U^-1 of the lowest nv vectors
+// (written to A(:, :nv)); eigenvalues ascending in w.  One host round trip
+// checks both statuses.
+void generalized_eigh_small(Blas &blas, int k, int nv, double2 *A, double2 *B, double *w, double *e, const char *what) {
+  hipStream_t s = blas.stream;
+  blas.eig_tau.ensure(size_t(k));
+  blas.eig_z.ensure(size_t(k) * k);
+  blas.eig_info.ensure(2);
+  double2 *T = blas.eig_z.get(), *tau = blas.eig_tau.get();
+  int *info = blas.eig_info.get();
+  potrf_inverse_batched(k, 1, B, k, 0, B, k, 0, info, false, nullptr, 0, s);
+  const double2 one = make_double2(1, 0), zero = make_double2(0, 0);
+  zgemm(blas, rocblas_operation_none, rocblas_operation_none, k, k, k, one, A, k, B, k, zero, T, k);  // A U^-1
+  zgemm(blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, k, k, k, one, B, k, T, k, zero, A, k);
+  hetrd_upper(k, 1, A, k, 0, w, 0, e, 0, tau, 0, s);
+  tridiag_eig_batched(k, 1, w, 0, e, 0, A, k, 0, tau, 0, nv, B, k, 0, 0.0, T, k, 0, w, 0, info + 1, nullptr, s);
+  HALF_HIP(hipMemcpyAsync(A, T, sizeof(double2) * size_t(k) * nv, hipMemcpyDeviceToDevice, s));
+  int h[2] = {0, 0};
+  HALF_HIP(hipMemcpyAsync(h, info, sizeof(h), hipMemcpyDeviceToHost, s));
+  HALF_HIP(hipStreamSynchronize(s));
+  if (h[0] != 0)
+    throw Error(kInternal, std::string(what) + " failed: overlap not positive definite (info " +
+                               std::to_string(h[0]) + ")");
+  if (h[1] != 0) throw Error(kInternal, std::string(what) + " failed (QL iterations)");
+}
+
 // Type-1 generalized Hermitian eigenproblem A y = e B y of order k, upper
 // triangles referenced (LAPACK zhegvd semantics): B = U^H U, C = U^-H A U^-1,
 // C v = e v, y = U^-1 v.  Eigenvectors replace A, eigenvalues go to w
@@ -346,6 +382,10 @@ void heevd_vectors(Blas &blas, int k, double2 *A, double *w, double *e, rocblas_
 // Without vectors A is left holding the reduced matrix's workspace.
 void generalized_eigh(Blas &blas, int k, double2 *A, double2 *B, double *w, double *e, rocblas_int *info,
                       const char *what, bool want_vectors = true) {
+  if (g_smalleig == 2 && want_vectors && k <= kSmallMaxOrder) {
+    generalized_eigh_small(blas, k, k, A, B, w, e, what);
+    return;
+  }
   const rocblas_double_complex one{1.0, 0.0};
   HALF_BLAS(rocsolver_zpotrf(blas.h, rocblas_fill_upper, k, rz(B), k, info));
   HALF_BLAS(rocblas_ztrsm(blas.h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_conjugate_transpose,
@@ -376,6 +416,10 @@ void generalized_eigh(Blas &blas, int k, double2 *A, double2 *B, double *w, doub
 // Larger orders take generalized_eigh.
 void generalized_eigh_ritz(Blas &blas, int k, int nv, double2 *A, double2 *B, double *w, double *e,
                            rocblas_int *info, const char *what) {
+  if (g_smalleig == 2 && k <= kSmallMaxOrder) {
+    generalized_eigh_small(blas, k, nv, A, B, w, e, what);
+    return;
+  }
   if (!g_hetrd || k > kTridiagMaxOrder) {
     generalized_eigh(blas, k, A, B, w, e, info, what);
     return;
@@ -448,6 +492,11 @@ bool set_solver_tuning(const std::string &key, int value) {
     g_fp32_algebra = value;
     return true;
   }
+  if (key == "solver.smalleig") {
+    if (value < 0 || value > 3) return false;
+    g_smalleig = value;
+    return true;
+  }
   if (key != "solver.hetrd" || value < 0 || value > 2) return false;
   g_hetrd = value != 0;
   g_hetrd_mode = value;
@@ -456,6 +505,7 @@ bool set_solver_tuning(const std::string &key, int value) {
 
 bool solver_locking() { return g_lock != 0; }
 bool solver_fp32_algebra() { return g_fp32_algebra != 0; }
+int solver_smalleig() { return g_smalleig; }
 
 int AccControls::lock_mode() const { return lock < 0 ? g_lock : lock; }
 int AccControls::precond_mode() const { return precond < 0 ? g_precond : precond; }
@@ -472,6 +522,8 @@ bool get_solver_tuning(const std::string &key, int &value) {
     value = g_refresh;
   } else if (key == "solver.fp32algebra") {
     value = g_fp32_algebra;
+  } else if (key == "solver.smalleig") {
+    value = g_smalleig;
   } else if (key == "solver.hetrd") {
     value = g_hetrd_mode;
   } else {

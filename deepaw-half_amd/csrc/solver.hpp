@@ -18,6 +18,7 @@ struct Blas {
   rocblas_handle h = nullptr;
   hipStream_t stream = nullptr;
   DeviceArray<double2> eig_z, eig_tau, eig_coop;  // workspace of the small eigensolver
+  DeviceArray<int> eig_info;                      // statuses of the one-workgroup eigensolver
   void init(hipStream_t s);
   void release();
   ~Blas();
@@ -109,6 +110,10 @@ bool set_solver_tuning(const std::string &key, int value);
 // key solver.fp32algebra: FP32 products in the dense algebra of a mixed
 // solve's FP32 phase
 bool solver_fp32_algebra();
+// Key "solver.smalleig" (0 rocSOLVER for every subspace problem, 1 the k-point
+// mesh on the one-workgroup kernels, 2 also single-k problems of order <= 96,
+// 3 as 1 with the mesh's metric always in the eigen-decomposition form).
+int solver_smalleig();
 bool get_solver_tuning(const std::string &key, int &value);
 // Whether the all-band solves drop the correction columns of converged bands
 // (tuning key "solver.lock").

@@ -56,7 +56,7 @@ __device__ inline double wave_total4(const double *red) { return (red[0] + red[1
 template <int NMAX>
 __global__ __launch_bounds__(kTrdThreads) void hetrd_upper_kernel(int n, double2 *A, int64_t lda, int64_t sA,
                                                                    double *D, int64_t sD, double *E, int64_t sE,
-                                                                   double2 *Tau, int64_t sT) {
+                                                                   double2 *Tau, int64_t sT, const int *only) {
   constexpr int RC = NMAX / 64;  // row chunks per lane
   static_assert(NMAX <= kTrdThreads && NMAX % 64 == 0, "one thread per row in the vector phases");
   __shared__ double2 vb[2][NMAX], wb[2][NMAX];  // v, w of this step and of the previous one
@@ -65,6 +65,7 @@ __global__ __launch_bounds__(kTrdThreads) void hetrd_upper_kernel(int n, double2
   __shared__ double red[kTrdWaves];             // per-wave partial v^H A v
   __shared__ double2 s_alpha, s_tau;
 
+  if (only && only[blockIdx.x] == 0) return;  // the whole workgroup, before any barrier
   A += blockIdx.x * sA;
   D += blockIdx.x * sD;
   E += blockIdx.x * sE;
@@ -417,19 +418,19 @@ bool hetrd_upper_coop(int n, double2 *A, int64_t lda, double *d, double *e, doub
 size_t coop_work_elems(int n, int groups) { return 4 * size_t(n) + size_t(groups); }
 
 void hetrd_upper(int n, int batch, double2 *A, int64_t lda, int64_t sA, double *d, int64_t sd, double *e,
-                 int64_t se, double2 *tau, int64_t stau, hipStream_t s) {
+                 int64_t se, double2 *tau, int64_t stau, hipStream_t s, const int *only) {
   if (n < 1 || n > kTridiagMaxOrder || lda < n || batch < 1)
     throw Error(kInvalidArgument, "hetrd_upper: order " + std::to_string(n) + " outside 1.." +
                                       std::to_string(kTridiagMaxOrder) + " or bad leading dimension / batch");
   const dim3 grid(batch), block(kTrdThreads);
   if (n <= 64)
-    hipLaunchKernelGGL(hetrd_upper_kernel<64>, grid, block, 0, s, n, A, lda, sA, d, sd, e, se, tau, stau);
+    hipLaunchKernelGGL(hetrd_upper_kernel<64>, grid, block, 0, s, n, A, lda, sA, d, sd, e, se, tau, stau, only);
   else if (n <= 128)
-    hipLaunchKernelGGL(hetrd_upper_kernel<128>, grid, block, 0, s, n, A, lda, sA, d, sd, e, se, tau, stau);
+    hipLaunchKernelGGL(hetrd_upper_kernel<128>, grid, block, 0, s, n, A, lda, sA, d, sd, e, se, tau, stau, only);
   else if (n <= 192)
-    hipLaunchKernelGGL(hetrd_upper_kernel<192>, grid, block, 0, s, n, A, lda, sA, d, sd, e, se, tau, stau);
+    hipLaunchKernelGGL(hetrd_upper_kernel<192>, grid, block, 0, s, n, A, lda, sA, d, sd, e, se, tau, stau, only);
   else
-    hipLaunchKernelGGL(hetrd_upper_kernel<256>, grid, block, 0, s, n, A, lda, sA, d, sd, e, se, tau, stau);
+    hipLaunchKernelGGL(hetrd_upper_kernel<256>, grid, block, 0, s, n, A, lda, sA, d, sd, e, se, tau, stau, only);
   HALF_HIP(hipGetLastError());
 }
 

@@ -18,8 +18,10 @@ constexpr int kTridiagMaxOrder = 256;
 // Q = H(n-2) ... H(0), H(i) = I - tau_i v v^H, v(i) = 1, v(0:i-1) stored in
 // A(0:i-1, i+1), v(i+1:) = 0; the rest of A is overwritten.  The output
 // feeds rocSOLVER's zstedc / dsterf (d, e) and zunmtr (A, tau) unchanged.
+// With `only` given, matrix b is skipped (nothing read or written) where
+// only[b] == 0.
 void hetrd_upper(int n, int batch, double2 *A, int64_t lda, int64_t sA, double *d, int64_t sd, double *e,
-                 int64_t se, double2 *tau, int64_t stau, hipStream_t s);
+                 int64_t se, double2 *tau, int64_t stau, hipStream_t s, const int *only = nullptr);
 
 // The same reduction (uplo = 'U', one matrix, the full Hermitian matrix in
 // A on entry) by a cooperative launch of several workgroups, for orders above

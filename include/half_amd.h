@@ -104,8 +104,12 @@ int half_amd_solve_history(half_handle handle, int capacity, double *residuals_e
 
 /* All-band solves at nk k points (fractional, nk x 3) of one context in
  * lockstep: the H.psi of every k point is one launch per stage, the
- * Rayleigh-Ritz algebra strided-batched over k, and the correction block's
- * H.psi replayed from a captured hipGraph when graph != 0.  Controls and
+ * Rayleigh-Ritz algebra batched over k, and, when graph != 0 and every
+ * subspace problem has order <= 96 ("solver.smalleig"), each iteration
+ * after the host's choice of correction columns (their H.psi,
+ * orthogonalisation, normalisation, Rayleigh-Ritz, rotations, residual
+ * norms) replayed from a hipGraph captured once per precision and
+ * correction width; one host round trip per iteration.  Controls and
  * FP32-phase exit rules as in half_amd_solve_acc; "solver.lock" 1 and 2
  * both give soft locking here, with one correction width for all k points
  * (the largest active count rounded up to a multiple of 8); the solve stops
@@ -242,7 +246,13 @@ int half_amd_trace_read(int max_kernels, char *names, double *total_ms, int64_t 
  * the skinny 16x16-tile kernels, 0: on the 64-wide contractions with a
  * separate D/Q mix); "solver.hetrd" (1: Hermitian eigensolves with vectors
  * of order <= 256 reduce to tridiagonal form in one workgroup before
- * rocSOLVER's zstedc / zunmtr, 0: rocSOLVER's zheevd throughout).
+ * rocSOLVER's zstedc / zunmtr, 0: rocSOLVER's zheevd throughout);
+ * "solver.smalleig" (1, the default: the k-point mesh solve's subspace
+ * problems of order <= 96 on one-workgroup-per-k kernels with whole
+ * iterations captured in hipGraphs; 0: rocSOLVER's batched routines with
+ * eager launches; 2: also single-k Rayleigh-Ritz problems and dense
+ * generalized solves of order <= 96; 3: as 1 with the correction-block
+ * metric always in the eigen-decomposition form).
  * Process-wide. */
 int half_amd_set_tuning(const char *key, int value, char *error, int error_capacity);
 int half_amd_get_tuning(const char *key, int *value, char *error, int error_capacity);

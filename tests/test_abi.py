@@ -142,14 +142,16 @@ def test_solver_tuning_keys_host_only():
         assert H.get_tuning("solver.stall") == 7
         H.set_tuning("solver.lock", 1)
         assert H.get_tuning("solver.lock") == 1
-        for key, good in (("solver.precond", 1), ("solver.refresh", 8), ("solver.fp32algebra", 0)):
+        assert H.get_tuning("solver.smalleig") == 1
+        for key, good in (("solver.precond", 1), ("solver.refresh", 8), ("solver.fp32algebra", 0),
+                          ("solver.smalleig", 2), ("solver.smalleig", 0), ("solver.smalleig", 3)):
             before = H.get_tuning(key)
             H.set_tuning(key, good)
             assert H.get_tuning(key) == good
             H.set_tuning(key, before)
         for key, bad in (("solver.lock", 3), ("solver.lock", -1), ("solver.stall", -1), ("solver.stall", 1001),
                          ("solver.precond", 2), ("solver.refresh", -1), ("solver.fp32algebra", 2),
-                         ("solver.nope", 0)):
+                         ("solver.smalleig", 4), ("solver.smalleig", -1), ("solver.nope", 0)):
             with pytest.raises(Exception, match="unknown tuning key or value out of range"):
                 H.set_tuning(key, bad)
         assert H.get_tuning("solver.stall") == 7

@@ -1,7 +1,7 @@
 """k-point mesh batching: all-band solves of every k point of a mesh in
-lockstep (one launch per operator stage for all k, strided-batched
-Rayleigh-Ritz, hipGraph-replayed correction block) against per-k dense
-solves and the reference's golden eigenvalues.
+lockstep (one launch per operator stage for all k, batched Rayleigh-Ritz
+on one-workgroup-per-k kernels, whole iterations replayed from hipGraphs)
+against per-k dense solves and the reference's golden eigenvalues.
 
 Tolerances: FP64 <= 1e-9 eV per band, mixed <= 1e-8 eV per band (the
 per-k solver's bars); graph replay against eager launches bit-identical
@@ -107,6 +107,35 @@ def test_kmesh_soft_locking(si2ctx, mixed):
         assert st["max_residual_eV"] <= tol
         assert np.abs(w - ref).max() < bar
     assert sl["iterations"] == se["iterations"] and np.array_equal(wl, we)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_kmesh_small_kernels_match_library(si2ctx, mixed):
+    """The mesh's subspace problems on the one-workgroup kernels with whole
+    iterations replayed from hipGraphs (solver.smalleig 1, default) against
+    rocSOLVER's batched factorisations and eigensolves with eager launches
+    (0), and with the correction-block metric in the eigen-decomposition form
+    at every k point and iteration (3): the same eigenvalues at the per-k
+    dense solves' bar."""
+    nb = 16
+    ref = _dense(si2ctx, nb, SI2_MESH)
+    tol, bar = (1e-6, 1e-8) if mixed else (1e-7, 1e-9)
+    saved = H.get_tuning("solver.smalleig")
+    out = {}
+    try:
+        for mode in (0, 1, 3):
+            H.set_tuning("solver.smalleig", mode)
+            print(f"[gpu] si2 kmesh mixed={mixed} solver.smalleig={mode}", flush=True)
+            out[mode] = si2ctx.solve_kmesh(nb, SI2_MESH, mixed=mixed, tol=tol, maxiter=200, extra=4)
+    finally:
+        H.set_tuning("solver.smalleig", saved)
+    for mode, (w, st) in out.items():
+        print(f"[gpu]   mode {mode}: {st} |dw| {np.abs(w - ref).max():.2e}", flush=True)
+        assert st["max_residual_eV"] <= tol
+        assert np.abs(w - ref).max() < bar
+    assert out[0][1]["metric_fallbacks"] == 0 and out[1][1]["metric_fallbacks"] == 0
+    # every iteration's metric took the eigen-decomposition form
+    assert out[3][1]["metric_fallbacks"] >= out[3][1]["iterations"] - 2
 
 
 def test_kmesh_overlap_matches_in_order(si2ctx):

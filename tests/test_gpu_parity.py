@@ -8,6 +8,7 @@ and projector stages <= 2e-6 relative to FP64; the mixed-precision all-band
 solve <= 1e-8 eV per band after its FP64 correction sweeps."""
 import numpy as np
 import pytest
+import scipy.linalg
 from conftest import golden, psi_block, require_same_inputs, workload_files
 
 import half_amd as H
@@ -81,6 +82,44 @@ def test_dense_eigensolvers(si2):
         else:
             assert abs(smin - g["overlap_min"]) < 1e-12 and abs(smax - g["overlap_max"]) < 1e-12
     ctx.set_solver(H.SOLVER_EVD)
+
+
+def test_smalleig_dense_matches_library(si2):
+    """Dense generalized solves of order <= 96 (the si2 cell at reduced
+    cutoffs; Gamma has degenerate levels) on the one-workgroup kernels
+    (Cholesky and its inverse in LDS, reduction to tridiagonal form, implicit
+    QL, back-transformation: solver.smalleig 2) against rocSOLVER's
+    zpotrf / ztrsm / zheevd path (0): the same eigenvalues, eigenvectors that
+    satisfy H v = e S v and are S-orthonormal."""
+    name, g, _ = si2
+    if name != "si2_pbe":
+        pytest.skip("one dataset suffices")
+    saved = H.get_tuning("solver.smalleig")
+    for frac in (0.15, 0.22):
+        ctx = H.Context(*SI2, frac * g["encut"], g["xc"], H.BACKEND_CUDA, H.SOLVER_EVD)
+        try:
+            h, s = ctx.assemble_hs(g["kpoint"])
+            n = h.shape[0]
+            assert 8 <= n <= 96, n
+            nb = n
+            out = {}
+            for mode in (2, 0):
+                H.set_tuning("solver.smalleig", mode)
+                try:
+                    print(f"[gpu] dense order {n} solver.smalleig={mode}", flush=True)
+                    out[mode] = ctx.solve(nb, g["kpoint"], vectors=True)
+                finally:
+                    H.set_tuning("solver.smalleig", saved)
+        finally:
+            ctx.close()
+        w2, v2 = out[2][0], out[2][1]
+        ref = scipy.linalg.eigh(h, s, eigvals_only=True)
+        print(f"[gpu] order {n}: |dw| library {np.abs(w2 - out[0][0]).max():.2e} "
+              f"scipy {np.abs(w2 - ref).max():.2e}", flush=True)
+        assert np.abs(w2 - out[0][0]).max() < 1e-10
+        assert np.abs(w2 - ref).max() < 1e-9
+        assert np.abs(h @ v2 - (s @ v2) * w2[None, :]).max() < 1e-9
+        assert np.abs(v2.conj().T @ s @ v2 - np.eye(nb)).max() < 1e-11
 
 
 def test_small_eigensolver_matches_library(si2):
@@ -442,6 +481,25 @@ def test_hfo2_acc_fp64_and_mixed(hfo2):
     print("[gpu] hfo2 fp64 acc, library eigensolver", st0, flush=True)
     assert np.abs(w0 - ref).max() < 1e-9
     assert np.abs(w0 - w64).max() < 1e-10
+
+
+def test_hfo2_acc_smalleig(hfo2):
+    """Single-k all-band solves with the Rayleigh-Ritz problems of order
+    <= 96 on the one-workgroup kernels (solver.smalleig 2; locking keeps most
+    of them that small): the golden eigenvalues, FP64 and mixed."""
+    g, ctx = hfo2
+    ref = np.array(g["eigenvalues"][:64])
+    saved = H.get_tuning("solver.smalleig")
+    try:
+        for mode in (saved, 2):
+            H.set_tuning("solver.smalleig", mode)
+            for mixed in (False, True):
+                w, _, st = ctx.solve_acc(64, mixed=mixed, tol=1e-6, maxiter=300, extra=8)
+                print(f"[gpu] hfo2 acc solver.smalleig={mode} mixed={mixed}", st, np.abs(w - ref).max(), flush=True)
+                assert st["max_residual_eV"] <= 1e-6
+                assert np.abs(w - ref).max() < (1e-8 if mixed else 1e-9)
+    finally:
+        H.set_tuning("solver.smalleig", saved)
 
 
 def test_hfo2_mixed_96_bands(hfo2):
