@@ -115,7 +115,7 @@ struct KOperator {
   DeviceArray<cplx_t<R>> BD, BQ;  // B D and B Q (per-atom blocks), leading dimension ldb; formed on first use
   LocalMap lmap;  // plane-wave sphere on the box (pruned transforms)
   // workspace
-  int box_batch = 0;  // bands per FFT launch; 0 = all bands within a 2 GiB workspace
+  int box_batch = 0;  // bands per FFT launch; 0 = lines and planes within 256 MiB (>= 16), boxes within 2 GiB
   DeviceArray<cplx_t<R>> boxes;
   DeviceArray<cplx_t<R>> coeff, dcoeff, qcoeff, partial;
   CoeffWork<R> cwork;

@@ -77,7 +77,7 @@ struct StallWatch {
 struct AccControls {
   double tol = 1e-4;      // max residual norm, eV (reference default)
   int maxiter = 40;
-  int block = 0;          // bands per FFT launch (0: all within 2 GiB; the reference used 16)
+  int block = 0;          // bands per FFT launch (0: the operator's cache rule; the reference used 16)
   bool mixed = false;     // FP32 H.psi iterations before FP64 correction sweeps
   double switch_tol = 2e-3;  // mixed form: residual at which FP64 sweeps start
   int extra = 0;          // buffer bands carried internally, not reported

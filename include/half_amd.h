@@ -73,7 +73,8 @@ typedef struct half_amd_solve_stats {
 
 /* All-band solve with explicit controls (the reference ABI fixes tol 1e-4 eV,
  * 40 iterations, 16-band blocks: src/half_cuda_iterative_solver.cuf:113-119).
- * block: bands per FFT launch, 0 = all bands within a 2 GiB workspace.
+ * block: bands per FFT launch, 0 = as many as keep the pass-to-pass lines and
+ * planes within 256 MiB (at least 16) and the boxes within 2 GiB.
  * mixed != 0 runs FP32 H.psi iterations, then FP64 H.psi sweeps to tol.  The
  * FP32 phase ends at the first of: max residual <= max(switch_tol (0: 2e-3),
  * tol, the FP32 operator's error measured on a probe block at the start);
