@@ -12,10 +12,12 @@ kinetic, projector contraction B^H psi on the matrix cores, D/Q mixing,
 expansion B(Dc), B(Qc).  value = band applications per second over the timed
 steps (inputs resident in HBM, no host traffic in the timed region).
 
-Also reported: the FP32 operator's rate, the all-band solve times of the
-plain FP64 and the mixed form with their eigenvalue error against a tightly
+Also reported: the FP32 operator's rate; the all-band solve times of the FP64
+and the mixed form under the same solver settings (locking, preconditioner)
+and of the plain FP64 form (the reference's iteration: full 2m subspace,
+reference preconditioner), each with its eigenvalue error against a tightly
 converged FP64 solve (or the reference's golden eigenvalues where they cover
-the bands), the dominant kernel's roofline and every operator kernel's, and
+the bands); the dominant kernel's roofline and every operator kernel's; and
 the oracle's CPU rate on a bounded sample of the same workload with all host
 threads.
 """
@@ -52,6 +54,8 @@ def main():
     ap.add_argument("--bands", type=int, default=0)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain FP64 solve (the reference's iteration) beside the chosen form")
     ap.add_argument("--extra", type=int, default=-1, help="buffer bands of the all-band solves (-1: bands / 8, "
                                                           "at least 8)")
     ap.add_argument("--tol", type=float, default=1e-5, help="residual tolerance (eV) of the all-band solves")
@@ -207,6 +211,31 @@ def main():
         solve["controls"] = {"tol_eV": args.tol, "extra_bands": extra, "maxiter": 400,
                              "switch_tol_eV": args.switch_tol or "library default",
                              "lock": H.get_tuning("solver.lock"), "precond": H.get_tuning("solver.precond")}
+        if not args.no_plain and "mixed" in solve:
+            # the plain FP64 form: the reference's iteration (every band's
+            # correction column, the full 2m-wide Rayleigh-Ritz subspace, the
+            # reference preconditioner) with the FP64 operator, same tolerance
+            saved = {key: H.get_tuning(key) for key in ("solver.lock", "solver.precond")}
+            H.set_tuning("solver.lock", 0)
+            H.set_tuning("solver.precond", 0)
+            try:
+                t = time.perf_counter()
+                wp, _, st = ctx.solve_acc(nb, k, tol=args.tol, maxiter=800, extra=extra)
+                el = time.perf_counter() - t
+            finally:
+                for key, v in saved.items():
+                    H.set_tuning(key, v)
+            err = np.abs(wp - ref)
+            solve["plain_fp64"] = {"kind": "FP64 operator, the reference's iteration: full 2m subspace (solver.lock "
+                                           "0), reference preconditioner (solver.precond 0)",
+                                   "seconds": el, "iterations": st["iterations"],
+                                   "max_residual_eV": st["max_residual_eV"], "rr_order_mean": st["rr_order_mean"],
+                                   "max_abs_err_eV": float(err.max())}
+            # the chosen form (mixed precision with the locking and
+            # preconditioner of `controls`) against it; the like-for-like
+            # ratio is speedup_mixed_vs_fp64
+            solve["speedup_chosen_vs_plain_fp64"] = el / solve["mixed"]["seconds"]
+            print(f"bench: plain FP64 all-band solve {solve['plain_fp64']}", file=sys.stderr, flush=True)
 
     if not args.no_solve and n <= 6000:
         # dense FP64 solve (assembly, overlap spectrum, generalized eigensolve),
