@@ -284,6 +284,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(chg, pot, w, k, args.cpu_seconds)
+        if not args.no_solve:
+            cpu["acc_iteration"] = cpu_acc_iteration(chg, pot, w, k, nb + extra)
+            plain = solve.get("plain_fp64")
+            if plain:
+                cpu["acc_iteration"]["gpu_plain_fp64_s"] = plain["seconds"] / max(1, plain["iterations"])
 
     out = {
         "metric": "H.psi band-applications/s + all-band solve time at 1e-8 eV, mixed vs FP64, 1 GPU",
@@ -512,6 +517,40 @@ def cpu_baseline(chg, pot, w, k, seconds):
             "cpu_model": cpu_model(), "blas": blas, "fft": f"scipy.fft, {threads} workers",
             "sample": f"{done} band applications of the oracle's matrix-free H.psi (FP64, numpy/scipy) in "
                       f"{el:.1f} s, same workload, {nb}-band blocks, {threads} host threads"}
+
+
+def cpu_acc_iteration(chg, pot, w, k, m):
+    """One iteration of the reference's all-band iteration (the oracle's
+    restatement: full 2m subspace, reference preconditioner, FP64) on the
+    host cores, at the solve's band count m: the time from the start of the
+    first iteration's H.psi to that of the second (corrections' H.psi in
+    64-band blocks, orthogonalisation, metric eigen-decomposition, the 2m x 2m
+    generalized eigensolve, rotations), all host threads for FFTs and BLAS.
+    The GPU figure beside it is the plain FP64 form's time per iteration."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import half_oracle as O
+    from threadpoolctl import threadpool_limits
+    threads = host_threads()
+    ctx = O.Context(chg, pot, w.encut, w.xc)
+    basis, paw = ctx.prepare(k)
+    starts = []
+
+    def apply(x):
+        starts.append(time.perf_counter())
+        hx, sx = np.empty_like(x), np.empty_like(x)
+        for j in range(0, x.shape[1], 64):
+            hx[:, j:j + 64], sx[:, j:j + 64] = O.apply_hs(ctx.veff, basis, paw, x[:, j:j + 64], workers=threads)
+        return hx, sx
+
+    with threadpool_limits(threads):
+        t0 = time.perf_counter()
+        O.acc_solve(apply, basis.kinetic, m, tol=1e-12, maxiter=2)
+        total = time.perf_counter() - t0
+    it = starts[2] - starts[1] if len(starts) >= 3 else None
+    return {"seconds": it, "cores": threads, "kind": "port",
+            "sample": f"one iteration of the oracle's restatement of the reference's all-band iteration, {m} bands "
+                      f"(FP64, numpy/scipy, {threads} host threads; the bounded run took {total:.1f} s)"}
 
 
 if __name__ == "__main__":
