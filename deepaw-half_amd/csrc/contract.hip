@@ -646,7 +646,9 @@ void gemm_tall_batched(int batch, int n, int m, int k, R alpha, const cplx_t<R> 
   if (grid.y > 65535) throw Error(kInvalidArgument, "gemm_tall: too many columns");
   const int nbl = split ? 1 : nb;  // outputs per workgroup
   if constexpr (std::is_same<R, double>::value) {
-    const int form = product_form<double>(g_tall_gauss && !inplace && nbl == 1);
+    // (in place too: a workgroup owns whole rows and has read all of them
+    // before its epilogue stores, whichever product form it runs)
+    const int form = product_form<double>(g_tall_gauss && nbl == 1);
     auto go = [&](auto s_tag, auto r_tag, auto parts) {
       using SS = decltype(s_tag);
       using RR = decltype(r_tag);
