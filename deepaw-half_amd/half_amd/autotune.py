@@ -16,6 +16,8 @@ block while sweeping, per precision:
   contract.<p>.gram.wt     tile width of the coefficient contraction B^H psi
   contract.<p>.gram.fill   round fill its row-chunk count must reach
   contract.<p>.tall.wt     tile width of the expansion B (D c), B (Q c)
+  contract.<p>.tall.order  its workgroup order: row tiles first (0), column
+                           tiles first (1), 1 with an XCD-contiguous renumbering (2)
   contract.fp64.*.gauss    FP64 complex products with three MFMAs or four
 The tiles are swept with the separate passes (fused off, so every pass runs):
 each candidate is applied to all three passes at once and every pass keeps
@@ -168,6 +170,16 @@ def tune(workload="hfo2", bands=0, steps=10, verbose=True):
             result["best"]["contract.fp64.tall.gauss"] = ga
         set_tuning(f"contract.{p}.tall.wt", wt)
         result["best"][f"contract.{p}.tall.wt"] = wt
+        obest = (1, float("inf"))
+        for order in (0, 1, 2):
+            set_tuning(f"contract.{p}.tall.order", order)
+            kern = run()
+            ms = kern["proj_expand_mfma"][0] / steps if "proj_expand_mfma" in kern else float("inf")
+            result["timings"][f"{p}.tall.order={order}"] = ms
+            if ms < obest[1]:
+                obest = (order, ms)
+        set_tuning(f"contract.{p}.tall.order", obest[0])
+        result["best"][f"contract.{p}.tall.order"] = obest[0]
         # the Vx rows go to LDS only with band groups of two or more
         forms = ([(0, 1, 0, ov, pj) for ov in (0, 1) for pj in (0, 1)]
                  + [(1, 1, 0, ov, pj) for ov in (0, 1) for pj in (0, 1)]

@@ -245,7 +245,10 @@ int half_amd_trace_read(int max_kernels, char *names, double *total_ms, int64_t 
  * LDS); "fp32" keys for the FP32 operator; "operator.overlap" (projector
  * stage on a second stream) and "operator.projector" (1: projector stage on
  * the skinny 16x16-tile kernels, 0: on the 64-wide contractions with a
- * separate D/Q mix); "solver.hetrd" (1: Hermitian eigensolves with vectors
+ * separate D/Q mix); "contract.fp64.tall.order" / "contract.fp32.tall.order"
+ * (workgroup order of those contractions' expansion B (D c), B (Q c):
+ * 0 row tiles first, 1 column tiles first, the default, 2 as 1 with each
+ * XCD taking a contiguous range of row tiles); "solver.hetrd" (1: Hermitian eigensolves with vectors
  * of order <= 256 reduce to tridiagonal form in one workgroup before
  * rocSOLVER's zstedc / zunmtr, 0: rocSOLVER's zheevd throughout);
  * "solver.smalleig" (1, the default: the k-point mesh solve's subspace

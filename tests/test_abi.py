@@ -179,23 +179,25 @@ def test_solver_tuning_keys_host_only():
 
 def test_contract_tuning_keys_host_only():
     """contract.<fp64|fp32>.gram.wt / tall.wt (0 = shape rule, 1, 2),
-    contract.<fp64|fp32>.gram.fill (10..100 percent) and
+    contract.<fp64|fp32>.gram.fill (10..100 percent),
+    contract.<fp64|fp32>.tall.order (0, 1, 2; default 1) and
     contract.fp64.gram.gauss / tall.gauss (0, 1) are host settings: set, read
     back, restored; out-of-range values and unknown keys are rejected."""
-    keys = [f"contract.{p}.{k}" for p in ("fp64", "fp32") for k in ("gram.wt", "tall.wt", "gram.fill")]
+    keys = [f"contract.{p}.{k}" for p in ("fp64", "fp32") for k in ("gram.wt", "tall.wt", "gram.fill", "tall.order")]
     keys += ["contract.fp64.gram.gauss", "contract.fp64.tall.gauss"]
     saved = {k: H.get_tuning(k) for k in keys}
     assert all(saved[k] == 0 for k in keys if k.endswith(".wt"))
-    assert all(saved[k] == 1 for k in keys if k.endswith(".gauss"))
+    assert all(saved[k] == 1 for k in keys if k.endswith(".gauss") or k.endswith(".order"))
     assert all(saved[k] == 90 for k in keys if k.endswith(".fill"))
     try:
         for k in keys:
-            good = 75 if k.endswith(".fill") else 0 if k.endswith(".gauss") else 1
+            good = 75 if k.endswith(".fill") else 0 if k.endswith(".gauss") else 2 if k.endswith(".order") else 1
             H.set_tuning(k, good)
             assert H.get_tuning(k) == good
         for key, bad in (("contract.fp64.gram.wt", 3), ("contract.fp32.tall.wt", -1),
                          ("contract.fp64.gram.fill", 9), ("contract.fp32.gram.fill", 101), ("contract.fp16.gram.wt", 1),
-                         ("contract.fp64.tall.gauss", 2), ("contract.fp32.tall.gauss", 1)):
+                         ("contract.fp64.tall.gauss", 2), ("contract.fp32.tall.gauss", 1),
+                         ("contract.fp64.tall.order", 3), ("contract.fp32.tall.order", -1)):
             with pytest.raises(Exception, match="unknown tuning key or value out of range"):
                 H.set_tuning(key, bad)
     finally:

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 SI2 = ("tests/golden/inputs/si2.CHGCAR", "tests/golden/inputs/si2.POTCAR")
 PER_PREC = ["local.{p}.tile.x", "local.{p}.tile.y", "local.{p}.tile.z", "local.{p}.xgroup", "local.{p}.xconv",
             "local.{p}.fused", "local.{p}.ygroup", "local.{p}.vxlds", "contract.{p}.gram.wt",
-            "contract.{p}.gram.fill", "contract.{p}.tall.wt"]
+            "contract.{p}.gram.fill", "contract.{p}.tall.wt", "contract.{p}.tall.order"]
 
 
 def test_autotune_si2_keys_and_result():
@@ -31,6 +31,7 @@ def test_autotune_si2_keys_and_result():
                 assert k.format(p=p) in best
             assert f"{p}.tuned_step_ms" in res["timings"] and res["timings"][f"{p}.tuned_step_ms"] > 0
             assert f"{p}.xconv=2" in res["timings"]
+            assert all(f"{p}.tall.order={o}" in res["timings"] for o in (0, 1, 2))
         assert "fp64.tall.gauss=1,wt=2" in res["timings"] and "fp32.tall.gauss=None,wt=2" in res["timings"]
         assert best["contract.fp64.gram.gauss"] in (0, 1) and best["contract.fp64.tall.gauss"] in (0, 1)
         # the FP32 choices are in effect now; the shared operator keys hold the FP32 sweep's

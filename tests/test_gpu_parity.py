@@ -451,6 +451,43 @@ def test_hfo2_veff_and_dense(hfo2):
     assert abs(smin - g["overlap_min"]) < 1e-12 and abs(smax - g["overlap_max"]) < 1e-12
 
 
+def test_hfo2_expansion_workgroup_orders(hfo2):
+    """The projector stage on the 64-wide contractions (operator.projector
+    0) with each workgroup order of the expansion (contract.<p>.tall.order
+    0, 1, 2): 96 bands give two column tiles and the FP64 form splits its
+    two outputs over the grid.  Every order computes each output tile with
+    the same products in the same sequence, so the results are identical;
+    FP64 against the skinny-kernel form within 1e-13 relative."""
+    from half_amd import hip
+    g, ctx = hfo2
+    nb = 96
+    psi = psi_block(g["npw"], nb)
+    keys = ["operator.projector"] + [f"contract.fp{p}.tall.order" for p in (64, 32)]
+    saved = {k: H.get_tuning(k) for k in keys}
+    try:
+        for prec, dt in ((64, np.complex128), (32, np.complex64)):
+            op = ctx.operator(g["kpoint"], prec)
+            x = hip.Buffer.from_numpy(np.ascontiguousarray(psi.T).astype(dt))
+            out = {}
+            for proj, order in ((1, 1), (0, 0), (0, 1), (0, 2)):
+                print(f"[gpu] hfo2 fp{prec} projector={proj} tall.order={order}", flush=True)
+                H.set_tuning("operator.projector", proj)
+                H.set_tuning(f"contract.fp{prec}.tall.order", order)
+                hp, sp = hip.Buffer(x.shape, dt), hip.Buffer(x.shape, dt)
+                op.apply_device(nb, x.ptr, hp.ptr, sp.ptr)
+                hip.synchronize()
+                out[(proj, order)] = (hp.numpy(), sp.numpy())
+            op.close()
+            for i in range(2):
+                assert np.array_equal(out[(0, 0)][i], out[(0, 1)][i]), (prec, i)
+                assert np.array_equal(out[(0, 0)][i], out[(0, 2)][i]), (prec, i)
+                a, b = out[(1, 1)][i].astype(np.complex128), out[(0, 1)][i].astype(np.complex128)
+                assert np.abs(a - b).max() / np.abs(a).max() < (1e-13 if prec == 64 else 2e-6), (prec, i)
+    finally:
+        for k, v in saved.items():
+            H.set_tuning(k, v)
+
+
 def test_hfo2_forces_finite_difference(hfo2):
     g, ctx = hfo2
     nb = 32
