@@ -114,9 +114,12 @@ def main():
 
     w = synth.workload(args.workload)
     nb = args.bands or w.nbands
-    # buffer bands: 216-atom cell, 512 bands, locking mode 2, TPA: 64 of them
-    # took 63 iterations (FP64 1.65 s, mixed 1.46 s), 128 took 25 (1.42 s,
-    # 1.30 s); 48 left the FP64 solve stagnating above tol for 600 iterations
+    # buffer bands: 216-atom cell, 512 bands, locking mode 2, TPA, no buffer
+    # correction columns (solver.bufcols 0): 96 of them took 50 iterations
+    # (FP64 1.23 s, mixed 1.12 s), 128 took 39 (1.19 s, 1.09 s), 160 36
+    # (1.19, 1.09 s), 192 28 (1.15, 1.09 s), 256 27 (1.16, 1.13 s); with
+    # every buffer band's column 128 took 25 (1.39, 1.27 s), 64 took 63 and
+    # 48 left the FP64 solve stagnating above tol for 600 iterations
     extra = args.extra if args.extra >= 0 else max(8, nb // 4)
     tmp = tempfile.mkdtemp(prefix="halfbench")
     chg, pot, _ = synth.make_inputs(args.workload, tmp)

@@ -88,7 +88,9 @@ typedef struct half_amd_solve_stats {
  * stay S-orthogonal to them; the FP32 phase's frozen bands rejoin at the
  * switch to FP64), and one Rayleigh-Ritz step over all bands ends the solve;
  * 0 off.  The stopping rule still reads every requested band.
- * extra buffer bands are carried internally and not reported.  eigenvectors
+ * extra buffer bands are carried internally and not reported; with locking
+ * on, only the lowest "solver.bufcols" percent of them (default 0) get
+ * correction columns (all of them once no requested band needs one).  eigenvectors
  * (host, ld >= npw) may be null.  ACC through half_solve_kpoint never locks
  * (the reference's iteration). */
 int half_amd_solve_acc(half_handle handle, const double kpoint[3], int nbands, int mixed, double tol,

@@ -613,6 +613,33 @@ def test_hfo2_acc_soft_locking(hfo2):
     assert H.get_tuning("solver.lock") == saved
 
 
+def test_hfo2_acc_buffer_columns(hfo2):
+    """solver.bufcols: with every buffer band's correction column (100), half
+    of them (50) or none (0, the default) the locked solve converges to the
+    reference eigenvalues, FP64 and mixed; fewer buffer columns give narrower
+    Rayleigh-Ritz problems."""
+    g, ctx = hfo2
+    ref = np.array(g["eigenvalues"][:64])
+    saved = H.get_tuning("solver.bufcols")
+    assert saved == 0
+    out = {}
+    try:
+        for pct in (100, 50, 0):
+            H.set_tuning("solver.bufcols", pct)
+            for mixed in (False, True):
+                w, _, st = ctx.solve_acc(64, mixed=mixed, tol=1e-6, maxiter=300, extra=16)
+                print(f"[gpu] hfo2 acc bufcols={pct} mixed={mixed}", st, np.abs(w - ref).max(), flush=True)
+                out[pct, mixed] = (w, st)
+    finally:
+        H.set_tuning("solver.bufcols", saved)
+    for mixed in (False, True):
+        for pct in (100, 50, 0):
+            w, st = out[pct, mixed]
+            assert st["max_residual_eV"] <= 1e-6
+            assert np.abs(w - ref).max() < (1e-8 if mixed else 1e-9), (pct, mixed)
+        assert out[0, mixed][1]["rr_order_mean"] < out[100, mixed][1]["rr_order_mean"]
+
+
 # --------------------------------------------------------------------------
 # potential-dependent D correction (QDEP) on the device
 # --------------------------------------------------------------------------
