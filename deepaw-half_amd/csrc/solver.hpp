@@ -19,6 +19,7 @@ struct Blas {
   hipStream_t stream = nullptr;
   DeviceArray<double2> eig_z, eig_tau, eig_coop;  // workspace of the small eigensolver
   DeviceArray<int> eig_info;                      // statuses of the one-workgroup eigensolver
+  DeviceArray<double> eig_dev;                    // per-workgroup shares of a metric's deviation from I
   void init(hipStream_t s);
   void release();
   ~Blas();

@@ -148,9 +148,11 @@ def test_no_device_is_reported_not_aborted():
 def test_solver_tuning_keys_host_only():
     """solver.lock (0 off, 1 soft locking, 2 soft locking plus a frozen
     prefix of converged bands, the default), solver.stall (FP32-phase
-    stagnation window, 0..1000) and solver.bufcols (percent of the buffer
-    bands given correction columns, default 0) are host settings: set, read
-    back, restored; out-of-range values are rejected with INVALID_ARGUMENT."""
+    stagnation window, 0..1000), solver.bufcols (percent of the buffer
+    bands given correction columns, default 0) and solver.identity (0, or
+    6..15: the exponent of the unit-metric threshold, default 10) are host
+    settings: set, read back, restored; out-of-range values are rejected
+    with INVALID_ARGUMENT."""
     saved = {k: H.get_tuning(k) for k in ("solver.lock", "solver.stall", "solver.hetrd")}
     assert saved["solver.lock"] == 2 and saved["solver.stall"] == 4
     try:
@@ -161,9 +163,11 @@ def test_solver_tuning_keys_host_only():
         H.set_tuning("solver.lock", 1)
         assert H.get_tuning("solver.lock") == 1
         assert H.get_tuning("solver.smalleig") == 1 and H.get_tuning("solver.bufcols") == 0
+        assert H.get_tuning("solver.identity") == 10
         for key, good in (("solver.precond", 1), ("solver.refresh", 8), ("solver.fp32algebra", 0),
                           ("solver.smalleig", 2), ("solver.smalleig", 0), ("solver.smalleig", 3),
-                          ("solver.bufcols", 100), ("solver.bufcols", 25)):
+                          ("solver.bufcols", 100), ("solver.bufcols", 25), ("solver.identity", 0),
+                          ("solver.identity", 6), ("solver.identity", 15)):
             before = H.get_tuning(key)
             H.set_tuning(key, good)
             assert H.get_tuning(key) == good
@@ -171,7 +175,8 @@ def test_solver_tuning_keys_host_only():
         for key, bad in (("solver.lock", 3), ("solver.lock", -1), ("solver.stall", -1), ("solver.stall", 1001),
                          ("solver.precond", 2), ("solver.refresh", -1), ("solver.fp32algebra", 2),
                          ("solver.smalleig", 4), ("solver.smalleig", -1), ("solver.bufcols", 101),
-                         ("solver.bufcols", -1), ("solver.nope", 0)):
+                         ("solver.bufcols", -1), ("solver.identity", 5), ("solver.identity", 16),
+                         ("solver.nope", 0)):
             with pytest.raises(Exception, match="unknown tuning key or value out of range"):
                 H.set_tuning(key, bad)
         assert H.get_tuning("solver.stall") == 7
