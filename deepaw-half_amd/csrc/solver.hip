@@ -470,15 +470,19 @@ void generalized_eigh(Blas &blas, int k, double2 *A, double2 *B, double *w, doub
 // The standard problem A v = e' v is solved instead and each kept pair is
 // corrected by its metric Rayleigh quotient, b = v^H B v: e = e' / b,
 // y = v / sqrt(b).  The residual of (e, y) in A y = e B y is then
-// e (b - 1) y - e E y / sqrt(b), orthogonal to y to first order in E, so e
-// is exact to second order (|e| |E|^2 / gap); the kept vectors are B-normal
-// and B-orthogonal to within |E|.  Where the normalisation was ill
-// conditioned (pivots near its 1e-10 floor), or in FP32-product iterations
-// (E ~ 1e-7), the deviation exceeds the threshold and the generalized form
-// runs.  216-atom cell, 512 + 128 bands, tol 1e-5 eV: eigensolves 485 ->
-// 414 ms per FP64 solve at 1e-10 (1.18 -> 1.11 s; eigenvalues unchanged to
-// 1e-13 eV); HfO2 0.050 -> 0.043 s.
-int g_identity = 10;
+// e (b - 1) y - e E y / sqrt(b), orthogonal to y to first order in E, so an
+// eigenvalue set apart from the others by more than |e| |E| is exact to
+// second order (|e| |E|^2 / gap); within a (nearly) degenerate cluster the
+// off-diagonal part of E still splits the values at first order, by at most
+// |e| max|E| (1e-10 eV for |e| <= 10 eV at the default 1e-11).  The kept
+// vectors are B-normal and B-orthogonal to within |E|.  Where the
+// normalisation was ill conditioned (pivots near its 1e-10 floor), or in
+// FP32-product iterations (E ~ 1e-7), the deviation exceeds the threshold
+// and the generalized form runs.  216-atom cell, 512 + 128 bands, tol 1e-5
+// eV: eigensolves 485 -> 415 ms per FP64 solve at 1e-11 (the same steps as at
+// 1e-10; 1.18 -> 1.11 s; eigenvalues unchanged to 1e-13 eV); HfO2 0.050 ->
+// 0.043 s.
+int g_identity = 11;
 constexpr int kDeviationGroups = 256;
 
 // max |B - I| over B's upper triangle (order k), read back to the host

@@ -92,10 +92,10 @@ typedef struct half_amd_solve_stats {
  * on, only the lowest "solver.bufcols" percent of them (default 0) get
  * correction columns (all of them once no requested band needs one).
  * Rayleigh-Ritz steps whose subspace metric is the identity to within
- * 10^-"solver.identity" entrywise (default 10; 0 = never) solve the standard
+ * 10^-"solver.identity" entrywise (default 11; 0 = never) solve the standard
  * problem and correct each kept pair by its metric Rayleigh quotient
  * instead of the Cholesky reduction (eigenvalues exact to second order in
- * the deviation).  eigenvectors
+ * the deviation, within degenerate clusters to |eigenvalue| x 10^-11).  eigenvectors
  * (host, ld >= npw) may be null.  ACC through half_solve_kpoint never locks
  * (the reference's iteration). */
 int half_amd_solve_acc(half_handle handle, const double kpoint[3], int nbands, int mixed, double tol,

@@ -150,7 +150,7 @@ def test_solver_tuning_keys_host_only():
     prefix of converged bands, the default), solver.stall (FP32-phase
     stagnation window, 0..1000), solver.bufcols (percent of the buffer
     bands given correction columns, default 0) and solver.identity (0, or
-    6..15: the exponent of the unit-metric threshold, default 10) are host
+    6..15: the exponent of the unit-metric threshold, default 11) are host
     settings: set, read back, restored; out-of-range values are rejected
     with INVALID_ARGUMENT."""
     saved = {k: H.get_tuning(k) for k in ("solver.lock", "solver.stall", "solver.hetrd")}
@@ -163,7 +163,7 @@ def test_solver_tuning_keys_host_only():
         H.set_tuning("solver.lock", 1)
         assert H.get_tuning("solver.lock") == 1
         assert H.get_tuning("solver.smalleig") == 1 and H.get_tuning("solver.bufcols") == 0
-        assert H.get_tuning("solver.identity") == 10
+        assert H.get_tuning("solver.identity") == 11
         for key, good in (("solver.precond", 1), ("solver.refresh", 8), ("solver.fp32algebra", 0),
                           ("solver.smalleig", 2), ("solver.smalleig", 0), ("solver.smalleig", 3),
                           ("solver.bufcols", 100), ("solver.bufcols", 25), ("solver.identity", 0),
