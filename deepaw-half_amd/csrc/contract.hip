@@ -713,6 +713,7 @@ void gemm_tall(int n, int m, int k, R alpha, const cplx_t<R> *A, int64_t lda, co
 
 ContractFp32Scope::ContractFp32Scope(bool on) : prev_(t_fp32_products) { t_fp32_products = on; }
 ContractFp32Scope::~ContractFp32Scope() { t_fp32_products = prev_; }
+bool contract_fp32_products() { return t_fp32_products; }
 
 bool set_contract_tuning(const std::string &key, int value) {
   for (int p = 0; p < 2; ++p) {

@@ -68,6 +68,10 @@ class ContractFp32Scope {
   bool prev_;
 };
 
+// Whether the calling thread's FP64 contractions currently form FP32
+// products (inside a ContractFp32Scope(true)).
+bool contract_fp32_products();
+
 // Tuning keys contract.<fp64|fp32>.gram.wt / tall.wt (0 = shape rule, 1 =
 // 32-wide tiles, 2 = 64-wide) and contract.<fp64|fp32>.gram.fill (10-100:
 // percent of the device's workgroup slots every round of a row-chunked gram

@@ -483,6 +483,13 @@ void generalized_eigh(Blas &blas, int k, double2 *A, double2 *B, double *w, doub
 // 1e-10; 1.18 -> 1.11 s; eigenvalues unchanged to 1e-13 eV); HfO2 0.050 ->
 // 0.043 s.
 int g_identity = 11;
+// The same for the Rayleigh-Ritz steps of the mixed form's FP32 phase, whose
+// Grams are formed with FP32 products (key "solver.identity32", default 5,
+// 0 = never): their metrics deviate by ~1e-7, the pairs they give only steer
+// the FP32 iterations (the switch to FP64 redoes the Rayleigh-Ritz step with
+// FP64 products and the FP64 threshold), and a cluster split of |e| 1e-5 is
+// far below the FP32 operator's own error there (~2e-4 eV in the residual).
+int g_identity32 = 5;
 constexpr int kDeviationGroups = 256;
 
 // max |B - I| over B's upper triangle (order k), read back to the host
@@ -512,7 +519,8 @@ void generalized_eigh_ritz(Blas &blas, int k, int nv, double2 *A, double2 *B, do
     generalized_eigh_small(blas, k, nv, A, B, w, e, what);
     return;
   }
-  if (g_identity > 0 && identity_deviation(blas, k, B) <= std::pow(10.0, -g_identity)) {
+  const int level = contract_fp32_products() ? g_identity32 : g_identity;
+  if (level > 0 && identity_deviation(blas, k, B) <= std::pow(10.0, -level)) {
     trace_mark("acc_subspace_eigensolve_unit_metric", blas.stream);
     heevd_vectors(blas, k, A, w, e, info + 1);
     // metric Rayleigh quotients of the kept pairs: T = B V(:, :nv)
@@ -629,6 +637,11 @@ bool set_solver_tuning(const std::string &key, int value) {
     g_identity = value;
     return true;
   }
+  if (key == "solver.identity32") {
+    if (value != 0 && (value < 3 || value > 15)) return false;
+    g_identity32 = value;
+    return true;
+  }
   if (key != "solver.hetrd" || value < 0 || value > 2) return false;
   g_hetrd = value != 0;
   g_hetrd_mode = value;
@@ -660,6 +673,8 @@ bool get_solver_tuning(const std::string &key, int &value) {
     value = g_bufcols;
   } else if (key == "solver.identity") {
     value = g_identity;
+  } else if (key == "solver.identity32") {
+    value = g_identity32;
   } else if (key == "solver.hetrd") {
     value = g_hetrd_mode;
   } else {

@@ -95,7 +95,9 @@ typedef struct half_amd_solve_stats {
  * 10^-"solver.identity" entrywise (default 11; 0 = never) solve the standard
  * problem and correct each kept pair by its metric Rayleigh quotient
  * instead of the Cholesky reduction (eigenvalues exact to second order in
- * the deviation, within degenerate clusters to |eigenvalue| x 10^-11).  eigenvectors
+ * the deviation, within degenerate clusters to |eigenvalue| x 10^-11);
+ * "solver.identity32" (default 5) is the same threshold for the mixed
+ * form's FP32-phase iterations.  eigenvectors
  * (host, ld >= npw) may be null.  ACC through half_solve_kpoint never locks
  * (the reference's iteration). */
 int half_amd_solve_acc(half_handle handle, const double kpoint[3], int nbands, int mixed, double tol,

@@ -149,10 +149,11 @@ def test_solver_tuning_keys_host_only():
     """solver.lock (0 off, 1 soft locking, 2 soft locking plus a frozen
     prefix of converged bands, the default), solver.stall (FP32-phase
     stagnation window, 0..1000), solver.bufcols (percent of the buffer
-    bands given correction columns, default 0) and solver.identity (0, or
-    6..15: the exponent of the unit-metric threshold, default 11) are host
-    settings: set, read back, restored; out-of-range values are rejected
-    with INVALID_ARGUMENT."""
+    bands given correction columns, default 0), solver.identity (0, or
+    6..15: the exponent of the unit-metric threshold, default 11) and
+    solver.identity32 (0, or 3..15, the same for FP32-product iterations,
+    default 5) are host settings: set, read back, restored; out-of-range
+    values are rejected with INVALID_ARGUMENT."""
     saved = {k: H.get_tuning(k) for k in ("solver.lock", "solver.stall", "solver.hetrd")}
     assert saved["solver.lock"] == 2 and saved["solver.stall"] == 4
     try:
@@ -163,11 +164,12 @@ def test_solver_tuning_keys_host_only():
         H.set_tuning("solver.lock", 1)
         assert H.get_tuning("solver.lock") == 1
         assert H.get_tuning("solver.smalleig") == 1 and H.get_tuning("solver.bufcols") == 0
-        assert H.get_tuning("solver.identity") == 11
+        assert H.get_tuning("solver.identity") == 11 and H.get_tuning("solver.identity32") == 5
         for key, good in (("solver.precond", 1), ("solver.refresh", 8), ("solver.fp32algebra", 0),
                           ("solver.smalleig", 2), ("solver.smalleig", 0), ("solver.smalleig", 3),
                           ("solver.bufcols", 100), ("solver.bufcols", 25), ("solver.identity", 0),
-                          ("solver.identity", 6), ("solver.identity", 15)):
+                          ("solver.identity", 6), ("solver.identity", 15), ("solver.identity32", 0),
+                          ("solver.identity32", 3)):
             before = H.get_tuning(key)
             H.set_tuning(key, good)
             assert H.get_tuning(key) == good
@@ -176,6 +178,7 @@ def test_solver_tuning_keys_host_only():
                          ("solver.precond", 2), ("solver.refresh", -1), ("solver.fp32algebra", 2),
                          ("solver.smalleig", 4), ("solver.smalleig", -1), ("solver.bufcols", 101),
                          ("solver.bufcols", -1), ("solver.identity", 5), ("solver.identity", 16),
+                         ("solver.identity32", 2), ("solver.identity32", 16),
                          ("solver.nope", 0)):
             with pytest.raises(Exception, match="unknown tuning key or value out of range"):
                 H.set_tuning(key, bad)

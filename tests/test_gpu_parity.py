@@ -674,6 +674,30 @@ def test_hfo2_acc_unit_metric_rayleigh_ritz(hfo2):
     assert np.abs(out[11][0] - out[0][0]).max() < 1e-10
 
 
+def test_hfo2_mixed_unit_metric_fp32_phase(hfo2):
+    """solver.identity32: in the mixed form's FP32 phase (Grams with FP32
+    products, metrics off the identity by ~1e-7) Rayleigh-Ritz steps within
+    1e-5 take the standard-problem form (the default) or never (0); either
+    way the FP64 sweeps bring every band within 1e-8 eV of the reference."""
+    g, ctx = hfo2
+    ref = np.array(g["eigenvalues"][:64])
+    saved = H.get_tuning("solver.identity32")
+    assert saved == 5
+    out = {}
+    try:
+        for level in (5, 0):
+            H.set_tuning("solver.identity32", level)
+            w, _, st = ctx.solve_acc(64, mixed=True, tol=1e-6, maxiter=300, extra=16)
+            print(f"[gpu] hfo2 mixed identity32={level}", st, np.abs(w - ref).max(), flush=True)
+            out[level] = (w, st)
+    finally:
+        H.set_tuning("solver.identity32", saved)
+    for level in (5, 0):
+        w, st = out[level]
+        assert st["max_residual_eV"] <= 1e-6 and st["fp64_iterations"] >= 1
+        assert np.abs(w - ref).max() < 1e-8, level
+
+
 # --------------------------------------------------------------------------
 # potential-dependent D correction (QDEP) on the device
 # --------------------------------------------------------------------------
