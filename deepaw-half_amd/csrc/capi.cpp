@@ -298,10 +298,12 @@ void run_solve(Context &c, const double *k, int nbands, bool want_vectors, Devic
   if (nbands < 1 || nbands > op.n) throw Error(kInvalidArgument, "invalid requested band count");
   if (c.solver == kSolverAcc) {
     // reference controls: tol 1e-4 eV, 40 iterations, every band's correction
-    // column in each Rayleigh-Ritz step (src/half_cuda_iterative_solver.cuf:113-119)
+    // column in each Rayleigh-Ritz step, each step through the Cholesky
+    // reduction (src/half_cuda_iterative_solver.cuf:113-119)
     AccControls ctl;
     ctl.lock = 0;
     ctl.precond = 0;
+    ctl.identity = 0;
     solve_acc(c.blas, op, nullptr, nbands, ctl, want_vectors, vec, res);
   } else {
     solve_dense(c.blas, op, c.solver, nbands, want_vectors, vec, res);

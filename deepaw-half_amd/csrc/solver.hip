@@ -514,12 +514,12 @@ double identity_deviation(Blas &blas, int k, const double2 *B) {
 // about 50 us each at these orders, the inverse and three products less.
 // Larger orders take generalized_eigh.
 void generalized_eigh_ritz(Blas &blas, int k, int nv, double2 *A, double2 *B, double *w, double *e,
-                           rocblas_int *info, const char *what) {
+                           rocblas_int *info, const char *what, bool unit_metric = true) {
   if (g_smalleig == 2 && k <= kSmallMaxOrder) {
     generalized_eigh_small(blas, k, nv, A, B, w, e, what);
     return;
   }
-  const int level = contract_fp32_products() ? g_identity32 : g_identity;
+  const int level = unit_metric ? (contract_fp32_products() ? g_identity32 : g_identity) : 0;
   if (level > 0 && identity_deviation(blas, k, B) <= std::pow(10.0, -level)) {
     trace_mark("acc_subspace_eigensolve_unit_metric", blas.stream);
     heevd_vectors(blas, k, A, w, e, info + 1);
@@ -904,7 +904,7 @@ void solve_acc(Blas &blas, KOperator<double> &op64, KOperator<float> *op32, int 
   // the lowest nv (default m) eigenvectors in hsub(:, :nv), values from w + w0
   auto small_generalized = [&](int k, int nv = -1, int w0 = 0) {
     generalized_eigh_ritz(blas, k, nv < 0 ? m : nv, wk.hsub.get(), wk.ssub.get(), wk.w.get() + w0, wk.E.get(),
-                          wk.info.get(), "ACC small generalized eigensolve");
+                          wk.info.get(), "ACC small generalized eigensolve", ctl.identity != 0);
   };
   // (hsub, ssub) = (A^H HA, A^H SA) for k columns of Z-shaped blocks
   auto gram_hs = [&](int k, const double2 *A, const double2 *HA, const double2 *SA) {

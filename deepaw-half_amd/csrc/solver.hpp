@@ -87,6 +87,9 @@ struct AccControls {
                           // converged bands, -1 the process-wide key "solver.lock"
   int precond = -1;       // 0 the reference's preconditioner, 1 Teter-Payne-Allan,
                           // -1 the process-wide key "solver.precond"
+  int identity = -1;      // 0: every Rayleigh-Ritz step through the Cholesky
+                          // reduction (the reference's zhegv); -1 the
+                          // process-wide keys "solver.identity" / "solver.identity32"
   int lock_mode() const;
   int precond_mode() const;
   bool locking() const;

@@ -99,7 +99,8 @@ typedef struct half_amd_solve_stats {
  * "solver.identity32" (default 5) is the same threshold for the mixed
  * form's FP32-phase iterations.  eigenvectors
  * (host, ld >= npw) may be null.  ACC through half_solve_kpoint never locks
- * (the reference's iteration). */
+ * and reduces every Rayleigh-Ritz step by Cholesky (the reference's
+ * iteration). */
 int half_amd_solve_acc(half_handle handle, const double kpoint[3], int nbands, int mixed, double tol,
                        double switch_tol, int maxiter, int block, int extra, double *eigenvalues_eV,
                        HALF_COMPLEX *eigenvectors, int64_t ld, half_amd_solve_stats *stats,

@@ -203,27 +203,27 @@ def test_acc_abi_defaults(si2):
     _, s = ctx.assemble_hs(g["kpoint"])
     assert np.abs(v.conj().T @ s @ v - np.eye(g["nbands"])).max() < 1e-10
     # the drop-in path runs the reference's iteration (full 2m-wide subspace,
-    # the reference's preconditioner) whatever the process-wide locking key
-    # says: the same Ritz pairs under any key value, and the same as the
-    # extension's solve with locking off at the reference's controls
-    saved = H.get_tuning("solver.lock")
-    saved_pc = H.get_tuning("solver.precond")
+    # the reference's preconditioner, every Rayleigh-Ritz step through the
+    # Cholesky reduction) whatever the process-wide keys say: the same Ritz
+    # pairs under any key values, and the same as the extension's solve with
+    # those three switched to the reference's at the reference's controls
+    keys = ("solver.lock", "solver.precond", "solver.identity")
+    saved = {k: H.get_tuning(k) for k in keys}
     try:
-        # and whatever the preconditioner key says
-        for lock, pc in ((2, 1), (1, 0), (0, 1)):
-            H.set_tuning("solver.lock", lock)
-            H.set_tuning("solver.precond", pc)
+        for values in ((2, 1, 11), (1, 0, 8), (0, 1, 0)):
+            for k, val in zip(keys, values):
+                H.set_tuning(k, val)
             ctx.set_solver(H.SOLVER_ACC)
             wa, va, _, _ = ctx.solve(g["nbands"], g["kpoint"], vectors=True)
             ctx.set_solver(H.SOLVER_EVD)
             assert np.abs(wa - w).max() == 0.0 and np.abs(va - v).max() == 0.0
-        H.set_tuning("solver.lock", 0)
-        H.set_tuning("solver.precond", 0)
+        for k in keys:
+            H.set_tuning(k, 0)
         we, ve, st = ctx.solve_acc(g["nbands"], g["kpoint"], tol=1e-4, maxiter=40, vectors=True)
     finally:
         ctx.set_solver(H.SOLVER_EVD)
-        H.set_tuning("solver.lock", saved)
-        H.set_tuning("solver.precond", saved_pc)
+        for k, val in saved.items():
+            H.set_tuning(k, val)
     assert st["rr_order_mean"] == 2 * g["nbands"]
     assert np.abs(we - w).max() < 1e-12 and np.abs(ve - v).max() < 1e-10
 
