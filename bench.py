@@ -229,6 +229,8 @@ def main():
             H.set_tuning("solver.precond", 0)
             H.set_tuning("solver.identity", 0)
             try:
+                # warm-up: the reduction's library kernels load on first use
+                ctx.solve_acc(nb, k, tol=args.tol, maxiter=2, extra=extra)
                 t = time.perf_counter()
                 wp, _, st = ctx.solve_acc(nb, k, tol=args.tol, maxiter=800, extra=extra)
                 el = time.perf_counter() - t
