@@ -519,8 +519,12 @@ void generalized_eigh_ritz(Blas &blas, int k, int nv, double2 *A, double2 *B, do
     generalized_eigh_small(blas, k, nv, A, B, w, e, what);
     return;
   }
+  // (k max|E| bounds |E| in the 2-norm: at most 0.1 keeps every b = v^H B v
+  // within 10% of 1 whatever the threshold key allows)
   const int level = unit_metric ? (contract_fp32_products() ? g_identity32 : g_identity) : 0;
-  if (level > 0 && identity_deviation(blas, k, B) <= std::pow(10.0, -level)) {
+  double dev = HUGE_VAL;
+  if (level > 0) dev = identity_deviation(blas, k, B);
+  if (dev <= std::pow(10.0, -level) && dev * k <= 0.1) {
     trace_mark("acc_subspace_eigensolve_unit_metric", blas.stream);
     heevd_vectors(blas, k, A, w, e, info + 1);
     // metric Rayleigh quotients of the kept pairs: T = B V(:, :nv)
