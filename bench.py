@@ -82,8 +82,8 @@ def main():
                          "for profiles) and print their rate")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the k-point and band sharding design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: one GPU only; --gpus {args.gpus} is not supported (the library runs on one "
+              "GPU; sharding k points and bands across GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
