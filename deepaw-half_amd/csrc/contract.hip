@@ -29,7 +29,9 @@ namespace half {
 namespace {
 
 constexpr int KS = 16;  // inner-index rows per LDS slab
-constexpr int PAD = 4;  // LDS row padding (elements)
+// LDS row padding (elements): rows of 65 let the loaders' stores, four
+// threads per column at rows 0, 4, 8, 12 (+i), fall in distinct banks
+constexpr int PAD = 1;
 constexpr int kXcds = 8;  // XCDs of an MI355X: workgroup id w is dispatched to XCD w % 8
 
 template <typename R>
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     tz = g.split ? bx / g.col_tiles : blockIdx.z;
   }
   const int g0 = tr * TW, c0 = tc * TW;
-  const int akc = tid / TPA, ar0 = (tid % TPA) * L;  // A: inner column akc, rows ar0..
+  const int akc = tid / TPA, ar0 = tid % TPA;  // A: inner column akc, rows ar0 + i TPA
   const int wcol = tid / TPW, wk0 = (tid % TPW) * L;  // W: column wcol, inner rows wk0..
   const bool wcv = c0 + wcol < g.m;
   const int64_t bz = g.split ? 0 : tz;
@@ -254,7 +256,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
     const bool kv = kc < g.k;
     const VS *pa = g.A + bz * g.bA + int64_t(min(kc, g.k - 1)) * g.lda;
     for (int i = 0; i < L; ++i) {
-      const int r = g0 + ar0 + i;
+      const int r = g0 + ar0 + i * TPA;
       ra[i] = (kv && r < g.n) ? pa[r] : mk<VS>(0, 0);
     }
     for (int j = 0; j < NB; ++j) {
@@ -274,8 +276,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
   for (int k0 = 0; k0 < g.k; k0 += KS) {
     __syncthreads();
     for (int i = 0; i < L; ++i) {
-      sa[0][akc][ar0 + i] = static_cast<R>(ra[i].x);
-      sa[1][akc][ar0 + i] = static_cast<R>(ra[i].y);
+      sa[0][akc][ar0 + i * TPA] = static_cast<R>(ra[i].x);
+      sa[1][akc][ar0 + i * TPA] = static_cast<R>(ra[i].y);
       for (int j = 0; j < NB; ++j) {
         sw[j][0][wk0 + i][wcol] = static_cast<R>(rw[j][i].x);
         sw[j][1][wk0 + i][wcol] = static_cast<R>(rw[j][i].y);
